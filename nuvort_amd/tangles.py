@@ -1,0 +1,125 @@
+"""Seeded synthetic vortex tangles (initial conditions).
+
+The reference's own initial conditions draw from Fortran ``random_number``,
+whose stream differs between compilers (src/initial_cond.f90:145-148), so the
+project generates its tangles itself and hands them to every implementation
+(the HIP library, the oracle and the reference build) as the same state.
+
+Random numbers come from ``numpy.random.Generator(PCG64(seed))``.  Filaments
+are discretised with chords of at most ``0.98 * delta`` so that neither point
+insertion (``|x - x_infront| > delta``, src/line.f90:40) nor removal
+(``|x_i - x_{i+2}| < 0.99 delta``, src/line.f90:115) fires at t = 0.
+Positions are wrapped into the periodic box the way src/boundary.f90:91-138
+wraps them (a single +-L shift per component).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .state import FilamentState
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _wrap(x: np.ndarray, box) -> np.ndarray:
+    box = np.asarray(box, np.float64)
+    x = x.copy()
+    for k in range(3):
+        L = box[k]
+        hi = x[:, k] > L / 2.0
+        x[hi, k] -= L
+        lo = x[:, k] < -L / 2.0
+        x[lo, k] += L
+    return x
+
+
+def _assemble(loops, box) -> FilamentState:
+    """Concatenate closed loops (each an (n,3) array, consecutive points along
+    the line) into one state; point k of a loop has infront k+1 (cyclic)."""
+    n = sum(len(p) for p in loops)
+    st = FilamentState.empty(n)
+    off = 0
+    for pts in loops:
+        m = len(pts)
+        idx = np.arange(m)
+        st.x[off:off + m] = pts
+        st.infront[off:off + m] = off + (idx + 1) % m + 1
+        st.behind[off:off + m] = off + (idx - 1) % m + 1
+        off += m
+    st.x = _wrap(st.x, box)
+    return st
+
+
+def ring(center, radius, normal, delta, phase=0.0, chord=0.98):
+    """Points of one circular loop (chord <= chord*delta)."""
+    normal = np.asarray(normal, np.float64)
+    normal = normal / np.linalg.norm(normal)
+    a = np.array([1.0, 0.0, 0.0]) if abs(normal[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(normal, a)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(normal, e1)
+    n = max(5, math.ceil(2.0 * math.pi * radius / (chord * delta)))
+    phi = phase + 2.0 * math.pi * np.arange(n) / n
+    return (np.asarray(center, np.float64)[None, :] + radius * np.cos(phi)[:, None] * e1[None, :]
+            + radius * np.sin(phi)[:, None] * e2[None, :])
+
+
+def random_rings(seed: int, nrings: int, rmin: float, rmax: float, box: float, delta: float,
+                 target_points: int | None = None) -> FilamentState:
+    """Rings with radii ~U[rmin, rmax], centres ~U(box), normals uniform on
+    the sphere (normalised Gaussian), phase ~U[0, 2pi).  If ``target_points``
+    is given, rings are added until the point count reaches it instead."""
+    g = _rng(seed)
+    loops, total = [], 0
+    k = 0
+    while (target_points is None and k < nrings) or (target_points is not None and total < target_points):
+        r = g.uniform(rmin, rmax)
+        c = g.uniform(-box / 2.0, box / 2.0, size=3)
+        nv = g.normal(size=3)
+        ph = g.uniform(0.0, 2.0 * math.pi)
+        pts = ring(c, r, nv, delta, ph)
+        loops.append(pts)
+        total += len(pts)
+        k += 1
+    return _assemble(loops, (box, box, box))
+
+
+def kelvin_lattice(seed: int, box: float = 0.1, nx: int = 4, ny: int = 2, amp: float = 0.002,
+                   waves: int = 3, delta: float = 5e-4) -> FilamentState:
+    """Straight periodic lines along z on an nx-by-ny lattice (the lattice
+    formula of src/initial_cond.f90:225-228), each carrying a helical Kelvin
+    wave x += A cos(kz + phi), y += A sin(kz + phi) (the form of
+    src/initial_cond.f90:367-375) with k = 2 pi waves / box and a random phase
+    per line.  infront points towards -z as in src/initial_cond.f90:233."""
+    g = _rng(seed)
+    k = 2.0 * math.pi * waves / box
+    n = math.ceil(box * math.sqrt(1.0 + (amp * k) ** 2) / (0.98 * delta))
+    loops = []
+    for i in range(nx):
+        for j in range(ny):
+            x0 = -box / 2.0 + box * (2 * i + 1) / (2 * nx)
+            y0 = -box / 2.0 + box * (2 * j + 1) / (2 * ny)
+            phi = 2.0 * math.pi * g.uniform()
+            z = -box / 2.0 + box * np.arange(n) / n
+            pts = np.stack([x0 + amp * np.cos(k * z + phi), y0 + amp * np.sin(k * z + phi), z], 1)
+            loops.append(pts[::-1].copy())  # consecutive points run towards -z
+    return _assemble(loops, (box, box, box))
+
+
+def ensemble_sizes(seed: int, ntangles: int, nmin: int, nmax: int) -> np.ndarray:
+    """Target point counts log-uniform in [nmin, nmax]."""
+    g = _rng(seed)
+    return np.floor(np.exp(g.uniform(math.log(nmin), math.log(nmax), size=ntangles))).astype(np.int64)
+
+
+def ensemble(seed: int, ntangles: int, nmin: int, nmax: int, box: float = 0.1, delta: float = 5e-4,
+             rmin: float = 0.01, rmax: float = 0.03):
+    """Independent random-ring tangles, one per target size; returns a list of
+    states (tangle k seeded with seed*1000003 + k)."""
+    sizes = ensemble_sizes(seed, ntangles, nmin, nmax)
+    return [random_rings(seed * 1000003 + k, 0, rmin, rmax, box, delta, target_points=int(s))
+            for k, s in enumerate(sizes)]

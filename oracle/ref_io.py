@@ -1,0 +1,99 @@
+"""Test infrastructure only: state files of the reference build's driver
+(oracle/ref_harness.f90) and a runner for it.
+
+The file layout is the one ref_harness.f90 documents: int32 pcount, int32
+itime, int32 recon_count, real8 t, then every per-point field as a
+contiguous array (vectors component by component).
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+from nuvort_amd.state import FilamentState
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_BIN = os.path.join(HERE, "_ref", "nuvort_ref")
+
+
+def write_state(path: str, st: FilamentState) -> None:
+    n = st.pcount
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<iiid", n, st.itime, st.recon_count, st.t))
+        for name, arr in st.arrays():
+            if arr.ndim == 2:
+                for k in range(3):
+                    fh.write(np.ascontiguousarray(arr[:, k]).astype(arr.dtype).tobytes())
+            else:
+                fh.write(np.ascontiguousarray(arr).tobytes())
+
+
+def read_state(path: str) -> FilamentState:
+    with open(path, "rb") as fh:
+        data = fh.read()
+    n, itime, rc, t = struct.unpack_from("<iiid", data, 0)
+    off = 20
+    st = FilamentState.empty(n)
+    st.itime, st.recon_count, st.t = itime, rc, t
+    for name, arr in st.arrays():
+        dt = arr.dtype
+        if arr.ndim == 2:
+            for k in range(3):
+                arr[:, k] = np.frombuffer(data, dt, n, off)
+                off += n * dt.itemsize
+        else:
+            arr[:] = np.frombuffer(data, dt, n, off)
+            off += n * dt.itemsize
+    assert off == len(data), (off, len(data))
+    return st
+
+
+TREE_REC = np.dtype([("pos", "<f8", 3), ("width", "<f8"), ("cent", "<f8", 3), ("circ", "<f8", 3),
+                     ("pcount", "<i4"), ("leaf_infront", "<i4")])
+
+
+def read_tree(path: str) -> np.ndarray:
+    return np.fromfile(path, TREE_REC)
+
+
+def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, dump_phases: bool = True,
+                  threads: int = 1) -> dict:
+    """Run the reference build on ``st`` for ``nsteps`` steps in ``workdir``.
+    Returns the dumped states (ghost, tree, motion, insert, close, recon,
+    remove for the first step; final after the last), the reconnection log
+    lines and the timing record."""
+    os.makedirs(os.path.join(workdir, "data"), exist_ok=True)
+    with open(os.path.join(workdir, "run.in"), "w") as fh:
+        fh.write(run_text)
+    log = os.path.join(workdir, "data", "recon_location.log")
+    if os.path.exists(log):
+        os.remove(log)
+    write_state(os.path.join(workdir, "state_in.bin"), st)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    subprocess.run([REF_BIN, "state_in.bin", "out", str(nsteps), "1" if dump_phases else "0"],
+                   cwd=workdir, env=env, check=True, capture_output=True)
+    out = {}
+    for ph in ("ghost", "motion", "insert", "close", "recon", "remove", "final"):
+        p = os.path.join(workdir, "out_%s.bin" % ph)
+        if os.path.exists(p):
+            out[ph] = read_state(p)
+    p = os.path.join(workdir, "out_tree.bin")
+    if os.path.exists(p):
+        out["tree"] = read_tree(p)
+    # list-directed output may wrap one record over several lines: read the
+    # log as a token stream of 6 values per reconnection (t, midpoint(3), angle, dl)
+    vals = []
+    if os.path.exists(log):
+        with open(log) as fh:
+            vals = [float(v) for v in fh.read().split()]
+    out["recon_log"] = np.array(vals, np.float64).reshape(-1, 6)
+    timing = {}
+    with open(os.path.join(workdir, "out_timing.txt")) as fh:
+        for line in fh:
+            k, v = line.split()
+            timing[k] = float(v)
+    out["timing"] = timing
+    return out

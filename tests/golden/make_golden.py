@@ -1,0 +1,165 @@
+"""Write the golden fixtures under tests/golden/ from the reference build.
+
+Each fixture is an input state made by this project's seeded generators,
+run through oracle/_ref/nuvort_ref (the reference's own Fortran modules,
+compiled unmodified by ``make -C oracle ref``) with OMP_NUM_THREADS=1, and the
+phase-by-phase outputs it dumps.  Run in the development container only (it
+needs the reference tree and amdflang):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Fixture file layout (npz): ``run_text`` (the run.in used), ``nsteps``,
+``in__<field>`` (input state), ``<phase>__<field>`` for the phases ghost,
+motion, insert, close, recon, remove (first step) and final (after nsteps),
+``tree__<field>`` (depth-first node list of the first step's tree when the
+tree is on) and ``recon_log`` (rows of src/reconnection.f90:48).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from nuvort_amd import tangles  # noqa: E402
+from nuvort_amd.runfile import RunParams, format_run_text  # noqa: E402
+from nuvort_amd.state import FilamentState  # noqa: E402
+from oracle import ref_io  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+FIELDS = ("x", "u", "u1", "u2", "u_sup", "u_mf", "ghosti", "ghostb", "infront", "behind", "closest",
+          "closestd", "pinnedi", "pinnedb")
+
+
+def base_params(**kw) -> RunParams:
+    p = RunParams(nsteps=1, shots=1, dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1),
+                  velocity="BS", overide_timestep_check=True)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def pack_state(prefix: str, st: FilamentState, d: dict) -> None:
+    for name in FIELDS:
+        d["%s__%s" % (prefix, name)] = getattr(st, name)
+    d["%s__meta" % prefix] = np.array([st.t, st.itime, st.recon_count], np.float64)
+
+
+ALL_PHASES = ("ghost", "motion", "insert", "close", "recon", "remove", "final")
+
+
+def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, phases=ALL_PHASES) -> None:
+    txt = format_run_text(p)
+    with tempfile.TemporaryDirectory(dir="/tmp") as wd:
+        out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=True, threads=1)
+    d = {"run_text": np.array(txt), "nsteps": np.array(nsteps), "note": np.array(note)}
+    pack_state("in", st, d)
+    for ph in tuple(phases) + ("final",):
+        if ph in out and "%s__x" % ph not in d:
+            pack_state(ph, out[ph], d)
+    if "tree" in out:
+        for k in out["tree"].dtype.names:
+            d["tree__" + k] = out["tree"][k]
+    d["recon_log"] = out["recon_log"]
+    d["timing"] = np.array(json.dumps(out["timing"]))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+    print("%-22s N=%5d steps=%3d recon=%d final_pcount=%d" % (
+        name, st.pcount, nsteps, len(out["recon_log"]), out["final"].pcount))
+
+
+def crossing_rings(y0: float, delta: float) -> FilamentState:
+    """Ring A in the xy-plane (centre 0, R=0.01); ring B in the plane y=y0,
+    centred at (0.004, y0, 0) with R=0.006, so B passes y0 away from A's
+    point (0.01, 0, 0) with a perpendicular tangent: a reconnection pair."""
+    def loop(center, r, e1, e2, n):
+        phi = 2 * math.pi * np.arange(n) / n
+        return center[None, :] + r * np.cos(phi)[:, None] * e1[None, :] + r * np.sin(phi)[:, None] * e2[None, :]
+    nA = math.ceil(2 * math.pi * 0.01 / (0.98 * delta))
+    nB = math.ceil(2 * math.pi * 0.006 / (0.98 * delta))
+    A = loop(np.zeros(3), 0.01, np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), nA)
+    B = loop(np.array([0.004, y0, 0.0]), 0.006, np.array([1.0, 0, 0]), np.array([0, 0, 1.0]), nB)
+    return tangles._assemble([A, B], (0.1, 0.1, 0.1))
+
+
+def remesh_case(delta: float) -> FilamentState:
+    """Three rings: one stretched (chord ~1.3 delta, insertions), one
+    compressed (chord ~0.45 delta, removals), one regular; two empty slots
+    interleaved so insertions reuse them (first empty slot, src/line.f90:44-47)."""
+    A = tangles.ring([0.01, 0.0, 0.0], 0.006, [0, 0, 1], delta, chord=1.3)
+    B = tangles.ring([-0.02, 0.01, 0.0], 0.004, [0, 1, 0], delta, chord=0.45)
+    Cc = tangles.ring([0.0, -0.025, 0.01], 0.005, [1, 1, 0], delta)
+    st = tangles._assemble([A, B, Cc], (0.1, 0.1, 0.1))
+    n = st.pcount
+    holes = {len(A), len(A) + len(B) + len(Cc) // 2 + 1}   # rows left empty
+    rows = []
+    for old in range(n):
+        while len(rows) in holes:
+            rows.append(-1)
+        rows.append(old)
+    mapping = np.zeros(n + 1, np.int64)
+    for r, old in enumerate(rows):
+        if old >= 0:
+            mapping[old + 1] = r + 1
+    out = FilamentState.empty(len(rows))
+    for r, old in enumerate(rows):
+        if old < 0:
+            continue
+        for name, arr in st.arrays():
+            getattr(out, name)[r] = arr[old]
+        out.infront[r] = mapping[st.infront[old]]
+        out.behind[r] = mapping[st.behind[old]]
+    return out
+
+
+def main() -> None:
+    d5 = 5e-4
+    rings = tangles.random_rings(7, 4, 0.004, 0.012, 0.1, d5)
+    make("bs_rings_1step", rings, base_params(), 1, "BS, 4 random rings, one step with phase dumps")
+    make("bs_rings_5step", rings, base_params(), 5, "BS, Euler->AB2->AB3 start-up over 5 steps", ("final",))
+    make("tree05_rings", rings, base_params(velocity="Tree", tree_theta=0.5), 2,
+         "Tree theta=0.5, tree node list + pclose_tree")
+    mf = base_params(velocity="Tree", tree_theta=0.4, normal_velocity="xflow", norm_vel_xflow=0.5,
+                     alpha=(0.206, 0.0083))
+    make("tree04_xflow", rings, mf, 2, "Tree theta=0.4 with xflow mutual friction",
+         ("motion", "final"))
+    make("bs_xflow", rings, base_params(normal_velocity="xflow", norm_vel_xflow=0.5, alpha=(0.206, 0.0083),
+                                        super_velocity=(0.0, 0.01, 0.0)), 1,
+         "BS with xflow mutual friction and an imposed super_velocity", ("motion",))
+    ab = rings.copy()
+    g = np.random.Generator(np.random.PCG64(11))
+    ab.u1[:] = g.normal(scale=0.05, size=ab.u1.shape)
+    ab.u2[:] = g.normal(scale=0.05, size=ab.u2.shape)
+    sel = g.uniform(size=ab.pcount)
+    ab.u1[sel < 0.3] = 0.0          # Euler
+    ab.u2[(sel >= 0.3) & (sel < 0.6)] = 0.0   # AB2
+    ab.u1[sel > 0.95, 0] = -0.0     # a negative zero among the history
+    make("bs_ab_startup", ab, base_params(), 1, "per-point Euler/AB2/AB3 selection from u1/u2 history",
+         ("motion",))
+    kel = tangles.kelvin_lattice(3, box=0.1, nx=2, ny=1, amp=0.002, waves=3, delta=2e-3)
+    make("kelvin_traj20", kel, base_params(delta=2e-3, dt=1e-5), 20, "2 Kelvin-wave lines, 20-step trajectory",
+         ("motion", "final"))
+    cr = crossing_rings(1.0e-4, d5)
+    make("recon_bs", cr, base_params(), 3, "crossing rings: brute close search + reconnection")
+    make("recon_tree", cr, base_params(velocity="Tree", tree_theta=0.5), 3,
+         "crossing rings: stale-tree close search + reconnection")
+    rm = remesh_case(d5)
+    make("remesh_bs", rm, base_params(), 2, "stretched/compressed rings with empty slots: pinsert/premove")
+    make("remesh_tree", rm, base_params(velocity="Tree", tree_theta=0.5), 2,
+         "remesh case under Tree: stale tree with inserted points")
+    dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
+    few = ("close", "final")
+    make("dense_bs_10step", dense, base_params(box_size=(0.025, 0.025, 0.025)), 10,
+         "dense random rings in a 0.025 cm box: 10 steps with reconnections", few)
+    make("dense_tree_10step", dense, base_params(box_size=(0.025, 0.025, 0.025), velocity="Tree",
+                                                 tree_theta=0.5), 10,
+         "dense random rings, Tree theta=0.5, 10 steps", few)
+
+
+if __name__ == "__main__":
+    main()
