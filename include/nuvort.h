@@ -1,0 +1,177 @@
+/*
+ * nuvort.h -- C-ABI of the MI355X filament velocity and timestep library.
+ *
+ * The reference (NuVort, Fortran) has no plugin or FFI layer: its hot path
+ * is a set of argument-less module procedures that read and write module
+ * globals (the point array f(:) of src/cdata.f90:31, run.in parameters of
+ * src/cdata.f90:76-127, the octree vtree of src/tree.f90:34) and are called
+ * in a fixed order from the main loop (src/run.f90:229-270).  Each entry
+ * point below replaces one of those procedures; the citation on each says
+ * which.  A caller (the Fortran shim nuvort_amd/fortran/nuvort_hip.f90, or
+ * ctypes) keeps the reference's call sequence and only swaps the callee.
+ *
+ * Conventions
+ *  - Points are 1-based as in the reference: point p lives in row p-1 of
+ *    every array; infront/behind/closest hold 1-based indices; infront == 0
+ *    marks an empty slot (src/cdata.f90:19-29).  Vector fields are
+ *    [rows][3] doubles, row-major.
+ *  - The library copies host arrays in and out inside nv_set_state /
+ *    nv_get_state and never keeps a host pointer (the reference reallocates
+ *    f with move_alloc while inserting points, src/line.f90:51-58).
+ *  - State lives on the device between calls.  Every call is synchronous
+ *    with respect to the host: it returns after its kernels finished.
+ *  - Every call returns NV_OK or an error code; nv_last_error() describes
+ *    the last error.  The Fortran shim maps non-zero codes to the
+ *    reference's fatal_error (src/cdata.f90:245-255).
+ *  - One context per host thread; contexts are independent.
+ */
+#ifndef NUVORT_H
+#define NUVORT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NV_OK = 0,
+    NV_ERR_ARG = 1,          /* bad argument or parameter */
+    NV_ERR_HIP = 2,          /* HIP runtime failure */
+    NV_ERR_CAPACITY = 3,     /* caller's array too small (nv_get_state) */
+    NV_ERR_ZERO_LENGTH = 4,  /* no active point: src/line.f90:25-27 */
+    NV_ERR_SINGULAR = 5,     /* tree singularity: src/tree.f90:289-295 */
+    NV_ERR_NAN = 6,          /* NaN in a position: src/general.f90:172-191 */
+    NV_ERR_UNSUPPORTED = 7,  /* solid walls / pinned points / other modes */
+    NV_ERR_TREE_DEPTH = 8,   /* octree deeper than NV_TREE_MAX_LEVELS */
+    NV_ERR_TIMESTEP = 9,     /* dt above the stability bound: src/initial.f90:169-180 */
+    NV_ERR_STATE = 10        /* call out of order (e.g. walk without a tree) */
+};
+
+#define NV_TREE_MAX_LEVELS 16
+
+enum { NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2 };
+enum { NV_BC_OPEN = 0, NV_BC_PERIODIC = 1 };
+enum { NV_NF_ZERO = 0, NV_NF_XFLOW = 1 };
+enum { NV_PREC_FP64 = 0, NV_PREC_FP32C = 1 };
+enum { NV_CLOSE_AUTO = 0, NV_CLOSE_BRUTE = 1, NV_CLOSE_TREE = 2 };
+
+/* run.in parameters of src/cdata.f90:76-127 that the path reads, followed by
+ * this library's own extensions. */
+typedef struct {
+    double delta;                /* run.in delta */
+    double dt;                   /* run.in dt */
+    double quant_circ;           /* run.in quant_circ (default 9.97e-4) */
+    double corea;                /* run.in corea (default 8.244023e-9) */
+    double box_size[3];          /* run.in box_size */
+    int32_t boundary[3];         /* run.in boundary_x/y/z: NV_BC_* */
+    int32_t velocity;            /* run.in velocity: NV_VEL_* */
+    double tree_theta;           /* run.in tree_theta (0 = no tree) */
+    int32_t tree_extra_correction; /* run.in tree_extra_correction (default 1) */
+    int32_t normal_velocity;     /* run.in normal_velocity: NV_NF_* */
+    double norm_vel_xflow;       /* run.in norm_vel_xflow (default 0.5) */
+    double alpha[2];             /* run.in alpha */
+    double super_velocity[3];    /* run.in super_velocity */
+    double normal_fluid_cutoff;  /* run.in normal_fluid_cutoff (default 1e8) */
+    double t_zero_normal_fluid;  /* run.in t_zero_normal_fluid (default 1e8) */
+    int32_t recon_shots;         /* run.in recon_shots (default 1) */
+    int32_t overide_timestep_check; /* run.in overide_timestep_check */
+    /* extensions (no reference counterpart) */
+    int32_t precision;           /* NV_PREC_FP64 (reference arithmetic) or NV_PREC_FP32C */
+    double near_radius;          /* FP32C: FP64 near field radius, in units of delta */
+    int32_t close_search;        /* NV_CLOSE_*: AUTO = tree when tree_theta > 0 */
+    int32_t exact_order;         /* 1: sum each point's terms in the reference's order (one
+                                    chunk per point, bit-exact FP64, slower); 0: chunked */
+    int32_t capacity;            /* initial device rows (0 = 2 x pcount) */
+    int32_t block_targets;       /* direct sum targets per workgroup (0 = tuned) */
+    int32_t tile_sources;        /* direct sum sources per LDS tile (0 = tuned) */
+} nv_params;
+
+/* Host view of the filament point array, structure of arrays (the fields
+ * of type qvort, src/cdata.f90:19-29, that the path reads or writes). */
+typedef struct {
+    int32_t pcount;      /* rows in use (the reference's pcount) */
+    int32_t capacity;    /* rows the arrays below can hold */
+    int32_t itime;       /* the reference's itime */
+    int32_t recon_count; /* the reference's recon_count */
+    int32_t remove_count;/* the reference's remove_count */
+    int32_t pad;
+    double t;            /* the reference's t */
+    double *x, *u, *u1, *u2, *u_sup, *u_mf, *ghosti, *ghostb; /* [capacity][3] */
+    int32_t *infront, *behind, *closest;                      /* [capacity] */
+    double *closestd;                                         /* [capacity] */
+    int32_t *pinnedi, *pinnedb;                               /* [capacity] 0/1 */
+    int32_t *wpinned;                                         /* [capacity][3] */
+} nv_state;
+
+/* One reconnection, as src/reconnection.f90:46-49 logs it plus the pair. */
+typedef struct {
+    int32_t itime, i, j, pad;
+    double t, mid[3], angle, dl;
+} nv_recon_event;
+
+/* One octree node in depth-first pre-order (child order of
+ * src/tree.f90:311-314), the fields of type node (src/tree.f90:20-32). */
+typedef struct {
+    double pos[3], width, cent[3], circ[3];
+    int32_t pcount, leaf_infront;
+} nv_tree_node;
+
+typedef struct {
+    int64_t pair_evals;        /* direct sum (target, segment, image) evaluations done */
+    int64_t near_evals;        /* of which evaluated in FP64 by the FP32C kernel */
+    int64_t tree_evals;        /* tree interactions (the reference's tree_eval) */
+    int64_t tree_nodes_visited;/* node records read by tree walks */
+    int32_t tree_nodes;        /* nodes of the last tree, empty ones included */
+    int32_t tree_levels;       /* levels of the last tree */
+    double ms_velocity;        /* last velocity kernel time (HIP events) */
+    double ms_tree_build;      /* last tree build time */
+    double ms_close;           /* last close-point search time */
+    double ms_topology;        /* last pinsert+precon+premove time */
+    double ms_step;            /* last nv_step call, per step */
+} nv_stats;
+
+/* lifecycle -------------------------------------------------------------- */
+/* Replaces the setup part of init_setup (src/initial.f90:14-146): checks the
+ * parameters (timestep_check, open boundaries with a tree, ...) and creates
+ * the periodic shift table (src/boundary.f90:143-175). */
+int nv_create(const nv_params *params, void **ctx);
+int nv_destroy(void *ctx);
+const char *nv_last_error(void *ctx);
+
+/* state ------------------------------------------------------------------ */
+/* Upload f(1:pcount) (replaces data_restore / the ICs, src/initial.f90:151-166). */
+int nv_set_state(void *ctx, const nv_state *host);
+/* Download into caller arrays; on NV_ERR_CAPACITY host->pcount holds the rows needed. */
+int nv_get_state(void *ctx, nv_state *host);
+int nv_set_time(void *ctx, double t, int32_t itime);
+
+/* phases (in the order src/run.f90:229-270 calls them) -------------------- */
+int nv_ghostp(void *ctx);           /* src/boundary.f90:10-19 ghostp */
+int nv_construct_tree(void *ctx);   /* src/tree.f90:38-59 construct_tree */
+int nv_pmotion(void *ctx);          /* src/timestep.f90:16-45 pmotion */
+int nv_velocity(void *ctx);         /* src/timestep.f90:21-27: velocities only, no AB update */
+int nv_pinsert(void *ctx);          /* src/line.f90:18-96 pinsert */
+int nv_pclose(void *ctx);           /* src/line.f90:135-160 pclose */
+int nv_pclose_tree(void *ctx);      /* src/tree.f90:190-203 pclose_tree */
+int nv_precon(void *ctx);           /* src/reconnection.f90:11-61 precon */
+int nv_premove(void *ctx);          /* src/line.f90:102-130 premove */
+int nv_enforce_boundary(void *ctx); /* src/boundary.f90:91-138 enforce_boundary */
+int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.f90:263-267 */
+int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */
+
+/* nsteps iterations of the src/run.f90:229-270 loop body (diagnostics and
+ * file output excluded), state kept on the device throughout. */
+int nv_step(void *ctx, int32_t nsteps);
+
+/* outputs ---------------------------------------------------------------- */
+/* Reconnections since the last call (the pairs of src/reconnection.f90:11-61). */
+int nv_get_recon_events(void *ctx, nv_recon_event *buf, int32_t cap, int32_t *n);
+/* The current tree in depth-first order (what tree_print writes,
+ * src/tree.f90:96-102, plus internal nodes and moments). */
+int nv_get_tree(void *ctx, nv_tree_node *buf, int32_t cap, int32_t *n);
+int nv_get_stats(void *ctx, nv_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,469 @@
+// Per-point boundary kernels, the brute-force close-point search and the
+// serial line-topology operations.
+//
+// pinsert / precon / premove (src/line.f90:18-130, src/reconnection.f90:11-61)
+// mutate links in index order and later decisions read earlier mutations,
+// so their semantics are serial.  Each runs as a parallel pre-filter (every
+// row tests the reference's trigger predicate on the state at entry), a
+// stable compaction of the rows that fire, and one device thread that walks
+// those rows in ascending order, re-evaluating each against the current
+// state.  Rows whose inputs a mutation changes after the pre-filter saw them
+// (and that the reference loop has not reached yet) are queued as "dirty"
+// and visited in order too, so the visited set always contains every row
+// the reference loop would act on.
+#include "nv_device.h"
+#include "nv_kernels.h"
+
+namespace {
+
+constexpr int DIRTY_MAX = 8192;
+
+__device__ __forceinline__ void clear_row(const DState &s, int r) {
+    // src/general.f90:10-20 clear_particle (u_sup, u_mf untouched)
+    const int cap = s.cap;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.x[k * cap + r] = 0.0;
+        s.u[k * cap + r] = 0.0;
+        s.u1[k * cap + r] = 0.0;
+        s.u2[k * cap + r] = 0.0;
+        s.gi[k * cap + r] = 0.0;
+        s.gb[k * cap + r] = 0.0;
+    }
+    s.infront[r] = 0;
+    s.behind[r] = 0;
+    s.closest[r] = 0;
+    s.closestd[r] = 0.0;
+}
+
+__device__ __forceinline__ void store_ghosts(const DState &s, const Phys &p, int r) {
+    double gi[3], gb[3];
+    ghost_of(s, p, r, gi, gb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.gi[k * s.cap + r] = gi[k];
+        s.gb[k * s.cap + r] = gb[k];
+    }
+}
+
+// src/reconnection.f90:155-182 loop_killer.  The reference walks the whole
+// loop, but its outcome only depends on whether the loop closes within three
+// hops (counter < 4); the walk stops as soon as counter reaches 4, which
+// gives the same outcome (no pinned points exist on this path).
+__device__ void loop_killer(const DState &s, int particle, int n) {
+    int counter = 1, next = particle;
+    bool closed = false;
+    for (int it = 1; it <= n; ++it) {
+        next = s.infront[next] - 1;
+        if (next == particle) { closed = true; break; }
+        if (next < 0) return;
+        counter = counter + 1;
+        if (counter >= 4) return;
+    }
+    (void)closed;
+    if (counter < 4) {
+        next = particle;
+        for (int it = 1; it <= n; ++it) {
+            int store_next = s.infront[next] - 1;
+            clear_row(s, next);
+            next = store_next;
+            if (next == particle || next < 0) break;
+        }
+    }
+}
+
+struct Dirty {
+    int v[DIRTY_MAX];
+    int hd, nd;
+};
+
+__device__ bool dirty_push(Dirty &d, int val) {
+    for (int q = d.hd; q < d.nd; ++q)
+        if (d.v[q] == val) return true;
+    if (d.nd >= DIRTY_MAX) {
+        if (d.hd == 0) return false;
+        for (int q = d.hd; q < d.nd; ++q) d.v[q - d.hd] = d.v[q];
+        d.nd -= d.hd;
+        d.hd = 0;
+    }
+    int q = d.nd++;
+    while (q > d.hd && d.v[q - 1] > val) {
+        d.v[q] = d.v[q - 1];
+        --q;
+    }
+    d.v[q] = val;
+    return true;
+}
+
+// merge cursor over the ascending candidate list and the dirty queue
+__device__ __forceinline__ int next_row(const int *cand, int ncand, int &ci, Dirty &d) {
+    int rc = ci < ncand ? cand[ci] : 0x7fffffff;
+    int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+    int r = rc < rd ? rc : rd;
+    if (rc == r && r != 0x7fffffff) ++ci;
+    if (rd == r && r != 0x7fffffff) ++d.hd;
+    return r;
+}
+
+// --------------------------------------------------------------- per row
+__global__ void k_ghosts(DState s, Phys p, int n) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || s.infront[r] == 0) return;
+    store_ghosts(s, p, r);
+}
+
+__global__ void k_wrap(DState s, Phys p, int n) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || s.infront[r] == 0) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (!p.periodic[k]) continue;
+        double L = p.box[k];
+        double x = s.x[k * s.cap + r];
+        if (x > (L / 2.0)) x = x - L;
+        else if (x < (-L / 2.0)) x = x + L;
+        s.x[k * s.cap + r] = x;
+    }
+}
+
+__global__ void k_nan(DState s, int n, DCtl *ctl) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    if (isnan(s.x[r]) || isnan(s.x[s.cap + r]) || isnan(s.x[2 * s.cap + r])) atomicOr(&ctl->status, NVK_NAN);
+}
+
+// ---------------------------------------------------------------- pinsert
+__global__ void k_pinsert_pre(DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const bool act = s.infront[r] != 0;
+    empty[r] = act ? 0 : 1;
+    int c = 0;
+    if (act) {
+        double disti = dist_gen(s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r], s.gi[r], s.gi[s.cap + r],
+                                s.gi[2 * s.cap + r]);
+        c = disti > p.delta ? 1 : 0;
+    }
+    cand[r] = c;
+    // a ballot-free presence test for the zero-length stop
+    if (act) ctl->nactive = 1;
+}
+
+__global__ void k_pinsert_serial(DState s, Phys p, int n_old, const int *cand, const int *empty, DCtl *ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ Dirty d;
+    d.hd = d.nd = 0;
+    if (ctl->nactive == 0) { atomicOr(&ctl->status, NVK_ZERO_LENGTH); return; }
+    const int ncand = ctl->ncand, nempty = ctl->nempty;
+    const int cap = s.cap;
+    int n = ctl->n;
+    int ci = 0, ei = 0;
+    while (true) {
+        const int r = next_row(cand, ncand, ci, d);
+        if (r == 0x7fffffff) break;
+        if (s.infront[r] == 0) continue;
+        double x[3], gi[3], gb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            x[k] = s.x[k * cap + r];
+            gi[k] = s.gi[k * cap + r];
+            gb[k] = s.gb[k * cap + r];
+        }
+        const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
+        if (!(disti > p.delta)) continue;
+        int slot;
+        if (ei < nempty) {
+            slot = empty[ei++];
+        } else {
+            if (n >= cap) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+            slot = n++;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                s.usup[k * cap + slot] = 0.0;
+                s.umf[k * cap + slot] = 0.0;
+            }
+            clear_row(s, slot);
+        }
+        double fd[3], fdd[3];
+        derivs(x, gi, gb, fd, fdd);
+        double curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
+        double xn[3];
+        if (curv > 1e-5) {
+            curv = 1.0 / curv;
+            if (curv * curv - 0.25 * (disti * disti) > 0.0) {
+                const double rr = (sqrt(curv * curv - 0.25 * (disti * disti)) - curv) * curv;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]) + rr * fdd[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
+        }
+        const int fi = s.infront[r] - 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s.x[k * cap + slot] = xn[k];
+            s.u[k * cap + slot] = 0.5 * (s.u[k * cap + r] + s.u[k * cap + fi]);
+            s.u1[k * cap + slot] = 0.0;
+            s.u2[k * cap + slot] = 0.0;
+        }
+        s.behind[slot] = r + 1;
+        s.infront[slot] = s.infront[r];
+        store_ghosts(s, p, slot);
+        s.behind[fi] = slot + 1;
+        store_ghosts(s, p, fi);
+        s.infront[r] = slot + 1;
+        store_ghosts(s, p, r);
+        // rows the reference loop still has to reach whose inputs changed
+        if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+        if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+    }
+    ctl->n = n;
+}
+
+// ----------------------------------------------------------------- precon
+__global__ void k_precon_pre(DState s, Phys p, int n, int *cand) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double cd = s.closestd[r];
+    cand[r] = (s.infront[r] != 0 && (cd < p.delta / 2.0) && (cd > NV_EPS)) ? 1 : 0;
+}
+
+// src/general.f90:106-117 norm_tanf
+__device__ __forceinline__ void norm_tanf(const DState &s, int r, double t[3]) {
+    double x[3], gi[3], gb[3], fdd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = s.x[k * s.cap + r];
+        gi[k] = s.gi[k * s.cap + r];
+        gb[k] = s.gb[k * s.cap + r];
+    }
+    derivs(x, gi, gb, t, fdd);
+    double length = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = t[k] / length;
+}
+
+__device__ __forceinline__ double dist_rows(const DState &s, const double *a, int ra, const double *b, int rb) {
+    return dist_gen(s.x[ra], s.x[s.cap + ra], s.x[2 * s.cap + ra], b[rb], b[s.cap + rb], b[2 * s.cap + rb]);
+}
+
+__global__ void k_precon_serial(DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev, int ev_cap,
+                                double t, int itime) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int ncand = ctl->ncand;
+    const int n = ctl->n;
+    const int cap = s.cap;
+    for (int ci = 0; ci < ncand; ++ci) {
+        const int r = cand[ci];
+        if (s.infront[r] == 0) continue;
+        const int pari = s.infront[r];
+        const double cd = s.closestd[r];
+        if (!((cd < p.delta / 2.0) && (cd > NV_EPS))) continue;
+        const int j = s.closest[r];
+        if (j == pari) continue;
+        if (j == s.behind[r]) continue;
+        if (j == 0) continue;
+        const int rj = j - 1;
+        if (s.infront[rj] == 0) continue;
+        const int parjb = s.behind[rj];
+        double t1[3], t2[3];
+        norm_tanf(s, r, t1);
+        norm_tanf(s, rj, t2);
+        const double dot_val = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
+        if (dot_val > 0.9) continue;
+        const double l_before = dist_rows(s, s.x, r, s.gi, r) + dist_rows(s, s.x, r, s.gb, r) +
+                                dist_rows(s, s.x, rj, s.gi, rj) + dist_rows(s, s.x, rj, s.gb, rj);
+        const double l_after = dist_rows(s, s.x, r, s.gb, r) + dist_rows(s, s.x, r, s.x, rj) +
+                               dist_rows(s, s.x, rj, s.gi, rj) + dist_rows(s, s.x, pari - 1, s.x, parjb - 1);
+        if (l_after <= l_before) {
+            ctl->recon_count += 1;
+            const int q = ctl->nevents++;
+            if (q < ev_cap) {
+                nv_recon_event e;
+                e.itime = itime; e.i = r + 1; e.j = j; e.pad = 0; e.t = t;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) e.mid[k] = 0.5 * (s.x[k * cap + r] + s.x[k * cap + rj]);
+                e.angle = acos(dot_val);
+                e.dl = l_before - l_after;
+                ev[q] = e;
+            }
+            s.infront[parjb - 1] = pari;
+            s.behind[pari - 1] = parjb;
+            s.infront[r] = j;
+            s.behind[rj] = r + 1;
+            loop_killer(s, pari - 1, n);
+            loop_killer(s, r, n);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- premove
+__global__ void k_premove_pre(DState s, Phys p, int n, int *cand) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int c = 0;
+    if (s.infront[r] != 0) {
+        const int ii = s.infront[s.infront[r] - 1] - 1;
+        c = (ii >= 0 && dist_rows(s, s.x, r, s.x, ii) < .99 * p.delta) ? 1 : 0;
+    }
+    cand[r] = c;
+}
+
+__global__ void k_premove_serial(DState s, Phys p, const int *cand, DCtl *ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ Dirty d;
+    d.hd = d.nd = 0;
+    const int ncand = ctl->ncand;
+    const int n = ctl->n;
+    int ci = 0;
+    while (true) {
+        const int r = next_row(cand, ncand, ci, d);
+        if (r == 0x7fffffff) break;
+        if (s.infront[r] == 0) continue;
+        const int inf = s.infront[r] - 1;
+        const int ii = s.infront[inf] - 1;
+        if (ii < 0) continue;
+        const double distii = dist_rows(s, s.x, r, s.x, ii);
+        if (distii < .99 * p.delta) {
+            const int tinfront = ii;
+            s.behind[tinfront] = r + 1;
+            s.infront[r] = tinfront + 1;
+            clear_row(s, inf);
+            loop_killer(s, r, n);
+            ctl->remove_count += 1;
+            if (s.infront[r] != 0) {
+                const int b = s.behind[r] - 1;
+                if (b > r && !dirty_push(d, b)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ brute close search
+// src/line.f90:135-160 pclose: the first (lowest) j with the smallest
+// distf(i,j) among non-empty j not in {i, infront(i), behind(i)}; sources
+// split into S chunks, each keeping its first minimum; k_pclose_fin keeps
+// the first chunk's on ties.  Distances are the reference's correctly
+// rounded sqrt, compared strictly.
+template <int BT, int T>
+__global__ __launch_bounds__(BT) void k_pclose(DState s, int n, int S, double *__restrict__ pd, int *__restrict__ pj,
+                                               int npad) {
+    __shared__ double tile[T * 4];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * BT + tid;
+    const bool act = i < n && s.infront[i] != 0;
+    double x0 = 0, x1 = 0, x2 = 0;
+    int fi = -2, bi = -2;
+    if (act) {
+        x0 = s.x[i]; x1 = s.x[s.cap + i]; x2 = s.x[2 * s.cap + i];
+        fi = s.infront[i] - 1; bi = s.behind[i] - 1;
+    }
+    const int ntiles = (n + T - 1) / T;
+    const int c = blockIdx.y;
+    const int t0 = (int)((long)ntiles * c / S), t1 = (int)((long)ntiles * (c + 1) / S);
+    double best = 100.0, bound = 100.0 * 100.0 * (1.0 + 8e-16);
+    int bj = -1;
+    for (int t = t0; t < t1; ++t) {
+        __syncthreads();
+        for (int q = tid; q < T; q += BT) {
+            const int j = t * T + q;
+            const bool v = j < n && s.infront[j] != 0;
+            tile[q * 4 + 0] = v ? s.x[j] : 0.0;
+            tile[q * 4 + 1] = v ? s.x[s.cap + j] : 0.0;
+            tile[q * 4 + 2] = v ? s.x[2 * s.cap + j] : 0.0;
+            tile[q * 4 + 3] = v ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (!act) continue;
+        for (int jj = 0; jj < T; ++jj) {
+            const double *q = &tile[jj * 4];
+            if (q[3] == 0.0) continue;
+            const int j = t * T + jj;
+            if (j == i || j == fi || j == bi) continue;
+            const double d0 = x0 - q[0], d1 = x1 - q[1], d2 = x2 - q[2];
+            const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+            if (dd < bound) {
+                const double dist = sqrt(dd);
+                if (dist < best) {
+                    best = dist;
+                    bound = dist * dist * (1.0 + 8e-16);
+                    bj = j;
+                }
+            }
+        }
+    }
+    if (i < npad) {
+        pd[(size_t)c * npad + i] = best;
+        pj[(size_t)c * npad + i] = bj;
+    }
+}
+
+__global__ void k_pclose_fin(DState s, int n, int S, const double *pd, const int *pj, int npad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || s.infront[i] == 0) return;
+    double best = 100.0;
+    int bj = -1;
+    for (int c = 0; c < S; ++c) {
+        const double d = pd[(size_t)c * npad + i];
+        if (pj[(size_t)c * npad + i] >= 0 && d < best) {
+            best = d;
+            bj = pj[(size_t)c * npad + i];
+        }
+    }
+    s.closestd[i] = best;
+    if (bj >= 0) s.closest[i] = bj + 1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launch
+#define GRID(n) dim3(((n) + 255) / 256), dim3(256)
+
+hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_ghosts, GRID(n), 0, st, s, p, n);
+    return hipGetLastError();
+}
+hipError_t nvk_wrap(hipStream_t st, DState s, Phys p, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_wrap, GRID(n), 0, st, s, p, n);
+    return hipGetLastError();
+}
+hipError_t nvk_nan(hipStream_t st, DState s, int n, DCtl *ctl) {
+    if (n > 0) hipLaunchKernelGGL(k_nan, GRID(n), 0, st, s, n, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl) {
+    if (n > 0) hipLaunchKernelGGL(k_pinsert_pre, GRID(n), 0, st, s, p, n, cand, empty, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, int n_old, const int *cand, const int *empty,
+                              DCtl *ctl) {
+    hipLaunchKernelGGL(k_pinsert_serial, dim3(1), dim3(64), 0, st, s, p, n_old, cand, empty, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
+    if (n > 0) hipLaunchKernelGGL(k_precon_pre, GRID(n), 0, st, s, p, n, cand);
+    return hipGetLastError();
+}
+hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev,
+                             int ev_cap, double t, int itime) {
+    hipLaunchKernelGGL(k_precon_serial, dim3(1), dim3(64), 0, st, s, p, cand, ctl, ev, ev_cap, t, itime);
+    return hipGetLastError();
+}
+hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
+    if (n > 0) hipLaunchKernelGGL(k_premove_pre, GRID(n), 0, st, s, p, n, cand);
+    return hipGetLastError();
+}
+hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl) {
+    hipLaunchKernelGGL(k_premove_serial, dim3(1), dim3(64), 0, st, s, p, cand, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_pclose(hipStream_t st, DState s, int n, int npad, int S, double *pd, int *pj) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_pclose<256, 256>), dim3(npad / 256, S), dim3(256), 0, st, s, n, S, pd, pj, npad);
+    hipLaunchKernelGGL(k_pclose_fin, GRID(n), 0, st, s, n, S, pd, pj, npad);
+    return hipGetLastError();
+}

@@ -1,0 +1,708 @@
+// Barnes-Hut octree of src/tree.f90 built and walked on the device.
+//
+// Build (construct_tree / build_tree / new_tree, src/tree.f90:38-157):
+//  1. every active point computes its path through the octree from the
+//     exact FP64 boxes the reference builds (corner accumulated as
+//     pos + width/2, child width width/2, half-open membership
+//     [pos, pos+width), child order fbl fbr ftl ftr bbl bbr btl btr i.e.
+//     c = 4*y + 2*z + x).  Four bits per level: 0..7 the child, 8 = in the
+//     parent but in no child (the sliver a rounded upper bound leaves).
+//  2. a stable LSD radix sort orders the points by path; every node is then
+//     a contiguous range and its children are sub-ranges in child order.
+//  3. nodes are created level by level: a node with more than one point
+//     gets 8 children (empty ones included), ranges found by binary search.
+//  4. moments (centroid, circulation) bottom-up; sizes and depth-first
+//     positions give a flat array of the non-empty nodes in the reference's
+//     recursion order, each with the index just past its subtree ("skip").
+//
+// Walks (tree_walk :250-316, closest_tree :206-244): one wavefront takes 64
+// consecutive points (neighbours along a filament, so their traversals
+// overlap) and follows ONE cursor through the depth-first array.  A lane is
+// active at the cursor unless it already accepted an ancestor; the cursor
+// descends when any active lane opens the node and skips the subtree
+// otherwise.  Every lane therefore accepts exactly the nodes its own
+// recursive walk would, in the same order, while all node reads are
+// wave-uniform (scalar loads).
+#include <algorithm>
+#include <vector>
+
+#include "nv_tree.h"
+#include "nv_util.h"
+
+namespace {
+
+constexpr int LEVELS = NV_TREE_MAX_LEVELS;   // 16 levels x 4 bits = 64-bit key
+
+struct __align__(16) WNode {
+    double cent[3];
+    double circ[3];
+    double width, geo;
+    int pcount, leaf_infront, skip, pad;
+};
+
+__device__ __forceinline__ int digit_at(unsigned long long key, int level) {
+    return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
+}
+
+__global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals, int *nact) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    vals[r] = r;
+    if (s.infront[r] == 0) {
+        keys[r] = ~0ull;
+        return;
+    }
+    atomicAdd(nact, 1);
+    const double x[3] = {s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r]};
+    double pos[3] = {-bsize / 2.0, -bsize / 2.0, -bsize / 2.0};
+    double w = bsize;
+    unsigned long long key = 0;
+    for (int lev = 1; lev <= LEVELS; ++lev) {
+        int dig = 0;
+        bool in = true;
+        double lo[3];
+        const double cw = w / 2.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double split = pos[k] + (w / 2);
+            const int b = x[k] >= split ? 1 : 0;
+            lo[k] = b ? split : pos[k];
+            in = in && (x[k] >= lo[k]) && (x[k] < (lo[k] + cw));
+            dig |= b << (k == 0 ? 0 : (k == 1 ? 2 : 1));
+        }
+        if (!in) {
+            key |= 8ull << (4 * (LEVELS - lev));
+            break;
+        }
+        key |= (unsigned long long)dig << (4 * (LEVELS - lev));
+        pos[0] = lo[0]; pos[1] = lo[1]; pos[2] = lo[2];
+        w = cw;
+    }
+    keys[r] = key;
+}
+
+// ------------------------------------------------------------ radix sort
+constexpr int RS_BT = 256;
+constexpr int RS_IPT = 16;
+constexpr int RS_TILE = RS_BT * RS_IPT;
+
+__global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int *hist, int nblk) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_IPT; ++k) {
+        int q = base + k * RS_BT + threadIdx.x;
+        if (q < n) atomicAdd(&h[(keys[q] >> shift) & 0xFF], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, int shift, const int *hist,
+                             int nblk, unsigned long long *okeys, int *ovals) {
+    __shared__ int running[256];
+    __shared__ int wcnt[RS_BT / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    running[tid] = hist[tid * nblk + blockIdx.x];
+    const int base = blockIdx.x * RS_TILE;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int k = 0; k < RS_IPT; ++k) {
+        for (int w = 0; w < RS_BT / 64; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        const int q = base + k * RS_BT + tid;
+        const bool ok = q < n;
+        unsigned long long key = ok ? keys[q] : ~0ull;
+        const int dg = ok ? (int)((key >> shift) & 0xFF) : 256;
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(ok);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dg >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        const bool last = ok && ((peers >> lane) >> 1) == 0ull;
+        if (last) wcnt[wv][dg] = rank + 1;
+        __syncthreads();
+        if (ok) {
+            int off = running[dg];
+            for (int w = 0; w < wv; ++w) off += wcnt[w][dg];
+            okeys[off + rank] = key;
+            ovals[off + rank] = vals[q];
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < RS_BT / 64; ++w) add += wcnt[w][tid];
+        running[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ nodes
+struct Nodes {
+    int *beg, *cnt, *pcount, *dropb, *first, *parent, *minrow, *size, *dfs, *leafinf;
+    double *pos, *w, *sum, *circ;   // pos/sum/circ: [3][ncap]
+    int ncap;
+};
+
+__global__ void k_root(Nodes nd, DState s, int n, double bsize, const int *nact) {
+    nd.beg[0] = 0;
+    nd.cnt[0] = *nact;
+    nd.pcount[0] = n;   // the root holds every slot (src/tree.f90:48)
+    nd.parent[0] = -1;
+    nd.first[0] = -1;
+    for (int k = 0; k < 3; ++k) nd.pos[k * nd.ncap] = -bsize / 2.0;
+    nd.w[0] = bsize;
+    nd.dropb[0] = *nact;
+}
+
+__global__ void k_split_flags(Nodes nd, int L0, int L1, int *flags, int at_max, DCtl *ctl) {
+    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L1) return;
+    const int sp = nd.pcount[q] > 1 ? 1 : 0;
+    flags[q - L0] = sp;
+    if (sp && at_max) atomicOr(&ctl->status, NVK_TREE_DEPTH);
+}
+
+__device__ __forceinline__ int lower_digit(const unsigned long long *keys, int lo, int hi, int level, int d) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (digit_at(keys[mid], level) < d) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// children of the split nodes of level `level` (nodes [L0, L1)) into L1 + 8*off
+__global__ void k_children(Nodes nd, int L0, int L1, const int *flags, const int *off,
+                           const unsigned long long *keys, int level) {
+    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L1 || !flags[q - L0]) return;
+    const int base = L1 + 8 * off[q - L0];
+    nd.first[q] = base;
+    const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
+    int bnd[9];
+    bnd[0] = b0;
+    for (int d = 1; d <= 8; ++d) bnd[d] = lower_digit(keys, bnd[d - 1], b1, level + 1, d);
+    nd.dropb[q] = bnd[8];
+    const double w = nd.w[q];
+    const double p0 = nd.pos[q], p1 = nd.pos[nd.ncap + q], p2 = nd.pos[2 * nd.ncap + q];
+    for (int c = 0; c < 8; ++c) {
+        const int ch = base + c;
+        const int xb = c & 1, zb = (c >> 1) & 1, yb = (c >> 2) & 1;
+        nd.pos[ch] = xb ? p0 + (w / 2) : p0;
+        nd.pos[nd.ncap + ch] = yb ? p1 + (w / 2) : p1;
+        nd.pos[2 * nd.ncap + ch] = zb ? p2 + (w / 2) : p2;
+        nd.w[ch] = w / 2.0;
+        nd.beg[ch] = bnd[c];
+        nd.cnt[ch] = bnd[c + 1] - bnd[c];
+        nd.pcount[ch] = bnd[c + 1] - bnd[c];
+        nd.parent[ch] = q;
+        nd.first[ch] = -1;
+        nd.dropb[ch] = bnd[c + 1];
+    }
+}
+
+// moments, minimum row and non-empty subtree size, children before parents
+__global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals) {
+    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L1) return;
+    const int cap = s.cap, nc = nd.ncap;
+    double sm[3] = {0.0, 0.0, 0.0}, ci[3] = {0.0, 0.0, 0.0};
+    int mr = 0x7fffffff, size;
+    if (nd.first[q] < 0) {
+        // a leaf: the reference's new_tree sums over its points in index
+        // order; with at most one point that is exact: cent = x, circ = (0 + gi) - x
+        if (nd.cnt[q] == 1 && nd.pcount[q] == 1) {
+            const int p = vals[nd.beg[q]];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double xk = s.x[k * cap + p];
+                sm[k] = 0.0 + xk;
+                ci[k] = (0.0 + s.gi[k * cap + p]) - xk;
+            }
+            mr = p;
+        } else {
+            for (int j = nd.beg[q]; j < nd.beg[q] + nd.cnt[q]; ++j) {
+                const int p = vals[j];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double xk = s.x[k * cap + p];
+                    sm[k] = sm[k] + xk;
+                    ci[k] = ci[k] + s.gi[k * cap + p] - xk;
+                }
+                mr = min(mr, p);
+            }
+        }
+        size = nd.pcount[q] > 0 ? 1 : 0;
+    } else {
+        size = 1;
+        for (int c = 0; c < 8; ++c) {
+            const int ch = nd.first[q] + c;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sm[k] = sm[k] + nd.sum[k * nc + ch];
+                ci[k] = ci[k] + nd.circ[k * nc + ch];
+            }
+            mr = min(mr, nd.minrow[ch]);
+            size += nd.size[ch];
+        }
+        for (int j = nd.dropb[q]; j < nd.beg[q] + nd.cnt[q]; ++j) {
+            const int p = vals[j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double xk = s.x[k * cap + p];
+                sm[k] = sm[k] + xk;
+                ci[k] = ci[k] + s.gi[k * cap + p] - xk;
+            }
+            mr = min(mr, p);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        nd.sum[k * nc + q] = sm[k];
+        nd.circ[k * nc + q] = ci[k];
+    }
+    nd.minrow[q] = mr;
+    nd.size[q] = size;
+}
+
+// depth-first positions of the children of level-`L` nodes and the walk
+// records of the level-`L` nodes themselves
+__global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk) {
+    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L1) return;
+    const int nc = nd.ncap;
+    const int pc = nd.pcount[q];
+    if (q == 0) nd.dfs[0] = 0;
+    // leaf_infront: parray(1)%infront at build time; the root's parray(1)
+    // is f(1) whatever it holds (src/tree.f90:49)
+    int li = 0;
+    if (q == 0) li = s.infront[0];
+    else if (pc >= 1 && nd.minrow[q] != 0x7fffffff) li = s.infront[nd.minrow[q]];
+    nd.leafinf[q] = li;
+    if (pc == 0) return;
+    const int my = nd.dfs[q];
+    if (nd.first[q] >= 0) {
+        int acc = my + 1;
+        for (int c = 0; c < 8; ++c) {
+            const int ch = nd.first[q] + c;
+            nd.dfs[ch] = acc;
+            acc += nd.size[ch];
+        }
+    }
+    WNode w;
+    const double cntd = (q == 0) ? (double)nact : (double)pc;
+    double cent[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        cent[k] = nd.sum[k * nc + q] / cntd;
+        w.cent[k] = cent[k];
+        w.circ[k] = (q == 0) ? 0.0 : nd.circ[k * nc + q];
+    }
+    w.width = nd.w[q];
+    const double g0 = cent[0] - (nd.pos[q] + nd.w[q] / 2.0);
+    const double g1 = cent[1] - (nd.pos[nc + q] + nd.w[q] / 2.0);
+    const double g2 = cent[2] - (nd.pos[2 * nc + q] + nd.w[q] / 2.0);
+    w.geo = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    w.pcount = pc;
+    w.leaf_infront = li;
+    w.skip = my + nd.size[q];
+    w.pad = 0;
+    walk[my] = w;
+}
+
+// ------------------------------------------------------------ walks
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
+                                                 const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
+                                                 DCtl *ctl, unsigned long long *counters) {
+    const int r = blockIdx.x * BT + threadIdx.x;
+    const bool act = r < n && s.infront[r] != 0;
+    double xi[3] = {0, 0, 0};
+    int beh = -5, me = -5;
+    if (act) {
+        xi[0] = s.x[r]; xi[1] = s.x[s.cap + r]; xi[2] = s.x[2 * s.cap + r];
+        beh = s.behind[r];
+        me = r + 1;
+    }
+    const double kappa = p.kappa, theta_c = p.tree_theta;
+    const int corr = p.extra_corr;
+    double a0[3] = {0, 0, 0}, as[3] = {0, 0, 0};
+    unsigned long long nev = 0, nvis = 0;
+    bool sing = false;
+    for (int m = 0; m < sh.n; ++m) {
+        const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+        const bool img0 = (m == 0);
+        int lskip = act ? 0 : 0x7fffffff;
+        int cur = 0;
+        while (cur < nwalk) {
+            cur = wave_uniform(cur);
+            const WNode nd = walk[cur];
+            bool open = false;
+            if (cur >= lskip) {
+                ++nvis;
+                const double e0 = xi[0] - (nd.cent[0] + s0);
+                const double e1 = xi[1] - (nd.cent[1] + s1);
+                const double e2 = xi[2] - (nd.cent[2] + s2);
+                const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+                double theta;
+                if (corr) {
+                    if (dist - nd.geo > 0.0) theta = nd.width / (dist - nd.geo);
+                    else theta = nd.width / dist;
+                } else {
+                    theta = nd.width / dist;
+                }
+                if (nd.pcount == 1 || theta < theta_c) {
+                    lskip = nd.skip;
+                    bool use = true;
+                    if (nd.pcount == 1) {
+                        const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
+                        if (img0 && (j == me || j == beh)) use = false;
+                        if (use && dist < NV_EPS) { sing = true; use = false; }
+                    }
+                    if (use) {
+                        const double v0 = ((nd.cent[0] + s0) - xi[0]);
+                        const double v1 = ((nd.cent[1] + s1) - xi[1]);
+                        const double v2 = ((nd.cent[2] + s2) - xi[2]);
+                        const double a = dist * dist;
+                        const double b = 2.0 * (v0 * nd.circ[0] + v1 * nd.circ[1] + v2 * nd.circ[2]);
+                        const double c = nd.circ[0] * nd.circ[0] + nd.circ[1] * nd.circ[1] + nd.circ[2] * nd.circ[2];
+                        const double D = 4.0 * a * c - b * b;
+                        if (!(D < NV_EPS)) {
+                            const double den = (2.0 * NV_PI) * D;
+                            const double fac = (2.0 * c + b) / sqrt(a + b + c) - (b / sqrt(a));
+                            const double u0 = ((v1 * nd.circ[2] - v2 * nd.circ[1]) * kappa / den) * fac;
+                            const double u1 = ((v2 * nd.circ[0] - v0 * nd.circ[2]) * kappa / den) * fac;
+                            const double u2 = ((v0 * nd.circ[1] - v1 * nd.circ[0]) * kappa / den) * fac;
+                            if (img0) { a0[0] = a0[0] + u0; a0[1] = a0[1] + u1; a0[2] = a0[2] + u2; }
+                            else { as[0] = as[0] + u0; as[1] = as[1] + u1; as[2] = as[2] + u2; }
+                            ++nev;
+                        }
+                    }
+                } else {
+                    open = true;
+                }
+            }
+            cur = __any(open) ? cur + 1 : nd.skip;
+        }
+    }
+    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
+    if (r < npad) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            part[k * npad + r] = a0[k];
+            part[(3 + k) * npad + r] = as[k];
+        }
+    }
+    atomicAdd(&counters[0], nev);
+    atomicAdd(&counters[1], nvis);
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const WNode *__restrict__ walk, int nwalk) {
+    const int r = blockIdx.x * BT + threadIdx.x;
+    if (r < n) s.closestd[r] = 100.0;   // set for every slot (src/tree.f90:196)
+    const bool act = r < n && s.infront[r] != 0;
+    double xi[3] = {0, 0, 0};
+    int inf = -5, beh = -5, me = -5;
+    if (act) {
+        xi[0] = s.x[r]; xi[1] = s.x[s.cap + r]; xi[2] = s.x[2 * s.cap + r];
+        inf = s.infront[r];
+        beh = s.behind[r];
+        me = r + 1;
+    }
+    double best = 100.0;
+    int closest = act ? s.closest[r] : 0;
+    int lskip = act ? 0 : 0x7fffffff;
+    int cur = 0;
+    while (cur < nwalk) {
+        cur = wave_uniform(cur);
+        const WNode nd = walk[cur];
+        bool open = false;
+        if (cur >= lskip) {
+            const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
+            const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            const double theta = nd.width / dist;
+            if (nd.pcount == 1 || theta < .5) {
+                lskip = nd.skip;
+                if (dist < best) {
+                    const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
+                    if (nd.pcount > 1) {
+                        closest = 0;
+                    } else if ((me != j) && (inf != j) && (beh != j)) {
+                        closest = j;
+                        best = dist;
+                    }
+                }
+            } else {
+                open = true;
+            }
+        }
+        cur = __any(open) ? cur + 1 : nd.skip;
+    }
+    if (act) {
+        s.closest[r] = closest;
+        s.closestd[r] = best;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+struct TreeBuf {
+    int cap = 0;                         // rows the key buffers hold
+    unsigned long long *k0 = nullptr, *k1 = nullptr;
+    int *v0 = nullptr, *v1 = nullptr;
+    int *hist = nullptr, *tmp = nullptr;
+    int hist_cap = 0;
+    int *nact = nullptr;                 // device
+    Nodes nd{};
+    int *flags = nullptr, *offs = nullptr, *scan_tmp = nullptr;
+    int flag_cap = 0;
+    WNode *walk = nullptr;
+    int walk_cap = 0;
+    int nwalk = 0, nnodes = 0, nact_h = 0;
+    std::vector<int> lvl;                // level start offsets
+    unsigned long long *counters = nullptr;
+    int *h_int = nullptr;                // pinned scratch
+    const int *vals = nullptr;           // sorted rows of the last build
+};
+
+#define TCHK(x)                                     \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return NV_ERR_HIP;    \
+    } while (0)
+
+TreeBuf *nv_tree_new() {
+    TreeBuf *t = new TreeBuf();
+    if (hipMalloc(&t->nact, sizeof(int)) != hipSuccess || hipMalloc(&t->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc((void **)&t->h_int, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+
+static void free_nodes(Nodes &nd) {
+    int **ip[] = {&nd.beg, &nd.cnt, &nd.pcount, &nd.dropb, &nd.first, &nd.parent, &nd.minrow, &nd.size, &nd.dfs,
+                  &nd.leafinf};
+    for (int **p : ip)
+        if (*p) { hipFree(*p); *p = nullptr; }
+    double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ};
+    for (double **p : dp)
+        if (*p) { hipFree(*p); *p = nullptr; }
+    nd.ncap = 0;
+}
+
+void nv_tree_free(TreeBuf *t) {
+    if (!t) return;
+    hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
+    hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
+    hipFree(t->counters);
+    hipHostFree(t->h_int);
+    free_nodes(t->nd);
+    delete t;
+}
+
+// grow node arrays to hold `want` nodes, keeping the first `keep`
+static int grow_nodes(TreeBuf *t, hipStream_t st, int want, int keep) {
+    if (want <= t->nd.ncap) return NV_OK;
+    int ncap = std::max(want, t->nd.ncap * 2);
+    Nodes o = t->nd, n{};
+    n.ncap = ncap;
+    int **ip[] = {&n.beg, &n.cnt, &n.pcount, &n.dropb, &n.first, &n.parent, &n.minrow, &n.size, &n.dfs, &n.leafinf};
+    int *oi[] = {o.beg, o.cnt, o.pcount, o.dropb, o.first, o.parent, o.minrow, o.size, o.dfs, o.leafinf};
+    for (int q = 0; q < 10; ++q) {
+        TCHK(hipMalloc(ip[q], sizeof(int) * (size_t)ncap));
+        if (oi[q] && keep) TCHK(hipMemcpyAsync(*ip[q], oi[q], sizeof(int) * keep, hipMemcpyDeviceToDevice, st));
+    }
+    TCHK(hipMalloc(&n.w, sizeof(double) * (size_t)ncap));
+    if (o.w && keep) TCHK(hipMemcpyAsync(n.w, o.w, sizeof(double) * keep, hipMemcpyDeviceToDevice, st));
+    double **dp[] = {&n.pos, &n.sum, &n.circ};
+    double *od[] = {o.pos, o.sum, o.circ};
+    for (int q = 0; q < 3; ++q) {
+        TCHK(hipMalloc(dp[q], sizeof(double) * 3 * (size_t)ncap));
+        if (od[q] && keep)
+            for (int k = 0; k < 3; ++k)
+                TCHK(hipMemcpyAsync(*dp[q] + (size_t)k * ncap, od[q] + (size_t)k * o.ncap, sizeof(double) * keep,
+                                    hipMemcpyDeviceToDevice, st));
+    }
+    TCHK(hipStreamSynchronize(st));
+    free_nodes(o);
+    t->nd = n;
+    return NV_OK;
+}
+
+static int ensure_rows(TreeBuf *t, int n) {
+    if (n <= t->cap) return NV_OK;
+    int cap = std::max(n, t->cap + t->cap / 2);
+    hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
+    TCHK(hipMalloc(&t->k0, sizeof(unsigned long long) * cap));
+    TCHK(hipMalloc(&t->k1, sizeof(unsigned long long) * cap));
+    TCHK(hipMalloc(&t->v0, sizeof(int) * cap));
+    TCHK(hipMalloc(&t->v1, sizeof(int) * cap));
+    int nblk = (cap + RS_TILE - 1) / RS_TILE;
+    TCHK(hipMalloc(&t->hist, sizeof(int) * 256 * (size_t)nblk));
+    TCHK(hipMalloc(&t->tmp, sizeof(int) * (size_t)nv_scan_tmp_ints(256 * nblk)));
+    t->cap = cap;
+    return NV_OK;
+}
+
+static int ensure_flags(TreeBuf *t, int m) {
+    if (m <= t->flag_cap) return NV_OK;
+    int cap = std::max(m, 2 * t->flag_cap);
+    hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp);
+    TCHK(hipMalloc(&t->flags, sizeof(int) * cap));
+    TCHK(hipMalloc(&t->offs, sizeof(int) * cap));
+    TCHK(hipMalloc(&t->scan_tmp, sizeof(int) * (size_t)nv_scan_tmp_ints(cap)));
+    t->flag_cap = cap;
+    return NV_OK;
+}
+
+int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl, nv_stats *stats) {
+    if (n <= 0) return NV_ERR_ZERO_LENGTH;
+    if (int rc = ensure_rows(t, n)) return rc;
+    const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+    TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0, t->nact);
+    TCHK(hipGetLastError());
+    // stable LSD radix sort, 8 bits per pass
+    const int nblk = (n + RS_TILE - 1) / RS_TILE;
+    unsigned long long *ka = t->k0, *kb = t->k1;
+    int *va = t->v0, *vb = t->v1;
+    for (int shift = 0; shift < 64; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, t->hist, nblk);
+        TCHK(hipGetLastError());
+        TCHK(nv_exclusive_scan(st, t->hist, 256 * nblk, t->tmp));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, t->hist, nblk, kb, vb);
+        TCHK(hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    // after an even number of passes the result is back in k0/v0
+    t->vals = va;
+    TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    const int nact = t->h_int[0];
+    t->nact_h = nact;
+    if (int rc = grow_nodes(t, st, std::max(1024, 4 * nact + 64), 0)) return rc;
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact);
+    TCHK(hipGetLastError());
+    t->lvl.assign(1, 0);
+    t->lvl.push_back(1);
+    int level = 0;
+    while (true) {
+        const int L0 = t->lvl[level], L1 = t->lvl[level + 1], m = L1 - L0;
+        if (m == 0) break;
+        if (int rc = ensure_flags(t, m)) return rc;
+        hipLaunchKernelGGL(k_split_flags, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->flags,
+                           level == LEVELS ? 1 : 0, ctl);
+        TCHK(hipGetLastError());
+        TCHK(hipMemcpyAsync(t->offs, t->flags, sizeof(int) * m, hipMemcpyDeviceToDevice, st));
+        TCHK(nv_exclusive_scan(st, t->offs, m, t->scan_tmp));
+        TCHK(hipMemcpyAsync(t->h_int, t->offs + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        TCHK(hipMemcpyAsync(t->h_int + 1, t->flags + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        TCHK(hipStreamSynchronize(st));
+        const int nsplit = t->h_int[0] + t->h_int[1];
+        if (nsplit == 0 || level == LEVELS) {
+            if (nsplit && level == LEVELS) return NV_ERR_TREE_DEPTH;
+            break;
+        }
+        if (int rc = grow_nodes(t, st, L1 + 8 * nsplit, L1)) return rc;
+        hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->flags, t->offs,
+                           ka, level);
+        TCHK(hipGetLastError());
+        t->lvl.push_back(L1 + 8 * nsplit);
+        ++level;
+    }
+    const int nlev = (int)t->lvl.size() - 1;
+    t->nnodes = t->lvl.back();
+    for (int lv = nlev - 1; lv >= 0; --lv) {
+        const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
+        if (L1 > L0)
+            hipLaunchKernelGGL(k_up, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->vals);
+        TCHK(hipGetLastError());
+    }
+    TCHK(hipMemcpyAsync(t->h_int, t->nd.size, sizeof(int), hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    t->nwalk = t->h_int[0];
+    if (t->nwalk > t->walk_cap) {
+        hipFree(t->walk);
+        t->walk_cap = std::max(t->nwalk, 2 * t->walk_cap);
+        TCHK(hipMalloc(&t->walk, sizeof(WNode) * (size_t)t->walk_cap));
+    }
+    for (int lv = 0; lv < nlev; ++lv) {
+        const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
+        if (L1 > L0)
+            hipLaunchKernelGGL(k_down, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, nact,
+                               t->walk);
+        TCHK(hipGetLastError());
+    }
+    stats->tree_nodes = t->nnodes;
+    stats->tree_levels = nlev;
+    return NV_OK;
+}
+
+int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
+                          int precision, double *part, DCtl *ctl, nv_stats *stats) {
+    (void)precision;
+    TCHK(hipMemsetAsync(t->counters, 0, 4 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256), dim3(256), 0, st, d, p, sh, n, npad, t->walk, t->nwalk,
+                       part, ctl, t->counters);
+    TCHK(hipGetLastError());
+    unsigned long long h[2];
+    TCHK(hipMemcpyAsync(h, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    stats->tree_evals = (int64_t)h[0];
+    stats->tree_nodes_visited = (int64_t)h[1];
+    return NV_OK;
+}
+
+int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats) {
+    (void)stats;
+    if (n <= 0) return NV_OK;
+    hipLaunchKernelGGL(k_walk_close<256>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, t->walk, t->nwalk);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *nout) {
+    const int nn = t->nnodes, nc = t->nd.ncap;
+    *nout = nn;
+    if (!buf || cap < nn) return NV_OK;
+    std::vector<int> first(nn), pc(nn), li(nn);
+    std::vector<double> pos(3 * (size_t)nc), w(nn), sum(3 * (size_t)nc), circ(3 * (size_t)nc);
+    TCHK(hipMemcpyAsync(first.data(), t->nd.first, sizeof(int) * nn, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(pc.data(), t->nd.pcount, sizeof(int) * nn, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(li.data(), t->nd.leafinf, sizeof(int) * nn, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(w.data(), t->nd.w, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(pos.data(), t->nd.pos, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(sum.data(), t->nd.sum, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
+    TCHK(hipMemcpyAsync(circ.data(), t->nd.circ, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    int o = 0;
+    std::vector<int> stack;
+    stack.push_back(0);
+    while (!stack.empty()) {
+        int q = stack.back();
+        stack.pop_back();
+        nv_tree_node &r = buf[o++];
+        const double cntd = q == 0 ? (double)t->nact_h : (double)pc[q];
+        for (int k = 0; k < 3; ++k) {
+            r.pos[k] = pos[(size_t)k * nc + q];
+            r.cent[k] = pc[q] > 0 ? sum[(size_t)k * nc + q] / cntd : 0.0;
+            r.circ[k] = q == 0 ? 0.0 : circ[(size_t)k * nc + q];
+        }
+        r.width = w[q];
+        r.pcount = pc[q];
+        r.leaf_infront = li[q];
+        if (first[q] >= 0)
+            for (int c = 7; c >= 0; --c) stack.push_back(first[q] + c);
+    }
+    return NV_OK;
+}

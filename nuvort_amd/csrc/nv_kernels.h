@@ -1,0 +1,49 @@
+// Launch wrappers of every kernel, and the device control block.
+#pragma once
+#include "nv_device.h"
+#include "../../include/nuvort.h"
+
+// status bits set by kernels (mapped to NV_ERR_* by the host)
+#define NVK_NAN 1
+#define NVK_ZERO_LENGTH 2
+#define NVK_CAPACITY 4
+#define NVK_SINGULAR 8
+#define NVK_TREE_DEPTH 16
+
+// device-resident counters shared by the topology kernels
+struct DCtl {
+    int n;             // pcount
+    int status;        // NVK_* bits
+    int recon_count;
+    int remove_count;
+    int nevents;       // reconnection events logged since the host last read them
+    int nactive;       // pinsert: 1 if any point is active
+    int ncand;         // rows in the current candidate list
+    int nempty;        // rows in the current empty-slot list
+    int pad[8];
+};
+
+// velocity (k_velocity.hip)
+hipError_t nvk_src64(hipStream_t st, DState s, int n, int npad, double *rec);
+hipError_t nvk_bs64(hipStream_t st, DState s, int n, int npad, const double *rec, const Shifts *sh, int ntiles, int S,
+                    double kappa, double *part);
+hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, double *origin);
+hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
+                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part);
+hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
+                      int do_ab);
+size_t nvk_rec32_bytes();
+
+// boundary / topology / close search (k_topology.hip)
+hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n);
+hipError_t nvk_wrap(hipStream_t st, DState s, Phys p, int n);
+hipError_t nvk_nan(hipStream_t st, DState s, int n, DCtl *ctl);
+hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl);
+hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, int n_old, const int *cand, const int *empty,
+                              DCtl *ctl);
+hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
+hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev,
+                             int ev_cap, double t, int itime);
+hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
+hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl);
+hipError_t nvk_pclose(hipStream_t st, DState s, int n, int npad, int S, double *pd, int *pj);

@@ -1,0 +1,18 @@
+// Barnes-Hut octree on the device (src/tree.f90): build, walks, dump.
+#pragma once
+#include "nv_kernels.h"
+
+struct TreeBuf;
+
+TreeBuf *nv_tree_new();
+void nv_tree_free(TreeBuf *t);
+// construct_tree (src/tree.f90:38-157) from the current positions/ghosts
+int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl, nv_stats *stats);
+// tree_walk (src/tree.f90:250-316) for every point and image: writes the
+// unshifted and image sums to part[0..6)[npad] (the layout k_finish reads)
+int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
+                          int precision, double *part, DCtl *ctl, nv_stats *stats);
+// pclose_tree / closest_tree (src/tree.f90:190-244) over the (stale) tree
+int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats);
+// every node, depth-first, empty ones included
+int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *n);

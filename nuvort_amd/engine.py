@@ -1,0 +1,172 @@
+"""Host-side mirror of the reference's hot-path procedures over the HIP C-ABI.
+
+``FilamentSolver`` keeps the filament state on the GPU and exposes one method
+per reference routine, with the reference's names and call order
+(src/run.f90:229-270): ``ghostp``, ``construct_tree``, ``pmotion``,
+``pinsert``, ``pclose`` / ``pclose_tree``, ``precon``, ``premove``,
+``enforce_boundary``, ``empty_tree``, ``nan_finder``; ``step(k)`` runs k
+iterations of the main-loop body on the device.  Errors raise
+``NuvortError`` where the reference calls fatal_error (src/cdata.f90:245-255).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _L
+from .runfile import RunParams, validate
+from .state import FilamentState
+
+
+def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.NvParams:
+    s = _L.NvParams()
+    s.delta, s.dt, s.quant_circ, s.corea = p.delta, p.dt, p.quant_circ, p.corea
+    for k in range(3):
+        s.box_size[k] = p.box_size[k]
+        s.super_velocity[k] = p.super_velocity[k]
+    for k, b in enumerate((p.boundary_x, p.boundary_y, p.boundary_z)):
+        s.boundary[k] = {"open": 0, "periodic": 1}.get(b, 2)
+    s.velocity = {"LIA": 0, "BS": 1, "Tree": 2}[p.velocity]
+    s.tree_theta = p.tree_theta
+    s.tree_extra_correction = 1 if p.tree_extra_correction else 0
+    s.normal_velocity = {"zero": 0, "xflow": 1}.get(p.normal_velocity, 9)
+    s.norm_vel_xflow = p.norm_vel_xflow
+    s.alpha[0], s.alpha[1] = p.alpha
+    s.normal_fluid_cutoff, s.t_zero_normal_fluid = p.normal_fluid_cutoff, p.t_zero_normal_fluid
+    s.recon_shots = p.recon_shots
+    s.overide_timestep_check = 1 if p.overide_timestep_check else 0
+    s.precision = {"fp64": 0, "fp32c": 1}[p.hip_precision]
+    s.near_radius = p.hip_near_radius
+    s.close_search = {"auto": 0, "brute": 1, "tree": 2}[p.hip_close_search]
+    s.exact_order = 1 if exact_order else 0
+    s.capacity = capacity
+    return s
+
+
+class FilamentSolver:
+    def __init__(self, params: RunParams, exact_order: bool = False, capacity: int = 0, check: bool = True):
+        if check:
+            validate(params)
+        self.params = params
+        self.L = _L.load()
+        self.h = C.c_void_p()
+        rc = self.L.nv_create(C.byref(nv_params(params, exact_order, capacity)), C.byref(self.h))
+        if rc != 0:
+            raise _L.NuvortError(rc, "nv_create rejected the parameters")
+        self._events = []
+
+    # ------------------------------------------------------------ plumbing
+    def _call(self, name, *args):
+        rc = getattr(self.L, name)(self.h, *args)
+        if rc != 0:
+            raise _L.NuvortError(rc, self.L.nv_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.nv_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _view(st: FilamentState, cap: int | None = None) -> _L.NvState:
+        v = _L.NvState()
+        v.pcount = st.pcount
+        v.capacity = cap if cap is not None else st.pcount
+        v.itime, v.recon_count, v.t = st.itime, st.recon_count, st.t
+        v.remove_count = int(st.extra.get("remove_count", 0))
+        m = {"x": st.x, "u": st.u, "u1": st.u1, "u2": st.u2, "u_sup": st.u_sup, "u_mf": st.u_mf,
+             "ghosti": st.ghosti, "ghostb": st.ghostb, "closestd": st.closestd}
+        for k, a in m.items():
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+            setattr(v, k, a.ctypes.data_as(_L._dp))
+        for k in ("infront", "behind", "closest", "pinnedi", "pinnedb", "wpinned"):
+            a = getattr(st, k)
+            assert a.dtype == np.int32 and a.flags.c_contiguous
+            setattr(v, k, a.ctypes.data_as(_L._ip))
+        return v
+
+    def set_state(self, st: FilamentState) -> None:
+        self._keep = st
+        self._call("nv_set_state", C.byref(self._view(st)))
+
+    def get_state(self) -> FilamentState:
+        probe = _L.NvState()
+        probe.capacity = 0
+        self.L.nv_get_state(self.h, C.byref(probe))
+        st = FilamentState.empty(probe.pcount)
+        v = self._view(st, st.pcount)
+        self._call("nv_get_state", C.byref(v))
+        st.t, st.itime, st.recon_count = v.t, v.itime, v.recon_count
+        st.extra["remove_count"] = v.remove_count
+        return st
+
+    def set_time(self, t: float, itime: int) -> None:
+        self._call("nv_set_time", C.c_double(t), C.c_int32(itime))
+
+    # ------------------------------------------------ reference procedures
+    def ghostp(self):
+        self._call("nv_ghostp")
+
+    def construct_tree(self):
+        self._call("nv_construct_tree")
+
+    def pmotion(self):
+        self._call("nv_pmotion")
+
+    def velocity(self):
+        self._call("nv_velocity")
+
+    def pinsert(self):
+        self._call("nv_pinsert")
+
+    def pclose(self):
+        self._call("nv_pclose")
+
+    def pclose_tree(self):
+        self._call("nv_pclose_tree")
+
+    def precon(self):
+        self._call("nv_precon")
+
+    def premove(self):
+        self._call("nv_premove")
+
+    def enforce_boundary(self):
+        self._call("nv_enforce_boundary")
+
+    def empty_tree(self):
+        self._call("nv_empty_tree")
+
+    def nan_finder(self):
+        self._call("nv_nan_check")
+
+    def step(self, nsteps: int = 1):
+        self._call("nv_step", C.c_int32(nsteps))
+
+    # ------------------------------------------------------------ outputs
+    def recon_events(self):
+        buf = (_L.NvReconEvent * 65536)()
+        n = C.c_int32(0)
+        self._call("nv_get_recon_events", buf, 65536, C.byref(n))
+        return [(e.itime, e.i, e.j, e.t, tuple(e.mid), e.angle, e.dl) for e in buf[:min(n.value, 65536)]]
+
+    def tree_nodes(self) -> np.ndarray:
+        n = C.c_int32(0)
+        self._call("nv_get_tree", None, 0, C.byref(n))
+        buf = (_L.NvTreeNode * max(1, n.value))()
+        self._call("nv_get_tree", buf, n.value, C.byref(n))
+        dt = np.dtype([("pos", "<f8", 3), ("width", "<f8"), ("cent", "<f8", 3), ("circ", "<f8", 3),
+                       ("pcount", "<i4"), ("leaf_infront", "<i4")])
+        return np.frombuffer(bytes(buf), dt, n.value).copy()
+
+    def stats(self) -> dict:
+        s = _L.NvStats()
+        self._call("nv_get_stats", C.byref(s))
+        return {f: getattr(s, f) for f, _ in _L.NvStats._fields_}
