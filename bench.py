@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the filament velocity + timestep path on one MI355X.
+
+Workload (the headline configuration): 64 vortex rings, radii U[0.01, 0.03]
+cm, centres uniform in a periodic 0.1 cm box, orientations uniform on the
+sphere, delta = 5e-4 cm (~16k points), kappa = 9.97e-4, core 1.3e-8 cm,
+dt = 5e-6 s, direct Biot-Savart over straight segments with the 26
+periodic images, reconnections and re-meshing on, 3rd-order Adams-Bashforth.
+Seed 1236 (1234 + configuration number 2).  Synthetic data, generated here.
+
+A "step" is one pass of the whole reference main-loop body on the device:
+ghosts -> velocity (FP32 Kahan-compensated far field, FP64 near field,
+local term, mutual friction) -> AB update -> point insertion -> brute
+close-point search -> reconnection -> point removal -> periodic wrap.
+``value`` = point-updates/s = sum over the timed steps of the points moved,
+divided by the wall time of those steps (state resident in HBM).
+
+Also reported: the same step with the plain FP64 kernel, the Barnes-Hut
+(theta = 0.5) step, the single-step velocity error of the FP32 form against
+the FP64 kernel (all points) and against the FP64 oracle (sampled points),
+the roofline of the direct-sum kernel, and the oracle's FP64 CPU velocity
+timed on a bounded sample of targets as the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "vortex point-updates/s (points×AB steps) + max rel. velocity err vs FP64"
+FLOP_PER_PAIR = 43          # reference arithmetic per (target, segment, image) evaluation
+PEAK_FP32 = 157.3           # TFLOP/s, MI355X vector FP32
+PEAK_FP64 = 78.6            # TFLOP/s, MI355X vector FP64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_err_stats(u_ref: np.ndarray, u: np.ndarray) -> dict:
+    """Per-point e_i = |u_i - u_ref_i| / |u_ref_i| over points with
+    |u_ref_i| >= 1e-3 max|u_ref|; normwise = max |du| / max |u_ref|."""
+    mag = np.linalg.norm(u_ref, axis=1)
+    err = np.linalg.norm(u - u_ref, axis=1)
+    sel = mag >= 1e-3 * mag.max()
+    e = err[sel] / mag[sel]
+    return {"max_rel": float(e.max()), "rms_rel": float(np.sqrt(np.mean(e ** 2))),
+            "normwise_max": float(err.max() / mag.max()), "points": int(sel.sum())}
+
+
+def workload():
+    from nuvort_amd import tangles
+    from nuvort_amd.runfile import RunParams
+    st = tangles.random_rings(1236, 64, 0.01, 0.03, 0.1, 5e-4)
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1), velocity="BS",
+                  hip_precision="fp32c", recon_shots=1, overide_timestep_check=False)
+    return st, p
+
+
+def timed_steps(p, st, steps, warmup):
+    from nuvort_amd.engine import FilamentSolver
+    s = FilamentSolver(p)
+    s.set_state(st)
+    if warmup:
+        s.step(warmup)
+    s0 = s.stats()
+    t0 = time.perf_counter()
+    s.step(steps)
+    t1 = time.perf_counter()
+    s1 = s.stats()
+    out = {"seconds": t1 - t0, "point_updates": s1["point_updates"] - s0["point_updates"],
+           "pair_ms": s1["pair_ms_total"] - s0["pair_ms_total"],
+           "pair_launches": s1["pair_launches"] - s0["pair_launches"],
+           "pair_evals": s1["pair_evals_total"] - s0["pair_evals_total"],
+           "walk_ms": s1["walk_ms_total"] - s0["walk_ms_total"],
+           "walk_launches": s1["walk_launches"] - s0["walk_launches"],
+           "walk_nodes": s1["walk_nodes_total"] - s0["walk_nodes_total"],
+           "final_points": s.get_state().active}
+    s.close()
+    return out
+
+
+def velocity_once(p, st, precision):
+    from nuvort_amd.engine import FilamentSolver
+    q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+    q.hip_precision = precision
+    s = FilamentSolver(q)
+    s.set_state(st)
+    s.ghostp()
+    s.velocity()
+    g = s.get_state()
+    s.close()
+    return g
+
+
+def cpu_baseline_parallel(st, p, sample: int, threads: int):
+    """The oracle's FP64 velocity (the reference's arithmetic, one OpenMP
+    loop over targets) for the first `sample` points against all sources and
+    the 27 images; returns seconds, the rows and the velocities."""
+    from oracle import port
+    L = port.lib()
+    ps = port.PortState(st)
+    pr = port.params_struct(p, threads)
+    L.nvo_ghostp(ps.ref(), C.byref(pr))
+    first = 1
+    u = np.zeros((sample, 3))
+    L.nvo_velocity(ps.ref(), C.byref(pr), None, first, 8, u.ctypes.data_as(port._dp))
+    t0 = time.perf_counter()
+    L.nvo_velocity(ps.ref(), C.byref(pr), None, first, sample, u.ctypes.data_as(port._dp))
+    t1 = time.perf_counter()
+    return t1 - t0, np.arange(sample), u
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=int, default=3072, help="targets the CPU baseline evaluates")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and tree comparison runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from nuvort_amd import lib as nvlib
+    nvlib.load()
+    st, p = workload()
+    n = st.active
+    log("workload: %d points" % n)
+
+    head = timed_steps(p, st, args.steps, args.warmup)
+    ms_step = head["seconds"] / args.steps * 1e3
+    value = head["point_updates"] / head["seconds"]
+    log("fp32c step: %.3f ms, %.4g point-updates/s" % (ms_step, value))
+
+    # roofline of the dominant kernel (the FP32-compensated direct sum)
+    t_kernel = head["pair_ms"] / max(1, head["pair_launches"]) * 1e-3
+    evals_per_launch = head["pair_evals"] / max(1, head["pair_launches"])
+    achieved = FLOP_PER_PAIR * evals_per_launch / t_kernel / 1e12
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "direct_fp32c_rings64_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32, 4), "traffic": traffic,
+                "kernel": "k_bs32c (direct sum, FP32 far field)", "kernel_ms": round(t_kernel * 1e3, 4),
+                "pair_evals_per_launch": int(evals_per_launch), "flop_per_pair": FLOP_PER_PAIR}
+
+    extra = {}
+    if not args.no_extra:
+        q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+        q.hip_precision = "fp64"
+        k64 = max(5, min(args.steps, 20))
+        r64 = timed_steps(q, st, k64, 2)
+        v64 = r64["point_updates"] / r64["seconds"]
+        t64 = r64["pair_ms"] / max(1, r64["pair_launches"]) * 1e-3
+        a64 = FLOP_PER_PAIR * (r64["pair_evals"] / max(1, r64["pair_launches"])) / t64 / 1e12
+        extra["fp64_plain"] = {"value": v64, "ms_per_step": r64["seconds"] / k64 * 1e3,
+                               "kernel_ms": t64 * 1e3, "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
+                               "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": t64 / t_kernel}
+        log("fp64 step: %.3f ms (x%.2f)" % (r64["seconds"] / k64 * 1e3, value / v64))
+        qt = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+        qt.hip_precision, qt.velocity, qt.tree_theta = "fp64", "Tree", 0.5
+        kt = max(5, min(args.steps, 50))
+        rt = timed_steps(qt, st, kt, 2)
+        tw = rt["walk_ms"] / max(1, rt["walk_launches"]) * 1e-3
+        walk_nodes = rt["walk_nodes"] / max(1, rt["walk_launches"])
+        # algorithmic bytes of one walk: node records read (80 B) + targets
+        # read (27 walks x 12 B... positions as 3 x 8 B FP64) + velocities written
+        walk_bytes = walk_nodes * 80 + n * 27 * 24 + n * 48
+        extra["tree_theta_0.5"] = {"value": rt["point_updates"] / rt["seconds"],
+                                   "ms_per_step": rt["seconds"] / kt * 1e3, "walk_ms": tw * 1e3,
+                                   "walk_GBps": walk_bytes / tw / 1e9, "hbm_peak_GBps": 8000.0,
+                                   "node_reads_per_walk": walk_nodes}
+        log("tree step: %.3f ms" % (rt["seconds"] / kt * 1e3))
+
+    # accuracy of the FP32 form: against the FP64 GPU kernel (every point)
+    g64 = velocity_once(p, st, "fp64")
+    g32 = velocity_once(p, st, "fp32c")
+    act = g64.infront != 0
+    acc = {"fp32c_vs_fp64_gpu": rel_err_stats(g64.u_sup[act], g32.u_sup[act])}
+
+    cpu = None
+    if not args.no_cpu:
+        secs, idx, ucpu = cpu_baseline_parallel(st, p, args.cpu_sample, args.cpu_threads)
+        rows = idx
+        # velocity of the sampled points from the oracle: u (with the local
+        # term and imposed flow, no friction in this workload) = u_sup
+        acc["fp32c_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g32.u[rows])
+        acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g64.u[rows])
+        cpu_rate = args.cpu_sample / secs
+        cpu = {"value": cpu_rate, "unit": "point-updates/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": "FP64 velocity (direct sum, 27 images, local term) of %d of the %d points of the same "
+                         "workload, oracle C restatement, OpenMP; %.2f s; the velocity is >99%% of a CPU step "
+                         "at this size, so point-updates/s = sampled points / time" % (args.cpu_sample, n, secs)}
+        log("cpu baseline: %.4g point-updates/s on %d threads" % (cpu_rate, args.cpu_threads))
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
+        "config": {"workload": "64 random vortex rings in a periodic 0.1 cm box, delta 5e-4 cm, direct "
+                               "Biot-Savart with 26 images, FP32 Kahan far field + FP64 near field, AB3, "
+                               "reconnection and re-meshing every step",
+                   "points": n, "rings": 64, "delta_cm": 5e-4, "dt_s": 5e-6, "box_cm": 0.1, "seed": 1236,
+                   "near_radius_delta": p.hip_near_radius, "parallelism": "1 GPU"},
+        "accuracy": acc,
+        "max_rel_velocity_err_vs_fp64": acc["fp32c_vs_fp64_gpu"]["max_rel"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

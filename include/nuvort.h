@@ -128,6 +128,13 @@ typedef struct {
     double ms_close;           /* last close-point search time */
     double ms_topology;        /* last pinsert+precon+premove time */
     double ms_step;            /* last nv_step call, per step */
+    int64_t point_updates;     /* sum over steps of the points moved by pmotion */
+    int64_t pair_launches;     /* direct-sum kernel launches */
+    double pair_ms_total;      /* summed duration of those launches (HIP events) */
+    int64_t pair_evals_total;  /* summed pair evaluations of those launches */
+    int64_t walk_launches;     /* tree velocity walk launches */
+    double walk_ms_total;      /* summed duration of those launches */
+    int64_t walk_nodes_total;  /* summed node records read by those walks */
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */

@@ -135,18 +135,18 @@ __global__ void k_nan(DState s, int n, DCtl *ctl) {
 // ---------------------------------------------------------------- pinsert
 __global__ void k_pinsert_pre(DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const bool act = s.infront[r] != 0;
-    empty[r] = act ? 0 : 1;
+    const bool act = r < n && s.infront[r] != 0;
+    if (r < n) empty[r] = act ? 0 : 1;
     int c = 0;
     if (act) {
         double disti = dist_gen(s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r], s.gi[r], s.gi[s.cap + r],
                                 s.gi[2 * s.cap + r]);
         c = disti > p.delta ? 1 : 0;
     }
-    cand[r] = c;
-    // a ballot-free presence test for the zero-length stop
-    if (act) ctl->nactive = 1;
+    if (r < n) cand[r] = c;
+    // active-point count for the zero-length stop (src/line.f90:25-27)
+    const unsigned long long m = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&ctl->nactive, (int)__popcll(m));
 }
 
 __global__ void k_pinsert_serial(DState s, Phys p, int n_old, const int *cand, const int *empty, DCtl *ctl) {

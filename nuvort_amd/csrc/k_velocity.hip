@@ -225,10 +225,13 @@ __global__ __launch_bounds__(BT) void k_bs32c(DState s, int n, int npad, const R
 // mode 0: part holds (acc0, accs) of k_bs64; mode 1: (far, near) of k_bs32c;
 // mode 2: no partials (LIA).
 __global__ void k_finish(DState s, Phys p, int n, int npad, const double *__restrict__ part, int S, int mode,
-                         double t, int do_ab) {
+                         double t, int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = r < n && s.infront[r] != 0;
+    const unsigned long long m = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&ctl->nvel, (int)__popcll(m));
     if (r >= n) return;
-    if (s.infront[r] == 0) {
+    if (!act) {
         if (do_ab) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) s.xn[k * s.cap + r] = s.x[k * s.cap + r];
@@ -292,8 +295,9 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec,
 }
 
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
-                      int do_ab) {
+                      int do_ab, DCtl *ctl) {
     if (n > 0)
-        hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, st, s, p, n, npad, part, S, mode, t, do_ab);
+        hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, st, s, p, n, npad, part, S, mode, t, do_ab,
+                           ctl);
     return hipGetLastError();
 }

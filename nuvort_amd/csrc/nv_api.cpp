@@ -44,6 +44,8 @@ struct Ctx {
     std::string err;
     TreeBuf *tree = nullptr;
     bool tree_valid = false;
+    int64_t last_pair_evals = 0;
+    int n_active = -1;           // active points, refreshed by nv_pinsert / nv_set_state
 };
 
 int fail(Ctx *c, int code, const std::string &msg) {
@@ -164,14 +166,18 @@ int velocity_impl(Ctx *c, int do_ab) {
     const int ntiles = npad / 256;
     HIPCHK(c, hipEventRecord(c->e0, c->st));
     int mode = 0, S = 1;
+    bool timed_pair = false, timed_walk = false;
     if (c->prm.velocity == NV_VEL_LIA) {
         mode = 2;
     } else if (c->prm.velocity == NV_VEL_TREE) {
         if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pmotion: tree velocity without construct_tree");
         if (int rc = ensure_part(c, (size_t)6 * npad)) return rc;
+        HIPCHK(c, hipEventRecord(c->e2, c->st));
         int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, n, npad, c->prm.precision, c->part,
                                        c->ctl, &c->stats);
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
+        HIPCHK(c, hipEventRecord(c->e3, c->st));
+        timed_walk = true;
         mode = 0;
         S = 1;
     } else {
@@ -182,20 +188,47 @@ int velocity_impl(Ctx *c, int do_ab) {
         if (c->prm.precision == NV_PREC_FP32C) {
             HIPCHK(c, nvk_src32(c->st, c->d, n, npad, c->rec32, c->origin));
             double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+            HIPCHK(c, hipEventRecord(c->e2, c->st));
             HIPCHK(c, nvk_bs32c(c->st, c->d, n, npad, c->rec32, c->origin, c->rec64, &c->sh, ntiles, S,
                                 c->phys.kappa, (float)(rn * rn), c->part));
+            HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 1;
         } else {
+            HIPCHK(c, hipEventRecord(c->e2, c->st));
             HIPCHK(c, nvk_bs64(c->st, c->d, n, npad, c->rec64, &c->sh, ntiles, S, c->phys.kappa, c->part));
+            HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 0;
         }
+        timed_pair = true;
     }
-    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, n, npad, c->part, S, mode, c->t, do_ab));
+    c->ctl_h->nvel = 0;
+    HIPCHK(c, hipMemcpyAsync(&c->ctl->nvel, &c->ctl_h->nvel, sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, n, npad, c->part, S, mode, c->t, do_ab, c->ctl));
+    HIPCHK(c, hipMemcpyAsync(&c->ctl_h->nvel, &c->ctl->nvel, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->e0, c->e1);
     c->stats.ms_velocity = ms;
+    const int64_t nv = c->ctl_h->nvel;
+    if (do_ab) c->stats.point_updates += nv;
+    // evaluated pairs: every active target against every active source in
+    // every image, less j == i (unshifted) and j == behind(i) (all images)
+    c->last_pair_evals = nv * ((int64_t)c->sh.n * nv - c->sh.n - 1);
+    if (timed_pair) {
+        float pm = 0.f;
+        hipEventElapsedTime(&pm, c->e2, c->e3);
+        c->stats.pair_ms_total += pm;
+        c->stats.pair_launches += 1;
+        c->stats.pair_evals_total += c->last_pair_evals;
+    }
+    if (timed_walk) {
+        float wm = 0.f;
+        hipEventElapsedTime(&wm, c->e2, c->e3);
+        c->stats.walk_ms_total += wm;
+        c->stats.walk_launches += 1;
+        c->stats.walk_nodes_total += c->stats.tree_nodes_visited;
+    }
     if (do_ab) swap_x(c);
     return NV_OK;
 }

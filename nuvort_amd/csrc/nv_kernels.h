@@ -17,10 +17,11 @@ struct DCtl {
     int recon_count;
     int remove_count;
     int nevents;       // reconnection events logged since the host last read them
-    int nactive;       // pinsert: 1 if any point is active
+    int nactive;       // active points counted by the pinsert pre-filter
     int ncand;         // rows in the current candidate list
     int nempty;        // rows in the current empty-slot list
-    int pad[8];
+    int nvel;          // points whose velocity the last velocity launch computed
+    int pad[7];
 };
 
 // velocity (k_velocity.hip)
@@ -31,7 +32,7 @@ hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, doubl
 hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
                      const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
-                      int do_ab);
+                      int do_ab, DCtl *ctl);
 size_t nvk_rec32_bytes();
 
 // boundary / topology / close search (k_topology.hip)

@@ -54,7 +54,9 @@ class NvStats(C.Structure):
     _fields_ = [("pair_evals", C.c_int64), ("near_evals", C.c_int64), ("tree_evals", C.c_int64),
                 ("tree_nodes_visited", C.c_int64), ("tree_nodes", C.c_int32), ("tree_levels", C.c_int32),
                 ("ms_velocity", C.c_double), ("ms_tree_build", C.c_double), ("ms_close", C.c_double),
-                ("ms_topology", C.c_double), ("ms_step", C.c_double)]
+                ("ms_topology", C.c_double), ("ms_step", C.c_double), ("point_updates", C.c_int64),
+                ("pair_launches", C.c_int64), ("pair_ms_total", C.c_double), ("pair_evals_total", C.c_int64),
+                ("walk_launches", C.c_int64), ("walk_ms_total", C.c_double), ("walk_nodes_total", C.c_int64)]
 
 
 EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_time", "nv_ghostp",

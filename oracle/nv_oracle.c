@@ -471,6 +471,9 @@ int nvo_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, int32_t 
     double sh[26][3];
     int nsh = shift_table_cache(p, sh);
     int err = NVO_OK;
+#ifdef _OPENMP
+    if (p->threads > 0) omp_set_num_threads(p->threads);
+#endif
     int64_t evals = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : evals)
     for (int32_t q = 0; q < count; q++) {
