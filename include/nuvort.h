@@ -82,8 +82,9 @@ typedef struct {
     int32_t exact_order;         /* 1: sum each point's terms in the reference's order (one
                                     chunk per point, bit-exact FP64, slower); 0: chunked */
     int32_t capacity;            /* initial device rows (0 = 2 x pcount) */
-    int32_t block_targets;       /* direct sum targets per workgroup (0 = tuned) */
-    int32_t tile_sources;        /* direct sum sources per LDS tile (0 = tuned) */
+    int32_t targets_per_lane;    /* FP32C direct sum: points per lane, 1 or 2 (0 = tuned) */
+    int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
+                                    adds, 1 (every term), 2, 4 or 8 (0 = default 8) */
 } nv_params;
 
 /* Host view of the filament point array, structure of arrays (the fields

@@ -313,6 +313,10 @@ __global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk
     walk[my] = w;
 }
 
+__global__ void k_zero_counters(unsigned long long *c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
+}
+
 // ------------------------------------------------------------ walks
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -651,10 +655,16 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
                           int precision, double *part, DCtl *ctl, nv_stats *stats) {
     (void)precision;
-    TCHK(hipMemsetAsync(t->counters, 0, 4 * sizeof(unsigned long long), st));
+    (void)stats;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
     hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256), dim3(256), 0, st, d, p, sh, n, npad, t->walk, t->nwalk,
                        part, ctl, t->counters);
     TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+// interaction and node-visit counters of the last velocity walk (synchronises)
+int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats) {
     unsigned long long h[2];
     TCHK(hipMemcpyAsync(h, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));

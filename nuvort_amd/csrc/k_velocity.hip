@@ -139,32 +139,43 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 }
 
 // FP32 compensated far field + FP64 near field.  Far pairs use the
-// well-conditioned form u = k/4pi (d x e)(r1+r2) / (r1 r2 (r1 r2 + R1.R2))
-// (algebraically equal to the reference's expression, without its two
-// cancellations) with d accurate to FP32 rounding of |d| thanks to
-// tile-local coordinates, and Kahan-compensated FP32 accumulation.  Pairs
-// with |d| < r_near, and the reference's excluded pairs, go through the FP64
-// reference arithmetic instead.  part: [S][6][npad] = far xyz (already times
-// k/4pi), near xyz.
-template <int BT, int T>
+// well-conditioned form u = k/4pi (d x e) (1/r1 + 1/r2) / (r1 r2 + R1.R2)
+// with R1 = d, R2 = d + e (algebraically equal to the reference's
+// expression, without its two cancellations); d is accurate to FP32
+// rounding of |d| because positions are stored relative to their tile's
+// origin.  Terms are accumulated in FP32, compensated (Kahan) every KB terms.
+// Pairs closer than r_near, and the pairs the reference excludes, go
+// through the reference's FP64 arithmetic instead.  Each lane carries TPT
+// target points, so one LDS record read serves TPT pairs.
+// part: [S][6][npad] = far xyz (already times k/4pi), near xyz.
+template <int BT, int T, int TPT, int KB>
 __global__ __launch_bounds__(BT) void k_bs32c(DState s, int n, int npad, const Rec32 *__restrict__ rec,
                                               const double *__restrict__ origin, const double *__restrict__ rec64,
                                               Shifts sh, int ntiles, int S, double kappa, float rn2,
                                               double *__restrict__ part) {
     __shared__ Rec32 tile[T];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * BT + tid;
-    const bool act = (i < n) && (s.infront[i] != 0);
-    double xi0 = 0, xi1 = 0, xi2 = 0;
-    int bi = -2;
-    if (act) {
-        xi0 = ld(s.x, s.cap, 0, i);
-        xi1 = ld(s.x, s.cap, 1, i);
-        xi2 = ld(s.x, s.cap, 2, i);
-        bi = s.behind[i] - 1;
+    int ii[TPT], bi[TPT];
+    bool act[TPT];
+    double xi[TPT][3];
+    float f[TPT][3], k[TPT][3];
+    double nr[TPT][3];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        ii[q] = blockIdx.x * BT * TPT + q * BT + tid;
+        act[q] = (ii[q] < n) && (s.infront[ii[q]] != 0);
+        bi[q] = -2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xi[q][c] = act[q] ? ld(s.x, s.cap, c, ii[q]) : 0.0;
+            f[q][c] = 0.f;
+            k[q][c] = 0.f;
+            nr[q][c] = 0.0;
+        }
+        if (act[q]) bi[q] = s.behind[ii[q]] - 1;
+        any = any || act[q];
     }
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f, k0 = 0.f, k1 = 0.f, k2 = 0.f;
-    double n0 = 0, n1 = 0, n2 = 0;
     const long U = (long)sh.n * ntiles;
     const int c = blockIdx.y;
     const long u0 = U * c / S, u1 = U * (c + 1) / S;
@@ -173,51 +184,68 @@ __global__ __launch_bounds__(BT) void k_bs32c(DState s, int n, int npad, const R
         __syncthreads();
         for (int q = tid; q < T; q += BT) tile[q] = rec[(size_t)t * T + q];
         __syncthreads();
-        if (!act) continue;
-        // target relative to the tile origin, shifted image: d = xr_j - ti
-        const float t0 = (float)((xi0 - sh.s[m][0]) - origin[t * 3 + 0]);
-        const float t1 = (float)((xi1 - sh.s[m][1]) - origin[t * 3 + 1]);
-        const float t2 = (float)((xi2 - sh.s[m][2]) - origin[t * 3 + 2]);
-#pragma unroll 2
-        for (int jj = 0; jj < T; ++jj) {
-            const Rec32 q = tile[jj];
-            const float d0 = q.x0 - t0, d1 = q.x1 - t1, d2 = q.x2 - t2;
-            const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-            if (r1sq < rn2 || q.j == bi) {
-                // near field or excluded pair: reference arithmetic in FP64
-                const int j = q.j;
-                if (j < 0 || j == bi || (m == 0 && j == i)) continue;
-                const double *qv = &rec64[(size_t)j * 8];
-                double out[3];
-                if (bs_term64((qv[0] + sh.s[m][0]) - xi0, (qv[1] + sh.s[m][1]) - xi1, (qv[2] + sh.s[m][2]) - xi2,
-                              qv[3], qv[4], qv[5], qv[6], kappa, out)) {
-                    n0 += out[0]; n1 += out[1]; n2 += out[2];
+        if (!any) continue;
+        const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+        float tt[TPT][3];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            tt[q][0] = (float)((xi[q][0] - s0) - origin[t * 3 + 0]);
+            tt[q][1] = (float)((xi[q][1] - s1) - origin[t * 3 + 1]);
+            tt[q][2] = (float)((xi[q][2] - s2) - origin[t * 3 + 2]);
+        }
+        for (int jb = 0; jb < T; jb += KB) {
+            float a[TPT][3];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < KB; ++jj) {
+                const Rec32 r = tile[jb + jj];
+#pragma unroll
+                for (int q = 0; q < TPT; ++q) {
+                    const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
+                    const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+                    if (r1sq < rn2 || r.j == bi[q]) {
+                        // near field or excluded pair: reference arithmetic in FP64
+                        const int j = r.j;
+                        if (!act[q] || j < 0 || j == bi[q] || (m == 0 && j == ii[q])) continue;
+                        const double *qv = &rec64[(size_t)j * 8];
+                        double out[3];
+                        if (bs_term64((qv[0] + s0) - xi[q][0], (qv[1] + s1) - xi[q][1], (qv[2] + s2) - xi[q][2],
+                                      qv[3], qv[4], qv[5], qv[6], kappa, out)) {
+                            nr[q][0] += out[0]; nr[q][1] += out[1]; nr[q][2] += out[2];
+                        }
+                        continue;
+                    }
+                    const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
+                    const float r2sq = fmaf(2.0f, de, r1sq + r.c);
+                    const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+                    const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+                    const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
+                    const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
+                    a[q][0] = fmaf(fmaf(d1, r.e2, -(d2 * r.e1)), fac, a[q][0]);
+                    a[q][1] = fmaf(fmaf(d2, r.e0, -(d0 * r.e2)), fac, a[q][1]);
+                    a[q][2] = fmaf(fmaf(d0, r.e1, -(d1 * r.e0)), fac, a[q][2]);
                 }
-                continue;
             }
-            const float de = fmaf(d0, q.e0, fmaf(d1, q.e1, d2 * q.e2));
-            const float r2sq = fmaf(2.0f, de, r1sq + q.c);
-            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
-            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
-            const float r1 = r1sq * inv1, r2 = r2sq * inv2;
-            const float r1r2 = r1 * r2;
-            const float den = r1r2 * (r1r2 + (r1sq + de));
-            const float fac = (r1 + r2) * __builtin_amdgcn_rcpf(den);
-            const float cx = fmaf(d1, q.e2, -(d2 * q.e1));
-            const float cy = fmaf(d2, q.e0, -(d0 * q.e2));
-            const float cz = fmaf(d0, q.e1, -(d1 * q.e0));
-            kahan_add(f0, k0, cx * fac);
-            kahan_add(f1, k1, cy * fac);
-            kahan_add(f2, k2, cz * fac);
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) {
+                kahan_add(f[q][0], k[q][0], a[q][0]);
+                kahan_add(f[q][1], k[q][1], a[q][1]);
+                kahan_add(f[q][2], k[q][2], a[q][2]);
+            }
         }
     }
-    if (i < npad) {
-        const double k4 = kappa / (4.0 * NV_PI);
-        double *pp = part + (size_t)c * 6 * npad;
-        pp[0 * npad + i] = ((double)f0 - (double)k0) * k4;
-        pp[1 * npad + i] = ((double)f1 - (double)k1) * k4;
-        pp[2 * npad + i] = ((double)f2 - (double)k2) * k4;
-        pp[3 * npad + i] = n0; pp[4 * npad + i] = n1; pp[5 * npad + i] = n2;
+    const double k4 = kappa / (4.0 * NV_PI);
+    double *pp = part + (size_t)c * 6 * npad;
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int i = ii[q];
+        if (i >= npad) continue;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            pp[cc * npad + i] = ((double)f[q][cc] - (double)k[q][cc]) * k4;
+            pp[(3 + cc) * npad + i] = nr[q][cc];
+        }
     }
 }
 
@@ -287,10 +315,24 @@ hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, doubl
 }
 
 hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
-                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part) {
-    dim3 grid(npad / 256, S);
-    hipLaunchKernelGGL((k_bs32c<256, 256>), grid, dim3(256), 0, st, s, n, npad, (const Rec32 *)rec, origin, rec64,
-                       *sh, ntiles, S, kappa, rn2, part);
+                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part,
+                     int tpt, int kb) {
+    const Rec32 *r = (const Rec32 *)rec;
+#define NV_BS32(TPT_, KB_)                                                                                   \
+    hipLaunchKernelGGL((k_bs32c<256, 256, TPT_, KB_>), dim3((npad + 256 * TPT_ - 1) / (256 * TPT_), S), dim3(256), \
+                       0, st, s, n, npad, r, origin, rec64, *sh, ntiles, S, kappa, rn2, part)
+    if (tpt == 2) {
+        if (kb == 8) NV_BS32(2, 8);
+        else if (kb == 4) NV_BS32(2, 4);
+        else if (kb == 2) NV_BS32(2, 2);
+        else NV_BS32(2, 1);
+    } else {
+        if (kb == 8) NV_BS32(1, 8);
+        else if (kb == 4) NV_BS32(1, 4);
+        else if (kb == 2) NV_BS32(1, 2);
+        else NV_BS32(1, 1);
+    }
+#undef NV_BS32
     return hipGetLastError();
 }
 

@@ -30,7 +30,8 @@ hipError_t nvk_bs64(hipStream_t st, DState s, int n, int npad, const double *rec
                     double kappa, double *part);
 hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, double *origin);
 hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
-                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part);
+                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part,
+                     int tpt, int kb);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
                       int do_ab, DCtl *ctl);
 size_t nvk_rec32_bytes();

@@ -12,6 +12,7 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
 // unshifted and image sums to part[0..6)[npad] (the layout k_finish reads)
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
                           int precision, double *part, DCtl *ctl, nv_stats *stats);
+int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats);
 // pclose_tree / closest_tree (src/tree.f90:190-244) over the (stale) tree
 int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats);
 // every node, depth-first, empty ones included
