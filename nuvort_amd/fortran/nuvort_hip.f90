@@ -1,0 +1,167 @@
+!> iso_c_binding interface to the HIP library (include/nuvort.h).
+!>
+!> The reference's hot-path procedures (src/run.f90:229-270 call sequence:
+!> ghostp, construct_tree, pmotion, pinsert, pclose / pclose_tree, precon,
+!> premove, enforce_boundary, empty_tree, NAN_finder) each have one nv_*
+!> counterpart here.  Every nv_* returns an integer status (NV_OK = 0); the
+!> caller maps a non-zero status to its fatal_error (src/cdata.f90:245-255),
+!> which nv_check below does.
+!>
+!> Arrays cross the boundary as structure-of-arrays: never pass type(qvort)
+!> itself, whose memory layout is processor-defined.  nv_state holds c_loc
+!> pointers to caller arrays x(3,n), u(3,n), ... and infront(n), ...; the
+!> library copies them inside nv_set_state / nv_get_state and keeps no pointer.
+module nuvort_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: NV_OK = 0
+  integer(c_int), parameter :: NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2
+  integer(c_int), parameter :: NV_BC_OPEN = 0, NV_BC_PERIODIC = 1
+  integer(c_int), parameter :: NV_NF_ZERO = 0, NV_NF_XFLOW = 1
+  integer(c_int), parameter :: NV_PREC_FP64 = 0, NV_PREC_FP32C = 1
+  integer(c_int), parameter :: NV_CLOSE_AUTO = 0, NV_CLOSE_BRUTE = 1, NV_CLOSE_TREE = 2
+
+  type, bind(C) :: nv_params
+    real(c_double) :: delta, dt, quant_circ, corea
+    real(c_double) :: box_size(3)
+    integer(c_int32_t) :: boundary(3)
+    integer(c_int32_t) :: velocity
+    real(c_double) :: tree_theta
+    integer(c_int32_t) :: tree_extra_correction
+    integer(c_int32_t) :: normal_velocity
+    real(c_double) :: norm_vel_xflow
+    real(c_double) :: alpha(2)
+    real(c_double) :: super_velocity(3)
+    real(c_double) :: normal_fluid_cutoff, t_zero_normal_fluid
+    integer(c_int32_t) :: recon_shots
+    integer(c_int32_t) :: overide_timestep_check
+    integer(c_int32_t) :: precision
+    real(c_double) :: near_radius
+    integer(c_int32_t) :: close_search
+    integer(c_int32_t) :: exact_order
+    integer(c_int32_t) :: capacity
+    integer(c_int32_t) :: targets_per_lane
+    integer(c_int32_t) :: kahan_block
+  end type
+
+  type, bind(C) :: nv_state
+    integer(c_int32_t) :: pcount, capacity, itime, recon_count, remove_count, pad
+    real(c_double) :: t
+    type(c_ptr) :: x, u, u1, u2, u_sup, u_mf, ghosti, ghostb
+    type(c_ptr) :: infront, behind, closest
+    type(c_ptr) :: closestd
+    type(c_ptr) :: pinnedi, pinnedb
+    type(c_ptr) :: wpinned
+  end type
+
+  interface
+    integer(c_int) function nv_create(params, ctx) bind(C, name='nv_create')
+      import :: nv_params, c_ptr, c_int
+      type(nv_params), intent(in) :: params
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function nv_destroy(ctx) bind(C, name='nv_destroy')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function nv_last_error(ctx) bind(C, name='nv_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_set_state(ctx, st) bind(C, name='nv_set_state')
+      import :: c_ptr, c_int, nv_state
+      type(c_ptr), value :: ctx
+      type(nv_state), intent(in) :: st
+    end function
+    integer(c_int) function nv_get_state(ctx, st) bind(C, name='nv_get_state')
+      import :: c_ptr, c_int, nv_state
+      type(c_ptr), value :: ctx
+      type(nv_state), intent(inout) :: st
+    end function
+    integer(c_int) function nv_set_time(ctx, t, itime) bind(C, name='nv_set_time')
+      import :: c_ptr, c_int, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t
+      integer(c_int32_t), value :: itime
+    end function
+    integer(c_int) function nv_ghostp(ctx) bind(C, name='nv_ghostp')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_construct_tree(ctx) bind(C, name='nv_construct_tree')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_pmotion(ctx) bind(C, name='nv_pmotion')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_pinsert(ctx) bind(C, name='nv_pinsert')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_pclose(ctx) bind(C, name='nv_pclose')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_pclose_tree(ctx) bind(C, name='nv_pclose_tree')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_precon(ctx) bind(C, name='nv_precon')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_premove(ctx) bind(C, name='nv_premove')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_enforce_boundary(ctx) bind(C, name='nv_enforce_boundary')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_empty_tree(ctx) bind(C, name='nv_empty_tree')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_nan_check(ctx) bind(C, name='nv_nan_check')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_step(ctx, nsteps) bind(C, name='nv_step')
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps
+    end function
+  end interface
+
+contains
+
+  !> the message of the last error, as a Fortran string
+  function nv_error_text(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: cs(:)
+    integer :: k
+    msg = ' '
+    call c_f_pointer(nv_last_error(ctx), cs, [256])
+    do k = 1, 256
+      if (cs(k) == c_null_char) exit
+      msg(k:k) = cs(k)
+    end do
+  end function
+
+  !> stop the run the way the reference's fatal_error does on a bad status
+  subroutine nv_check(ctx, status, location)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: location
+    if (status /= NV_OK) then
+      write (*,*) '-------------------------FATAL ERROR-------------------------'
+      write (*,*) trim(location), ': ', trim(nv_error_text(ctx)), ' (status ', status, ')'
+      write (*,*) '-------------------------------------------------------------'
+      stop 1
+    end if
+  end subroutine
+end module

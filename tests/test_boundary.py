@@ -1,0 +1,102 @@
+"""The drop-in boundary on the CPU: the C-ABI library loads and exports every
+entry point include/nuvort.h declares, the Fortran iso_c_binding types have
+the C layout, parameters are validated like the reference's init_setup, and
+the run.in parser follows src/cdata.f90:131-241.  No GPU work is launched."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from nuvort_amd import lib as nvlib
+from nuvort_amd.runfile import RunParams, dt_max, parse_run_text
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "nuvort.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(nv_\w+)\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(nvlib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(nvlib.LIB_PATH):
+        pytest.skip("library not built")
+    L = C.CDLL(nvlib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_fortran_types_have_the_c_layout():
+    exe = os.path.join(ROOT, "nuvort_amd", "fortran", "layout_check")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran shim not built")
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line.strip())
+    assert int(got["nv_params"]) == C.sizeof(nvlib.NvParams)
+    assert int(got["nv_state"]) == C.sizeof(nvlib.NvState)
+    assert int(got["nv_params.precision"]) == nvlib.NvParams.precision.offset
+    assert int(got["nv_params.kahan_block"]) == nvlib.NvParams.kahan_block.offset
+    assert int(got["nv_state.t"]) == nvlib.NvState.t.offset
+    assert int(got["nv_state.wpinned"]) == nvlib.NvState.wpinned.offset
+
+
+def _create(**kw):
+    from nuvort_amd.engine import nv_params
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1), velocity="BS")
+    for k, v in kw.items():
+        setattr(p, k, v)
+    L = nvlib.load()
+    h = C.c_void_p()
+    return L.nv_create(C.byref(nv_params(p)), C.byref(h))
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(dt=1e-5), 9),                               # src/initial.f90:174-179 timestep_check
+    (dict(box_size=(0.1, 0.0, 0.1)), 1),              # src/initial.f90:80 box size <= 0
+    (dict(delta=0.0), 1),                             # src/initial.f90:21 delta must be set
+    (dict(tree_theta=0.5, boundary_x="open"), 1),     # src/initial.f90:133-136 tree with open boundaries
+    (dict(boundary_y="solid"), 7),                    # walls are not on this path
+    (dict(velocity="Tree"), 1),                       # Tree without tree_theta walks a null tree
+])
+def test_parameter_checks(kw, code):
+    if not os.path.exists(nvlib.LIB_PATH):
+        pytest.skip("library not built")
+    assert _create(**kw) == code
+
+
+def test_timestep_bound_matches_reference_formula():
+    p = RunParams(delta=5e-4, corea=1.3e-8)
+    # BASELINE config 1 (dt = 1e-5) is above the bound and needs the override
+    assert 7.0e-6 < dt_max(p) < 7.4e-6
+
+
+def test_run_file_semantics():
+    text = "\n".join([
+        "nsteps 100",
+        "! a comment line is skipped",
+        "velocity Tree",
+        "tree_theta 0.4",
+        "alpha 0.206 0.0083",
+        "unknown_label 3",
+        "tree_extra_correction F",
+        "box_size 0.4",            # one value for a 3-vector: the read fails ...
+        "delta 0.01",              # ... and nothing after it is read
+    ])
+    p = parse_run_text(text)
+    assert p.nsteps == 100 and p.velocity == "Tree" and p.tree_theta == 0.4
+    assert p.alpha == (0.206, 0.0083) and p.tree_extra_correction is False
+    assert p.box_size == (0.0, 0.0, 0.0)
+    assert p.delta is None
+    assert p.quant_circ == 9.97e-4 and p.corea == 8.244023e-9 and p.recon_shots == 1
+
+
+def test_run_file_reads_fortran_forms():
+    p = parse_run_text("dt 1.0d-5\nNAN_test .false.\nbox_size 0.1,0.1, 0.1\nnsteps 0.4\nshots 5\n")
+    assert p.dt == 1e-5 and p.NAN_test is False and p.box_size == (0.1, 0.1, 0.1)
+    assert p.nsteps is None and p.shots is None   # '0.4' into an integer is a read error
