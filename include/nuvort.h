@@ -104,9 +104,10 @@ typedef struct {
     int32_t *wpinned;                                         /* [capacity][3] */
 } nv_state;
 
-/* One reconnection, as src/reconnection.f90:46-49 logs it plus the pair. */
+/* One reconnection, as src/reconnection.f90:46-49 logs it plus the pair
+ * (i, j local to the tangle, 1-based; tangle = 0 for a single system). */
 typedef struct {
-    int32_t itime, i, j, pad;
+    int32_t itime, i, j, tangle;
     double t, mid[3], angle, dl;
 } nv_recon_event;
 
@@ -152,6 +153,13 @@ int nv_set_state(void *ctx, const nv_state *host);
 /* Download into caller arrays; on NV_ERR_CAPACITY host->pcount holds the rows needed. */
 int nv_get_state(void *ctx, nv_state *host);
 int nv_set_time(void *ctx, double t, int32_t itime);
+/* Batches of independent tangles (no reference counterpart: the ensemble
+ * form of the same path).  The arrays hold the tangles one after another,
+ * tangle k with pcounts[k] rows and links local to it (1-based); every
+ * phase and nv_step then advance all tangles together, the direct sums of
+ * all of them in one launch.  The octree path takes one tangle. */
+int nv_set_state_batch(void *ctx, int32_t ntangles, const int32_t *pcounts, const nv_state *host);
+int nv_get_state_batch(void *ctx, int32_t ntangles, int32_t *pcounts, nv_state *host);
 
 /* phases (in the order src/run.f90:229-270 calls them) -------------------- */
 int nv_ghostp(void *ctx);           /* src/boundary.f90:10-19 ghostp */

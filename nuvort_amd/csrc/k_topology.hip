@@ -105,6 +105,17 @@ __device__ __forceinline__ int next_row(const int *cand, int ncand, int &ci, Dir
     return r;
 }
 
+// [lo, hi) of the ascending row list `list` (length n) with rows in [a, b)
+__device__ __forceinline__ int lower_row(const int *list, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (list[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // --------------------------------------------------------------- per row
 __global__ void k_ghosts(DState s, Phys p, int n) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,10 +144,14 @@ __global__ void k_nan(DState s, int n, DCtl *ctl) {
 }
 
 // ---------------------------------------------------------------- pinsert
-__global__ void k_pinsert_pre(DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl) {
+__global__ void k_pinsert_pre(DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < n && s.infront[r] != 0;
-    if (r < n) empty[r] = act ? 0 : 1;
+    if (r < n) {
+        // a free slot of the reference's f(1:pcount): empty and below pcount
+        const int k = tg.row2t[r];
+        empty[r] = (!act && r - tg.base[k] < tg.n[k]) ? 1 : 0;
+    }
     int c = 0;
     if (act) {
         double disti = dist_gen(s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r], s.gi[r], s.gi[s.cap + r],
@@ -144,22 +159,34 @@ __global__ void k_pinsert_pre(DState s, Phys p, int n, int *cand, int *empty, DC
         c = disti > p.delta ? 1 : 0;
     }
     if (r < n) cand[r] = c;
-    // active-point count for the zero-length stop (src/line.f90:25-27)
+    // active points per tangle for the zero-length stop (src/line.f90:25-27);
+    // tangles are 256-row aligned, so a wavefront lies in one tangle
     const unsigned long long m = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&ctl->nactive, (int)__popcll(m));
+    if ((threadIdx.x & 63) == 0 && m) {
+        atomicAdd(&ctl->nactive, (int)__popcll(m));
+        atomicAdd(&tg.nact[tg.row2t[r]], (int)__popcll(m));
+    }
 }
 
-__global__ void k_pinsert_serial(DState s, Phys p, int n_old, const int *cand, const int *empty, DCtl *ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand, const int *empty, DCtl *ctl) {
+    if (threadIdx.x != 0) return;
     __shared__ Dirty d;
     d.hd = d.nd = 0;
-    if (ctl->nactive == 0) { atomicOr(&ctl->status, NVK_ZERO_LENGTH); return; }
-    const int ncand = ctl->ncand, nempty = ctl->nempty;
+    const int tk = blockIdx.x;
+    const int base = tg.base[tk], tcap = tg.cap[tk];
+    int nl = tg.n[tk];                     // local pcount
+    const int n_old = base + nl;           // rows the reference loop visits end here
+    // this tangle's slices of the candidate and empty-slot lists
+    const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tcap);
+    const int e0 = lower_row(empty, ctl->nempty, base), e1 = lower_row(empty, ctl->nempty, base + tcap);
+    // src/line.f90:25-27: no active point in this tangle
+    if (tg.nact[tk] == 0) { atomicOr(&ctl->status, NVK_ZERO_LENGTH); return; }
+    const int *cl = cand + c0;
+    const int ncand = c1 - c0;
     const int cap = s.cap;
-    int n = ctl->n;
-    int ci = 0, ei = 0;
+    int ci = 0, ei = e0;
     while (true) {
-        const int r = next_row(cand, ncand, ci, d);
+        const int r = next_row(cl, ncand, ci, d);
         if (r == 0x7fffffff) break;
         if (s.infront[r] == 0) continue;
         double x[3], gi[3], gb[3];
@@ -172,11 +199,12 @@ __global__ void k_pinsert_serial(DState s, Phys p, int n_old, const int *cand, c
         const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
         if (!(disti > p.delta)) continue;
         int slot;
-        if (ei < nempty) {
+        if (ei < e1) {
             slot = empty[ei++];
         } else {
-            if (n >= cap) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
-            slot = n++;
+            if (nl >= tcap) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+            slot = base + nl;
+            ++nl;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 s.usup[k * cap + slot] = 0.0;
@@ -221,7 +249,7 @@ __global__ void k_pinsert_serial(DState s, Phys p, int n_old, const int *cand, c
         if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
         if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
     }
-    ctl->n = n;
+    tg.n[tk] = nl;
 }
 
 // ----------------------------------------------------------------- precon
@@ -251,13 +279,15 @@ __device__ __forceinline__ double dist_rows(const DState &s, const double *a, in
     return dist_gen(s.x[ra], s.x[s.cap + ra], s.x[2 * s.cap + ra], b[rb], b[s.cap + rb], b[2 * s.cap + rb]);
 }
 
-__global__ void k_precon_serial(DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev, int ev_cap,
-                                double t, int itime) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int ncand = ctl->ncand;
-    const int n = ctl->n;
+__global__ void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl, nv_recon_event *ev,
+                                int ev_cap, double t, int itime) {
+    if (threadIdx.x != 0) return;
+    const int tk = blockIdx.x;
+    const int base = tg.base[tk];
+    const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
+    const int n = tg.n[tk];
     const int cap = s.cap;
-    for (int ci = 0; ci < ncand; ++ci) {
+    for (int ci = c0; ci < c1; ++ci) {
         const int r = cand[ci];
         if (s.infront[r] == 0) continue;
         const int pari = s.infront[r];
@@ -280,11 +310,11 @@ __global__ void k_precon_serial(DState s, Phys p, const int *cand, DCtl *ctl, nv
         const double l_after = dist_rows(s, s.x, r, s.gb, r) + dist_rows(s, s.x, r, s.x, rj) +
                                dist_rows(s, s.x, rj, s.gi, rj) + dist_rows(s, s.x, pari - 1, s.x, parjb - 1);
         if (l_after <= l_before) {
-            ctl->recon_count += 1;
-            const int q = ctl->nevents++;
+            tg.recon[tk] += 1;
+            const int q = atomicAdd(&ctl->nevents, 1);
             if (q < ev_cap) {
                 nv_recon_event e;
-                e.itime = itime; e.i = r + 1; e.j = j; e.pad = 0; e.t = t;
+                e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) e.mid[k] = 0.5 * (s.x[k * cap + r] + s.x[k * cap + rj]);
                 e.angle = acos(dot_val);
@@ -313,15 +343,19 @@ __global__ void k_premove_pre(DState s, Phys p, int n, int *cand) {
     cand[r] = c;
 }
 
-__global__ void k_premove_serial(DState s, Phys p, const int *cand, DCtl *ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
+    if (threadIdx.x != 0) return;
     __shared__ Dirty d;
     d.hd = d.nd = 0;
-    const int ncand = ctl->ncand;
-    const int n = ctl->n;
+    const int tk = blockIdx.x;
+    const int base = tg.base[tk];
+    const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
+    const int *cl = cand + c0;
+    const int ncand = c1 - c0;
+    const int n = tg.n[tk];
     int ci = 0;
     while (true) {
-        const int r = next_row(cand, ncand, ci, d);
+        const int r = next_row(cl, ncand, ci, d);
         if (r == 0x7fffffff) break;
         if (s.infront[r] == 0) continue;
         const int inf = s.infront[r] - 1;
@@ -334,7 +368,7 @@ __global__ void k_premove_serial(DState s, Phys p, const int *cand, DCtl *ctl) {
             s.infront[r] = tinfront + 1;
             clear_row(s, inf);
             loop_killer(s, r, n);
-            ctl->remove_count += 1;
+            tg.remove[tk] += 1;
             if (s.infront[r] != 0) {
                 const int b = s.behind[r] - 1;
                 if (b > r && !dirty_push(d, b)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
@@ -345,17 +379,20 @@ __global__ void k_premove_serial(DState s, Phys p, const int *cand, DCtl *ctl) {
 
 // ------------------------------------------------------ brute close search
 // src/line.f90:135-160 pclose: the first (lowest) j with the smallest
-// distf(i,j) among non-empty j not in {i, infront(i), behind(i)}; sources
-// split into S chunks, each keeping its first minimum; k_pclose_fin keeps
-// the first chunk's on ties.  Distances are the reference's correctly
-// rounded sqrt, compared strictly.
-template <int BT, int T>
-__global__ __launch_bounds__(BT) void k_pclose(DState s, int n, int S, double *__restrict__ pd, int *__restrict__ pj,
-                                               int npad) {
+// distf(i,j) among non-empty j of the same tangle not in {i, infront(i),
+// behind(i)}.  Work items (tangle, target block, chunk of source tiles): each
+// chunk keeps its first minimum, k_pclose_fin keeps the first chunk's on
+// ties.  Distances are the reference's correctly rounded sqrt, compared
+// strictly.  Partial block of tangle k: [S][cap_k].
+__global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item *__restrict__ items,
+                                                double *__restrict__ pd, int *__restrict__ pj) {
+    constexpr int T = 256;
     __shared__ double tile[T * 4];
+    const Item I = items[blockIdx.x];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * BT + tid;
-    const bool act = i < n && s.infront[i] != 0;
+    const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+    const int loc = I.tb * 256 + tid, i = base + loc;
+    const bool act = loc < n && s.infront[i] != 0;
     double x0 = 0, x1 = 0, x2 = 0;
     int fi = -2, bi = -2;
     if (act) {
@@ -363,15 +400,14 @@ __global__ __launch_bounds__(BT) void k_pclose(DState s, int n, int S, double *_
         fi = s.infront[i] - 1; bi = s.behind[i] - 1;
     }
     const int ntiles = (n + T - 1) / T;
-    const int c = blockIdx.y;
-    const int t0 = (int)((long)ntiles * c / S), t1 = (int)((long)ntiles * (c + 1) / S);
+    const int t0 = (int)((long)ntiles * I.c / S), t1 = (int)((long)ntiles * (I.c + 1) / S);
     double best = 100.0, bound = 100.0 * 100.0 * (1.0 + 8e-16);
     int bj = -1;
     for (int t = t0; t < t1; ++t) {
         __syncthreads();
-        for (int q = tid; q < T; q += BT) {
-            const int j = t * T + q;
-            const bool v = j < n && s.infront[j] != 0;
+        for (int q = tid; q < T; q += 256) {
+            const int jl = t * T + q, j = base + jl;
+            const bool v = jl < n && s.infront[j] != 0;
             tile[q * 4 + 0] = v ? s.x[j] : 0.0;
             tile[q * 4 + 1] = v ? s.x[s.cap + j] : 0.0;
             tile[q * 4 + 2] = v ? s.x[2 * s.cap + j] : 0.0;
@@ -382,7 +418,7 @@ __global__ __launch_bounds__(BT) void k_pclose(DState s, int n, int S, double *_
         for (int jj = 0; jj < T; ++jj) {
             const double *q = &tile[jj * 4];
             if (q[3] == 0.0) continue;
-            const int j = t * T + jj;
+            const int j = base + t * T + jj;
             if (j == i || j == fi || j == bi) continue;
             const double d0 = x0 - q[0], d1 = x1 - q[1], d2 = x2 - q[2];
             const double dd = d0 * d0 + d1 * d1 + d2 * d2;
@@ -396,22 +432,26 @@ __global__ __launch_bounds__(BT) void k_pclose(DState s, int n, int S, double *_
             }
         }
     }
-    if (i < npad) {
-        pd[(size_t)c * npad + i] = best;
-        pj[(size_t)c * npad + i] = bj;
+    if (loc < capk) {
+        const size_t o = tg.poff[I.k] + (size_t)I.c * capk + loc;
+        pd[o] = best;
+        pj[o] = bj;
     }
 }
 
-__global__ void k_pclose_fin(DState s, int n, int S, const double *pd, const int *pj, int npad) {
+__global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, const int *pj) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || s.infront[i] == 0) return;
+    if (i >= rows || s.infront[i] == 0) return;
+    const int k = tg.row2t[i];
+    const int capk = tg.cap[k], loc = i - tg.base[k], S = tg.S[k];
     double best = 100.0;
     int bj = -1;
     for (int c = 0; c < S; ++c) {
-        const double d = pd[(size_t)c * npad + i];
-        if (pj[(size_t)c * npad + i] >= 0 && d < best) {
+        const size_t o = tg.poff[k] + (size_t)c * capk + loc;
+        const double d = pd[o];
+        if (pj[o] >= 0 && d < best) {
             best = d;
-            bj = pj[(size_t)c * npad + i];
+            bj = pj[o];
         }
     }
     s.closestd[i] = best;
@@ -435,35 +475,35 @@ hipError_t nvk_nan(hipStream_t st, DState s, int n, DCtl *ctl) {
     if (n > 0) hipLaunchKernelGGL(k_nan, GRID(n), 0, st, s, n, ctl);
     return hipGetLastError();
 }
-hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl) {
-    if (n > 0) hipLaunchKernelGGL(k_pinsert_pre, GRID(n), 0, st, s, p, n, cand, empty, ctl);
+hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl) {
+    if (n > 0) hipLaunchKernelGGL(k_pinsert_pre, GRID(n), 0, st, s, p, tg, n, cand, empty, ctl);
     return hipGetLastError();
 }
-hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, int n_old, const int *cand, const int *empty,
+hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, const int *empty,
                               DCtl *ctl) {
-    hipLaunchKernelGGL(k_pinsert_serial, dim3(1), dim3(64), 0, st, s, p, n_old, cand, empty, ctl);
+    hipLaunchKernelGGL(k_pinsert_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, empty, ctl);
     return hipGetLastError();
 }
 hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
     if (n > 0) hipLaunchKernelGGL(k_precon_pre, GRID(n), 0, st, s, p, n, cand);
     return hipGetLastError();
 }
-hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev,
-                             int ev_cap, double t, int itime) {
-    hipLaunchKernelGGL(k_precon_serial, dim3(1), dim3(64), 0, st, s, p, cand, ctl, ev, ev_cap, t, itime);
+hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
+                             nv_recon_event *ev, int ev_cap, double t, int itime) {
+    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl, ev, ev_cap, t, itime);
     return hipGetLastError();
 }
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
     if (n > 0) hipLaunchKernelGGL(k_premove_pre, GRID(n), 0, st, s, p, n, cand);
     return hipGetLastError();
 }
-hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl) {
-    hipLaunchKernelGGL(k_premove_serial, dim3(1), dim3(64), 0, st, s, p, cand, ctl);
+hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
+    hipLaunchKernelGGL(k_premove_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl);
     return hipGetLastError();
 }
-hipError_t nvk_pclose(hipStream_t st, DState s, int n, int npad, int S, double *pd, int *pj) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_pclose<256, 256>), dim3(npad / 256, S), dim3(256), 0, st, s, n, S, pd, pj, npad);
-    hipLaunchKernelGGL(k_pclose_fin, GRID(n), 0, st, s, n, S, pd, pj, npad);
+hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, double *pd,
+                      int *pj) {
+    if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, pd, pj);
+    if (rows > 0) hipLaunchKernelGGL(k_pclose_fin, GRID(rows), 0, st, s, tg, rows, pd, pj);
     return hipGetLastError();
 }

@@ -3,28 +3,33 @@
 // :175-198), fused with the local term, mutual friction and the
 // Adams-Bashforth update (src/timestep.f90:16-45).
 //
-// Work decomposition: targets are split into blocks of BT rows (one row per
-// thread); the (image, source tile) sequence of each target -- image-major,
-// tiles of T sources in ascending row order, exactly the order the
-// reference visits them -- is split into S contiguous chunks, one per
-// blockIdx.y.  Each block stages a tile of sources in LDS (coalesced SoA
-// reads from HBM/L2), every lane then reads the same LDS record (broadcast),
-// so the inner loop is pure VALU.  Chunk partial sums go to a partial
-// buffer that k_finish reduces in chunk order.  With S = 1 the summation
-// order per target is the reference's, term for term.
+// Work decomposition.  For every tangle, targets are split into blocks of
+// 256 lanes x TPT points; the (image, source tile) sequence of each target --
+// image-major, tiles of 256 sources in ascending row order, the order the
+// reference visits them -- is split into S contiguous chunks.  A work item is
+// (tangle, target block, chunk).  One persistent launch runs all items of all
+// tangles: every workgroup pulls the next item from an atomic queue (items
+// are listed largest tangle first), stages each source tile in LDS with
+// coalesced reads, and every lane then reads the same LDS record
+// (broadcast), so the inner loop is pure VALU.  Chunk partial sums go to a
+// per-tangle partial block that k_finish reduces in chunk order, so the
+// result does not depend on which workgroup ran which item.  With S = 1 the
+// summation order per target is the reference's, term for term.
 #include "nv_kernels.h"
 
 namespace {
 
 constexpr int REC64 = 8;   // x0 x1 x2 e0 e1 e2 c valid
+constexpr int TB = 256;    // lanes per workgroup
+constexpr int TS = 256;    // sources per tile
 
 // FP64 source records, one per row: x, e = ghosti - x, c = |e|^2
 // (dist_gen_sq(ghosti, x), src/timestep.f90:164), valid = infront != 0.
-__global__ void k_src64(DState s, int n, int npad, double *__restrict__ rec) {
+__global__ void k_src64(DState s, int rows, double *__restrict__ rec) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= npad) return;
+    if (r >= rows) return;
     double v[REC64] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (r < n && s.infront[r] != 0) {
+    if (s.infront[r] != 0) {
         double e[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -39,70 +44,83 @@ __global__ void k_src64(DState s, int n, int npad, double *__restrict__ rec) {
     for (int k = 0; k < REC64; ++k) rec[(size_t)r * REC64 + k] = v[k];
 }
 
-// FP64 plain direct sum.  part layout: [S][6][npad] (acc0 xyz, accs xyz).
-template <int BT, int T>
-__global__ __launch_bounds__(BT) void k_bs64(DState s, int n, int npad, const double *__restrict__ rec, Shifts sh,
-                                             int ntiles, int S, double kappa, double *__restrict__ part) {
-    __shared__ double tile[T * REC64];
+__device__ __forceinline__ int next_item(int *queue, int *slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = atomicAdd(queue, 1);
+    __syncthreads();
+    return *slot;
+}
+
+// FP64 plain direct sum, the reference's arithmetic.
+// Partial block of tangle k: [S][6][cap_k] (acc0 xyz, accs xyz).
+__global__ __launch_bounds__(TB) void k_bs64(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
+                                             int *queue, const double *__restrict__ rec, Shifts sh, double kappa,
+                                             double *__restrict__ part) {
+    __shared__ double tile[TS * REC64];
+    __shared__ int slot;
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * BT + tid;
-    const bool act = (i < n) && (s.infront[i] != 0);
-    double xi0 = 0, xi1 = 0, xi2 = 0;
-    int bi = -2;
-    if (act) {
-        xi0 = ld(s.x, s.cap, 0, i);
-        xi1 = ld(s.x, s.cap, 1, i);
-        xi2 = ld(s.x, s.cap, 2, i);
-        bi = s.behind[i] - 1;
-    }
-    double a00 = 0, a01 = 0, a02 = 0, as0 = 0, as1 = 0, as2 = 0;
-    const long U = (long)sh.n * ntiles;
-    const int c = blockIdx.y;
-    const long u0 = U * c / S, u1 = U * (c + 1) / S;
-    for (long un = u0; un < u1; ++un) {
-        const int m = (int)(un / ntiles), t = (int)(un % ntiles);
-        const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
-        __syncthreads();
-        for (int q = tid; q < T * REC64; q += BT) tile[q] = rec[(size_t)t * T * REC64 + q];
-        __syncthreads();
-        if (!act) continue;
-        const int base = t * T;
-        for (int jj = 0; jj < T; ++jj) {
-            const double *qv = &tile[jj * REC64];
-            if (qv[7] == 0.0) continue;
-            const int j = base + jj;
-            if (j == bi || (m == 0 && j == i)) continue;
-            double out[3];
-            if (!bs_term64((qv[0] + s0) - xi0, (qv[1] + s1) - xi1, (qv[2] + s2) - xi2, qv[3], qv[4], qv[5], qv[6],
-                           kappa, out))
-                continue;
-            // keep the reference's running-sum order: accumulate straight
-            // into the image accumulator, term by term
-            if (m == 0) { a00 = a00 + out[0]; a01 = a01 + out[1]; a02 = a02 + out[2]; }
-            else { as0 = as0 + out[0]; as1 = as1 + out[1]; as2 = as2 + out[2]; }
+    for (;;) {
+        const int it = next_item(queue, &slot);
+        if (it >= nitems) break;
+        const Item I = items[it];
+        const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+        const int loc = I.tb * TB + tid, i = base + loc;
+        const bool act = (loc < n) && (s.infront[i] != 0);
+        double xi0 = 0, xi1 = 0, xi2 = 0;
+        int bi = -2;
+        if (act) {
+            xi0 = ld(s.x, s.cap, 0, i);
+            xi1 = ld(s.x, s.cap, 1, i);
+            xi2 = ld(s.x, s.cap, 2, i);
+            bi = s.behind[i] - 1;
         }
-    }
-    if (i < npad) {
-        double *pp = part + (size_t)c * 6 * npad;
-        pp[0 * npad + i] = a00; pp[1 * npad + i] = a01; pp[2 * npad + i] = a02;
-        pp[3 * npad + i] = as0; pp[4 * npad + i] = as1; pp[5 * npad + i] = as2;
+        double a00 = 0, a01 = 0, a02 = 0, as0 = 0, as1 = 0, as2 = 0;
+        const int ntiles = (n + TS - 1) / TS;
+        const long U = (long)sh.n * ntiles;
+        const long u0 = U * I.c / S, u1 = U * (I.c + 1) / S;
+        for (long un = u0; un < u1; ++un) {
+            const int m = (int)(un / ntiles), t = (int)(un % ntiles);
+            const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+            __syncthreads();
+            for (int q = tid; q < TS * REC64; q += TB) tile[q] = rec[((size_t)base + (size_t)t * TS) * REC64 + q];
+            __syncthreads();
+            if (!act) continue;
+            const int jb = base + t * TS;
+            for (int jj = 0; jj < TS; ++jj) {
+                const double *qv = &tile[jj * REC64];
+                if (qv[7] == 0.0) continue;
+                const int j = jb + jj;
+                if (j == bi || (m == 0 && j == i)) continue;
+                double out[3];
+                if (!bs_term64((qv[0] + s0) - xi0, (qv[1] + s1) - xi1, (qv[2] + s2) - xi2, qv[3], qv[4], qv[5],
+                               qv[6], kappa, out))
+                    continue;
+                // the reference's running-sum order, term by term
+                if (m == 0) { a00 = a00 + out[0]; a01 = a01 + out[1]; a02 = a02 + out[2]; }
+                else { as0 = as0 + out[0]; as1 = as1 + out[1]; as2 = as2 + out[2]; }
+            }
+        }
+        if (loc < capk) {
+            double *pp = part + tg.poff[I.k] + (size_t)I.c * 6 * capk;
+            pp[0 * capk + loc] = a00; pp[1 * capk + loc] = a01; pp[2 * capk + loc] = a02;
+            pp[3 * capk + loc] = as0; pp[4 * capk + loc] = as1; pp[5 * capk + loc] = as2;
+        }
     }
 }
 
 // FP32 source records, one per row: position relative to its tile's origin
-// (the first row of the tile, FP64), e in FP32, c = |e|^2 in FP32, row.
+// (the first active row of the tile, FP64), e in FP32, c = |e|^2, row.
 struct __align__(16) Rec32 {
     float x0, x1, x2, e0, e1, e2, c;
     int j;
 };
 
-__global__ void k_src32(DState s, int n, int npad, int T, Rec32 *__restrict__ rec, double *__restrict__ origin) {
+__global__ void k_src32(DState s, int rows, Rec32 *__restrict__ rec, double *__restrict__ origin) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= npad) return;
-    int t = r / T, r0 = t * T;
+    if (r >= rows) return;
+    int t = r / TS, r0 = t * TS;
     double o[3] = {0.0, 0.0, 0.0};
-    // origin: first active row of the tile (fall back to 0)
-    for (int q = r0; q < r0 + T && q < n; ++q) {
+    for (int q = r0; q < r0 + TS && q < rows; ++q) {
         if (s.infront[q] != 0) {
             o[0] = ld(s.x, s.cap, 0, q); o[1] = ld(s.x, s.cap, 1, q); o[2] = ld(s.x, s.cap, 2, q);
             break;
@@ -110,7 +128,7 @@ __global__ void k_src32(DState s, int n, int npad, int T, Rec32 *__restrict__ re
     }
     if (r == r0) { origin[t * 3 + 0] = o[0]; origin[t * 3 + 1] = o[1]; origin[t * 3 + 2] = o[2]; }
     Rec32 v;
-    if (r < n && s.infront[r] != 0) {
+    if (s.infront[r] != 0) {
         double x[3], e[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -147,118 +165,126 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // Pairs closer than r_near, and the pairs the reference excludes, go
 // through the reference's FP64 arithmetic instead.  Each lane carries TPT
 // target points, so one LDS record read serves TPT pairs.
-// part: [S][6][npad] = far xyz (already times k/4pi), near xyz.
-template <int BT, int T, int TPT, int KB>
-__global__ __launch_bounds__(BT) void k_bs32c(DState s, int n, int npad, const Rec32 *__restrict__ rec,
+// Partial block of tangle k: [S][6][cap_k] = far xyz (times k/4pi), near xyz.
+template <int TPT, int KB>
+__global__ __launch_bounds__(TB) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
+                                              int *queue, const Rec32 *__restrict__ rec,
                                               const double *__restrict__ origin, const double *__restrict__ rec64,
-                                              Shifts sh, int ntiles, int S, double kappa, float rn2,
-                                              double *__restrict__ part) {
-    __shared__ Rec32 tile[T];
+                                              Shifts sh, double kappa, float rn2, double *__restrict__ part) {
+    __shared__ Rec32 tile[TS];
+    __shared__ int slot;
     const int tid = threadIdx.x;
-    int ii[TPT], bi[TPT];
-    bool act[TPT];
-    double xi[TPT][3];
-    float f[TPT][3], k[TPT][3];
-    double nr[TPT][3];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < TPT; ++q) {
-        ii[q] = blockIdx.x * BT * TPT + q * BT + tid;
-        act[q] = (ii[q] < n) && (s.infront[ii[q]] != 0);
-        bi[q] = -2;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xi[q][c] = act[q] ? ld(s.x, s.cap, c, ii[q]) : 0.0;
-            f[q][c] = 0.f;
-            k[q][c] = 0.f;
-            nr[q][c] = 0.0;
-        }
-        if (act[q]) bi[q] = s.behind[ii[q]] - 1;
-        any = any || act[q];
-    }
-    const long U = (long)sh.n * ntiles;
-    const int c = blockIdx.y;
-    const long u0 = U * c / S, u1 = U * (c + 1) / S;
-    for (long un = u0; un < u1; ++un) {
-        const int m = (int)(un / ntiles), t = (int)(un % ntiles);
-        __syncthreads();
-        for (int q = tid; q < T; q += BT) tile[q] = rec[(size_t)t * T + q];
-        __syncthreads();
-        if (!any) continue;
-        const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
-        float tt[TPT][3];
+    for (;;) {
+        const int it = next_item(queue, &slot);
+        if (it >= nitems) break;
+        const Item I = items[it];
+        const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+        int ii[TPT], loc[TPT], bi[TPT];
+        bool act[TPT];
+        double xi[TPT][3];
+        float f[TPT][3], k[TPT][3];
+        double nr[TPT][3];
+        bool any = false;
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
-            tt[q][0] = (float)((xi[q][0] - s0) - origin[t * 3 + 0]);
-            tt[q][1] = (float)((xi[q][1] - s1) - origin[t * 3 + 1]);
-            tt[q][2] = (float)((xi[q][2] - s2) - origin[t * 3 + 2]);
-        }
-        for (int jb = 0; jb < T; jb += KB) {
-            float a[TPT][3];
+            loc[q] = I.tb * TB * TPT + q * TB + tid;
+            ii[q] = base + loc[q];
+            act[q] = (loc[q] < n) && (s.infront[ii[q]] != 0);
+            bi[q] = -2;
 #pragma unroll
-            for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
-#pragma unroll
-            for (int jj = 0; jj < KB; ++jj) {
-                const Rec32 r = tile[jb + jj];
-#pragma unroll
-                for (int q = 0; q < TPT; ++q) {
-                    const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
-                    const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-                    if (r1sq < rn2 || r.j == bi[q]) {
-                        // near field or excluded pair: reference arithmetic in FP64
-                        const int j = r.j;
-                        if (!act[q] || j < 0 || j == bi[q] || (m == 0 && j == ii[q])) continue;
-                        const double *qv = &rec64[(size_t)j * 8];
-                        double out[3];
-                        if (bs_term64((qv[0] + s0) - xi[q][0], (qv[1] + s1) - xi[q][1], (qv[2] + s2) - xi[q][2],
-                                      qv[3], qv[4], qv[5], qv[6], kappa, out)) {
-                            nr[q][0] += out[0]; nr[q][1] += out[1]; nr[q][2] += out[2];
-                        }
-                        continue;
-                    }
-                    const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
-                    const float r2sq = fmaf(2.0f, de, r1sq + r.c);
-                    const float inv1 = __builtin_amdgcn_rsqf(r1sq);
-                    const float inv2 = __builtin_amdgcn_rsqf(r2sq);
-                    const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
-                    const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
-                    a[q][0] = fmaf(fmaf(d1, r.e2, -(d2 * r.e1)), fac, a[q][0]);
-                    a[q][1] = fmaf(fmaf(d2, r.e0, -(d0 * r.e2)), fac, a[q][1]);
-                    a[q][2] = fmaf(fmaf(d0, r.e1, -(d1 * r.e0)), fac, a[q][2]);
-                }
+            for (int c = 0; c < 3; ++c) {
+                xi[q][c] = act[q] ? ld(s.x, s.cap, c, ii[q]) : 0.0;
+                f[q][c] = 0.f;
+                k[q][c] = 0.f;
+                nr[q][c] = 0.0;
             }
+            if (act[q]) bi[q] = s.behind[ii[q]] - 1;
+            any = any || act[q];
+        }
+        const int ntiles = (n + TS - 1) / TS;
+        const long U = (long)sh.n * ntiles;
+        const long u0 = U * I.c / S, u1 = U * (I.c + 1) / S;
+        for (long un = u0; un < u1; ++un) {
+            const int m = (int)(un / ntiles), t = (int)(un % ntiles);
+            const int gt = base / TS + t;
+            __syncthreads();
+            for (int q = tid; q < TS; q += TB) tile[q] = rec[(size_t)gt * TS + q];
+            __syncthreads();
+            if (!any) continue;
+            const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+            float tt[TPT][3];
 #pragma unroll
             for (int q = 0; q < TPT; ++q) {
-                kahan_add(f[q][0], k[q][0], a[q][0]);
-                kahan_add(f[q][1], k[q][1], a[q][1]);
-                kahan_add(f[q][2], k[q][2], a[q][2]);
+                tt[q][0] = (float)((xi[q][0] - s0) - origin[gt * 3 + 0]);
+                tt[q][1] = (float)((xi[q][1] - s1) - origin[gt * 3 + 1]);
+                tt[q][2] = (float)((xi[q][2] - s2) - origin[gt * 3 + 2]);
+            }
+            for (int jb = 0; jb < TS; jb += KB) {
+                float a[TPT][3];
+#pragma unroll
+                for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
+#pragma unroll
+                for (int jj = 0; jj < KB; ++jj) {
+                    const Rec32 r = tile[jb + jj];
+#pragma unroll
+                    for (int q = 0; q < TPT; ++q) {
+                        const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
+                        const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+                        if (r1sq < rn2 || r.j == bi[q]) {
+                            // near field or excluded pair: reference arithmetic in FP64
+                            const int j = r.j;
+                            if (!act[q] || j < 0 || j == bi[q] || (m == 0 && j == ii[q])) continue;
+                            const double *qv = &rec64[(size_t)j * REC64];
+                            double out[3];
+                            if (bs_term64((qv[0] + s0) - xi[q][0], (qv[1] + s1) - xi[q][1],
+                                          (qv[2] + s2) - xi[q][2], qv[3], qv[4], qv[5], qv[6], kappa, out)) {
+                                nr[q][0] += out[0]; nr[q][1] += out[1]; nr[q][2] += out[2];
+                            }
+                            continue;
+                        }
+                        const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
+                        const float r2sq = fmaf(2.0f, de, r1sq + r.c);
+                        const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+                        const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+                        const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
+                        const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
+                        a[q][0] = fmaf(fmaf(d1, r.e2, -(d2 * r.e1)), fac, a[q][0]);
+                        a[q][1] = fmaf(fmaf(d2, r.e0, -(d0 * r.e2)), fac, a[q][1]);
+                        a[q][2] = fmaf(fmaf(d0, r.e1, -(d1 * r.e0)), fac, a[q][2]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < TPT; ++q) {
+                    kahan_add(f[q][0], k[q][0], a[q][0]);
+                    kahan_add(f[q][1], k[q][1], a[q][1]);
+                    kahan_add(f[q][2], k[q][2], a[q][2]);
+                }
             }
         }
-    }
-    const double k4 = kappa / (4.0 * NV_PI);
-    double *pp = part + (size_t)c * 6 * npad;
+        const double k4 = kappa / (4.0 * NV_PI);
+        double *pp = part + tg.poff[I.k] + (size_t)I.c * 6 * capk;
 #pragma unroll
-    for (int q = 0; q < TPT; ++q) {
-        const int i = ii[q];
-        if (i >= npad) continue;
+        for (int q = 0; q < TPT; ++q) {
+            if (loc[q] >= capk) continue;
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-            pp[cc * npad + i] = ((double)f[q][cc] - (double)k[q][cc]) * k4;
-            pp[(3 + cc) * npad + i] = nr[q][cc];
+            for (int cc = 0; cc < 3; ++cc) {
+                pp[cc * capk + loc[q]] = ((double)f[q][cc] - (double)k[q][cc]) * k4;
+                pp[(3 + cc) * capk + loc[q]] = nr[q][cc];
+            }
         }
     }
 }
 
 // Reduce chunk partials in chunk order and finish every point.
-// mode 0: part holds (acc0, accs) of k_bs64; mode 1: (far, near) of k_bs32c;
-// mode 2: no partials (LIA).
-__global__ void k_finish(DState s, Phys p, int n, int npad, const double *__restrict__ part, int S, int mode,
-                         double t, int do_ab, DCtl *ctl) {
+// mode 0: partials hold (acc0, accs) of k_bs64 or of the tree walk;
+// mode 1: (far, near) of k_bs32c; mode 2: no partials (LIA).
+__global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, double t,
+                         int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = r < n && s.infront[r] != 0;
-    const unsigned long long m = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&ctl->nvel, (int)__popcll(m));
-    if (r >= n) return;
+    const bool act = r < rows && s.infront[r] != 0;
+    const unsigned long long msk = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && msk) atomicAdd(&ctl->nvel, (int)__popcll(msk));
+    if (r >= rows) return;
     if (!act) {
         if (do_ab) {
 #pragma unroll
@@ -267,27 +293,32 @@ __global__ void k_finish(DState s, Phys p, int n, int npad, const double *__rest
         return;
     }
     double ub0[3] = {0.0, 0.0, 0.0}, ubs[3] = {0.0, 0.0, 0.0};
-    if (mode == 0) {
-        for (int c = 0; c < S; ++c) {
-            const double *pp = part + (size_t)c * 6 * npad;
+    if (mode != 2) {
+        const int kk = tg.row2t[r];
+        const int capk = tg.cap[kk], loc = r - tg.base[kk], S = tg.S[kk];
+        const double *pb = part + tg.poff[kk];
+        if (mode == 0) {
+            for (int c = 0; c < S; ++c) {
+                const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                ub0[k] = ub0[k] + pp[k * npad + r];
-                ubs[k] = ubs[k] + pp[(3 + k) * npad + r];
+                for (int k = 0; k < 3; ++k) {
+                    ub0[k] = ub0[k] + pp[k * capk + loc];
+                    ubs[k] = ubs[k] + pp[(3 + k) * capk + loc];
+                }
             }
-        }
-    } else if (mode == 1) {
-        double far[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
-        for (int c = 0; c < S; ++c) {
-            const double *pp = part + (size_t)c * 6 * npad;
+        } else {
+            double far[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
+            for (int c = 0; c < S; ++c) {
+                const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                far[k] += pp[k * npad + r];
-                nr[k] += pp[(3 + k) * npad + r];
+                for (int k = 0; k < 3; ++k) {
+                    far[k] += pp[k * capk + loc];
+                    nr[k] += pp[(3 + k) * capk + loc];
+                }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) ub0[k] = nr[k] + far[k];
+            for (int k = 0; k < 3; ++k) ub0[k] = nr[k] + far[k];
+        }
     }
     finish_point(s, p, r, ub0, ubs, t, do_ab);
 }
@@ -297,30 +328,50 @@ __global__ void k_finish(DState s, Phys p, int n, int npad, const double *__rest
 // ------------------------------------------------------------------ launch
 size_t nvk_rec32_bytes() { return sizeof(Rec32); }
 
-hipError_t nvk_src64(hipStream_t st, DState s, int n, int npad, double *rec) {
-    hipLaunchKernelGGL(k_src64, dim3((npad + 255) / 256), dim3(256), 0, st, s, n, npad, rec);
+// resident workgroups per CU, which sizes the persistent grids
+int nvk_occupancy_bs64() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs64, TB, 0) != hipSuccess || b < 1) b = 2;
+    return b;
+}
+
+int nvk_occupancy_bs32c(int tpt, int kb) {
+    int b = 0;
+    hipError_t e;
+    if (tpt == 2)
+        e = kb == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<2, 8>, TB, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<2, 1>, TB, 0);
+    else
+        e = kb == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<1, 8>, TB, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<1, 1>, TB, 0);
+    if (e != hipSuccess || b < 1) b = 2;
+    return b;
+}
+
+hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec) {
+    if (rows > 0) hipLaunchKernelGGL(k_src64, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, rec);
     return hipGetLastError();
 }
 
-hipError_t nvk_bs64(hipStream_t st, DState s, int n, int npad, const double *rec, const Shifts *sh, int ntiles, int S,
-                    double kappa, double *part) {
-    dim3 grid(npad / 256, S);
-    hipLaunchKernelGGL((k_bs64<256, 256>), grid, dim3(256), 0, st, s, n, npad, rec, *sh, ntiles, S, kappa, part);
+hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
+                    const double *rec, const Shifts *sh, double kappa, double *part) {
+    hipLaunchKernelGGL(k_bs64, dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, rec, *sh, kappa, part);
     return hipGetLastError();
 }
 
-hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, double *origin) {
-    hipLaunchKernelGGL(k_src32, dim3((npad + 255) / 256), dim3(256), 0, st, s, n, npad, 256, (Rec32 *)rec, origin);
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, double *origin) {
+    if (rows > 0)
+        hipLaunchKernelGGL(k_src32, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, (Rec32 *)rec, origin);
     return hipGetLastError();
 }
 
-hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
-                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part,
-                     int tpt, int kb) {
+hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
+                     const void *rec, const double *origin, const double *rec64, const Shifts *sh, double kappa,
+                     float rn2, double *part, int tpt, int kb) {
     const Rec32 *r = (const Rec32 *)rec;
 #define NV_BS32(TPT_, KB_)                                                                                   \
-    hipLaunchKernelGGL((k_bs32c<256, 256, TPT_, KB_>), dim3((npad + 256 * TPT_ - 1) / (256 * TPT_), S), dim3(256), \
-                       0, st, s, n, npad, r, origin, rec64, *sh, ntiles, S, kappa, rn2, part)
+    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, origin, \
+                       rec64, *sh, kappa, rn2, part)
     if (tpt == 2) {
         if (kb == 8) NV_BS32(2, 8);
         else if (kb == 4) NV_BS32(2, 4);
@@ -336,10 +387,10 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec,
     return hipGetLastError();
 }
 
-hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
+hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode, double t,
                       int do_ab, DCtl *ctl) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, st, s, p, n, npad, part, S, mode, t, do_ab,
+    if (rows > 0)
+        hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode, t, do_ab,
                            ctl);
     return hipGetLastError();
 }

@@ -1,7 +1,8 @@
 // Host side of the C-ABI declared in include/nuvort.h: the context, device
 // buffers, and one function per reference procedure (src/run.f90:229-270
-// call order).  All work is launched on the context's stream; every entry
-// point synchronises before it returns.
+// call order).  A context holds K >= 1 independent tangles (K = 1 for the
+// reference's single filament system); all work is launched on the
+// context's stream and every entry point synchronises before it returns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -19,22 +21,47 @@
 
 namespace {
 
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t m) {
+        if (m <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, sizeof(T) * (m > 0 ? m : 1));
+        if (e == hipSuccess) n = m;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 struct Ctx {
     nv_params prm{};
     Phys phys{};
     Shifts sh{};
     hipStream_t st = nullptr;
-    int cap = 0, n = 0;
+    int ncu = 256;
+    // tangle layout: host mirrors and device copies
+    int K = 0, rows = 0;             // rows = sum of tangle capacities = the state arrays' stride
+    std::vector<int> tbase, tcap, tn, trecon, tremove;
+    DevBuf<int> d_tbase, d_tcap, d_tn, d_trecon, d_tremove, d_tnact, d_row2t;
     DState d{};
-    DCtl *ctl = nullptr;      // device
-    DCtl *ctl_h = nullptr;    // pinned host mirror
-    int *cand = nullptr, *empty = nullptr, *lcand = nullptr, *lempty = nullptr, *tmp = nullptr;
-    double *rec64 = nullptr, *origin = nullptr, *part = nullptr;
-    void *rec32 = nullptr;
-    size_t part_elems = 0;
-    double *pd = nullptr;
-    int *pj = nullptr;
-    size_t pcl_elems = 0;
+    DCtl *ctl = nullptr;             // device
+    DCtl *ctl_h = nullptr;           // pinned host mirror
+    DevBuf<int> cand, empty, lcand, lempty, tmp;
+    DevBuf<double> rec64, origin, part, pd;
+    DevBuf<char> rec32;
+    DevBuf<int> pj;
+    // work lists of the last direct-sum and close-search launches
+    DevBuf<Item> items, pitems;
+    DevBuf<int> d_S, d_pS, queue;
+    DevBuf<long long> d_poff, d_ppoff;
     nv_recon_event *ev = nullptr;
     int ev_cap = 1 << 16;
     double t = 0.0;
@@ -44,9 +71,8 @@ struct Ctx {
     std::string err;
     TreeBuf *tree = nullptr;
     bool tree_valid = false;
-    int64_t last_pair_evals = 0;
-    int tpt = 1, kb = 1;         // FP32C kernel variant: targets per lane, compensation block
-    int n_active = -1;           // active points, refreshed by nv_pinsert / nv_set_state
+    int tpt = 2, kb = 8;             // FP32C kernel variant: targets per lane, compensation block
+    int occ64 = 4, occ32 = 4;        // resident workgroups per CU of the persistent kernels
 };
 
 int fail(Ctx *c, int code, const std::string &msg) {
@@ -61,84 +87,93 @@ int fail(Ctx *c, int code, const std::string &msg) {
             return fail((c), NV_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
-int npad_of(int n) { return ((n + 255) / 256) * 256; }
+int round256(long v) { return (int)(((v + 255) / 256) * 256); }
 
-int free_state(Ctx *c) {
+Tangles tangles(Ctx *c, const int *S, const long long *poff) {
+    Tangles g;
+    g.K = c->K;
+    g.base = c->d_tbase.p;
+    g.cap = c->d_tcap.p;
+    g.n = c->d_tn.p;
+    g.recon = c->d_trecon.p;
+    g.remove = c->d_tremove.p;
+    g.nact = c->d_tnact.p;
+    g.row2t = c->d_row2t.p;
+    g.S = S;
+    g.poff = poff;
+    return g;
+}
+
+void free_state(Ctx *c) {
     double **dp[] = {&c->d.x, &c->d.xn, &c->d.u, &c->d.u1, &c->d.u2, &c->d.usup, &c->d.umf, &c->d.gi, &c->d.gb,
                      &c->d.closestd};
     for (double **p : dp)
-        if (*p) { hipFree(*p); *p = nullptr; }
-    int **ip[] = {&c->d.infront, &c->d.behind, &c->d.closest, &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp};
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    int **ip[] = {&c->d.infront, &c->d.behind, &c->d.closest};
     for (int **p : ip)
-        if (*p) { hipFree(*p); *p = nullptr; }
-    c->cap = 0;
-    return NV_OK;
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
 
-// (re)allocate device rows for `want` points, keeping the first c->n rows
-int ensure_capacity(Ctx *c, int want) {
-    if (want <= c->cap) return NV_OK;
-    int ncap = std::max(want, c->cap + c->cap / 2);
-    ncap = npad_of(ncap);
-    DState o = c->d;
-    int ocap = c->cap;
-    DState d{};
-    d.cap = ncap;
-    double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb};
-    double *op[] = {o.x, o.xn, o.u, o.u1, o.u2, o.usup, o.umf, o.gi, o.gb};
-    for (int q = 0; q < 9; ++q) {
-        HIPCHK(c, hipMalloc(dp[q], sizeof(double) * 3 * (size_t)ncap));
-        HIPCHK(c, hipMemsetAsync(*dp[q], 0, sizeof(double) * 3 * (size_t)ncap, c->st));
-        if (op[q] && c->n > 0)
-            for (int k = 0; k < 3; ++k)
-                HIPCHK(c, hipMemcpyAsync(*dp[q] + (size_t)k * ncap, op[q] + (size_t)k * ocap,
-                                         sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->st));
-    }
-    HIPCHK(c, hipMalloc(&d.closestd, sizeof(double) * ncap));
-    HIPCHK(c, hipMemsetAsync(d.closestd, 0, sizeof(double) * ncap, c->st));
-    int **ip[] = {&d.infront, &d.behind, &d.closest};
-    int *oi[] = {o.infront, o.behind, o.closest};
-    for (int q = 0; q < 3; ++q) {
-        HIPCHK(c, hipMalloc(ip[q], sizeof(int) * (size_t)ncap));
-        HIPCHK(c, hipMemsetAsync(*ip[q], 0, sizeof(int) * (size_t)ncap, c->st));
-        if (oi[q] && c->n > 0)
-            HIPCHK(c, hipMemcpyAsync(*ip[q], oi[q], sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->st));
-    }
-    if (o.closestd && c->n > 0)
-        HIPCHK(c, hipMemcpyAsync(d.closestd, o.closestd, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    int nn = c->n;
+// allocate the state arrays for c->rows rows (contents zeroed)
+int alloc_state(Ctx *c) {
     free_state(c);
-    c->n = nn;
+    const int rows = c->rows;
+    DState d{};
+    d.cap = rows;
+    double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb};
+    for (double **p : dp) {
+        HIPCHK(c, hipMalloc(p, sizeof(double) * 3 * (size_t)rows));
+        HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(double) * 3 * (size_t)rows, c->st));
+    }
+    HIPCHK(c, hipMalloc(&d.closestd, sizeof(double) * rows));
+    HIPCHK(c, hipMemsetAsync(d.closestd, 0, sizeof(double) * rows, c->st));
+    int **ip[] = {&d.infront, &d.behind, &d.closest};
+    for (int **p : ip) {
+        HIPCHK(c, hipMalloc(p, sizeof(int) * (size_t)rows));
+        HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(int) * (size_t)rows, c->st));
+    }
     c->d = d;
-    c->cap = ncap;
-    HIPCHK(c, hipMalloc(&c->cand, sizeof(int) * ncap));
-    HIPCHK(c, hipMalloc(&c->empty, sizeof(int) * ncap));
-    HIPCHK(c, hipMalloc(&c->lcand, sizeof(int) * ncap));
-    HIPCHK(c, hipMalloc(&c->lempty, sizeof(int) * ncap));
-    HIPCHK(c, hipMalloc(&c->tmp, sizeof(int) * (size_t)nv_compact_tmp_ints(ncap)));
-    // velocity source records and chunk partials sized for ncap
-    if (c->rec64) hipFree(c->rec64);
-    if (c->rec32) hipFree(c->rec32);
-    if (c->origin) hipFree(c->origin);
-    HIPCHK(c, hipMalloc(&c->rec64, sizeof(double) * 8 * (size_t)ncap));
-    HIPCHK(c, hipMalloc(&c->rec32, nvk_rec32_bytes() * (size_t)ncap));
-    HIPCHK(c, hipMalloc(&c->origin, sizeof(double) * 3 * (size_t)(ncap / 256 + 1)));
+    HIPCHK(c, c->cand.reserve(rows));
+    HIPCHK(c, c->empty.reserve(rows));
+    HIPCHK(c, c->lcand.reserve(rows));
+    HIPCHK(c, c->lempty.reserve(rows));
+    HIPCHK(c, c->tmp.reserve((size_t)nv_compact_tmp_ints(rows)));
+    HIPCHK(c, c->rec64.reserve((size_t)8 * rows));
+    HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
+    HIPCHK(c, c->origin.reserve((size_t)3 * (rows / 256 + 1)));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
 }
 
-int ensure_part(Ctx *c, size_t elems) {
-    if (elems <= c->part_elems) return NV_OK;
-    if (c->part) hipFree(c->part);
-    HIPCHK(c, hipMalloc(&c->part, sizeof(double) * elems));
-    c->part_elems = elems;
+// upload the tangle tables (base, capacity, pcount, counters, row -> tangle)
+int upload_tables(Ctx *c) {
+    const int K = c->K;
+    HIPCHK(c, c->d_tbase.reserve(K));
+    HIPCHK(c, c->d_tcap.reserve(K));
+    HIPCHK(c, c->d_tn.reserve(K));
+    HIPCHK(c, c->d_trecon.reserve(K));
+    HIPCHK(c, c->d_tremove.reserve(K));
+    HIPCHK(c, c->d_tnact.reserve(K));
+    HIPCHK(c, c->d_row2t.reserve(c->rows));
+    HIPCHK(c, hipMemcpy(c->d_tbase.p, c->tbase.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tcap.p, c->tcap.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tn.p, c->tn.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_trecon.p, c->trecon.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tremove.p, c->tremove.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    std::vector<int> r2t(c->rows);
+    for (int k = 0; k < K; ++k) std::fill(r2t.begin() + c->tbase[k], r2t.begin() + c->tbase[k] + c->tcap[k], k);
+    HIPCHK(c, hipMemcpy(c->d_row2t.p, r2t.data(), sizeof(int) * c->rows, hipMemcpyHostToDevice));
     return NV_OK;
 }
 
 int read_ctl(Ctx *c) {
     HIPCHK(c, hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(DCtl), hipMemcpyDeviceToHost, c->st));
+    if (c->K > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->tn.data(), c->d_tn.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->trecon.data(), c->d_trecon.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->tremove.data(), c->d_tremove.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
-    c->n = c->ctl_h->n;
     int s = c->ctl_h->status;
     if (s & NVK_NAN) return fail(c, NV_ERR_NAN, "run.x: there is a NAN in the (filament) f%x array");
     if (s & NVK_ZERO_LENGTH) return fail(c, NV_ERR_ZERO_LENGTH, "line.mod:pinsert: vortex line length is 0, run over");
@@ -148,58 +183,289 @@ int read_ctl(Ctx *c) {
     return NV_OK;
 }
 
-// chunks per target block so that the grid holds ~4 workgroups per CU
-int pick_chunks(Ctx *c, int npad, long units) {
-    if (c->prm.exact_order) return 1;
-    int ntb = npad / 256;
-    long want = (4L * 256 + ntb - 1) / ntb;
-    if (want > units) want = units;
-    if (want < 1) want = 1;
-    return (int)want;
+// --------------------------------------------------------- state transfer
+// host arrays: the tangles concatenated, links local to each tangle (1-based)
+int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const std::vector<int> *caps) {
+    if (K < 1) return fail(c, NV_ERR_ARG, "at least one tangle");
+    long total = 0;
+    for (int k = 0; k < K; ++k) {
+        if (pcounts[k] < 0) return fail(c, NV_ERR_ARG, "negative pcount");
+        total += pcounts[k];
+    }
+    if (hs->capacity < total) return fail(c, NV_ERR_ARG, "state arrays smaller than the sum of pcounts");
+    if (K > 1 && c->prm.tree_theta > 0) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
+    // links and the absence of walls, tangle by tangle
+    long off = 0;
+    for (int k = 0; k < K; ++k) {
+        const int n = pcounts[k];
+        for (int q = 0; q < n; ++q) {
+            const long r = off + q;
+            if ((hs->pinnedi && hs->pinnedi[r]) || (hs->pinnedb && hs->pinnedb[r]) ||
+                (hs->wpinned && (hs->wpinned[3 * r] | hs->wpinned[3 * r + 1] | hs->wpinned[3 * r + 2])))
+                return fail(c, NV_ERR_UNSUPPORTED, "pinned points (solid walls) are not supported");
+            const int f = hs->infront[r], b = hs->behind[r];
+            if (f < 0 || f > n || (f != 0 && (b < 1 || b > n))) return fail(c, NV_ERR_ARG, "link out of range");
+        }
+        off += n;
+    }
+    // layout: 256-row aligned ranges with room for at least one step of insertions
+    c->K = K;
+    c->tbase.assign(K, 0);
+    c->tcap.assign(K, 0);
+    c->tn.assign(pcounts, pcounts + K);
+    c->trecon.assign(K, 0);
+    c->tremove.assign(K, 0);
+    long rows = 0;
+    for (int k = 0; k < K; ++k) {
+        const long want = caps ? (long)(*caps)[k]
+                               : std::max<long>(2L * pcounts[k] + (K == 1 ? 1024 : 256), (long)c->prm.capacity);
+        c->tbase[k] = (int)rows;
+        c->tcap[k] = round256(std::max<long>(want, 256));
+        rows += c->tcap[k];
+    }
+    if (rows > 0x7fffffffL / 8) return fail(c, NV_ERR_CAPACITY, "too many rows");
+    c->trecon[0] = hs->recon_count;
+    c->tremove[0] = hs->remove_count;
+    c->rows = (int)rows;
+    if (int rc = alloc_state(c)) return rc;
+    if (int rc = upload_tables(c)) return rc;
+    const int R = c->rows;
+    std::vector<double> tmp((size_t)3 * R);
+    const double *src[] = {hs->x, hs->u, hs->u1, hs->u2, hs->u_sup, hs->u_mf, hs->ghosti, hs->ghostb};
+    double *dst[] = {c->d.x, c->d.u, c->d.u1, c->d.u2, c->d.usup, c->d.umf, c->d.gi, c->d.gb};
+    for (int q = 0; q < 8; ++q) {
+        std::fill(tmp.begin(), tmp.end(), 0.0);
+        if (src[q]) {
+            long o = 0;
+            for (int k = 0; k < K; ++k) {
+                for (int r = 0; r < pcounts[k]; ++r)
+                    for (int j = 0; j < 3; ++j) tmp[(size_t)j * R + c->tbase[k] + r] = src[q][3 * (o + r) + j];
+                o += pcounts[k];
+            }
+        }
+        HIPCHK(c, hipMemcpy(dst[q], tmp.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice));
+    }
+    std::vector<int> zi(R);
+    const int32_t *isrc[] = {hs->infront, hs->behind, hs->closest};
+    int *idst[] = {c->d.infront, c->d.behind, c->d.closest};
+    for (int q = 0; q < 3; ++q) {
+        std::fill(zi.begin(), zi.end(), 0);
+        if (isrc[q]) {
+            long o = 0;
+            for (int k = 0; k < K; ++k) {
+                for (int r = 0; r < pcounts[k]; ++r) {
+                    const int v = isrc[q][o + r];
+                    zi[c->tbase[k] + r] = v > 0 ? v + c->tbase[k] : 0;   // global 1-based rows
+                }
+                o += pcounts[k];
+            }
+        }
+        HIPCHK(c, hipMemcpy(idst[q], zi.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    }
+    std::vector<double> cd(R, 0.0);
+    if (hs->closestd) {
+        long o = 0;
+        for (int k = 0; k < K; ++k) {
+            for (int r = 0; r < pcounts[k]; ++r) cd[c->tbase[k] + r] = hs->closestd[o + r];
+            o += pcounts[k];
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->d.closestd, cd.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    c->t = hs->t;
+    c->itime = hs->itime;
+    std::memset(c->ctl_h, 0, sizeof(DCtl));
+    HIPCHK(c, hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice));
+    return NV_OK;
+}
+
+int get_batch(Ctx *c, int32_t *pcounts, nv_state *hs) {
+    if (int rc = read_ctl(c)) (void)rc;
+    long total = 0;
+    for (int k = 0; k < c->K; ++k) total += c->tn[k];
+    if (pcounts)
+        for (int k = 0; k < c->K; ++k) pcounts[k] = c->tn[k];
+    if (hs->capacity < total) {
+        hs->pcount = (int32_t)total;
+        return fail(c, NV_ERR_CAPACITY, "nv_get_state: caller arrays hold fewer rows than pcount");
+    }
+    hs->pcount = (int32_t)total;
+    hs->t = c->t;
+    hs->itime = c->itime;
+    hs->recon_count = std::accumulate(c->trecon.begin(), c->trecon.end(), 0);
+    hs->remove_count = std::accumulate(c->tremove.begin(), c->tremove.end(), 0);
+    if (total == 0) return NV_OK;
+    const int R = c->rows;
+    std::vector<double> tmp((size_t)3 * R);
+    double *dst[] = {hs->x, hs->u, hs->u1, hs->u2, hs->u_sup, hs->u_mf, hs->ghosti, hs->ghostb};
+    const double *src[] = {c->d.x, c->d.u, c->d.u1, c->d.u2, c->d.usup, c->d.umf, c->d.gi, c->d.gb};
+    for (int q = 0; q < 8; ++q) {
+        if (!dst[q]) continue;
+        HIPCHK(c, hipMemcpy(tmp.data(), src[q], sizeof(double) * 3 * R, hipMemcpyDeviceToHost));
+        long o = 0;
+        for (int k = 0; k < c->K; ++k) {
+            for (int r = 0; r < c->tn[k]; ++r)
+                for (int j = 0; j < 3; ++j) dst[q][3 * (o + r) + j] = tmp[(size_t)j * R + c->tbase[k] + r];
+            o += c->tn[k];
+        }
+    }
+    std::vector<int> iv(R);
+    int32_t *idst[] = {hs->infront, hs->behind, hs->closest};
+    const int *isrc[] = {c->d.infront, c->d.behind, c->d.closest};
+    for (int q = 0; q < 3; ++q) {
+        if (!idst[q]) continue;
+        HIPCHK(c, hipMemcpy(iv.data(), isrc[q], sizeof(int) * R, hipMemcpyDeviceToHost));
+        long o = 0;
+        for (int k = 0; k < c->K; ++k) {
+            for (int r = 0; r < c->tn[k]; ++r) {
+                const int v = iv[c->tbase[k] + r];
+                idst[q][o + r] = v > 0 ? v - c->tbase[k] : 0;
+            }
+            o += c->tn[k];
+        }
+    }
+    if (hs->closestd) {
+        std::vector<double> cd(R);
+        HIPCHK(c, hipMemcpy(cd.data(), c->d.closestd, sizeof(double) * R, hipMemcpyDeviceToHost));
+        long o = 0;
+        for (int k = 0; k < c->K; ++k) {
+            for (int r = 0; r < c->tn[k]; ++r) hs->closestd[o + r] = cd[c->tbase[k] + r];
+            o += c->tn[k];
+        }
+    }
+    if (hs->pinnedi) std::memset(hs->pinnedi, 0, sizeof(int32_t) * total);
+    if (hs->pinnedb) std::memset(hs->pinnedb, 0, sizeof(int32_t) * total);
+    if (hs->wpinned) std::memset(hs->wpinned, 0, sizeof(int32_t) * 3 * total);
+    return NV_OK;
+}
+
+// Give every tangle room for another step of insertions (pinsert inserts at
+// most once per visited point, so 2 x pcount rows always suffice): move the
+// state to a wider layout through the host when one is short.
+int ensure_room(Ctx *c) {
+    bool need = false;
+    for (int k = 0; k < c->K; ++k)
+        if (c->tcap[k] < 2 * c->tn[k] + 256) need = true;
+    if (!need) return NV_OK;
+    long total = 0;
+    for (int k = 0; k < c->K; ++k) total += c->tn[k];
+    std::vector<double> v((size_t)8 * 3 * total + 24), cd(total + 1);
+    std::vector<int32_t> iv((size_t)3 * total + 3);
+    nv_state hs{};
+    hs.capacity = (int32_t)total;
+    double **dp[] = {&hs.x, &hs.u, &hs.u1, &hs.u2, &hs.u_sup, &hs.u_mf, &hs.ghosti, &hs.ghostb};
+    for (int q = 0; q < 8; ++q) *dp[q] = v.data() + (size_t)q * 3 * total;
+    hs.infront = iv.data();
+    hs.behind = iv.data() + total;
+    hs.closest = iv.data() + 2 * total;
+    hs.closestd = cd.data();
+    std::vector<int32_t> pc(c->K);
+    if (int rc = get_batch(c, pc.data(), &hs)) return rc;
+    std::vector<int> caps(c->K);
+    for (int k = 0; k < c->K; ++k) caps[k] = round256(3L * c->tn[k] + 1024);
+    const std::vector<int> rec = c->trecon, rem = c->tremove;
+    const double t = c->t;
+    const int it = c->itime;
+    if (int rc = set_batch(c, c->K, pc.data(), &hs, &caps)) return rc;
+    c->trecon = rec;
+    c->tremove = rem;
+    c->t = t;
+    c->itime = it;
+    return upload_tables(c);
+}
+
+// ------------------------------------------------------------ work lists
+// Direct-sum items, largest tangle first.  The chunk count of each tangle
+// keeps items of similar size and gives the launch >= 16 items per CU.
+int build_items(Ctx *c, int tpt, bool exact, int *nitems_out) {
+    const int K = c->K;
+    std::vector<int> order(K);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c->tn[a] > c->tn[b]; });
+    long work = 0;
+    std::vector<int> ntb(K), U(K);
+    for (int k = 0; k < K; ++k) {
+        ntb[k] = (c->tn[k] + 256 * tpt - 1) / (256 * tpt);
+        U[k] = c->sh.n * ((c->tn[k] + 255) / 256);
+        work += (long)ntb[k] * U[k];
+    }
+    const long want = 16L * c->ncu;
+    const long utarget = std::max<long>(4, std::min<long>(256, work / std::max<long>(1, want)));
+    std::vector<int> S(K);
+    std::vector<long long> poff(K);
+    long long po = 0;
+    for (int k = 0; k < K; ++k) {
+        S[k] = exact ? 1 : (int)std::max<long>(1, std::min<long>(U[k], (U[k] + utarget - 1) / utarget));
+        poff[k] = po;
+        po += (long long)S[k] * 6 * c->tcap[k];
+    }
+    std::vector<Item> it;
+    for (int k : order)
+        for (int tb = 0; tb < ntb[k]; ++tb)
+            for (int ch = 0; ch < S[k]; ++ch) it.push_back(Item{k, tb, ch, 0});
+    HIPCHK(c, c->items.reserve(std::max<size_t>(1, it.size())));
+    HIPCHK(c, c->d_S.reserve(K));
+    HIPCHK(c, c->d_poff.reserve(K));
+    HIPCHK(c, c->part.reserve((size_t)std::max<long long>(po, 1)));
+    if (!it.empty())
+        HIPCHK(c, hipMemcpyAsync(c->items.p, it.data(), sizeof(Item) * it.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_S.p, S.data(), sizeof(int) * K, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_poff.p, poff.data(), sizeof(long long) * K, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *nitems_out = (int)it.size();
+    return NV_OK;
 }
 
 void swap_x(Ctx *c) { std::swap(c->d.x, c->d.xn); }
 
 int velocity_impl(Ctx *c, int do_ab) {
-    const int n = c->n;
-    if (n == 0) return NV_OK;
-    const int npad = npad_of(n);
-    const int ntiles = npad / 256;
+    if (c->K == 0) return NV_OK;
+    const int R = c->rows;
     HIPCHK(c, hipEventRecord(c->e0, c->st));
-    int mode = 0, S = 1;
+    int mode = 0;
     bool timed_pair = false, timed_walk = false;
     if (c->prm.velocity == NV_VEL_LIA) {
         mode = 2;
+        HIPCHK(c, c->d_S.reserve(c->K));
+        HIPCHK(c, c->d_poff.reserve(c->K));
     } else if (c->prm.velocity == NV_VEL_TREE) {
         if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pmotion: tree velocity without construct_tree");
-        if (int rc = ensure_part(c, (size_t)6 * npad)) return rc;
+        // one chunk: partial block [6][rows] at offset 0
+        const int one = 1;
+        const long long zero = 0;
+        HIPCHK(c, c->d_S.reserve(1));
+        HIPCHK(c, c->d_poff.reserve(1));
+        HIPCHK(c, c->part.reserve((size_t)6 * R));
+        HIPCHK(c, hipMemcpyAsync(c->d_S.p, &one, sizeof(int), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->d_poff.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipEventRecord(c->e2, c->st));
-        int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, n, npad, c->prm.precision, c->part,
-                                       c->ctl, &c->stats);
+        int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision,
+                                       c->part.p, c->ctl, &c->stats);
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
         HIPCHK(c, hipEventRecord(c->e3, c->st));
         timed_walk = true;
         mode = 0;
-        S = 1;
     } else {
-        long units = (long)c->sh.n * ntiles;
-        // FP32C: each workgroup covers tpt x 256 targets
-        const int npadg = (c->prm.precision == NV_PREC_FP32C) ? ((npad + 256 * c->tpt - 1) / (256 * c->tpt)) * 256
-                                                              : npad;
-        S = pick_chunks(c, npadg, units);
-        if (int rc = ensure_part(c, (size_t)S * 6 * npad)) return rc;
-        HIPCHK(c, nvk_src64(c->st, c->d, n, npad, c->rec64));
-        if (c->prm.precision == NV_PREC_FP32C) {
-            HIPCHK(c, nvk_src32(c->st, c->d, n, npad, c->rec32, c->origin));
-            double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+        const bool fp32 = c->prm.precision == NV_PREC_FP32C;
+        int nitems = 0;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, &nitems)) return rc;
+        HIPCHK(c, c->queue.reserve(1));
+        HIPCHK(c, nv_fill_int(c->st, c->queue.p, 0, 1));
+        HIPCHK(c, nvk_src64(c->st, c->d, R, c->rec64.p));
+        Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
+        if (fp32) {
+            HIPCHK(c, nvk_src32(c->st, c->d, R, c->rec32.p, c->origin.p));
+            const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+            const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
             HIPCHK(c, hipEventRecord(c->e2, c->st));
-            HIPCHK(c, nvk_bs32c(c->st, c->d, n, npad, c->rec32, c->origin, c->rec64, &c->sh, ntiles, S,
-                                c->phys.kappa, (float)(rn * rn), c->part, c->tpt, c->kb));
+            HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->origin.p,
+                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 1;
         } else {
+            const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));
             HIPCHK(c, hipEventRecord(c->e2, c->st));
-            HIPCHK(c, nvk_bs64(c->st, c->d, n, npad, c->rec64, &c->sh, ntiles, S, c->phys.kappa, c->part));
+            HIPCHK(c, nvk_bs64(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec64.p, &c->sh,
+                               c->phys.kappa, c->part.p));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 0;
         }
@@ -207,29 +473,34 @@ int velocity_impl(Ctx *c, int do_ab) {
     }
     c->ctl_h->nvel = 0;
     HIPCHK(c, hipMemcpyAsync(&c->ctl->nvel, &c->ctl_h->nvel, sizeof(int), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, n, npad, c->part, S, mode, c->t, do_ab, c->ctl));
+    Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
+    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, tg, R, c->part.p, mode, c->t, do_ab, c->ctl));
     HIPCHK(c, hipMemcpyAsync(&c->ctl_h->nvel, &c->ctl->nvel, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->e0, c->e1);
+    (void)hipEventElapsedTime(&ms, c->e0, c->e1);
     c->stats.ms_velocity = ms;
     const int64_t nv = c->ctl_h->nvel;
     if (do_ab) c->stats.point_updates += nv;
-    // evaluated pairs: every active target against every active source in
-    // every image, less j == i (unshifted) and j == behind(i) (all images)
-    c->last_pair_evals = nv * ((int64_t)c->sh.n * nv - c->sh.n - 1);
     if (timed_pair) {
+        // evaluated pairs: every active target against every active source of
+        // its tangle in every image, less j == i (unshifted) and j == behind(i)
+        // (all images); batch tangle sizes use pcount (empty slots are rare)
+        int64_t pe = 0;
+        if (c->K == 1) pe = nv * ((int64_t)c->sh.n * nv - c->sh.n - 1);
+        else
+            for (int k = 0; k < c->K; ++k) pe += (int64_t)c->tn[k] * ((int64_t)c->sh.n * c->tn[k] - c->sh.n - 1);
         float pm = 0.f;
-        hipEventElapsedTime(&pm, c->e2, c->e3);
+        (void)hipEventElapsedTime(&pm, c->e2, c->e3);
         c->stats.pair_ms_total += pm;
         c->stats.pair_launches += 1;
-        c->stats.pair_evals_total += c->last_pair_evals;
+        c->stats.pair_evals_total += pe;
     }
     if (timed_walk) {
         if (int rc = nv_tree_walk_counters(c->tree, c->st, &c->stats)) return fail(c, rc, "tree counters");
         float wm = 0.f;
-        hipEventElapsedTime(&wm, c->e2, c->e3);
+        (void)hipEventElapsedTime(&wm, c->e2, c->e3);
         c->stats.walk_ms_total += wm;
         c->stats.walk_launches += 1;
         c->stats.walk_nodes_total += c->stats.tree_nodes_visited;
@@ -270,6 +541,11 @@ int set_phys(Ctx *c) {
     return NV_OK;
 }
 
+int compact(Ctx *c, DevBuf<int> &flags, DevBuf<int> &list, int *count) {
+    HIPCHK(c, nv_compact_flags(c->st, flags.p, c->rows, list.p, count, c->tmp.p));
+    return NV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -304,18 +580,18 @@ int nv_create(const nv_params *params, void **out) {
         if (!(p.dt < dt_max)) rc = fail(c, NV_ERR_TIMESTEP, "initial.mod:timestep_check: dt is too large");
     }
     if (rc) {
-        // keep the message reachable through a context-less call
-        static thread_local std::string last;
-        last = c->err;
         delete c;
         return rc;
     }
     set_phys(c);
-    c->tpt = p.targets_per_lane == 2 ? 2 : (p.targets_per_lane == 1 ? 1 : 2);
+    c->tpt = p.targets_per_lane == 1 ? 1 : 2;
     // default: compensated add every 8 terms (measured: same error as every
     // term on the 16k-point ring tangle, 1.27x faster)
     c->kb = (p.kahan_block == 1 || p.kahan_block == 2 || p.kahan_block == 4 || p.kahan_block == 8) ? p.kahan_block : 8;
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->ctl, sizeof(DCtl)) != hipSuccess ||
         hipHostMalloc((void **)&c->ctl_h, sizeof(DCtl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&c->ev, sizeof(nv_recon_event) * c->ev_cap) != hipSuccess || hipEventCreate(&c->e0) != hipSuccess ||
@@ -324,8 +600,11 @@ int nv_create(const nv_params *params, void **out) {
         delete c;
         return NV_ERR_HIP;
     }
+    c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    c->occ64 = nvk_occupancy_bs64();
+    c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb);
     std::memset(c->ctl_h, 0, sizeof(DCtl));
-    hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice);
     c->tree = nv_tree_new();
     *out = c;
     return NV_OK;
@@ -334,20 +613,20 @@ int nv_create(const nv_params *params, void **out) {
 int nv_destroy(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    hipStreamSynchronize(c->st);
+    (void)hipStreamSynchronize(c->st);
     free_state(c);
-    if (c->rec64) hipFree(c->rec64);
-    if (c->rec32) hipFree(c->rec32);
-    if (c->origin) hipFree(c->origin);
-    if (c->part) hipFree(c->part);
-    if (c->pd) hipFree(c->pd);
-    if (c->pj) hipFree(c->pj);
-    if (c->ev) hipFree(c->ev);
-    if (c->ctl) hipFree(c->ctl);
-    if (c->ctl_h) hipHostFree(c->ctl_h);
+    DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tn, &c->d_trecon, &c->d_tremove, &c->d_tnact, &c->d_row2t,
+                         &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp, &c->pj, &c->d_S, &c->d_pS, &c->queue};
+    for (auto *b : ib) b->release();
+    c->rec64.release(); c->origin.release(); c->part.release(); c->pd.release(); c->rec32.release();
+    c->items.release(); c->pitems.release(); c->d_poff.release(); c->d_ppoff.release();
+    if (c->ev) (void)hipFree(c->ev);
+    if (c->ctl) (void)hipFree(c->ctl);
+    if (c->ctl_h) (void)hipHostFree(c->ctl_h);
     if (c->tree) nv_tree_free(c->tree);
-    hipEventDestroy(c->e0); hipEventDestroy(c->e1); hipEventDestroy(c->e2); hipEventDestroy(c->e3);
-    hipStreamDestroy(c->st);
+    (void)hipEventDestroy(c->e0); (void)hipEventDestroy(c->e1);
+    (void)hipEventDestroy(c->e2); (void)hipEventDestroy(c->e3);
+    (void)hipStreamDestroy(c->st);
     delete c;
     return NV_OK;
 }
@@ -360,85 +639,35 @@ const char *nv_last_error(void *h) {
 int nv_set_state(void *h, const nv_state *hs) {
     Ctx *c = (Ctx *)h;
     if (!c || !hs || hs->pcount < 0 || hs->capacity < hs->pcount) return NV_ERR_ARG;
-    const int n = hs->pcount;
-    for (int r = 0; r < n; ++r) {
-        if ((hs->pinnedi && hs->pinnedi[r]) || (hs->pinnedb && hs->pinnedb[r]) ||
-            (hs->wpinned && (hs->wpinned[3 * r] | hs->wpinned[3 * r + 1] | hs->wpinned[3 * r + 2])))
-            return fail(c, NV_ERR_UNSUPPORTED, "pinned points (solid walls) are not supported");
-        int f = hs->infront[r], b = hs->behind[r];
-        if (f < 0 || f > n || (f != 0 && (b < 1 || b > n))) return fail(c, NV_ERR_ARG, "link out of range");
-    }
-    c->n = 0;
-    int want = std::max(c->prm.capacity, 2 * n + 1024);
-    if (int rc = ensure_capacity(c, want)) return rc;
-    c->n = n;
     // an existing tree stays valid: like the reference's vtree it holds its
     // own copies of the build-time data, so a close search after a state
     // upload sees the stale tree exactly as pclose_tree does
-    std::vector<double> tmp((size_t)3 * c->cap, 0.0);
-    const double *src[] = {hs->x, hs->u, hs->u1, hs->u2, hs->u_sup, hs->u_mf, hs->ghosti, hs->ghostb};
-    double *dst[] = {c->d.x, c->d.u, c->d.u1, c->d.u2, c->d.usup, c->d.umf, c->d.gi, c->d.gb};
-    for (int q = 0; q < 8; ++q) {
-        std::fill(tmp.begin(), tmp.end(), 0.0);
-        if (src[q])
-            for (int r = 0; r < n; ++r)
-                for (int k = 0; k < 3; ++k) tmp[(size_t)k * c->cap + r] = src[q][3 * r + k];
-        HIPCHK(c, hipMemcpy(dst[q], tmp.data(), sizeof(double) * 3 * c->cap, hipMemcpyHostToDevice));
-    }
-    std::vector<int> zi(c->cap, 0);
-    const int32_t *isrc[] = {hs->infront, hs->behind, hs->closest};
-    int *idst[] = {c->d.infront, c->d.behind, c->d.closest};
-    for (int q = 0; q < 3; ++q) {
-        std::fill(zi.begin(), zi.end(), 0);
-        if (isrc[q]) std::copy(isrc[q], isrc[q] + n, zi.begin());
-        HIPCHK(c, hipMemcpy(idst[q], zi.data(), sizeof(int) * c->cap, hipMemcpyHostToDevice));
-    }
-    std::vector<double> cd(c->cap, 0.0);
-    if (hs->closestd) std::copy(hs->closestd, hs->closestd + n, cd.begin());
-    HIPCHK(c, hipMemcpy(c->d.closestd, cd.data(), sizeof(double) * c->cap, hipMemcpyHostToDevice));
-    c->t = hs->t;
-    c->itime = hs->itime;
-    std::memset(c->ctl_h, 0, sizeof(DCtl));
-    c->ctl_h->n = n;
-    c->ctl_h->recon_count = hs->recon_count;
-    c->ctl_h->remove_count = hs->remove_count;
-    HIPCHK(c, hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice));
-    return NV_OK;
+    const int32_t n = hs->pcount;
+    return set_batch(c, 1, &n, hs, nullptr);
 }
 
 int nv_get_state(void *h, nv_state *hs) {
     Ctx *c = (Ctx *)h;
     if (!c || !hs) return NV_ERR_ARG;
-    if (int rc = read_ctl(c)) { (void)rc; }
-    const int n = c->n;
-    if (hs->capacity < n) {
-        hs->pcount = n;
-        return fail(c, NV_ERR_CAPACITY, "nv_get_state: caller arrays hold fewer rows than pcount");
+    if (c->K > 1) return fail(c, NV_ERR_STATE, "a batch of tangles: use nv_get_state_batch");
+    if (c->K == 0) {
+        hs->pcount = 0;
+        return NV_OK;
     }
-    hs->pcount = n;
-    hs->t = c->t;
-    hs->itime = c->itime;
-    hs->recon_count = c->ctl_h->recon_count;
-    hs->remove_count = c->ctl_h->remove_count;
-    if (n == 0) return NV_OK;
-    std::vector<double> tmp((size_t)3 * c->cap);
-    double *dst[] = {hs->x, hs->u, hs->u1, hs->u2, hs->u_sup, hs->u_mf, hs->ghosti, hs->ghostb};
-    const double *src[] = {c->d.x, c->d.u, c->d.u1, c->d.u2, c->d.usup, c->d.umf, c->d.gi, c->d.gb};
-    for (int q = 0; q < 8; ++q) {
-        if (!dst[q]) continue;
-        HIPCHK(c, hipMemcpy(tmp.data(), src[q], sizeof(double) * 3 * c->cap, hipMemcpyDeviceToHost));
-        for (int r = 0; r < n; ++r)
-            for (int k = 0; k < 3; ++k) dst[q][3 * r + k] = tmp[(size_t)k * c->cap + r];
-    }
-    int32_t *idst[] = {hs->infront, hs->behind, hs->closest};
-    const int *isrc[] = {c->d.infront, c->d.behind, c->d.closest};
-    for (int q = 0; q < 3; ++q)
-        if (idst[q]) HIPCHK(c, hipMemcpy(idst[q], isrc[q], sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (hs->closestd) HIPCHK(c, hipMemcpy(hs->closestd, c->d.closestd, sizeof(double) * n, hipMemcpyDeviceToHost));
-    if (hs->pinnedi) std::memset(hs->pinnedi, 0, sizeof(int32_t) * n);
-    if (hs->pinnedb) std::memset(hs->pinnedb, 0, sizeof(int32_t) * n);
-    if (hs->wpinned) std::memset(hs->wpinned, 0, sizeof(int32_t) * 3 * n);
-    return NV_OK;
+    return get_batch(c, nullptr, hs);
+}
+
+int nv_set_state_batch(void *h, int32_t ntangles, const int32_t *pcounts, const nv_state *hs) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !hs || !pcounts || ntangles < 1) return NV_ERR_ARG;
+    return set_batch(c, ntangles, pcounts, hs, nullptr);
+}
+
+int nv_get_state_batch(void *h, int32_t ntangles, int32_t *pcounts, nv_state *hs) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !hs || !pcounts) return NV_ERR_ARG;
+    if (ntangles != c->K) return fail(c, NV_ERR_ARG, "ntangles differs from the uploaded batch");
+    return get_batch(c, pcounts, hs);
 }
 
 int nv_set_time(void *h, double t, int32_t itime) {
@@ -452,7 +681,7 @@ int nv_set_time(void *h, double t, int32_t itime) {
 int nv_ghostp(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    HIPCHK(c, nvk_ghosts(c->st, c->d, c->phys, c->n));
+    HIPCHK(c, nvk_ghosts(c->st, c->d, c->phys, c->rows));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
 }
@@ -460,14 +689,15 @@ int nv_ghostp(void *h) {
 int nv_construct_tree(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    int rc = nv_tree_build(c->tree, c->st, c->d, c->phys, c->n, c->ctl, &c->stats);
+    int rc = nv_tree_build(c->tree, c->st, c->d, c->phys, c->tn[0], c->ctl, &c->stats);
     if (rc == NV_ERR_TREE_DEPTH) return fail(c, rc, "octree deeper than NV_TREE_MAX_LEVELS (coincident points?)");
     if (rc) return fail(c, rc, "tree build failed");
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->e2, c->e3);
+    (void)hipEventElapsedTime(&ms, c->e2, c->e3);
     c->stats.ms_tree_build = ms;
     c->tree_valid = true;
     return NV_OK;
@@ -488,40 +718,59 @@ int nv_pmotion(void *h) {
 int nv_pinsert(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    // every visited row inserts at most once: 2 x pcount rows always suffice
-    if (int rc = ensure_capacity(c, 2 * c->n + 256)) return rc;
-    const int n = c->n;
+    if (c->K == 0) return NV_OK;
+    if (int rc = ensure_room(c)) return rc;
+    Tangles tg = tangles(c, nullptr, nullptr);
     c->ctl_h->nactive = 0;
     HIPCHK(c, hipMemcpyAsync(&c->ctl->nactive, &c->ctl_h->nactive, sizeof(int), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, nvk_pinsert_pre(c->st, c->d, c->phys, n, c->cand, c->empty, c->ctl));
-    HIPCHK(c, nv_compact_flags(c->st, c->cand, n, c->lcand, &c->ctl->ncand, c->tmp));
-    HIPCHK(c, nv_compact_flags(c->st, c->empty, n, c->lempty, &c->ctl->nempty, c->tmp));
-    HIPCHK(c, nvk_pinsert_serial(c->st, c->d, c->phys, n, c->lcand, c->lempty, c->ctl));
+    HIPCHK(c, nv_fill_int(c->st, c->d_tnact.p, 0, c->K));
+    HIPCHK(c, nvk_pinsert_pre(c->st, c->d, c->phys, tg, c->rows, c->cand.p, c->empty.p, c->ctl));
+    if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
+    if (int rc = compact(c, c->empty, c->lempty, &c->ctl->nempty)) return rc;
+    HIPCHK(c, nvk_pinsert_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
     return read_ctl(c);
 }
 
 int nv_pclose(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    const int n = c->n;
-    if (n == 0) return NV_OK;
-    const int npad = npad_of(n);
-    int ntiles = npad / 256;
-    int S = pick_chunks(c, npad, ntiles);
-    size_t need = (size_t)S * npad;
-    if (need > c->pcl_elems) {
-        if (c->pd) hipFree(c->pd);
-        if (c->pj) hipFree(c->pj);
-        HIPCHK(c, hipMalloc(&c->pd, sizeof(double) * need));
-        HIPCHK(c, hipMalloc(&c->pj, sizeof(int) * need));
-        c->pcl_elems = need;
+    const int K = c->K;
+    if (K == 0) return NV_OK;
+    // items (tangle, target block, chunk of source tiles), ~16 per CU
+    long work = 0;
+    std::vector<int> ntb(K), S(K);
+    for (int k = 0; k < K; ++k) {
+        ntb[k] = (c->tn[k] + 255) / 256;
+        work += (long)ntb[k] * ntb[k];
     }
+    const long want = 16L * c->ncu;
+    const long per = std::max<long>(4, work / std::max<long>(1, want));
+    std::vector<long long> poff(K);
+    long long po = 0;
+    std::vector<Item> it;
+    for (int k = 0; k < K; ++k) {
+        S[k] = (int)std::max<long>(1, std::min<long>(ntb[k], (ntb[k] + per - 1) / per));
+        poff[k] = po;
+        po += (long long)S[k] * c->tcap[k];
+        for (int tb = 0; tb < ntb[k]; ++tb)
+            for (int ch = 0; ch < S[k]; ++ch) it.push_back(Item{k, tb, ch, 0});
+    }
+    HIPCHK(c, c->pitems.reserve(std::max<size_t>(1, it.size())));
+    HIPCHK(c, c->d_pS.reserve(K));
+    HIPCHK(c, c->d_ppoff.reserve(K));
+    HIPCHK(c, c->pd.reserve((size_t)std::max<long long>(po, 1)));
+    HIPCHK(c, c->pj.reserve((size_t)std::max<long long>(po, 1)));
+    if (!it.empty())
+        HIPCHK(c, hipMemcpyAsync(c->pitems.p, it.data(), sizeof(Item) * it.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_pS.p, S.data(), sizeof(int) * K, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_ppoff.p, poff.data(), sizeof(long long) * K, hipMemcpyHostToDevice, c->st));
+    Tangles tg = tangles(c, c->d_pS.p, c->d_ppoff.p);
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    HIPCHK(c, nvk_pclose(c->st, c->d, n, npad, S, c->pd, c->pj));
+    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, (int)it.size(), c->pd.p, c->pj.p));
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->e2, c->e3);
+    (void)hipEventElapsedTime(&ms, c->e2, c->e3);
     c->stats.ms_close = ms;
     return NV_OK;
 }
@@ -530,13 +779,14 @@ int nv_pclose_tree(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pclose_tree without construct_tree");
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    int rc = nv_tree_walk_close(c->tree, c->st, c->d, c->n, &c->stats);
+    int rc = nv_tree_walk_close(c->tree, c->st, c->d, c->tn[0], &c->stats);
     if (rc) return fail(c, NV_ERR_HIP, "tree close search launch failed");
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->e2, c->e3);
+    (void)hipEventElapsedTime(&ms, c->e2, c->e3);
     c->stats.ms_close = ms;
     return NV_OK;
 }
@@ -544,27 +794,29 @@ int nv_pclose_tree(void *h) {
 int nv_precon(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    const int n = c->n;
-    HIPCHK(c, nvk_precon_pre(c->st, c->d, c->phys, n, c->cand));
-    HIPCHK(c, nv_compact_flags(c->st, c->cand, n, c->lcand, &c->ctl->ncand, c->tmp));
-    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, c->lcand, c->ctl, c->ev, c->ev_cap, c->t, c->itime));
+    if (c->K == 0) return NV_OK;
+    Tangles tg = tangles(c, nullptr, nullptr);
+    HIPCHK(c, nvk_precon_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
+    if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
+    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap, c->t, c->itime));
     return read_ctl(c);
 }
 
 int nv_premove(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    const int n = c->n;
-    HIPCHK(c, nvk_premove_pre(c->st, c->d, c->phys, n, c->cand));
-    HIPCHK(c, nv_compact_flags(c->st, c->cand, n, c->lcand, &c->ctl->ncand, c->tmp));
-    HIPCHK(c, nvk_premove_serial(c->st, c->d, c->phys, c->lcand, c->ctl));
+    if (c->K == 0) return NV_OK;
+    Tangles tg = tangles(c, nullptr, nullptr);
+    HIPCHK(c, nvk_premove_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
+    if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
+    HIPCHK(c, nvk_premove_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl));
     return read_ctl(c);
 }
 
 int nv_enforce_boundary(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    HIPCHK(c, nvk_wrap(c->st, c->d, c->phys, c->n));
+    HIPCHK(c, nvk_wrap(c->st, c->d, c->phys, c->rows));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
 }
@@ -579,18 +831,17 @@ int nv_empty_tree(void *h) {
 int nv_nan_check(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    HIPCHK(c, nvk_nan(c->st, c->d, c->n, c->ctl));
+    HIPCHK(c, nvk_nan(c->st, c->d, c->rows, c->ctl));
     return read_ctl(c);
 }
 
 int nv_step(void *h, int32_t nsteps) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    HIPCHK(c, hipEventRecord(c->e0, c->st));
     hipEvent_t s0, s1;
-    hipEventCreate(&s0);
-    hipEventCreate(&s1);
-    hipEventRecord(s0, c->st);
+    HIPCHK(c, hipEventCreate(&s0));
+    HIPCHK(c, hipEventCreate(&s1));
+    HIPCHK(c, hipEventRecord(s0, c->st));
     const bool use_tree = c->prm.tree_theta > 0;
     int rc = NV_OK;
     for (int k = 0; k < nsteps && rc == NV_OK; ++k) {
@@ -600,8 +851,8 @@ int nv_step(void *h, int32_t nsteps) {
         if ((rc = nv_pmotion(c))) break;
         if ((rc = nv_pinsert(c))) break;
         if (c->itime % c->prm.recon_shots == 0) {
-            bool tree_search = c->prm.close_search == NV_CLOSE_TREE ||
-                               (c->prm.close_search == NV_CLOSE_AUTO && use_tree);
+            const bool tree_search = c->prm.close_search == NV_CLOSE_TREE ||
+                                     (c->prm.close_search == NV_CLOSE_AUTO && use_tree);
             if (tree_search) rc = nv_pclose_tree(c);
             else rc = nv_pclose(c);
             if (rc) break;
@@ -613,13 +864,13 @@ int nv_step(void *h, int32_t nsteps) {
         if (use_tree) nv_empty_tree(c);
         if ((rc = nv_nan_check(c))) break;
     }
-    hipEventRecord(s1, c->st);
-    hipEventSynchronize(s1);
+    (void)hipEventRecord(s1, c->st);
+    (void)hipEventSynchronize(s1);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, s0, s1);
+    (void)hipEventElapsedTime(&ms, s0, s1);
     c->stats.ms_step = nsteps > 0 ? ms / nsteps : 0.0;
-    hipEventDestroy(s0);
-    hipEventDestroy(s1);
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
     return rc;
 }
 
@@ -627,9 +878,18 @@ int nv_get_recon_events(void *h, nv_recon_event *buf, int32_t cap, int32_t *n) {
     Ctx *c = (Ctx *)h;
     if (!c || !n) return NV_ERR_ARG;
     if (int rc = read_ctl(c)) return rc;
-    int m = std::min(c->ctl_h->nevents, c->ev_cap);
-    int take = std::min(m, (int)cap);
-    if (take > 0 && buf) HIPCHK(c, hipMemcpy(buf, c->ev, sizeof(nv_recon_event) * take, hipMemcpyDeviceToHost));
+    const int m = std::min(c->ctl_h->nevents, c->ev_cap);
+    const int take = std::min(m, (int)cap);
+    if (take > 0 && buf) {
+        HIPCHK(c, hipMemcpy(buf, c->ev, sizeof(nv_recon_event) * take, hipMemcpyDeviceToHost));
+        // tangles log concurrently: order by (step, tangle, row); within a
+        // tangle the serial pass visits rows in ascending order
+        std::stable_sort(buf, buf + take, [](const nv_recon_event &a, const nv_recon_event &b) {
+            if (a.itime != b.itime) return a.itime < b.itime;
+            if (a.tangle != b.tangle) return a.tangle < b.tangle;
+            return a.i < b.i;
+        });
+    }
     *n = c->ctl_h->nevents;
     c->ctl_h->nevents = 0;
     HIPCHK(c, hipMemcpy(&c->ctl->nevents, &c->ctl_h->nevents, sizeof(int), hipMemcpyHostToDevice));

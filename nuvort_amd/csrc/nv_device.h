@@ -23,6 +23,27 @@ struct DState {
     double *closestd;
 };
 
+// Independent tangles sharing the state arrays.  Tangle k owns the rows
+// [base[k], base[k] + cap[k]) (both multiples of 256, so source tiles never
+// straddle tangles); its first n[k] rows are its point array f(1:pcount),
+// links hold global 1-based rows.  A single filament system is K = 1.
+struct Tangles {
+    int K;
+    const int *base, *cap;
+    int *n;                 // pcount of each tangle (updated on the device)
+    int *recon, *remove;    // recon_count / remove_count of each tangle
+    int *nact;              // active points of each tangle (pinsert pre-filter)
+    const int *row2t;       // tangle of every row
+    const int *S;           // direct-sum chunks per target of each tangle (this launch)
+    const long long *poff;  // offset of each tangle's partial block (this launch)
+};
+
+// one unit of direct-sum work: a block of targets of tangle k against the
+// c-th of S contiguous chunks of its (image, source tile) sequence
+struct Item {
+    int k, tb, c, pad;
+};
+
 // run.in parameters the kernels read (src/cdata.f90:76-127)
 struct Phys {
     double delta, dt, kappa, corea;

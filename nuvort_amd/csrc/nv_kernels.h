@@ -25,27 +25,30 @@ struct DCtl {
 };
 
 // velocity (k_velocity.hip)
-hipError_t nvk_src64(hipStream_t st, DState s, int n, int npad, double *rec);
-hipError_t nvk_bs64(hipStream_t st, DState s, int n, int npad, const double *rec, const Shifts *sh, int ntiles, int S,
-                    double kappa, double *part);
-hipError_t nvk_src32(hipStream_t st, DState s, int n, int npad, void *rec, double *origin);
-hipError_t nvk_bs32c(hipStream_t st, DState s, int n, int npad, const void *rec, const double *origin,
-                     const double *rec64, const Shifts *sh, int ntiles, int S, double kappa, float rn2, double *part,
-                     int tpt, int kb);
-hipError_t nvk_finish(hipStream_t st, DState s, Phys p, int n, int npad, const double *part, int S, int mode, double t,
+hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec);
+hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
+                    const double *rec, const Shifts *sh, double kappa, double *part);
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, double *origin);
+hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
+                     const void *rec, const double *origin, const double *rec64, const Shifts *sh, double kappa,
+                     float rn2, double *part, int tpt, int kb);
+hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode, double t,
                       int do_ab, DCtl *ctl);
 size_t nvk_rec32_bytes();
+int nvk_occupancy_bs64();
+int nvk_occupancy_bs32c(int tpt, int kb);
 
 // boundary / topology / close search (k_topology.hip)
 hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n);
 hipError_t nvk_wrap(hipStream_t st, DState s, Phys p, int n);
 hipError_t nvk_nan(hipStream_t st, DState s, int n, DCtl *ctl);
-hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, int n, int *cand, int *empty, DCtl *ctl);
-hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, int n_old, const int *cand, const int *empty,
+hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl);
+hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, const int *empty,
                               DCtl *ctl);
 hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
-hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl, nv_recon_event *ev,
-                             int ev_cap, double t, int itime);
+hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
+                             nv_recon_event *ev, int ev_cap, double t, int itime);
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
-hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, const int *cand, DCtl *ctl);
-hipError_t nvk_pclose(hipStream_t st, DState s, int n, int npad, int S, double *pd, int *pj);
+hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl);
+hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, double *pd,
+                      int *pj);

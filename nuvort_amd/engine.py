@@ -107,6 +107,43 @@ class FilamentSolver:
         st.extra["remove_count"] = v.remove_count
         return st
 
+    def set_state_batch(self, states) -> None:
+        """Upload independent tangles; each keeps its own 1-based links."""
+        pcounts = [st.pcount for st in states]
+        allst = FilamentState.empty(sum(pcounts))
+        off = 0
+        for st in states:
+            n = st.pcount
+            for name, arr in st.arrays():
+                getattr(allst, name)[off:off + n] = arr
+            off += n
+        allst.t, allst.itime = states[0].t, states[0].itime
+        self._keep = allst
+        self._ntangles = len(states)
+        pc = (C.c_int32 * len(states))(*pcounts)
+        self._call("nv_set_state_batch", C.c_int32(len(states)), pc, C.byref(self._view(allst)))
+
+    def get_state_batch(self):
+        """The tangles back, one FilamentState each."""
+        k = self._ntangles
+        pc = (C.c_int32 * k)()
+        probe = _L.NvState()
+        probe.capacity = 0
+        self.L.nv_get_state_batch(self.h, C.c_int32(k), pc, C.byref(probe))
+        allst = FilamentState.empty(probe.pcount)
+        v = self._view(allst, allst.pcount)
+        self._call("nv_get_state_batch", C.c_int32(k), pc, C.byref(v))
+        out, off = [], 0
+        for q in range(k):
+            n = pc[q]
+            st = FilamentState.empty(n)
+            for name, arr in st.arrays():
+                arr[:] = getattr(allst, name)[off:off + n]
+            st.t, st.itime = v.t, v.itime
+            out.append(st)
+            off += n
+        return out
+
     def set_time(self, t: float, itime: int) -> None:
         self._call("nv_set_time", C.c_double(t), C.c_int32(itime))
 
@@ -155,7 +192,7 @@ class FilamentSolver:
         buf = (_L.NvReconEvent * 65536)()
         n = C.c_int32(0)
         self._call("nv_get_recon_events", buf, 65536, C.byref(n))
-        return [(e.itime, e.i, e.j, e.t, tuple(e.mid), e.angle, e.dl) for e in buf[:min(n.value, 65536)]]
+        return [(e.itime, e.i, e.j, e.t, tuple(e.mid), e.angle, e.dl, e.tangle) for e in buf[:min(n.value, 65536)]]
 
     def tree_nodes(self) -> np.ndarray:
         n = C.c_int32(0)

@@ -41,7 +41,7 @@ class NvState(C.Structure):
 
 
 class NvReconEvent(C.Structure):
-    _fields_ = [("itime", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("itime", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("tangle", C.c_int32),
                 ("t", C.c_double), ("mid", C.c_double * 3), ("angle", C.c_double), ("dl", C.c_double)]
 
 
@@ -59,7 +59,8 @@ class NvStats(C.Structure):
                 ("walk_launches", C.c_int64), ("walk_ms_total", C.c_double), ("walk_nodes_total", C.c_int64)]
 
 
-EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_time", "nv_ghostp",
+EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_state_batch",
+           "nv_get_state_batch", "nv_set_time", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
            "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
            "nv_get_recon_events", "nv_get_tree", "nv_get_stats")
@@ -85,6 +86,8 @@ def load(path: str = LIB_PATH):
     sig = {
         "nv_create": [C.POINTER(NvParams), C.POINTER(vp)], "nv_destroy": [vp], "nv_set_state": [vp, C.POINTER(NvState)],
         "nv_get_state": [vp, C.POINTER(NvState)], "nv_set_time": [vp, C.c_double, C.c_int32],
+        "nv_set_state_batch": [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(NvState)],
+        "nv_get_state_batch": [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(NvState)],
         "nv_step": [vp, C.c_int32],
         "nv_get_recon_events": [vp, C.POINTER(NvReconEvent), C.c_int32, C.POINTER(C.c_int32)],
         "nv_get_tree": [vp, C.POINTER(NvTreeNode), C.c_int32, C.POINTER(C.c_int32)],
