@@ -320,11 +320,16 @@ __global__ void k_zero_counters(unsigned long long *c) {
 // ------------------------------------------------------------ walks
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// one (target block, image) per workgroup: blockIdx.y = image; partial block
+// [images][6][npad], image 0 in the unshifted half, images 1..26 in the
+// image half, so k_finish's chunk-ordered reduction rebuilds the reference's
+// two sums (src/timestep.f90:107-115)
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
                                                  const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
                                                  DCtl *ctl, unsigned long long *counters) {
     const int r = blockIdx.x * BT + threadIdx.x;
+    const int m = blockIdx.y;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
     int beh = -5, me = -5;
@@ -335,75 +340,79 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     }
     const double kappa = p.kappa, theta_c = p.tree_theta;
     const int corr = p.extra_corr;
-    double a0[3] = {0, 0, 0}, as[3] = {0, 0, 0};
+    double acc[3] = {0, 0, 0};
     unsigned long long nev = 0, nvis = 0;
     bool sing = false;
-    for (int m = 0; m < sh.n; ++m) {
-        const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
-        const bool img0 = (m == 0);
-        int lskip = act ? 0 : 0x7fffffff;
-        int cur = 0;
-        while (cur < nwalk) {
-            cur = wave_uniform(cur);
-            const WNode nd = walk[cur];
-            bool open = false;
-            if (cur >= lskip) {
-                ++nvis;
-                const double e0 = xi[0] - (nd.cent[0] + s0);
-                const double e1 = xi[1] - (nd.cent[1] + s1);
-                const double e2 = xi[2] - (nd.cent[2] + s2);
-                const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-                double theta;
-                if (corr) {
-                    if (dist - nd.geo > 0.0) theta = nd.width / (dist - nd.geo);
-                    else theta = nd.width / dist;
-                } else {
-                    theta = nd.width / dist;
-                }
-                if (nd.pcount == 1 || theta < theta_c) {
-                    lskip = nd.skip;
-                    bool use = true;
-                    if (nd.pcount == 1) {
-                        const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
-                        if (img0 && (j == me || j == beh)) use = false;
-                        if (use && dist < NV_EPS) { sing = true; use = false; }
-                    }
-                    if (use) {
-                        const double v0 = ((nd.cent[0] + s0) - xi[0]);
-                        const double v1 = ((nd.cent[1] + s1) - xi[1]);
-                        const double v2 = ((nd.cent[2] + s2) - xi[2]);
-                        const double a = dist * dist;
-                        const double b = 2.0 * (v0 * nd.circ[0] + v1 * nd.circ[1] + v2 * nd.circ[2]);
-                        const double c = nd.circ[0] * nd.circ[0] + nd.circ[1] * nd.circ[1] + nd.circ[2] * nd.circ[2];
-                        const double D = 4.0 * a * c - b * b;
-                        if (!(D < NV_EPS)) {
-                            const double den = (2.0 * NV_PI) * D;
-                            const double fac = (2.0 * c + b) / sqrt(a + b + c) - (b / sqrt(a));
-                            const double u0 = ((v1 * nd.circ[2] - v2 * nd.circ[1]) * kappa / den) * fac;
-                            const double u1 = ((v2 * nd.circ[0] - v0 * nd.circ[2]) * kappa / den) * fac;
-                            const double u2 = ((v0 * nd.circ[1] - v1 * nd.circ[0]) * kappa / den) * fac;
-                            if (img0) { a0[0] = a0[0] + u0; a0[1] = a0[1] + u1; a0[2] = a0[2] + u2; }
-                            else { as[0] = as[0] + u0; as[1] = as[1] + u1; as[2] = as[2] + u2; }
-                            ++nev;
-                        }
-                    }
-                } else {
-                    open = true;
-                }
+    const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+    const bool img0 = (m == 0);
+    int lskip = act ? 0 : 0x7fffffff;
+    int cur = 0;
+    while (cur < nwalk) {
+        cur = wave_uniform(cur);
+        const WNode nd = walk[cur];
+        bool open = false;
+        if (cur >= lskip) {
+            ++nvis;
+            const double e0 = xi[0] - (nd.cent[0] + s0);
+            const double e1 = xi[1] - (nd.cent[1] + s1);
+            const double e2 = xi[2] - (nd.cent[2] + s2);
+            const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            double theta;
+            if (corr) {
+                if (dist - nd.geo > 0.0) theta = nd.width / (dist - nd.geo);
+                else theta = nd.width / dist;
+            } else {
+                theta = nd.width / dist;
             }
-            cur = __any(open) ? cur + 1 : nd.skip;
+            if (nd.pcount == 1 || theta < theta_c) {
+                lskip = nd.skip;
+                bool use = true;
+                if (nd.pcount == 1) {
+                    const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
+                    if (img0 && (j == me || j == beh)) use = false;
+                    if (use && dist < NV_EPS) { sing = true; use = false; }
+                }
+                if (use) {
+                    const double v0 = ((nd.cent[0] + s0) - xi[0]);
+                    const double v1 = ((nd.cent[1] + s1) - xi[1]);
+                    const double v2 = ((nd.cent[2] + s2) - xi[2]);
+                    const double a = dist * dist;
+                    const double b = 2.0 * (v0 * nd.circ[0] + v1 * nd.circ[1] + v2 * nd.circ[2]);
+                    const double c = nd.circ[0] * nd.circ[0] + nd.circ[1] * nd.circ[1] + nd.circ[2] * nd.circ[2];
+                    const double D = 4.0 * a * c - b * b;
+                    if (!(D < NV_EPS)) {
+                        const double den = (2.0 * NV_PI) * D;
+                        const double fac = (2.0 * c + b) / sqrt(a + b + c) - (b / sqrt(a));
+                        acc[0] = acc[0] + ((v1 * nd.circ[2] - v2 * nd.circ[1]) * kappa / den) * fac;
+                        acc[1] = acc[1] + ((v2 * nd.circ[0] - v0 * nd.circ[2]) * kappa / den) * fac;
+                        acc[2] = acc[2] + ((v0 * nd.circ[1] - v1 * nd.circ[0]) * kappa / den) * fac;
+                        ++nev;
+                    }
+                }
+            } else {
+                open = true;
+            }
         }
+        cur = __any(open) ? cur + 1 : nd.skip;
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     if (r < npad) {
+        double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            part[k * npad + r] = a0[k];
-            part[(3 + k) * npad + r] = as[k];
+            pp[(img0 ? k : 3 + k) * npad + r] = acc[k];
+            pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
         }
     }
-    atomicAdd(&counters[0], nev);
-    atomicAdd(&counters[1], nvis);
+    // wave-level counter partials
+    for (int off = 32; off > 0; off >>= 1) {
+        nev += __shfl_down(nev, off);
+        nvis += __shfl_down(nvis, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counters[0], nev);
+        atomicAdd(&counters[1], nvis);
+    }
 }
 
 template <int BT>
@@ -657,8 +666,8 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     (void)precision;
     (void)stats;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
-    hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256), dim3(256), 0, st, d, p, sh, n, npad, t->walk, t->nwalk,
-                       part, ctl, t->counters);
+    hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad, t->walk,
+                       t->nwalk, part, ctl, t->counters);
     TCHK(hipGetLastError());
     return NV_OK;
 }

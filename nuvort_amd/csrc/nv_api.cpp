@@ -429,13 +429,13 @@ int velocity_impl(Ctx *c, int do_ab) {
         HIPCHK(c, c->d_poff.reserve(c->K));
     } else if (c->prm.velocity == NV_VEL_TREE) {
         if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pmotion: tree velocity without construct_tree");
-        // one chunk: partial block [6][rows] at offset 0
-        const int one = 1;
+        // one chunk per image: partial block [images][6][rows] at offset 0
+        const int nimg = c->sh.n;
         const long long zero = 0;
         HIPCHK(c, c->d_S.reserve(1));
         HIPCHK(c, c->d_poff.reserve(1));
-        HIPCHK(c, c->part.reserve((size_t)6 * R));
-        HIPCHK(c, hipMemcpyAsync(c->d_S.p, &one, sizeof(int), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, c->part.reserve((size_t)nimg * 6 * R));
+        HIPCHK(c, hipMemcpyAsync(c->d_S.p, &nimg, sizeof(int), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipMemcpyAsync(c->d_poff.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipEventRecord(c->e2, c->st));
         int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision,
