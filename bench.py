@@ -11,7 +11,8 @@ Seed 1236 (1234 + configuration number 2).  Synthetic data, generated here.
 A "step" is one pass of the whole reference main-loop body on the device:
 ghosts -> velocity (FP32 Kahan-compensated far field, FP64 near field,
 local term, mutual friction) -> AB update -> point insertion -> brute
-close-point search -> reconnection -> point removal -> periodic wrap.
+close-point search -> reconnection -> point removal -> periodic wrap,
+recorded once as a HIP graph and replayed every step.
 ``value`` = point-updates/s = sum over the timed steps of the points moved,
 divided by the wall time of those steps (state resident in HBM).
 
@@ -66,18 +67,29 @@ def workload():
     return st, p
 
 
-def timed_steps(p, st, steps, warmup):
+def timed_steps(p, st, steps, warmup, blocks=1):
+    """Warm up, then time `steps` steps as `blocks` consecutive nv_step calls
+    (wall clock around each call; the library synchronises before it
+    returns).  Returns totals and the per-block ms/step."""
     from nuvort_amd.engine import FilamentSolver
     s = FilamentSolver(p)
     s.set_state(st)
     if warmup:
         s.step(warmup)
     s0 = s.stats()
+    sizes = [steps // blocks + (1 if b < steps % blocks else 0) for b in range(blocks)]
+    per_block = []
     t0 = time.perf_counter()
-    s.step(steps)
+    for k in sizes:
+        if k == 0:
+            continue
+        tb = time.perf_counter()
+        s.step(k)
+        per_block.append((time.perf_counter() - tb) / k * 1e3)
     t1 = time.perf_counter()
     s1 = s.stats()
     out = {"seconds": t1 - t0, "point_updates": s1["point_updates"] - s0["point_updates"],
+           "block_ms_per_step": per_block,
            "pair_ms": s1["pair_ms_total"] - s0["pair_ms_total"],
            "pair_launches": s1["pair_launches"] - s0["pair_launches"],
            "pair_evals": s1["pair_evals_total"] - s0["pair_evals_total"],
@@ -87,6 +99,25 @@ def timed_steps(p, st, steps, warmup):
            "final_points": s.get_state().active}
     s.close()
     return out
+
+
+def kernel_timing(p, st, reps):
+    """Average duration of the direct-sum launch: `reps` velocity passes on
+    the workload's state, each launch bracketed by HIP events on the
+    library's stream (the recorded step carries no events)."""
+    from nuvort_amd.engine import FilamentSolver
+    s = FilamentSolver(p)
+    s.set_state(st)
+    s.ghostp()
+    s.velocity()
+    s0 = s.stats()
+    for _ in range(reps):
+        s.velocity()
+    s1 = s.stats()
+    s.close()
+    launches = s1["pair_launches"] - s0["pair_launches"]
+    return ((s1["pair_ms_total"] - s0["pair_ms_total"]) / launches * 1e-3,
+            (s1["pair_evals_total"] - s0["pair_evals_total"]) / launches, launches)
 
 
 def velocity_once(p, st, precision):
@@ -141,14 +172,15 @@ def main():
     n = st.active
     log("workload: %d points" % n)
 
-    head = timed_steps(p, st, args.steps, args.warmup)
+    head = timed_steps(p, st, args.steps, args.warmup, blocks=min(5, args.steps))
     ms_step = head["seconds"] / args.steps * 1e3
     value = head["point_updates"] / head["seconds"]
-    log("fp32c step: %.3f ms, %.4g point-updates/s" % (ms_step, value))
+    blk = head["block_ms_per_step"]
+    log("fp32c step: %.3f ms, %.4g point-updates/s (blocks %s)" % (ms_step, value,
+                                                                  " ".join("%.3f" % b for b in blk)))
 
     # roofline of the dominant kernel (the FP32-compensated direct sum)
-    t_kernel = head["pair_ms"] / max(1, head["pair_launches"]) * 1e-3
-    evals_per_launch = head["pair_evals"] / max(1, head["pair_launches"])
+    t_kernel, evals_per_launch, nlaunch = kernel_timing(p, st, max(10, min(args.steps, 50)))
     achieved = FLOP_PER_PAIR * evals_per_launch / t_kernel / 1e12
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "direct_fp32c_rings64_traffic.json")
@@ -158,17 +190,29 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32, 4), "traffic": traffic,
                 "kernel": "k_bs32c (direct sum, FP32 far field)", "kernel_ms": round(t_kernel * 1e3, 4),
-                "pair_evals_per_launch": int(evals_per_launch), "flop_per_pair": FLOP_PER_PAIR}
+                "pair_evals_per_launch": int(evals_per_launch), "flop_per_pair": FLOP_PER_PAIR,
+                "launches_timed": nlaunch}
 
-    extra = {}
+    extra = {"step_ms_blocks": {"median": float(np.median(blk)), "min": float(min(blk)), "max": float(max(blk)),
+                                "blocks": len(blk), "note": "ms/step of each of the consecutive nv_step calls "
+                                                            "the timed steps were split into"}}
     if not args.no_extra:
+        qe = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+        qe.hip_step_launch = "eager"
+        ke = max(5, min(args.steps, 20))
+        re_ = timed_steps(qe, st, ke, 2)
+        extra["eager_launch"] = {"ms_per_step": re_["seconds"] / ke * 1e3,
+                                 "value": re_["point_updates"] / re_["seconds"],
+                                 "note": "the same step with every kernel launched eagerly and the host "
+                                         "synchronising after each phase, instead of one recorded HIP graph"}
+        log("eager step: %.3f ms" % (re_["seconds"] / ke * 1e3))
         q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
         q.hip_precision = "fp64"
         k64 = max(5, min(args.steps, 20))
         r64 = timed_steps(q, st, k64, 2)
         v64 = r64["point_updates"] / r64["seconds"]
-        t64 = r64["pair_ms"] / max(1, r64["pair_launches"]) * 1e-3
-        a64 = FLOP_PER_PAIR * (r64["pair_evals"] / max(1, r64["pair_launches"])) / t64 / 1e12
+        t64, e64, _ = kernel_timing(q, st, 5)
+        a64 = FLOP_PER_PAIR * e64 / t64 / 1e12
         extra["fp64_plain"] = {"value": v64, "ms_per_step": r64["seconds"] / k64 * 1e3,
                                "kernel_ms": t64 * 1e3, "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
                                "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": t64 / t_kernel}

@@ -85,6 +85,9 @@ typedef struct {
     int32_t targets_per_lane;    /* FP32C direct sum: points per lane, 1 or 2 (0 = tuned) */
     int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
                                     adds, 1 (every term), 2, 4 or 8 (0 = default 8) */
+    int32_t step_launch;         /* nv_step: 0 = record one step's launches in a HIP graph and
+                                    replay it every step (direct sum and LIA; the octree step
+                                    launches eagerly), 1 = eager launches */
 } nv_params;
 
 /* Host view of the filament point array, structure of arrays (the fields

@@ -280,8 +280,10 @@ __device__ __forceinline__ double dist_rows(const DState &s, const double *a, in
 }
 
 __global__ void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl, nv_recon_event *ev,
-                                int ev_cap, double t, int itime) {
+                                int ev_cap) {
     if (threadIdx.x != 0) return;
+    const double t = ctl->t;
+    const int itime = ctl->itime;
     const int tk = blockIdx.x;
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
@@ -391,6 +393,7 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
     const Item I = items[blockIdx.x];
     const int tid = threadIdx.x;
     const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+    if (I.tb * 256 >= n) return;   // item listed for rows the tangle does not fill (yet)
     const int loc = I.tb * 256 + tid, i = base + loc;
     const bool act = loc < n && s.infront[i] != 0;
     double x0 = 0, x1 = 0, x2 = 0;
@@ -458,6 +461,22 @@ __global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, c
     if (bj >= 0) s.closest[i] = bj + 1;
 }
 
+// ------------------------------------------------------------ clock
+__global__ void k_clock_set(DCtl *ctl, double t, int itime) {
+    ctl->t = t;
+    ctl->itime = itime;
+}
+
+// src/run.f90: the loop index advances before the body runs
+__global__ void k_clock_begin(DCtl *ctl) { ctl->itime = ctl->itime + 1; }
+
+// src/run.f90: t=t+dt after enforce_boundary; the step's moved points are
+// added to the running total
+__global__ void k_clock_end(DCtl *ctl, double dt) {
+    ctl->t = ctl->t + dt;
+    ctl->nvel_total += (unsigned long long)ctl->nvel;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launch
@@ -489,8 +508,8 @@ hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
     return hipGetLastError();
 }
 hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
-                             nv_recon_event *ev, int ev_cap, double t, int itime) {
-    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl, ev, ev_cap, t, itime);
+                             nv_recon_event *ev, int ev_cap) {
+    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl, ev, ev_cap);
     return hipGetLastError();
 }
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
@@ -505,5 +524,18 @@ hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item
                       int *pj) {
     if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, pd, pj);
     if (rows > 0) hipLaunchKernelGGL(k_pclose_fin, GRID(rows), 0, st, s, tg, rows, pd, pj);
+    return hipGetLastError();
+}
+
+hipError_t nvk_clock_set(hipStream_t st, DCtl *ctl, double t, int itime) {
+    hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(1), 0, st, ctl, t, itime);
+    return hipGetLastError();
+}
+hipError_t nvk_clock_begin_step(hipStream_t st, DCtl *ctl) {
+    hipLaunchKernelGGL(k_clock_begin, dim3(1), dim3(1), 0, st, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_clock_end_step(hipStream_t st, DCtl *ctl, double dt) {
+    hipLaunchKernelGGL(k_clock_end, dim3(1), dim3(1), 0, st, ctl, dt);
     return hipGetLastError();
 }

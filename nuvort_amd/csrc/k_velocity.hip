@@ -64,6 +64,7 @@ __global__ __launch_bounds__(TB) void k_bs64(DState s, Tangles tg, const Item *_
         if (it >= nitems) break;
         const Item I = items[it];
         const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+        if (I.tb * TB >= n) continue;   // item listed for rows the tangle does not fill (yet)
         const int loc = I.tb * TB + tid, i = base + loc;
         const bool act = (loc < n) && (s.infront[i] != 0);
         double xi0 = 0, xi1 = 0, xi2 = 0;
@@ -179,6 +180,7 @@ __global__ __launch_bounds__(TB) void k_bs32c(DState s, Tangles tg, const Item *
         if (it >= nitems) break;
         const Item I = items[it];
         const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
+        if (I.tb * TB * TPT >= n) continue;   // item listed for rows the tangle does not fill (yet)
         int ii[TPT], loc[TPT], bi[TPT];
         bool act[TPT];
         double xi[TPT][3];
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(TB) void k_bs32c(DState s, Tangles tg, const Item *
 // Reduce chunk partials in chunk order and finish every point.
 // mode 0: partials hold (acc0, accs) of k_bs64 or of the tree walk;
 // mode 1: (far, near) of k_bs32c; mode 2: no partials (LIA).
-__global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, double t,
+__global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode,
                          int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < rows && s.infront[r] != 0;
@@ -320,7 +322,7 @@ __global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *_
             for (int k = 0; k < 3; ++k) ub0[k] = nr[k] + far[k];
         }
     }
-    finish_point(s, p, r, ub0, ubs, t, do_ab);
+    finish_point(s, p, r, ub0, ubs, ctl->t, do_ab);
 }
 
 }  // namespace
@@ -387,10 +389,10 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
     return hipGetLastError();
 }
 
-hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode, double t,
+hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl) {
     if (rows > 0)
-        hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode, t, do_ab,
+        hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode, do_ab,
                            ctl);
     return hipGetLastError();
 }

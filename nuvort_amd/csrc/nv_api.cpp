@@ -73,6 +73,15 @@ struct Ctx {
     bool tree_valid = false;
     int tpt = 2, kb = 8;             // FP32C kernel variant: targets per lane, compensation block
     int occ64 = 4, occ32 = 4;        // resident workgroups per CU of the persistent kernels
+    // recorded steps, [x/xn parity][reconnection phase on], valid while
+    // gen == layout_gen (a new layout moves every buffer)
+    struct StepGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        long gen = -1;
+    } sg[2][2];
+    long layout_gen = 0;
+    double *xa = nullptr;            // the buffer alloc_state gave to x
 };
 
 int fail(Ctx *c, int code, const std::string &msg) {
@@ -133,6 +142,8 @@ int alloc_state(Ctx *c) {
         HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(int) * (size_t)rows, c->st));
     }
     c->d = d;
+    c->xa = d.x;
+    ++c->layout_gen;
     HIPCHK(c, c->cand.reserve(rows));
     HIPCHK(c, c->empty.reserve(rows));
     HIPCHK(c, c->lcand.reserve(rows));
@@ -365,18 +376,27 @@ int ensure_room(Ctx *c) {
     const std::vector<int> rec = c->trecon, rem = c->tremove;
     const double t = c->t;
     const int it = c->itime;
+    // the control block (event count, device clock, running totals)
+    // survives the move
+    DCtl saved;
+    HIPCHK(c, hipMemcpy(&saved, c->ctl, sizeof(DCtl), hipMemcpyDeviceToHost));
     if (int rc = set_batch(c, c->K, pc.data(), &hs, &caps)) return rc;
     c->trecon = rec;
     c->tremove = rem;
     c->t = t;
     c->itime = it;
+    *c->ctl_h = saved;
+    HIPCHK(c, hipMemcpy(c->ctl, &saved, sizeof(DCtl), hipMemcpyHostToDevice));
     return upload_tables(c);
 }
 
 // ------------------------------------------------------------ work lists
 // Direct-sum items, largest tangle first.  The chunk count of each tangle
 // keeps items of similar size and gives the launch >= 16 items per CU.
-int build_items(Ctx *c, int tpt, bool exact, int *nitems_out) {
+// With cap_rows the list covers every row of each tangle's range, so a
+// recorded launch stays valid while pcount grows; the kernels skip blocks
+// past the tangle's current pcount.
+int build_items(Ctx *c, int tpt, bool exact, bool cap_rows, int *nitems_out) {
     const int K = c->K;
     std::vector<int> order(K);
     std::iota(order.begin(), order.end(), 0);
@@ -399,9 +419,12 @@ int build_items(Ctx *c, int tpt, bool exact, int *nitems_out) {
         po += (long long)S[k] * 6 * c->tcap[k];
     }
     std::vector<Item> it;
-    for (int k : order)
-        for (int tb = 0; tb < ntb[k]; ++tb)
+    for (int k : order) {
+        const int nb = cap_rows ? (c->tcap[k] + 256 * tpt - 1) / (256 * tpt) : ntb[k];
+        for (int tb = 0; tb < nb; ++tb)
             for (int ch = 0; ch < S[k]; ++ch) it.push_back(Item{k, tb, ch, 0});
+    }
+    if (!cap_rows) ++c->layout_gen;   // recorded steps read these lists
     HIPCHK(c, c->items.reserve(std::max<size_t>(1, it.size())));
     HIPCHK(c, c->d_S.reserve(K));
     HIPCHK(c, c->d_poff.reserve(K));
@@ -417,6 +440,18 @@ int build_items(Ctx *c, int tpt, bool exact, int *nitems_out) {
 
 void swap_x(Ctx *c) { std::swap(c->d.x, c->d.xn); }
 
+// evaluated pairs of one direct-sum launch: every active target against
+// every active source of its tangle in every image, less j == i (unshifted)
+// and j == behind(i) (all images); nv = active points of a single tangle,
+// batch tangle sizes use pcount (empty slots are rare)
+int64_t pair_evals(const Ctx *c, int64_t nv, const std::vector<int> &tn) {
+    int64_t pe = 0;
+    if (c->K == 1) pe = nv * ((int64_t)c->sh.n * nv - c->sh.n - 1);
+    else
+        for (int k = 0; k < c->K; ++k) pe += (int64_t)tn[k] * ((int64_t)c->sh.n * tn[k] - c->sh.n - 1);
+    return pe;
+}
+
 int velocity_impl(Ctx *c, int do_ab) {
     if (c->K == 0) return NV_OK;
     const int R = c->rows;
@@ -425,6 +460,7 @@ int velocity_impl(Ctx *c, int do_ab) {
     bool timed_pair = false, timed_walk = false;
     if (c->prm.velocity == NV_VEL_LIA) {
         mode = 2;
+        ++c->layout_gen;
         HIPCHK(c, c->d_S.reserve(c->K));
         HIPCHK(c, c->d_poff.reserve(c->K));
     } else if (c->prm.velocity == NV_VEL_TREE) {
@@ -432,6 +468,7 @@ int velocity_impl(Ctx *c, int do_ab) {
         // one chunk per image: partial block [images][6][rows] at offset 0
         const int nimg = c->sh.n;
         const long long zero = 0;
+        ++c->layout_gen;   // rewrites d_S / d_poff that recorded steps read
         HIPCHK(c, c->d_S.reserve(1));
         HIPCHK(c, c->d_poff.reserve(1));
         HIPCHK(c, c->part.reserve((size_t)nimg * 6 * R));
@@ -447,7 +484,7 @@ int velocity_impl(Ctx *c, int do_ab) {
     } else {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
         int nitems = 0;
-        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, &nitems)) return rc;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, false, &nitems)) return rc;
         HIPCHK(c, c->queue.reserve(1));
         HIPCHK(c, nv_fill_int(c->st, c->queue.p, 0, 1));
         HIPCHK(c, nvk_src64(c->st, c->d, R, c->rec64.p));
@@ -471,10 +508,10 @@ int velocity_impl(Ctx *c, int do_ab) {
         }
         timed_pair = true;
     }
-    c->ctl_h->nvel = 0;
-    HIPCHK(c, hipMemcpyAsync(&c->ctl->nvel, &c->ctl_h->nvel, sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, nv_fill_int(c->st, &c->ctl->nvel, 0, 1));
+    HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
     Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
-    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, tg, R, c->part.p, mode, c->t, do_ab, c->ctl));
+    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, tg, R, c->part.p, mode, do_ab, c->ctl));
     HIPCHK(c, hipMemcpyAsync(&c->ctl_h->nvel, &c->ctl->nvel, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -484,13 +521,7 @@ int velocity_impl(Ctx *c, int do_ab) {
     const int64_t nv = c->ctl_h->nvel;
     if (do_ab) c->stats.point_updates += nv;
     if (timed_pair) {
-        // evaluated pairs: every active target against every active source of
-        // its tangle in every image, less j == i (unshifted) and j == behind(i)
-        // (all images); batch tangle sizes use pcount (empty slots are rare)
-        int64_t pe = 0;
-        if (c->K == 1) pe = nv * ((int64_t)c->sh.n * nv - c->sh.n - 1);
-        else
-            for (int k = 0; k < c->K; ++k) pe += (int64_t)c->tn[k] * ((int64_t)c->sh.n * c->tn[k] - c->sh.n - 1);
+        const int64_t pe = pair_evals(c, nv, c->tn);
         float pm = 0.f;
         (void)hipEventElapsedTime(&pm, c->e2, c->e3);
         c->stats.pair_ms_total += pm;
@@ -506,6 +537,181 @@ int velocity_impl(Ctx *c, int do_ab) {
         c->stats.walk_nodes_total += c->stats.tree_nodes_visited;
     }
     if (do_ab) swap_x(c);
+    return NV_OK;
+}
+
+// Brute close-search items (tangle, target block, chunk of source tiles),
+// ~16 per CU; cap_rows as for build_items.
+int build_close_items(Ctx *c, bool cap_rows, int *nitems_out) {
+    const int K = c->K;
+    long work = 0;
+    std::vector<int> ntb(K), S(K);
+    for (int k = 0; k < K; ++k) {
+        ntb[k] = (c->tn[k] + 255) / 256;
+        work += (long)ntb[k] * ntb[k];
+    }
+    const long want = 16L * c->ncu;
+    const long per = std::max<long>(4, work / std::max<long>(1, want));
+    std::vector<long long> poff(K);
+    long long po = 0;
+    std::vector<Item> it;
+    for (int k = 0; k < K; ++k) {
+        S[k] = (int)std::max<long>(1, std::min<long>(ntb[k], (ntb[k] + per - 1) / per));
+        poff[k] = po;
+        po += (long long)S[k] * c->tcap[k];
+        const int nb = cap_rows ? c->tcap[k] / 256 : ntb[k];
+        for (int tb = 0; tb < nb; ++tb)
+            for (int ch = 0; ch < S[k]; ++ch) it.push_back(Item{k, tb, ch, 0});
+    }
+    if (!cap_rows) ++c->layout_gen;
+    HIPCHK(c, c->pitems.reserve(std::max<size_t>(1, it.size())));
+    HIPCHK(c, c->d_pS.reserve(K));
+    HIPCHK(c, c->d_ppoff.reserve(K));
+    HIPCHK(c, c->pd.reserve((size_t)std::max<long long>(po, 1)));
+    HIPCHK(c, c->pj.reserve((size_t)std::max<long long>(po, 1)));
+    if (!it.empty())
+        HIPCHK(c, hipMemcpyAsync(c->pitems.p, it.data(), sizeof(Item) * it.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_pS.p, S.data(), sizeof(int) * K, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_ppoff.p, poff.data(), sizeof(long long) * K, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *nitems_out = (int)it.size();
+    return NV_OK;
+}
+
+// ------------------------------------------------------- recorded steps
+// One pass of the src/run.f90 loop body (direct sum or LIA) as kernel
+// launches only: no copies, fills by kernel, no host reads, the step's
+// itime and t advanced on the device.  Work lists cover every row of the
+// current layout; the x / xn exchange of the AB update is resolved at
+// record time (one recording per parity).
+int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
+    hipStream_t st = c->st;
+    const int R = c->rows;
+    HIPCHK(c, nvk_clock_begin_step(st, c->ctl));
+    HIPCHK(c, nvk_ghosts(st, d, c->phys, R));
+    int mode = 2;
+    Tangles tv = tangles(c, c->d_S.p, c->d_poff.p);
+    if (c->prm.velocity == NV_VEL_BS) {
+        const bool fp32 = c->prm.precision == NV_PREC_FP32C;
+        HIPCHK(c, nv_fill_int(st, c->queue.p, 0, 1));
+        HIPCHK(c, nvk_src64(st, d, R, c->rec64.p));
+        if (fp32) {
+            HIPCHK(c, nvk_src32(st, d, R, c->rec32.p, c->origin.p));
+            const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+            const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
+            HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->origin.p,
+                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
+            mode = 1;
+        } else {
+            const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));
+            HIPCHK(c, nvk_bs64(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec64.p, &c->sh, c->phys.kappa,
+                               c->part.p));
+            mode = 0;
+        }
+    }
+    HIPCHK(c, nv_fill_int(st, &c->ctl->nvel, 0, 1));
+    HIPCHK(c, nvk_finish(st, d, c->phys, tv, R, c->part.p, mode, 1, c->ctl));
+    std::swap(d.x, d.xn);
+    Tangles tg = tangles(c, nullptr, nullptr);
+    // pinsert
+    HIPCHK(c, nv_fill_int(st, &c->ctl->nactive, 0, 1));
+    HIPCHK(c, nv_fill_int(st, c->d_tnact.p, 0, c->K));
+    HIPCHK(c, nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
+    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
+    HIPCHK(c, nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    if (recon) {
+        Tangles tp = tangles(c, c->d_pS.p, c->d_ppoff.p);
+        HIPCHK(c, nvk_pclose(st, d, tp, R, c->pitems.p, npitems, c->pd.p, c->pj.p));
+        HIPCHK(c, nvk_precon_pre(st, d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
+        HIPCHK(c, nvk_premove_pre(st, d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl));
+    }
+    HIPCHK(c, nvk_wrap(st, d, c->phys, R));
+    HIPCHK(c, nvk_clock_end_step(st, c->ctl, c->prm.dt));
+    HIPCHK(c, nvk_nan(st, d, R, c->ctl));
+    return NV_OK;
+}
+
+void drop_graph(Ctx::StepGraph &g) {
+    if (g.e) (void)hipGraphExecDestroy(g.e);
+    if (g.g) (void)hipGraphDestroy(g.g);
+    g.e = nullptr;
+    g.g = nullptr;
+    g.gen = -1;
+}
+
+// Record every step variant from one set of work lists: both x / xn
+// parities and, when recon_shots > 1, steps with and without the
+// reconnection phase.
+int record_steps(Ctx *c) {
+    for (auto &row : c->sg)
+        for (auto &g : row) drop_graph(g);
+    int nitems = 0, npitems = 0;
+    if (c->prm.velocity == NV_VEL_BS) {
+        const bool fp32 = c->prm.precision == NV_PREC_FP32C;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, true, &nitems)) return rc;
+        HIPCHK(c, c->queue.reserve(1));
+    } else {
+        HIPCHK(c, c->d_S.reserve(c->K));
+        HIPCHK(c, c->d_poff.reserve(c->K));
+    }
+    if (int rc = build_close_items(c, true, &npitems)) return rc;
+    DState d0 = c->d;
+    if (d0.x != c->xa) std::swap(d0.x, d0.xn);
+    for (int par = 0; par < 2; ++par) {
+        DState d = d0;
+        if (par) std::swap(d.x, d.xn);
+        for (int recon = 0; recon < 2; ++recon) {
+            if (!recon && c->prm.recon_shots == 1) continue;
+            Ctx::StepGraph &g = c->sg[par][recon];
+            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_step(c, d, recon != 0, nitems, npitems);
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(c->st, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ee != hipSuccess)
+                return fail(c, NV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+            g.g = graph;
+            HIPCHK(c, hipGraphInstantiate(&g.e, graph, nullptr, nullptr, 0));
+            g.gen = c->layout_gen;
+        }
+    }
+    return NV_OK;
+}
+
+// nv_step for the direct sum and LIA: a recorded step replayed per step,
+// one synchronisation per step (status, pcounts, then the capacity check)
+int step_recorded(Ctx *c, int nsteps) {
+    HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
+    HIPCHK(c, hipMemcpyAsync(&c->ctl_h->nvel_total, &c->ctl->nvel_total, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    unsigned long long pu = c->ctl_h->nvel_total;
+    for (int k = 0; k < nsteps; ++k) {
+        if (int rc = ensure_room(c)) return rc;
+        if (c->d.x != c->xa && c->d.xn != c->xa) return fail(c, NV_ERR_STATE, "position buffers lost");
+        const int par = c->d.x == c->xa ? 0 : 1;
+        const bool recon = (c->itime + 1) % c->prm.recon_shots == 0;
+        Ctx::StepGraph &g = c->sg[par][recon ? 1 : 0];
+        if (g.gen != c->layout_gen)
+            if (int rc = record_steps(c)) return rc;
+        HIPCHK(c, hipGraphLaunch(g.e, c->st));
+        c->itime += 1;
+        swap_x(c);
+        const int rc = read_ctl(c);
+        c->t = c->t + c->prm.dt;
+        c->stats.point_updates += (int64_t)(c->ctl_h->nvel_total - pu);
+        pu = c->ctl_h->nvel_total;
+        if (rc) return rc;
+
+    }
     return NV_OK;
 }
 
@@ -614,6 +820,8 @@ int nv_destroy(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     (void)hipStreamSynchronize(c->st);
+    for (auto &row : c->sg)
+        for (auto &g : row) drop_graph(g);
     free_state(c);
     DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tn, &c->d_trecon, &c->d_tremove, &c->d_tnact, &c->d_row2t,
                          &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp, &c->pj, &c->d_S, &c->d_pS, &c->queue};
@@ -734,39 +942,12 @@ int nv_pinsert(void *h) {
 int nv_pclose(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
-    const int K = c->K;
-    if (K == 0) return NV_OK;
-    // items (tangle, target block, chunk of source tiles), ~16 per CU
-    long work = 0;
-    std::vector<int> ntb(K), S(K);
-    for (int k = 0; k < K; ++k) {
-        ntb[k] = (c->tn[k] + 255) / 256;
-        work += (long)ntb[k] * ntb[k];
-    }
-    const long want = 16L * c->ncu;
-    const long per = std::max<long>(4, work / std::max<long>(1, want));
-    std::vector<long long> poff(K);
-    long long po = 0;
-    std::vector<Item> it;
-    for (int k = 0; k < K; ++k) {
-        S[k] = (int)std::max<long>(1, std::min<long>(ntb[k], (ntb[k] + per - 1) / per));
-        poff[k] = po;
-        po += (long long)S[k] * c->tcap[k];
-        for (int tb = 0; tb < ntb[k]; ++tb)
-            for (int ch = 0; ch < S[k]; ++ch) it.push_back(Item{k, tb, ch, 0});
-    }
-    HIPCHK(c, c->pitems.reserve(std::max<size_t>(1, it.size())));
-    HIPCHK(c, c->d_pS.reserve(K));
-    HIPCHK(c, c->d_ppoff.reserve(K));
-    HIPCHK(c, c->pd.reserve((size_t)std::max<long long>(po, 1)));
-    HIPCHK(c, c->pj.reserve((size_t)std::max<long long>(po, 1)));
-    if (!it.empty())
-        HIPCHK(c, hipMemcpyAsync(c->pitems.p, it.data(), sizeof(Item) * it.size(), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->d_pS.p, S.data(), sizeof(int) * K, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->d_ppoff.p, poff.data(), sizeof(long long) * K, hipMemcpyHostToDevice, c->st));
+    if (c->K == 0) return NV_OK;
+    int nitems = 0;
+    if (int rc = build_close_items(c, false, &nitems)) return rc;
     Tangles tg = tangles(c, c->d_pS.p, c->d_ppoff.p);
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, (int)it.size(), c->pd.p, c->pj.p));
+    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, nitems, c->pd.p, c->pj.p));
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
@@ -798,7 +979,8 @@ int nv_precon(void *h) {
     Tangles tg = tangles(c, nullptr, nullptr);
     HIPCHK(c, nvk_precon_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
-    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap, c->t, c->itime));
+    HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
+    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
     return read_ctl(c);
 }
 
@@ -838,12 +1020,17 @@ int nv_nan_check(void *h) {
 int nv_step(void *h, int32_t nsteps) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    const int total = nsteps;
     hipEvent_t s0, s1;
     HIPCHK(c, hipEventCreate(&s0));
     HIPCHK(c, hipEventCreate(&s1));
     HIPCHK(c, hipEventRecord(s0, c->st));
     const bool use_tree = c->prm.tree_theta > 0;
     int rc = NV_OK;
+    if (!use_tree && c->prm.step_launch == 0 && c->K > 0 && nsteps > 0) {
+        rc = step_recorded(c, nsteps);
+        nsteps = 0;
+    }
     for (int k = 0; k < nsteps && rc == NV_OK; ++k) {
         c->itime += 1;
         if ((rc = nv_ghostp(c))) break;
@@ -868,7 +1055,7 @@ int nv_step(void *h, int32_t nsteps) {
     (void)hipEventSynchronize(s1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, s0, s1);
-    c->stats.ms_step = nsteps > 0 ? ms / nsteps : 0.0;
+    c->stats.ms_step = total > 0 ? ms / total : 0.0;
     (void)hipEventDestroy(s0);
     (void)hipEventDestroy(s1);
     return rc;

@@ -21,7 +21,10 @@ struct DCtl {
     int ncand;         // rows in the current candidate list
     int nempty;        // rows in the current empty-slot list
     int nvel;          // points whose velocity the last velocity launch computed
-    int pad[7];
+    int itime;         // the step being taken (src/run.f90 loop index)
+    double t;          // simulation time
+    unsigned long long nvel_total;  // nvel summed over the steps k_clock_tick closed
+    int pad[2];
 };
 
 // velocity (k_velocity.hip)
@@ -32,11 +35,17 @@ hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, double *orig
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
                      const void *rec, const double *origin, const double *rec64, const Shifts *sh, double kappa,
                      float rn2, double *part, int tpt, int kb);
-hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode, double t,
+hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl);
 size_t nvk_rec32_bytes();
 int nvk_occupancy_bs64();
 int nvk_occupancy_bs32c(int tpt, int kb);
+
+// device clock (k_topology.hip): the step's itime and t live in DCtl so that a
+// captured step graph reads them at replay time
+hipError_t nvk_clock_set(hipStream_t st, DCtl *ctl, double t, int itime);
+hipError_t nvk_clock_begin_step(hipStream_t st, DCtl *ctl);
+hipError_t nvk_clock_end_step(hipStream_t st, DCtl *ctl, double dt);
 
 // boundary / topology / close search (k_topology.hip)
 hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n);
@@ -47,7 +56,7 @@ hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, Tangles tg, cons
                               DCtl *ctl);
 hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
 hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
-                             nv_recon_event *ev, int ev_cap, double t, int itime);
+                             nv_recon_event *ev, int ev_cap);
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
 hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl);
 hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, double *pd,

@@ -39,6 +39,7 @@ def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.
     s.precision = {"fp64": 0, "fp32c": 1}[p.hip_precision]
     s.near_radius = p.hip_near_radius
     s.close_search = {"auto": 0, "brute": 1, "tree": 2}[p.hip_close_search]
+    s.step_launch = {"graph": 0, "eager": 1}[p.hip_step_launch]
     s.exact_order = 1 if exact_order else 0
     s.capacity = capacity
     return s

@@ -43,6 +43,7 @@ module nuvort_hip
     integer(c_int32_t) :: capacity
     integer(c_int32_t) :: targets_per_lane
     integer(c_int32_t) :: kahan_block
+    integer(c_int32_t) :: step_launch
   end type
 
   type, bind(C) :: nv_state

@@ -29,7 +29,8 @@ class NvParams(C.Structure):
                 ("normal_fluid_cutoff", C.c_double), ("t_zero_normal_fluid", C.c_double),
                 ("recon_shots", C.c_int32), ("overide_timestep_check", C.c_int32), ("precision", C.c_int32),
                 ("near_radius", C.c_double), ("close_search", C.c_int32), ("exact_order", C.c_int32),
-                ("capacity", C.c_int32), ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32)]
+                ("capacity", C.c_int32), ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32),
+                ("step_launch", C.c_int32)]
 
 
 class NvState(C.Structure):

@@ -66,6 +66,7 @@ class RunParams:
     hip_precision: str = "fp64"          # fp64 | fp32c
     hip_near_radius: float = 8.0         # FP64 near field, in units of delta
     hip_close_search: str = "auto"       # auto | brute | tree
+    hip_step_launch: str = "graph"       # graph | eager (how nv_step issues a step's kernels)
     extra: dict = field(default_factory=dict)
 
 
@@ -92,6 +93,7 @@ _KEYS = {
     "norm_shear_omega": ("norm_shear_omega", "r", 1), "lattice_ratio": ("lattice_ratio", "r", 1),
     "hip_precision": ("hip_precision", "c30", 1), "hip_near_radius": ("hip_near_radius", "r", 1),
     "hip_close_search": ("hip_close_search", "c30", 1),
+    "hip_step_launch": ("hip_step_launch", "c30", 1),
 }
 
 
@@ -195,6 +197,8 @@ def validate(p: RunParams) -> None:
         raise RunFileError("normal_velocity %r is not supported by this library" % p.normal_velocity)
     if p.hip_precision not in ("fp64", "fp32c"):
         raise RunFileError("hip_precision must be fp64 or fp32c")
+    if p.hip_step_launch not in ("graph", "eager"):
+        raise RunFileError("hip_step_launch must be graph or eager")
 
 
 def format_run_text(p: RunParams, keys=None) -> str:

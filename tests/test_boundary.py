@@ -42,6 +42,7 @@ def test_fortran_types_have_the_c_layout():
     assert int(got["nv_state"]) == C.sizeof(nvlib.NvState)
     assert int(got["nv_params.precision"]) == nvlib.NvParams.precision.offset
     assert int(got["nv_params.kahan_block"]) == nvlib.NvParams.kahan_block.offset
+    assert int(got["nv_params.step_launch"]) == nvlib.NvParams.step_launch.offset
     assert int(got["nv_state.t"]) == nvlib.NvState.t.offset
     assert int(got["nv_state.wpinned"]) == nvlib.NvState.wpinned.offset
 
