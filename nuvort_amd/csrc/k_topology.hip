@@ -380,14 +380,79 @@ __global__ void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, 
 }
 
 // ------------------------------------------------------ brute close search
+// Bounding sphere of the active points of each 256-row tile (raw
+// coordinates, FP64); rad < 0 marks a tile without active points.  The
+// radius is rounded up so that it bounds the exact distances.
+struct Sphere64 {
+    double c[3], rad;
+};
+
+__device__ __forceinline__ double wave_min64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ double wave_max64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_tile_spheres(DState s, int rows, Sphere64 *__restrict__ sp) {
+    __shared__ double red[4][6];
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const bool act = r < rows && s.infront[r] != 0;
+    double lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double v = act ? s.x[k * s.cap + r] : 0.0;
+        lo[k] = wave_min64(act ? v : 1.0e300);
+        hi[k] = wave_max64(act ? v : -1.0e300);
+    }
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { red[w][k] = lo[k]; red[w][3 + k] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 1; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], red[q][k]);
+            hi[k] = fmax(hi[k], red[q][3 + k]);
+        }
+    Sphere64 o;
+    if (lo[0] > hi[0]) {
+        o.c[0] = o.c[1] = o.c[2] = 0.0;
+        o.rad = -1.0;
+    } else {
+        double d2 = 0.0, m = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            o.c[k] = 0.5 * (lo[k] + hi[k]);
+            d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
+            m += fabs(o.c[k]);
+        }
+        const double rad = 0.5 * sqrt(d2);
+        o.rad = rad * (1.0 + 1.0e-12) + 1.0e-15 * (m + rad);
+    }
+    sp[blockIdx.x] = o;
+}
+
 // src/line.f90:135-160 pclose: the first (lowest) j with the smallest
 // distf(i,j) among non-empty j of the same tangle not in {i, infront(i),
 // behind(i)}.  Work items (tangle, target block, chunk of source tiles): each
 // chunk keeps its first minimum, k_pclose_fin keeps the first chunk's on
 // ties.  Distances are the reference's correctly rounded sqrt, compared
 // strictly.  Partial block of tangle k: [S][cap_k].
+//
+// Pruning, exact: every target starts from the distance to a valid
+// candidate, infront(infront(i)), as an upper bound just above it (so that
+// candidate, and any point at least as close, is still accepted by the
+// strict comparison; a chunk holding none of them reports no candidate), and
+// a tile is skipped when its bounding sphere lies beyond the current best of
+// every target of the workgroup.
 __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item *__restrict__ items,
-                                                double *__restrict__ pd, int *__restrict__ pj) {
+                                                const Sphere64 *__restrict__ sph, double *__restrict__ pd,
+                                                int *__restrict__ pj) {
     constexpr int T = 256;
     __shared__ double tile[T * 4];
     const Item I = items[blockIdx.x];
@@ -398,16 +463,30 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
     const bool act = loc < n && s.infront[i] != 0;
     double x0 = 0, x1 = 0, x2 = 0;
     int fi = -2, bi = -2;
+    double best = 100.0;
     if (act) {
         x0 = s.x[i]; x1 = s.x[s.cap + i]; x2 = s.x[2 * s.cap + i];
         fi = s.infront[i] - 1; bi = s.behind[i] - 1;
+        const int j2 = s.infront[fi] - 1;
+        if (j2 >= 0 && j2 != i && j2 != fi && j2 != bi && s.infront[j2] != 0) {
+            const double d0 = x0 - s.x[j2], d1 = x1 - s.x[s.cap + j2], d2 = x2 - s.x[2 * s.cap + j2];
+            const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if (d < 100.0) best = fmin(100.0, d * (1.0 + 4.0e-16) + 1.0e-300);
+        }
     }
+    double bound = best * best * (1.0 + 8e-16);
     const int ntiles = (n + T - 1) / T;
     const int t0 = (int)((long)ntiles * I.c / S), t1 = (int)((long)ntiles * (I.c + 1) / S);
-    double best = 100.0, bound = 100.0 * 100.0 * (1.0 + 8e-16);
     int bj = -1;
     for (int t = t0; t < t1; ++t) {
-        __syncthreads();
+        const Sphere64 sp = sph[base / T + t];
+        bool skip = true;
+        if (act && sp.rad >= 0.0) {
+            const double c0 = x0 - sp.c[0], c1 = x1 - sp.c[1], c2 = x2 - sp.c[2];
+            const double dc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            skip = (dc - sp.rad) * (1.0 - 1.0e-12) > best;
+        }
+        if (__syncthreads_and(skip)) continue;
         for (int q = tid; q < T; q += 256) {
             const int jl = t * T + q, j = base + jl;
             const bool v = jl < n && s.infront[j] != 0;
@@ -417,20 +496,21 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
             tile[q * 4 + 3] = v ? 1.0 : 0.0;
         }
         __syncthreads();
-        if (!act) continue;
-        for (int jj = 0; jj < T; ++jj) {
-            const double *q = &tile[jj * 4];
-            if (q[3] == 0.0) continue;
-            const int j = base + t * T + jj;
-            if (j == i || j == fi || j == bi) continue;
-            const double d0 = x0 - q[0], d1 = x1 - q[1], d2 = x2 - q[2];
-            const double dd = d0 * d0 + d1 * d1 + d2 * d2;
-            if (dd < bound) {
-                const double dist = sqrt(dd);
-                if (dist < best) {
-                    best = dist;
-                    bound = dist * dist * (1.0 + 8e-16);
-                    bj = j;
+        if (!skip) {
+            for (int jj = 0; jj < T; ++jj) {
+                const double *q = &tile[jj * 4];
+                if (q[3] == 0.0) continue;
+                const int j = base + t * T + jj;
+                if (j == i || j == fi || j == bi) continue;
+                const double d0 = x0 - q[0], d1 = x1 - q[1], d2 = x2 - q[2];
+                const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+                if (dd < bound) {
+                    const double dist = sqrt(dd);
+                    if (dist < best) {
+                        best = dist;
+                        bound = dist * dist * (1.0 + 8e-16);
+                        bj = j;
+                    }
                 }
             }
         }
@@ -520,9 +600,13 @@ hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, cons
     hipLaunchKernelGGL(k_premove_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl);
     return hipGetLastError();
 }
-hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, double *pd,
-                      int *pj) {
-    if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, pd, pj);
+size_t nvk_sphere_bytes() { return sizeof(Sphere64); }
+
+hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, void *spheres,
+                      double *pd, int *pj) {
+    Sphere64 *sp = (Sphere64 *)spheres;
+    if (rows > 0) hipLaunchKernelGGL(k_tile_spheres, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, sp);
+    if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, sp, pd, pj);
     if (rows > 0) hipLaunchKernelGGL(k_pclose_fin, GRID(rows), 0, st, s, tg, rows, pd, pj);
     return hipGetLastError();
 }

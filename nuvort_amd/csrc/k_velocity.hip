@@ -116,20 +116,41 @@ struct __align__(16) Rec32 {
     int j;
 };
 
-__global__ void k_src32(DState s, int rows, Rec32 *__restrict__ rec, double *__restrict__ origin) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    int t = r / TS, r0 = t * TS;
+// Per source tile: the FP64 origin its FP32 records are relative to, and
+// a sphere (FP32, relative to the origin) holding its active sources;
+// rad < 0 marks a tile without active sources.
+struct TileInfo {
+    double o[3];
+    float c[3], rad;
+};
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// one workgroup per tile of TS rows (rows is a multiple of TS)
+__global__ __launch_bounds__(TS) void k_src32(DState s, int rows, Rec32 *__restrict__ rec,
+                                              TileInfo *__restrict__ tiles) {
+    __shared__ float red[TS / 64][6];
+    __shared__ int first;
+    const int r = blockIdx.x * TS + threadIdx.x;
+    const bool act = r < rows && s.infront[r] != 0;
+    // origin: the tile's first active row
+    if (threadIdx.x == 0) first = 0x7fffffff;
+    __syncthreads();
+    if (act) atomicMin(&first, r);
+    __syncthreads();
     double o[3] = {0.0, 0.0, 0.0};
-    for (int q = r0; q < r0 + TS && q < rows; ++q) {
-        if (s.infront[q] != 0) {
-            o[0] = ld(s.x, s.cap, 0, q); o[1] = ld(s.x, s.cap, 1, q); o[2] = ld(s.x, s.cap, 2, q);
-            break;
-        }
+    if (first != 0x7fffffff) {
+        o[0] = ld(s.x, s.cap, 0, first); o[1] = ld(s.x, s.cap, 1, first); o[2] = ld(s.x, s.cap, 2, first);
     }
-    if (r == r0) { origin[t * 3 + 0] = o[0]; origin[t * 3 + 1] = o[1]; origin[t * 3 + 2] = o[2]; }
     Rec32 v;
-    if (s.infront[r] != 0) {
+    if (act) {
         double x[3], e[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -147,7 +168,47 @@ __global__ void k_src32(DState s, int rows, Rec32 *__restrict__ rec, double *__r
         v.c = 0.0f;
         v.j = -1;
     }
-    rec[r] = v;
+    if (r < rows) rec[r] = v;
+    // bounding box of the active sources, then its circumscribed sphere
+    float lo[3], hi[3];
+    lo[0] = act ? v.x0 : 3.0e38f; lo[1] = act ? v.x1 : 3.0e38f; lo[2] = act ? v.x2 : 3.0e38f;
+    hi[0] = act ? v.x0 : -3.0e38f; hi[1] = act ? v.x1 : -3.0e38f; hi[2] = act ? v.x2 : -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { red[w][k] = lo[k]; red[w][3 + k] = hi[k]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < TS / 64; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], red[q][k]);
+                hi[k] = fmaxf(hi[k], red[q][3 + k]);
+            }
+        TileInfo ti;
+        ti.o[0] = o[0]; ti.o[1] = o[1]; ti.o[2] = o[2];
+        if (lo[0] > hi[0]) {
+            ti.c[0] = ti.c[1] = ti.c[2] = 0.f;
+            ti.rad = -1.f;
+        } else {
+            float d2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                ti.c[k] = 0.5f * (lo[k] + hi[k]);
+                const float h = hi[k] - lo[k];
+                d2 += h * h;
+            }
+            // half the diagonal, rounded up by a relative 1e-5
+            ti.rad = 0.5f * sqrtf(d2) * 1.00001f;
+        }
+        tiles[blockIdx.x] = ti;
+    }
 }
 
 __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
@@ -167,14 +228,18 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // through the reference's FP64 arithmetic instead.  Each lane carries TPT
 // target points, so one LDS record read serves TPT pairs.
 // Partial block of tangle k: [S][6][cap_k] = far xyz (times k/4pi), near xyz.
+#ifndef NV_BS32_WAVES_PER_EU
+#define NV_BS32_WAVES_PER_EU 3
+#endif
 template <int TPT, int KB>
-__global__ __launch_bounds__(TB) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVES_PER_EU))) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
                                               int *queue, const Rec32 *__restrict__ rec,
-                                              const double *__restrict__ origin, const double *__restrict__ rec64,
+                                              const TileInfo *__restrict__ tiles, const double *__restrict__ rec64,
                                               Shifts sh, double kappa, float rn2, double *__restrict__ part) {
     __shared__ Rec32 tile[TS];
     __shared__ int slot;
     const int tid = threadIdx.x;
+    const float rnf = sqrtf(rn2);
     for (;;) {
         const int it = next_item(queue, &slot);
         if (it >= nitems) break;
@@ -213,13 +278,59 @@ __global__ __launch_bounds__(TB) void k_bs32c(DState s, Tangles tg, const Item *
             for (int q = tid; q < TS; q += TB) tile[q] = rec[(size_t)gt * TS + q];
             __syncthreads();
             if (!any) continue;
+            const TileInfo ti = tiles[gt];
+            if (ti.rad < 0.f) continue;     // no active source in this tile
             const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
             float tt[TPT][3];
+            bool far = true;
 #pragma unroll
             for (int q = 0; q < TPT; ++q) {
-                tt[q][0] = (float)((xi[q][0] - s0) - origin[gt * 3 + 0]);
-                tt[q][1] = (float)((xi[q][1] - s1) - origin[gt * 3 + 1]);
-                tt[q][2] = (float)((xi[q][2] - s2) - origin[gt * 3 + 2]);
+                tt[q][0] = (float)((xi[q][0] - s0) - ti.o[0]);
+                tt[q][1] = (float)((xi[q][1] - s1) - ti.o[1]);
+                tt[q][2] = (float)((xi[q][2] - s2) - ti.o[2]);
+                // every source of the tile is at least rn away from this
+                // target (with a relative margin far above FP32 rounding), and
+                // the target's behind point is not in the tile
+                const float c0 = tt[q][0] - ti.c[0], c1 = tt[q][1] - ti.c[1], c2 = tt[q][2] - ti.c[2];
+                const float dd = c0 * c0 + c1 * c1 + c2 * c2;
+                const float mag = fabsf(tt[q][0]) + fabsf(tt[q][1]) + fabsf(tt[q][2]) + ti.rad;
+                const float lim = (ti.rad + rnf) * 1.0001f + 1.0e-6f * mag;
+                const bool fq = dd > lim * lim && (unsigned)(bi[q] - gt * TS) >= (unsigned)TS;
+                far = far && (fq || !act[q]);
+            }
+            if (__all(far)) {
+                // far tile for the whole wave: no pair needs the near-field
+                // test or an exclusion, every term is FP32
+                for (int jb = 0; jb < TS; jb += KB) {
+                    float a[TPT][3];
+#pragma unroll
+                    for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
+#pragma unroll
+                    for (int jj = 0; jj < KB; ++jj) {
+                        const Rec32 r = tile[jb + jj];
+#pragma unroll
+                        for (int q = 0; q < TPT; ++q) {
+                            const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
+                            const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+                            const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
+                            const float r2sq = fmaf(2.0f, de, r1sq + r.c);
+                            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+                            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+                            const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
+                            const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
+                            a[q][0] = fmaf(fmaf(d1, r.e2, -(d2 * r.e1)), fac, a[q][0]);
+                            a[q][1] = fmaf(fmaf(d2, r.e0, -(d0 * r.e2)), fac, a[q][1]);
+                            a[q][2] = fmaf(fmaf(d0, r.e1, -(d1 * r.e0)), fac, a[q][2]);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < TPT; ++q) {
+                        kahan_add(f[q][0], k[q][0], a[q][0]);
+                        kahan_add(f[q][1], k[q][1], a[q][1]);
+                        kahan_add(f[q][2], k[q][2], a[q][2]);
+                    }
+                }
+                continue;
             }
             for (int jb = 0; jb < TS; jb += KB) {
                 float a[TPT][3];
@@ -361,18 +472,22 @@ hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int
     return hipGetLastError();
 }
 
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, double *origin) {
+size_t nvk_tileinfo_bytes() { return sizeof(TileInfo); }
+
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, void *tiles) {
     if (rows > 0)
-        hipLaunchKernelGGL(k_src32, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, (Rec32 *)rec, origin);
+        hipLaunchKernelGGL(k_src32, dim3((rows + TS - 1) / TS), dim3(TS), 0, st, s, rows, (Rec32 *)rec,
+                           (TileInfo *)tiles);
     return hipGetLastError();
 }
 
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
-                     const void *rec, const double *origin, const double *rec64, const Shifts *sh, double kappa,
+                     const void *rec, const void *tiles, const double *rec64, const Shifts *sh, double kappa,
                      float rn2, double *part, int tpt, int kb) {
     const Rec32 *r = (const Rec32 *)rec;
+    const TileInfo *ti = (const TileInfo *)tiles;
 #define NV_BS32(TPT_, KB_)                                                                                   \
-    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, origin, \
+    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, ti, \
                        rec64, *sh, kappa, rn2, part)
     if (tpt == 2) {
         if (kb == 8) NV_BS32(2, 8);

@@ -55,8 +55,8 @@ struct Ctx {
     DCtl *ctl = nullptr;             // device
     DCtl *ctl_h = nullptr;           // pinned host mirror
     DevBuf<int> cand, empty, lcand, lempty, tmp;
-    DevBuf<double> rec64, origin, part, pd;
-    DevBuf<char> rec32;
+    DevBuf<double> rec64, part, pd;
+    DevBuf<char> rec32, tinfo, sph;
     DevBuf<int> pj;
     // work lists of the last direct-sum and close-search launches
     DevBuf<Item> items, pitems;
@@ -151,7 +151,8 @@ int alloc_state(Ctx *c) {
     HIPCHK(c, c->tmp.reserve((size_t)nv_compact_tmp_ints(rows)));
     HIPCHK(c, c->rec64.reserve((size_t)8 * rows));
     HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
-    HIPCHK(c, c->origin.reserve((size_t)3 * (rows / 256 + 1)));
+    HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes() * (size_t)(rows / 256 + 1)));
+    HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 256 + 1)));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
 }
@@ -490,11 +491,11 @@ int velocity_impl(Ctx *c, int do_ab) {
         HIPCHK(c, nvk_src64(c->st, c->d, R, c->rec64.p));
         Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
         if (fp32) {
-            HIPCHK(c, nvk_src32(c->st, c->d, R, c->rec32.p, c->origin.p));
+            HIPCHK(c, nvk_src32(c->st, c->d, R, c->rec32.p, c->tinfo.p));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
             HIPCHK(c, hipEventRecord(c->e2, c->st));
-            HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->origin.p,
+            HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
                                 c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 1;
@@ -596,10 +597,10 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
         HIPCHK(c, nv_fill_int(st, c->queue.p, 0, 1));
         HIPCHK(c, nvk_src64(st, d, R, c->rec64.p));
         if (fp32) {
-            HIPCHK(c, nvk_src32(st, d, R, c->rec32.p, c->origin.p));
+            HIPCHK(c, nvk_src32(st, d, R, c->rec32.p, c->tinfo.p));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
-            HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->origin.p,
+            HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
                                 c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             mode = 1;
         } else {
@@ -622,7 +623,7 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
     HIPCHK(c, nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
     if (recon) {
         Tangles tp = tangles(c, c->d_pS.p, c->d_ppoff.p);
-        HIPCHK(c, nvk_pclose(st, d, tp, R, c->pitems.p, npitems, c->pd.p, c->pj.p));
+        HIPCHK(c, nvk_pclose(st, d, tp, R, c->pitems.p, npitems, c->sph.p, c->pd.p, c->pj.p));
         HIPCHK(c, nvk_precon_pre(st, d, c->phys, R, c->cand.p));
         HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
         HIPCHK(c, nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
@@ -826,7 +827,7 @@ int nv_destroy(void *h) {
     DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tn, &c->d_trecon, &c->d_tremove, &c->d_tnact, &c->d_row2t,
                          &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp, &c->pj, &c->d_S, &c->d_pS, &c->queue};
     for (auto *b : ib) b->release();
-    c->rec64.release(); c->origin.release(); c->part.release(); c->pd.release(); c->rec32.release();
+    c->rec64.release(); c->tinfo.release(); c->sph.release(); c->part.release(); c->pd.release(); c->rec32.release();
     c->items.release(); c->pitems.release(); c->d_poff.release(); c->d_ppoff.release();
     if (c->ev) (void)hipFree(c->ev);
     if (c->ctl) (void)hipFree(c->ctl);
@@ -947,7 +948,7 @@ int nv_pclose(void *h) {
     if (int rc = build_close_items(c, false, &nitems)) return rc;
     Tangles tg = tangles(c, c->d_pS.p, c->d_ppoff.p);
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, nitems, c->pd.p, c->pj.p));
+    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, nitems, c->sph.p, c->pd.p, c->pj.p));
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;

@@ -31,13 +31,14 @@ struct DCtl {
 hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec);
 hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
                     const double *rec, const Shifts *sh, double kappa, double *part);
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, double *origin);
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, void *tiles);
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
-                     const void *rec, const double *origin, const double *rec64, const Shifts *sh, double kappa,
+                     const void *rec, const void *tiles, const double *rec64, const Shifts *sh, double kappa,
                      float rn2, double *part, int tpt, int kb);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl);
 size_t nvk_rec32_bytes();
+size_t nvk_tileinfo_bytes();
 int nvk_occupancy_bs64();
 int nvk_occupancy_bs32c(int tpt, int kb);
 
@@ -59,5 +60,6 @@ hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const
                              nv_recon_event *ev, int ev_cap);
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
 hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl);
-hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, double *pd,
-                      int *pj);
+hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, void *spheres,
+                      double *pd, int *pj);
+size_t nvk_sphere_bytes();
