@@ -84,7 +84,7 @@ typedef struct {
     int32_t capacity;            /* initial device rows (0 = 2 x pcount) */
     int32_t targets_per_lane;    /* FP32C direct sum: points per lane, 1 or 2 (0 = tuned) */
     int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
-                                    adds, 1 (every term), 2, 4 or 8 (0 = default 8) */
+                                    adds, 1 (every term), 2, 4, 8 or 16 (0 = default 8) */
     int32_t step_launch;         /* nv_step: 0 = record one step's launches in a HIP graph and
                                     replay it every step (direct sum and LIA; the octree step
                                     launches eagerly), 1 = eager launches */
@@ -140,6 +140,11 @@ typedef struct {
     int64_t walk_launches;     /* tree velocity walk launches */
     double walk_ms_total;      /* summed duration of those launches */
     int64_t walk_nodes_total;  /* summed node records read by those walks */
+    int32_t targets_per_lane;  /* direct-sum variant in use: points per lane */
+    int32_t kahan_block;       /* FP32C terms between compensated adds */
+    int32_t items_per_cu;      /* direct-sum work items per CU (sets the chunk size) */
+    int32_t tuned;             /* 0 defaults, 1 timed by nv_autotune, 2 read from its cache */
+    double tune_ms;            /* direct-sum launch time of the chosen variant when tuned */
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */
@@ -189,6 +194,16 @@ int nv_get_recon_events(void *ctx, nv_recon_event *buf, int32_t cap, int32_t *n)
  * src/tree.f90:96-102, plus internal nodes and moments). */
 int nv_get_tree(void *ctx, nv_tree_node *buf, int32_t cap, int32_t *n);
 int nv_get_stats(void *ctx, nv_stats *stats);
+
+/* Autotuner (no reference counterpart).  Times the direct-sum variants on
+ * the uploaded state -- points per lane (targets_per_lane), Kahan block and
+ * work items per CU, each left at 0 in nv_params -- one coordinate at a
+ * time, and keeps the fastest.  The state is not modified (only scratch
+ * buffers are written).  With a cache_path, a choice is looked up first
+ * under (GPU architecture, precision, tangle count, log2 of the total and
+ * of the largest tangle's points) and a new one is appended to the file;
+ * NULL tunes without a cache.  Call it between steps. */
+int nv_autotune(void *ctx, const char *cache_path);
 
 #ifdef __cplusplus
 }

@@ -448,17 +448,30 @@ int nvk_occupancy_bs64() {
     return b;
 }
 
-int nvk_occupancy_bs32c(int tpt, int kb) {
+template <int TPT, int KB>
+int occ_bs32c() {
     int b = 0;
-    hipError_t e;
-    if (tpt == 2)
-        e = kb == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<2, 8>, TB, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<2, 1>, TB, 0);
-    else
-        e = kb == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<1, 8>, TB, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<1, 1>, TB, 0);
-    if (e != hipSuccess || b < 1) b = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<TPT, KB>, TB, 0) != hipSuccess || b < 1) b = 2;
     return b;
+}
+
+int nvk_occupancy_bs32c(int tpt, int kb) {
+    if (tpt == 2) {
+        switch (kb) {
+            case 1: return occ_bs32c<2, 1>();
+            case 2: return occ_bs32c<2, 2>();
+            case 4: return occ_bs32c<2, 4>();
+            case 16: return occ_bs32c<2, 16>();
+            default: return occ_bs32c<2, 8>();
+        }
+    }
+    switch (kb) {
+        case 1: return occ_bs32c<1, 1>();
+        case 2: return occ_bs32c<1, 2>();
+        case 4: return occ_bs32c<1, 4>();
+        case 16: return occ_bs32c<1, 16>();
+        default: return occ_bs32c<1, 8>();
+    }
 }
 
 hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec) {
@@ -491,11 +504,13 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
                        rec64, *sh, kappa, rn2, part)
     if (tpt == 2) {
         if (kb == 8) NV_BS32(2, 8);
+        else if (kb == 16) NV_BS32(2, 16);
         else if (kb == 4) NV_BS32(2, 4);
         else if (kb == 2) NV_BS32(2, 2);
         else NV_BS32(2, 1);
     } else {
         if (kb == 8) NV_BS32(1, 8);
+        else if (kb == 16) NV_BS32(1, 16);
         else if (kb == 4) NV_BS32(1, 4);
         else if (kb == 2) NV_BS32(1, 2);
         else NV_BS32(1, 1);

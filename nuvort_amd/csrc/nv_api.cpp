@@ -72,6 +72,7 @@ struct Ctx {
     TreeBuf *tree = nullptr;
     bool tree_valid = false;
     int tpt = 2, kb = 8;             // FP32C kernel variant: targets per lane, compensation block
+    int ipc = 16;                    // direct-sum work items per CU
     int occ64 = 4, occ32 = 4;        // resident workgroups per CU of the persistent kernels
     // recorded steps, [x/xn parity][reconnection phase on], valid while
     // gen == layout_gen (a new layout moves every buffer)
@@ -409,7 +410,7 @@ int build_items(Ctx *c, int tpt, bool exact, bool cap_rows, int *nitems_out) {
         U[k] = c->sh.n * ((c->tn[k] + 255) / 256);
         work += (long)ntb[k] * U[k];
     }
-    const long want = 16L * c->ncu;
+    const long want = (long)c->ipc * c->ncu;
     const long utarget = std::max<long>(4, std::min<long>(256, work / std::max<long>(1, want)));
     std::vector<int> S(K);
     std::vector<long long> poff(K);
@@ -794,7 +795,8 @@ int nv_create(const nv_params *params, void **out) {
     c->tpt = p.targets_per_lane == 1 ? 1 : 2;
     // default: compensated add every 8 terms (measured: same error as every
     // term on the 16k-point ring tangle, 1.27x faster)
-    c->kb = (p.kahan_block == 1 || p.kahan_block == 2 || p.kahan_block == 4 || p.kahan_block == 8) ? p.kahan_block : 8;
+    c->kb = (p.kahan_block == 1 || p.kahan_block == 2 || p.kahan_block == 4 || p.kahan_block == 8 ||
+             p.kahan_block == 16) ? p.kahan_block : 8;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
@@ -1096,7 +1098,136 @@ int nv_get_tree(void *h, nv_tree_node *buf, int32_t cap, int32_t *n) {
 int nv_get_stats(void *h, nv_stats *s) {
     Ctx *c = (Ctx *)h;
     if (!c || !s) return NV_ERR_ARG;
+    c->stats.targets_per_lane = c->prm.precision == NV_PREC_FP32C ? c->tpt : 1;
+    c->stats.kahan_block = c->prm.precision == NV_PREC_FP32C ? c->kb : 0;
+    c->stats.items_per_cu = c->ipc;
     *s = c->stats;
+    return NV_OK;
+}
+
+int nv_autotune(void *h, const char *cache_path) {
+    Ctx *c = (Ctx *)h;
+    if (!c) return NV_ERR_ARG;
+    if (c->prm.velocity != NV_VEL_BS) return NV_OK;   // only the direct sum has variants
+    if (c->K == 0) return fail(c, NV_ERR_STATE, "nv_autotune: no state uploaded");
+    if (cache_path && !cache_path[0]) cache_path = nullptr;   // "" = no cache
+    const bool fp32 = c->prm.precision == NV_PREC_FP32C;
+    long total = 0, largest = 0;
+    for (int k = 0; k < c->K; ++k) {
+        total += c->tn[k];
+        largest = std::max<long>(largest, c->tn[k]);
+    }
+    auto lg = [](long v) { int b = 0; while (v > 1) { v >>= 1; ++b; } return b; };
+    int dev = 0;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipGetDeviceProperties(&prop, dev));
+    std::string arch(prop.gcnArchName);
+    arch = arch.substr(0, arch.find(':'));
+    char key[160];
+    std::snprintf(key, sizeof key, "%s|%s|K=%d|n=%d|m=%d", arch.c_str(), fp32 ? "fp32c" : "fp64",
+                  c->K == 1 ? 1 : 1 << lg(c->K), lg(total), lg(largest));
+    auto apply = [&](int tpt, int kb, int ipc) {
+        c->tpt = tpt;
+        c->kb = kb;
+        c->ipc = ipc;
+        c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb);
+        ++c->layout_gen;   // recorded steps hold the old variant's work lists
+    };
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            char line[512];
+            int ht = 0, hk = 0, hi = 0;
+            double hm = 0;
+            bool hit = false;
+            while (std::fgets(line, sizeof line, f)) {
+                char k2[160];
+                int t2, b2, i2;
+                double m2;
+                if (std::sscanf(line, "%159s tpt=%d kb=%d ipc=%d ms=%lf", k2, &t2, &b2, &i2, &m2) == 5 &&
+                    std::strcmp(k2, key) == 0) {
+                    ht = t2; hk = b2; hi = i2; hm = m2;
+                    hit = true;   // the last matching line wins
+                }
+            }
+            std::fclose(f);
+            if (hit && (ht == 1 || ht == 2) && (hk == 1 || hk == 2 || hk == 4 || hk == 8 || hk == 16) && hi >= 1 &&
+                hi <= 256) {
+                apply(ht, hk, hi);
+                c->stats.tuned = 2;
+                c->stats.tune_ms = hm;
+                return NV_OK;
+            }
+        }
+    }
+    // time one variant: warm-up launches, then reps timed launches
+    int warm = 1, reps = 3;
+    auto time_variant = [&](int tpt, int kb, int ipc, float *ms_out) -> int {
+        apply(tpt, kb, ipc);
+        int nitems = 0;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, false, &nitems)) return rc;
+        HIPCHK(c, c->queue.reserve(1));
+        HIPCHK(c, nvk_src64(c->st, c->d, c->rows, c->rec64.p));
+        if (fp32) HIPCHK(c, nvk_src32(c->st, c->d, c->rows, c->rec32.p, c->tinfo.p));
+        Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
+        const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+        float tot = 0.f;
+        for (int r = 0; r < warm + reps; ++r) {
+            HIPCHK(c, nv_fill_int(c->st, c->queue.p, 0, 1));
+            HIPCHK(c, hipEventRecord(c->e2, c->st));
+            if (fp32) {
+                const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
+                HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
+                                    c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
+            } else {
+                const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));
+                HIPCHK(c, nvk_bs64(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec64.p, &c->sh,
+                                   c->phys.kappa, c->part.p));
+            }
+            HIPCHK(c, hipEventRecord(c->e3, c->st));
+            HIPCHK(c, hipEventSynchronize(c->e3));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->e2, c->e3));
+            if (r >= warm) tot += ms;
+        }
+        *ms_out = tot / reps;
+        return NV_OK;
+    };
+    const bool free_tpt = fp32 && c->prm.targets_per_lane == 0;
+    const bool free_kb = fp32 && c->prm.kahan_block == 0;
+    int bt = fp32 ? c->tpt : 1, bk = c->kb, bi = c->ipc;
+    float best = 0.f;
+    if (int rc = time_variant(bt, bk, bi, &best)) return rc;
+    // keep the whole search to a few seconds on large batches
+    if (best > 200.f) { warm = 0; reps = 1; }
+    else if (best > 20.f) reps = 2;
+    auto consider = [&](int t, int k, int i) -> int {
+        if (t == bt && k == bk && i == bi) return NV_OK;
+        float ms = 0.f;
+        if (int rc = time_variant(t, k, i, &ms)) return rc;
+        if (ms < best) {
+            best = ms;
+            bt = t; bk = k; bi = i;
+        }
+        return NV_OK;
+    };
+    for (int i : {8, 16, 32})
+        if (int rc = consider(bt, bk, i)) return rc;
+    if (free_tpt)
+        for (int t : {1, 2})
+            if (int rc = consider(t, bk, bi)) return rc;
+    if (free_kb)
+        for (int k : {4, 8, 16})
+            if (int rc = consider(bt, k, bi)) return rc;
+    apply(bt, bk, bi);
+    c->stats.tuned = 1;
+    c->stats.tune_ms = best;
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s tpt=%d kb=%d ipc=%d ms=%.4f\n", key, bt, bk, bi, best);
+            std::fclose(f);
+        }
+    }
     return NV_OK;
 }
 

@@ -204,6 +204,13 @@ class FilamentSolver:
                        ("pcount", "<i4"), ("leaf_infront", "<i4")])
         return np.frombuffer(bytes(buf), dt, n.value).copy()
 
+    def autotune(self, cache_path: str | None = None) -> dict:
+        """Pick the direct-sum variant by timing it on the uploaded state
+        (cached per GPU architecture and size class when a path is given)."""
+        self._call("nv_autotune", cache_path.encode() if cache_path else None)
+        st = self.stats()
+        return {k: st[k] for k in ("targets_per_lane", "kahan_block", "items_per_cu", "tuned", "tune_ms")}
+
     def stats(self) -> dict:
         s = _L.NvStats()
         self._call("nv_get_stats", C.byref(s))

@@ -135,6 +135,12 @@ module nuvort_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
     end function
+    !> cache_path: a c_null_char-terminated file name; c_null_char alone tunes without a cache
+    integer(c_int) function nv_autotune(ctx, cache_path) bind(C, name='nv_autotune')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: cache_path
+    end function
   end interface
 
 contains

@@ -57,14 +57,16 @@ class NvStats(C.Structure):
                 ("ms_velocity", C.c_double), ("ms_tree_build", C.c_double), ("ms_close", C.c_double),
                 ("ms_topology", C.c_double), ("ms_step", C.c_double), ("point_updates", C.c_int64),
                 ("pair_launches", C.c_int64), ("pair_ms_total", C.c_double), ("pair_evals_total", C.c_int64),
-                ("walk_launches", C.c_int64), ("walk_ms_total", C.c_double), ("walk_nodes_total", C.c_int64)]
+                ("walk_launches", C.c_int64), ("walk_ms_total", C.c_double), ("walk_nodes_total", C.c_int64),
+                ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32), ("items_per_cu", C.c_int32),
+                ("tuned", C.c_int32), ("tune_ms", C.c_double)]
 
 
 EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_state_batch",
            "nv_get_state_batch", "nv_set_time", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
            "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
-           "nv_get_recon_events", "nv_get_tree", "nv_get_stats")
+           "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune")
 
 _lib = None
 
@@ -93,6 +95,7 @@ def load(path: str = LIB_PATH):
         "nv_get_recon_events": [vp, C.POINTER(NvReconEvent), C.c_int32, C.POINTER(C.c_int32)],
         "nv_get_tree": [vp, C.POINTER(NvTreeNode), C.c_int32, C.POINTER(C.c_int32)],
         "nv_get_stats": [vp, C.POINTER(NvStats)],
+        "nv_autotune": [vp, C.c_char_p],
     }
     for name in EXPORTS:
         fn = getattr(L, name)
