@@ -380,8 +380,8 @@ __global__ void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, 
 }
 
 // ------------------------------------------------------ brute close search
-// Bounding sphere of the active points of each 256-row tile (raw
-// coordinates, FP64); rad < 0 marks a tile without active points.  The
+// Bounding sphere of the active points of each 64-row segment (raw
+// coordinates, FP64); rad < 0 marks a segment without active points.  The
 // radius is rounded up so that it bounds the exact distances.
 struct Sphere64 {
     double c[3], rad;
@@ -396,8 +396,8 @@ __device__ __forceinline__ double wave_max64(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_tile_spheres(DState s, int rows, Sphere64 *__restrict__ sp) {
-    __shared__ double red[4][6];
+// one wave per segment (rows is a multiple of 256)
+__global__ __launch_bounds__(256) void k_seg_spheres(DState s, int rows, Sphere64 *__restrict__ sp) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     const bool act = r < rows && s.infront[r] != 0;
     double lo[3], hi[3];
@@ -407,18 +407,7 @@ __global__ __launch_bounds__(256) void k_tile_spheres(DState s, int rows, Sphere
         lo[k] = wave_min64(act ? v : 1.0e300);
         hi[k] = wave_max64(act ? v : -1.0e300);
     }
-    const int w = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { red[w][k] = lo[k]; red[w][3 + k] = hi[k]; }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int q = 1; q < 4; ++q)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], red[q][k]);
-            hi[k] = fmax(hi[k], red[q][3 + k]);
-        }
+    if ((threadIdx.x & 63) != 0) return;
     Sphere64 o;
     if (lo[0] > hi[0]) {
         o.c[0] = o.c[1] = o.c[2] = 0.0;
@@ -434,7 +423,7 @@ __global__ __launch_bounds__(256) void k_tile_spheres(DState s, int rows, Sphere
         const double rad = 0.5 * sqrt(d2);
         o.rad = rad * (1.0 + 1.0e-12) + 1.0e-15 * (m + rad);
     }
-    sp[blockIdx.x] = o;
+    sp[r / 64] = o;
 }
 
 // src/line.f90:135-160 pclose: the first (lowest) j with the smallest
@@ -447,11 +436,12 @@ __global__ __launch_bounds__(256) void k_tile_spheres(DState s, int rows, Sphere
 // Pruning, exact: every target starts from the distance to a valid
 // candidate, infront(infront(i)), as an upper bound just above it (so that
 // candidate, and any point at least as close, is still accepted by the
-// strict comparison; a chunk holding none of them reports no candidate), and
-// a tile is skipped when its bounding sphere lies beyond the current best of
-// every target of the workgroup.
+// strict comparison; a chunk holding none of them reports no candidate).  A
+// wave scans a 64-row source segment only if the segment's bounding sphere
+// comes within the current best of one of its targets; a 256-row tile is
+// staged in LDS only if some wave needs one of its segments.
 __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item *__restrict__ items,
-                                                const Sphere64 *__restrict__ sph, double *__restrict__ pd,
+                                                const Sphere64 *__restrict__ seg, double *__restrict__ pd,
                                                 int *__restrict__ pj) {
     constexpr int T = 256;
     __shared__ double tile[T * 4];
@@ -479,14 +469,20 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
     const int t0 = (int)((long)ntiles * I.c / S), t1 = (int)((long)ntiles * (I.c + 1) / S);
     int bj = -1;
     for (int t = t0; t < t1; ++t) {
-        const Sphere64 sp = sph[base / T + t];
-        bool skip = true;
-        if (act && sp.rad >= 0.0) {
-            const double c0 = x0 - sp.c[0], c1 = x1 - sp.c[1], c2 = x2 - sp.c[2];
-            const double dc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-            skip = (dc - sp.rad) * (1.0 - 1.0e-12) > best;
+        // segments of this tile the wave needs (wave-uniform bits)
+        unsigned need = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const Sphere64 sp = seg[(base / T + t) * 4 + q];
+            bool lane = false;
+            if (act && sp.rad >= 0.0) {
+                const double c0 = x0 - sp.c[0], c1 = x1 - sp.c[1], c2 = x2 - sp.c[2];
+                const double dc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+                lane = !((dc - sp.rad) * (1.0 - 1.0e-12) > best);
+            }
+            if (__any(lane)) need |= 1u << q;
         }
-        if (__syncthreads_and(skip)) continue;
+        if (!__syncthreads_or(need != 0)) continue;
         for (int q = tid; q < T; q += 256) {
             const int jl = t * T + q, j = base + jl;
             const bool v = jl < n && s.infront[j] != 0;
@@ -496,13 +492,15 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
             tile[q * 4 + 3] = v ? 1.0 : 0.0;
         }
         __syncthreads();
-        if (!skip) {
-            for (int jj = 0; jj < T; ++jj) {
-                const double *q = &tile[jj * 4];
-                if (q[3] == 0.0) continue;
+        if (!act) continue;
+        for (int q = 0; q < 4; ++q) {
+            if (!(need & (1u << q))) continue;
+            for (int jj = q * 64; jj < q * 64 + 64; ++jj) {
+                const double *r = &tile[jj * 4];
+                if (r[3] == 0.0) continue;
                 const int j = base + t * T + jj;
                 if (j == i || j == fi || j == bi) continue;
-                const double d0 = x0 - q[0], d1 = x1 - q[1], d2 = x2 - q[2];
+                const double d0 = x0 - r[0], d1 = x1 - r[1], d2 = x2 - r[2];
                 const double dd = d0 * d0 + d1 * d1 + d2 * d2;
                 if (dd < bound) {
                     const double dist = sqrt(dd);
@@ -539,6 +537,34 @@ __global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, c
     }
     s.closestd[i] = best;
     if (bj >= 0) s.closest[i] = bj + 1;
+}
+
+// ------------------------------------------------------------ relayout
+// Copy every row of the old layout to its tangle's new range; links (1-based
+// global rows, 0 = none) move with their tangle.
+__global__ void k_relayout(DState o, DState nu, const int *__restrict__ row2t, const int *__restrict__ base_old,
+                           const int *__restrict__ base_new, int rows_old) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows_old) return;
+    const int k = row2t[r];
+    const int shift = base_new[k] - base_old[k];
+    const int q = r + shift;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        nu.x[c * nu.cap + q] = o.x[c * o.cap + r];
+        nu.u[c * nu.cap + q] = o.u[c * o.cap + r];
+        nu.u1[c * nu.cap + q] = o.u1[c * o.cap + r];
+        nu.u2[c * nu.cap + q] = o.u2[c * o.cap + r];
+        nu.usup[c * nu.cap + q] = o.usup[c * o.cap + r];
+        nu.umf[c * nu.cap + q] = o.umf[c * o.cap + r];
+        nu.gi[c * nu.cap + q] = o.gi[c * o.cap + r];
+        nu.gb[c * nu.cap + q] = o.gb[c * o.cap + r];
+    }
+    nu.closestd[q] = o.closestd[r];
+    const int f = o.infront[r], b = o.behind[r], cl = o.closest[r];
+    nu.infront[q] = f > 0 ? f + shift : 0;
+    nu.behind[q] = b > 0 ? b + shift : 0;
+    nu.closest[q] = cl > 0 ? cl + shift : 0;
 }
 
 // ------------------------------------------------------------ clock
@@ -605,7 +631,7 @@ size_t nvk_sphere_bytes() { return sizeof(Sphere64); }
 hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, void *spheres,
                       double *pd, int *pj) {
     Sphere64 *sp = (Sphere64 *)spheres;
-    if (rows > 0) hipLaunchKernelGGL(k_tile_spheres, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, sp);
+    if (rows > 0) hipLaunchKernelGGL(k_seg_spheres, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, sp);
     if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, sp, pd, pj);
     if (rows > 0) hipLaunchKernelGGL(k_pclose_fin, GRID(rows), 0, st, s, tg, rows, pd, pj);
     return hipGetLastError();
@@ -621,5 +647,12 @@ hipError_t nvk_clock_begin_step(hipStream_t st, DCtl *ctl) {
 }
 hipError_t nvk_clock_end_step(hipStream_t st, DCtl *ctl, double dt) {
     hipLaunchKernelGGL(k_clock_end, dim3(1), dim3(1), 0, st, ctl, dt);
+    return hipGetLastError();
+}
+
+hipError_t nvk_relayout(hipStream_t st, DState o, DState nu, const int *row2t, const int *base_old,
+                        const int *base_new, int rows_old) {
+    if (rows_old > 0)
+        hipLaunchKernelGGL(k_relayout, GRID(rows_old), 0, st, o, nu, row2t, base_old, base_new, rows_old);
     return hipGetLastError();
 }

@@ -114,37 +114,46 @@ Tangles tangles(Ctx *c, const int *S, const long long *poff) {
     return g;
 }
 
-void free_state(Ctx *c) {
-    double **dp[] = {&c->d.x, &c->d.xn, &c->d.u, &c->d.u1, &c->d.u2, &c->d.usup, &c->d.umf, &c->d.gi, &c->d.gb,
-                     &c->d.closestd};
+void free_dstate(DState &d) {
+    double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb, &d.closestd};
     for (double **p : dp)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    int **ip[] = {&c->d.infront, &c->d.behind, &c->d.closest};
+    int **ip[] = {&d.infront, &d.behind, &d.closest};
     for (int **p : ip)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
 
-// allocate the state arrays for c->rows rows (contents zeroed)
-int alloc_state(Ctx *c) {
-    free_state(c);
-    const int rows = c->rows;
+void free_state(Ctx *c) { free_dstate(c->d); }
+
+// state arrays for `rows` rows, zeroed
+hipError_t alloc_dstate(hipStream_t st, int rows, DState *out) {
     DState d{};
     d.cap = rows;
+    hipError_t e = hipSuccess;
     double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb};
     for (double **p : dp) {
-        HIPCHK(c, hipMalloc(p, sizeof(double) * 3 * (size_t)rows));
-        HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(double) * 3 * (size_t)rows, c->st));
+        if ((e = hipMalloc(p, sizeof(double) * 3 * (size_t)rows)) != hipSuccess) break;
+        if ((e = hipMemsetAsync(*p, 0, sizeof(double) * 3 * (size_t)rows, st)) != hipSuccess) break;
     }
-    HIPCHK(c, hipMalloc(&d.closestd, sizeof(double) * rows));
-    HIPCHK(c, hipMemsetAsync(d.closestd, 0, sizeof(double) * rows, c->st));
+    if (e == hipSuccess && (e = hipMalloc(&d.closestd, sizeof(double) * rows)) == hipSuccess)
+        e = hipMemsetAsync(d.closestd, 0, sizeof(double) * rows, st);
     int **ip[] = {&d.infront, &d.behind, &d.closest};
     for (int **p : ip) {
-        HIPCHK(c, hipMalloc(p, sizeof(int) * (size_t)rows));
-        HIPCHK(c, hipMemsetAsync(*p, 0, sizeof(int) * (size_t)rows, c->st));
+        if (e != hipSuccess) break;
+        if ((e = hipMalloc(p, sizeof(int) * (size_t)rows)) != hipSuccess) break;
+        e = hipMemsetAsync(*p, 0, sizeof(int) * (size_t)rows, st);
     }
-    c->d = d;
-    c->xa = d.x;
-    ++c->layout_gen;
+    if (e != hipSuccess) {
+        free_dstate(d);
+        return e;
+    }
+    *out = d;
+    return hipSuccess;
+}
+
+// row-sized scratch for c->rows rows
+int alloc_scratch(Ctx *c) {
+    const int rows = c->rows;
     HIPCHK(c, c->cand.reserve(rows));
     HIPCHK(c, c->empty.reserve(rows));
     HIPCHK(c, c->lcand.reserve(rows));
@@ -153,7 +162,19 @@ int alloc_state(Ctx *c) {
     HIPCHK(c, c->rec64.reserve((size_t)8 * rows));
     HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
     HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes() * (size_t)(rows / 256 + 1)));
-    HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 256 + 1)));
+    HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 64 + 4)));
+    return NV_OK;
+}
+
+// allocate the state arrays for c->rows rows (contents zeroed)
+int alloc_state(Ctx *c) {
+    free_state(c);
+    DState d{};
+    HIPCHK(c, alloc_dstate(c->st, c->rows, &d));
+    c->d = d;
+    c->xa = d.x;
+    ++c->layout_gen;
+    if (int rc = alloc_scratch(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
 }
@@ -352,43 +373,46 @@ int get_batch(Ctx *c, int32_t *pcounts, nv_state *hs) {
 }
 
 // Give every tangle room for another step of insertions (pinsert inserts at
-// most once per visited point, so 2 x pcount rows always suffice): move the
-// state to a wider layout through the host when one is short.
+// most once per visited point, so 2 x pcount rows always suffice): a tangle
+// that is short gets 3 x pcount + 1024 rows, and the state moves to the new
+// layout on the device (rows copied tangle by tangle, links remapped).
 int ensure_room(Ctx *c) {
     bool need = false;
     for (int k = 0; k < c->K; ++k)
         if (c->tcap[k] < 2 * c->tn[k] + 256) need = true;
     if (!need) return NV_OK;
-    long total = 0;
-    for (int k = 0; k < c->K; ++k) total += c->tn[k];
-    std::vector<double> v((size_t)8 * 3 * total + 24), cd(total + 1);
-    std::vector<int32_t> iv((size_t)3 * total + 3);
-    nv_state hs{};
-    hs.capacity = (int32_t)total;
-    double **dp[] = {&hs.x, &hs.u, &hs.u1, &hs.u2, &hs.u_sup, &hs.u_mf, &hs.ghosti, &hs.ghostb};
-    for (int q = 0; q < 8; ++q) *dp[q] = v.data() + (size_t)q * 3 * total;
-    hs.infront = iv.data();
-    hs.behind = iv.data() + total;
-    hs.closest = iv.data() + 2 * total;
-    hs.closestd = cd.data();
-    std::vector<int32_t> pc(c->K);
-    if (int rc = get_batch(c, pc.data(), &hs)) return rc;
-    std::vector<int> caps(c->K);
-    for (int k = 0; k < c->K; ++k) caps[k] = round256(3L * c->tn[k] + 1024);
-    const std::vector<int> rec = c->trecon, rem = c->tremove;
-    const double t = c->t;
-    const int it = c->itime;
-    // the control block (event count, device clock, running totals)
-    // survives the move
-    DCtl saved;
-    HIPCHK(c, hipMemcpy(&saved, c->ctl, sizeof(DCtl), hipMemcpyDeviceToHost));
-    if (int rc = set_batch(c, c->K, pc.data(), &hs, &caps)) return rc;
-    c->trecon = rec;
-    c->tremove = rem;
-    c->t = t;
-    c->itime = it;
-    *c->ctl_h = saved;
-    HIPCHK(c, hipMemcpy(c->ctl, &saved, sizeof(DCtl), hipMemcpyHostToDevice));
+    const int K = c->K;
+    std::vector<int> nbase(K), ncap(K);
+    long rows = 0;
+    for (int k = 0; k < K; ++k) {
+        ncap[k] = c->tcap[k] < 2 * c->tn[k] + 256 ? round256(3L * c->tn[k] + 1024) : c->tcap[k];
+        nbase[k] = (int)rows;
+        rows += ncap[k];
+    }
+    if (rows > 0x7fffffffL / 8) return fail(c, NV_ERR_CAPACITY, "too many rows");
+    DState nd{};
+    HIPCHK(c, alloc_dstate(c->st, (int)rows, &nd));
+    DevBuf<int> d_nbase;
+    if (d_nbase.reserve(K) != hipSuccess) {
+        free_dstate(nd);
+        return fail(c, NV_ERR_HIP, "relayout: hipMalloc failed");
+    }
+    hipError_t e = hipMemcpy(d_nbase.p, nbase.data(), sizeof(int) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = nvk_relayout(c->st, c->d, nd, c->d_row2t.p, c->d_tbase.p, d_nbase.p, c->rows);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    d_nbase.release();
+    if (e != hipSuccess) {
+        free_dstate(nd);
+        return fail(c, NV_ERR_HIP, std::string("relayout: ") + hipGetErrorString(e));
+    }
+    free_state(c);
+    c->d = nd;
+    c->xa = nd.x;
+    c->rows = (int)rows;
+    c->tbase = nbase;
+    c->tcap = ncap;
+    ++c->layout_gen;
+    if (int rc = alloc_scratch(c)) return rc;
     return upload_tables(c);
 }
 

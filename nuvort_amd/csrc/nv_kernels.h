@@ -42,6 +42,10 @@ size_t nvk_tileinfo_bytes();
 int nvk_occupancy_bs64();
 int nvk_occupancy_bs32c(int tpt, int kb);
 
+// move the state to a wider layout (k_topology.hip)
+hipError_t nvk_relayout(hipStream_t st, DState o, DState nu, const int *row2t, const int *base_old,
+                        const int *base_new, int rows_old);
+
 // device clock (k_topology.hip): the step's itime and t live in DCtl so that a
 // captured step graph reads them at replay time
 hipError_t nvk_clock_set(hipStream_t st, DCtl *ctl, double t, int itime);
