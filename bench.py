@@ -154,13 +154,14 @@ def ensemble_run(p, sts, precision, steps):
     return u, sec, pu, tune
 
 
-def velocity_once(p, st, precision):
+def velocity_once(p, st, precision, tune=True):
     from nuvort_amd.engine import FilamentSolver
     q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
     q.hip_precision = precision
     s = FilamentSolver(q)
     s.set_state(st)
-    s.autotune(TUNE_CACHE)
+    if tune:
+        s.autotune(TUNE_CACHE)
     s.ghostp()
     s.velocity()
     g = s.get_state()
@@ -186,6 +187,72 @@ def cpu_baseline_parallel(st, p, sample: int, threads: int):
     return t1 - t0, np.arange(sample), u
 
 
+def dense(args):
+    """800 rings, radii U[0.005, 0.04] cm, delta 2.5e-4 cm, periodic 0.1 cm
+    box (seed 1237): 466,977 points.  Relaxed 50 steps with the FP32C step
+    (dt 1.9e-6 s, below the 1.95e-6 s bound of src/initial.f90:169-180,
+    reconnection and re-meshing every step), then one velocity pass each in
+    FP32C and FP64 on the relaxed state (every point), and the oracle's FP64
+    velocity on a sample of targets."""
+    from nuvort_amd import tangles
+    from nuvort_amd.engine import FilamentSolver
+    from nuvort_amd.runfile import RunParams
+    st = tangles.random_rings(1237, 800, 0.005, 0.04, 0.1, 2.5e-4)
+    p = RunParams(dt=1.9e-6, delta=2.5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1), velocity="BS",
+                  hip_precision="fp32c", recon_shots=1)
+    log("dense: %d points" % st.active)
+    s = FilamentSolver(p)
+    s.set_state(st)
+    tune = s.autotune(TUNE_CACHE)
+    log("tuned: %s" % tune)
+    blocks, t_all, pu_all = [], 0.0, 0
+    for b in range(5):
+        s0 = s.stats()
+        t0 = time.perf_counter()
+        s.step(10)
+        dtb = time.perf_counter() - t0
+        pu = s.stats()["point_updates"] - s0["point_updates"]
+        blocks.append(dtb / 10 * 1e3)
+        t_all += dtb
+        pu_all += pu
+        log("relax steps %d-%d: %.1f ms/step" % (10 * b + 1, 10 * b + 10, dtb / 10 * 1e3))
+    relaxed = s.get_state()
+    ev = len(s.recon_events())
+    s.close()
+    n = relaxed.active
+    g32 = velocity_once(p, relaxed, "fp32c")
+    log("fp32c velocity done")
+    log("fp64 velocity (about 30 s at this size)")
+    g64 = velocity_once(p, relaxed, "fp64", tune=False)
+    log("fp64 velocity done")
+    act = g64.infront != 0
+    acc = {"fp32c_vs_fp64_gpu": rel_err_stats(g64.u_sup[act], g32.u_sup[act])}
+    t_k, e_k, _ = kernel_timing(p, relaxed, 3)
+    sample = 4096
+    log("oracle sample: %d targets on %d threads" % (sample, args.cpu_threads))
+    secs, idx, ucpu = cpu_baseline_parallel(relaxed, p, sample, args.cpu_threads)
+    acc["fp32c_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g32.u[idx])
+    acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g64.u[idx])
+    line = {
+        "metric": METRIC, "value": pu_all / t_all, "unit": "point-updates/s", "n_gpus": 1, "steps": 50,
+        "warmup": 0, "ms_per_step": t_all / 50 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
+        "config": {"workload": "dense tangle: 800 random rings, radii U[0.005,0.04] cm, periodic 0.1 cm box, "
+                               "delta 2.5e-4 cm, direct Biot-Savart with 26 images, FP32C, relaxed 50 steps",
+                   "points_initial": st.active, "points_relaxed": n, "seed": 1237, "dt_s": 1.9e-6,
+                   "reconnections_during_relaxation": ev, "parallelism": "1 GPU"},
+        "block_ms_per_step": blocks, "tuning": tune,
+        "accuracy": acc, "max_rel_velocity_err_vs_fp64": acc["fp32c_vs_fp64_gpu"]["max_rel"],
+        "kernel": {"name": "k_bs32c", "ms": t_k * 1e3, "pair_evals": int(e_k),
+                   "achieved_tflops": FLOP_PER_PAIR * e_k / t_k / 1e12,
+                   "frac": FLOP_PER_PAIR * e_k / t_k / 1e12 / PEAK_FP32},
+        "oracle_sample": {"targets": sample, "seconds": secs, "threads": args.cpu_threads},
+    }
+    print(json.dumps(line), flush=True)
+    if os.path.exists(TUNE_CACHE):
+        os.remove(TUNE_CACHE)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +262,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and tree comparison runs")
+    ap.add_argument("--workload", choices=("rings64", "dense460k"), default="rings64",
+                    help="rings64: the headline step (default); dense460k: the dense 800-ring tangle "
+                         "(467k points), relaxed 50 steps, FP32C error statistics against FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "for scaling HBM byte counters) and exit")
@@ -217,6 +287,9 @@ def main():
 
     from nuvort_amd import lib as nvlib
     nvlib.load()
+    if args.workload == "dense460k":
+        dense(args)
+        return
     st, p = workload()
     n = st.active
     log("workload: %d points" % n)

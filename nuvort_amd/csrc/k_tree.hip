@@ -81,64 +81,6 @@ __global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, 
     keys[r] = key;
 }
 
-// ------------------------------------------------------------ radix sort
-constexpr int RS_BT = 256;
-constexpr int RS_IPT = 16;
-constexpr int RS_TILE = RS_BT * RS_IPT;
-
-__global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int *hist, int nblk) {
-    __shared__ int h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
-    for (int k = 0; k < RS_IPT; ++k) {
-        int q = base + k * RS_BT + threadIdx.x;
-        if (q < n) atomicAdd(&h[(keys[q] >> shift) & 0xFF], 1);
-    }
-    __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-__global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, int shift, const int *hist,
-                             int nblk, unsigned long long *okeys, int *ovals) {
-    __shared__ int running[256];
-    __shared__ int wcnt[RS_BT / 64][256];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    running[tid] = hist[tid * nblk + blockIdx.x];
-    const int base = blockIdx.x * RS_TILE;
-    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    for (int k = 0; k < RS_IPT; ++k) {
-        for (int w = 0; w < RS_BT / 64; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
-        const int q = base + k * RS_BT + tid;
-        const bool ok = q < n;
-        unsigned long long key = ok ? keys[q] : ~0ull;
-        const int dg = ok ? (int)((key >> shift) & 0xFF) : 256;
-        // lanes of this wave holding the same digit
-        unsigned long long peers = __ballot(ok);
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (dg >> b) & 1;
-            const unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const int rank = __popcll(peers & lt);
-        const bool last = ok && ((peers >> lane) >> 1) == 0ull;
-        if (last) wcnt[wv][dg] = rank + 1;
-        __syncthreads();
-        if (ok) {
-            int off = running[dg];
-            for (int w = 0; w < wv; ++w) off += wcnt[w][dg];
-            okeys[off + rank] = key;
-            ovals[off + rank] = vals[q];
-        }
-        __syncthreads();
-        int add = 0;
-        for (int w = 0; w < RS_BT / 64; ++w) add += wcnt[w][tid];
-        running[tid] += add;
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------ nodes
 struct Nodes {
     int *beg, *cnt, *pcount, *dropb, *first, *parent, *minrow, *size, *dfs, *leafinf;
@@ -559,7 +501,7 @@ static int ensure_rows(TreeBuf *t, int n) {
     TCHK(hipMalloc(&t->k1, sizeof(unsigned long long) * cap));
     TCHK(hipMalloc(&t->v0, sizeof(int) * cap));
     TCHK(hipMalloc(&t->v1, sizeof(int) * cap));
-    int nblk = (cap + RS_TILE - 1) / RS_TILE;
+    int nblk = nv_radix_blocks(cap);
     TCHK(hipMalloc(&t->hist, sizeof(int) * 256 * (size_t)nblk));
     TCHK(hipMalloc(&t->tmp, sizeof(int) * (size_t)nv_scan_tmp_ints(256 * nblk)));
     t->cap = cap;
@@ -584,21 +526,9 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0, t->nact);
     TCHK(hipGetLastError());
-    // stable LSD radix sort, 8 bits per pass
-    const int nblk = (n + RS_TILE - 1) / RS_TILE;
-    unsigned long long *ka = t->k0, *kb = t->k1;
-    int *va = t->v0, *vb = t->v1;
-    for (int shift = 0; shift < 64; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, t->hist, nblk);
-        TCHK(hipGetLastError());
-        TCHK(nv_exclusive_scan(st, t->hist, 256 * nblk, t->tmp));
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, t->hist, nblk, kb, vb);
-        TCHK(hipGetLastError());
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-    // after an even number of passes the result is back in k0/v0
-    t->vals = va;
+    // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
+    TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+    t->vals = t->v0;
     TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     const int nact = t->h_int[0];
@@ -628,7 +558,7 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
         }
         if (int rc = grow_nodes(t, st, L1 + 8 * nsplit, L1)) return rc;
         hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->flags, t->offs,
-                           ka, level);
+                           t->k0, level);
         TCHK(hipGetLastError());
         t->lvl.push_back(L1 + 8 * nsplit);
         ++level;

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include "nv_util.h"
 
+#include <utility>
+
 namespace {
 
 constexpr int SCAN_BT = 256;
@@ -86,6 +88,66 @@ __global__ void k_fill_int(int *p, int v, int n) {
     if (r < n) p[r] = v;
 }
 
+// Stable LSD radix sort, 8 bits per pass: per-block digit histograms, one
+// exclusive scan over (digit, block), then a scatter in which each wave
+// ranks equal digits with ballots (stable within and across waves).
+constexpr int RS_BT = 256;
+constexpr int RS_IPT = 16;
+constexpr int RS_TILE = RS_BT * RS_IPT;
+
+__global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int *hist, int nblk) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_IPT; ++k) {
+        int q = base + k * RS_BT + threadIdx.x;
+        if (q < n) atomicAdd(&h[(keys[q] >> shift) & 0xFF], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, int shift, const int *hist,
+                             int nblk, unsigned long long *okeys, int *ovals) {
+    __shared__ int running[256];
+    __shared__ int wcnt[RS_BT / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    running[tid] = hist[tid * nblk + blockIdx.x];
+    const int base = blockIdx.x * RS_TILE;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int k = 0; k < RS_IPT; ++k) {
+        for (int w = 0; w < RS_BT / 64; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        const int q = base + k * RS_BT + tid;
+        const bool ok = q < n;
+        unsigned long long key = ok ? keys[q] : ~0ull;
+        const int dg = ok ? (int)((key >> shift) & 0xFF) : 256;
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(ok);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dg >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        const bool last = ok && ((peers >> lane) >> 1) == 0ull;
+        if (last) wcnt[wv][dg] = rank + 1;
+        __syncthreads();
+        if (ok) {
+            int off = running[dg];
+            for (int w = 0; w < wv; ++w) off += wcnt[w][dg];
+            okeys[off + rank] = key;
+            ovals[off + rank] = vals[q];
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < RS_BT / 64; ++w) add += wcnt[w][tid];
+        running[tid] += add;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 // a fill kernel rather than hipMemsetAsync, so that code a caller may
@@ -135,3 +197,26 @@ hipError_t nv_compact_flags(hipStream_t st, const int *flags, int n, int *out, i
 }
 
 int nv_compact_tmp_ints(int n) { return nv_scan_tmp_ints((n + SCAN_TILE - 1) / SCAN_TILE) + (n + SCAN_TILE - 1) / SCAN_TILE + 4; }
+
+int nv_radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+int nv_radix_hist_ints(int n) { return 256 * nv_radix_blocks(n); }
+
+hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
+                               int bits, int *hist, int *scan_tmp) {
+    if (n <= 0) return hipSuccess;
+    const int nblk = nv_radix_blocks(n);
+    unsigned long long *ka = k0, *kb = k1;
+    int *va = v0, *vb = v1;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, hist, nblk);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if ((e = nv_exclusive_scan(st, hist, 256 * nblk, scan_tmp)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, hist, nblk, kb, vb);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    return hipSuccess;
+}

@@ -16,6 +16,7 @@
 // result does not depend on which workgroup ran which item.  With S = 1 the
 // summation order per target is the reference's, term for term.
 #include "nv_kernels.h"
+#include "nv_util.h"
 
 namespace {
 
@@ -116,6 +117,48 @@ struct __align__(16) Rec32 {
     int j;
 };
 
+// Spatial order of the FP32 source records.  Within each tangle, active
+// rows by the Morton code of their cell (10 bits per axis over the periodic
+// box), then the inactive rows in row order; key = tangle << 32 | code.
+// Any order is correct (records carry their row); a compact tile keeps the
+// FP32 positions small relative to its origin and makes more tiles far.
+__device__ __forceinline__ unsigned spread10(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_order_keys(DState s, Tangles tg, int rows, double bx, double by, double bz,
+                             unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int k = tg.row2t[r];
+    unsigned code;
+    if (s.infront[r] != 0) {
+        const double bb[3] = {bx, by, bz};
+        unsigned c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double u = ld(s.x, s.cap, a, r) / bb[a] + 0.5;
+            const int q = (int)floor(u * 1024.0);
+            c[a] = (unsigned)(q < 0 ? 0 : q > 1023 ? 1023 : q);
+        }
+        code = spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2);
+    } else {
+        code = 0x80000000u | (unsigned)(r - tg.base[k]);
+    }
+    keys[r] = ((unsigned long long)(unsigned)k << 32) | code;
+    vals[r] = r;
+}
+
+__global__ void k_order_inverse(const int *__restrict__ perm, int rows, int *__restrict__ iperm) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < rows) iperm[perm[q]] = q;
+}
+
 // Per source tile: the FP64 origin its FP32 records are relative to, and
 // a sphere (FP32, relative to the origin) holding its active sources;
 // rad < 0 marks a tile without active sources.
@@ -133,21 +176,24 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// one workgroup per tile of TS rows (rows is a multiple of TS)
-__global__ __launch_bounds__(TS) void k_src32(DState s, int rows, Rec32 *__restrict__ rec,
-                                              TileInfo *__restrict__ tiles) {
+// One workgroup per tile of TS records (rows is a multiple of TS); record
+// q of the tile sequence holds row perm[q].
+__global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__restrict__ perm,
+                                              Rec32 *__restrict__ rec, TileInfo *__restrict__ tiles) {
     __shared__ float red[TS / 64][6];
     __shared__ int first;
-    const int r = blockIdx.x * TS + threadIdx.x;
-    const bool act = r < rows && s.infront[r] != 0;
-    // origin: the tile's first active row
+    const int q0 = blockIdx.x * TS + threadIdx.x;
+    const int r = q0 < rows ? perm[q0] : 0;
+    const bool act = q0 < rows && s.infront[r] != 0;
+    // origin: the row of the tile's first active record
     if (threadIdx.x == 0) first = 0x7fffffff;
     __syncthreads();
-    if (act) atomicMin(&first, r);
+    if (act) atomicMin(&first, q0);
     __syncthreads();
     double o[3] = {0.0, 0.0, 0.0};
     if (first != 0x7fffffff) {
-        o[0] = ld(s.x, s.cap, 0, first); o[1] = ld(s.x, s.cap, 1, first); o[2] = ld(s.x, s.cap, 2, first);
+        const int rf = perm[first];
+        o[0] = ld(s.x, s.cap, 0, rf); o[1] = ld(s.x, s.cap, 1, rf); o[2] = ld(s.x, s.cap, 2, rf);
     }
     Rec32 v;
     if (act) {
@@ -168,7 +214,7 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, Rec32 *__restr
         v.c = 0.0f;
         v.j = -1;
     }
-    if (r < rows) rec[r] = v;
+    if (q0 < rows) rec[q0] = v;
     // bounding box of the active sources, then its circumscribed sphere
     float lo[3], hi[3];
     lo[0] = act ? v.x0 : 3.0e38f; lo[1] = act ? v.x1 : 3.0e38f; lo[2] = act ? v.x2 : 3.0e38f;
@@ -234,7 +280,8 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 template <int TPT, int KB>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVES_PER_EU))) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
                                               int *queue, const Rec32 *__restrict__ rec,
-                                              const TileInfo *__restrict__ tiles, const double *__restrict__ rec64,
+                                              const TileInfo *__restrict__ tiles, const int *__restrict__ iperm,
+                                              const double *__restrict__ rec64,
                                               Shifts sh, double kappa, float rn2, double *__restrict__ part) {
     __shared__ Rec32 tile[TS];
     __shared__ int slot;
@@ -246,7 +293,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
         const Item I = items[it];
         const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
         if (I.tb * TB * TPT >= n) continue;   // item listed for rows the tangle does not fill (yet)
-        int ii[TPT], loc[TPT], bi[TPT];
+        int ii[TPT], loc[TPT], bi[TPT], bpos[TPT];
         bool act[TPT];
         double xi[TPT][3];
         float f[TPT][3], k[TPT][3];
@@ -266,6 +313,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
                 nr[q][c] = 0.0;
             }
             if (act[q]) bi[q] = s.behind[ii[q]] - 1;
+            bpos[q] = bi[q] >= 0 ? iperm[bi[q]] : -1;   // record position of the behind row
             any = any || act[q];
         }
         const int ntiles = (n + TS - 1) / TS;
@@ -295,7 +343,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
                 const float dd = c0 * c0 + c1 * c1 + c2 * c2;
                 const float mag = fabsf(tt[q][0]) + fabsf(tt[q][1]) + fabsf(tt[q][2]) + ti.rad;
                 const float lim = (ti.rad + rnf) * 1.0001f + 1.0e-6f * mag;
-                const bool fq = dd > lim * lim && (unsigned)(bi[q] - gt * TS) >= (unsigned)TS;
+                const bool fq = dd > lim * lim && (unsigned)(bpos[q] - gt * TS) >= (unsigned)TS;
                 far = far && (fq || !act[q]);
             }
             if (__all(far)) {
@@ -487,20 +535,34 @@ hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int
 
 size_t nvk_tileinfo_bytes() { return sizeof(TileInfo); }
 
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, void *tiles) {
+hipError_t nvk_order(hipStream_t st, DState s, Tangles tg, int rows, const double *box, unsigned long long *k0,
+                     int *v0, unsigned long long *k1, int *v1, int *iperm, int *hist, int *scan_tmp) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_order_keys, dim3((rows + 255) / 256), dim3(256), 0, st, s, tg, rows, box[0], box[1], box[2],
+                       k0, v0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int bits = tg.K <= 1 ? 32 : 48;
+    if ((e = nv_radix_sort_pairs(st, k0, v0, k1, v1, rows, bits, hist, scan_tmp)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order_inverse, dim3((rows + 255) / 256), dim3(256), 0, st, v0, rows, iperm);
+    return hipGetLastError();
+}
+
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles) {
     if (rows > 0)
-        hipLaunchKernelGGL(k_src32, dim3((rows + TS - 1) / TS), dim3(TS), 0, st, s, rows, (Rec32 *)rec,
+        hipLaunchKernelGGL(k_src32, dim3((rows + TS - 1) / TS), dim3(TS), 0, st, s, rows, perm, (Rec32 *)rec,
                            (TileInfo *)tiles);
     return hipGetLastError();
 }
 
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
-                     const void *rec, const void *tiles, const double *rec64, const Shifts *sh, double kappa,
+                     const void *rec, const void *tiles, const int *iperm, const double *rec64, const Shifts *sh,
+                     double kappa,
                      float rn2, double *part, int tpt, int kb) {
     const Rec32 *r = (const Rec32 *)rec;
     const TileInfo *ti = (const TileInfo *)tiles;
 #define NV_BS32(TPT_, KB_)                                                                                   \
-    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, ti, \
+    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, ti, iperm, \
                        rec64, *sh, kappa, rn2, part)
     if (tpt == 2) {
         if (kb == 8) NV_BS32(2, 8);

@@ -83,7 +83,16 @@ struct Ctx {
     } sg[2][2];
     long layout_gen = 0;
     double *xa = nullptr;            // the buffer alloc_state gave to x
+    // spatial order of the FP32 source records (perm in operm, its inverse
+    // in iperm), valid for the layout generation it was built in; rebuilt
+    // every few steps as points move
+    DevBuf<unsigned long long> okey0, okey1;
+    DevBuf<int> operm, oval1, iperm, ohist, oscan;
+    long state_gen = 0, order_gen = -1;
+    int order_age = 0;
 };
+
+constexpr int ORDER_EVERY = 8;       // steps between two source orderings
 
 int fail(Ctx *c, int code, const std::string &msg) {
     c->err = msg;
@@ -112,6 +121,18 @@ Tangles tangles(Ctx *c, const int *S, const long long *poff) {
     g.S = S;
     g.poff = poff;
     return g;
+}
+
+// (re)build the spatial order of the FP32 source records when it belongs to
+// another layout or has aged (launches only, no synchronisation)
+int order_sources(Ctx *c, bool force) {
+    if (!force && c->order_gen == c->state_gen && c->order_age < ORDER_EVERY) return NV_OK;
+    Tangles tg = tangles(c, nullptr, nullptr);
+    HIPCHK(c, nvk_order(c->st, c->d, tg, c->rows, c->phys.box, c->okey0.p, c->operm.p, c->okey1.p, c->oval1.p,
+                        c->iperm.p, c->ohist.p, c->oscan.p));
+    c->order_gen = c->state_gen;
+    c->order_age = 0;
+    return NV_OK;
 }
 
 void free_dstate(DState &d) {
@@ -163,6 +184,13 @@ int alloc_scratch(Ctx *c) {
     HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
     HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes() * (size_t)(rows / 256 + 1)));
     HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 64 + 4)));
+    HIPCHK(c, c->okey0.reserve(rows));
+    HIPCHK(c, c->okey1.reserve(rows));
+    HIPCHK(c, c->operm.reserve(rows));
+    HIPCHK(c, c->oval1.reserve(rows));
+    HIPCHK(c, c->iperm.reserve(rows));
+    HIPCHK(c, c->ohist.reserve((size_t)nv_radix_hist_ints(rows)));
+    HIPCHK(c, c->oscan.reserve((size_t)nv_scan_tmp_ints(nv_radix_hist_ints(rows))));
     return NV_OK;
 }
 
@@ -174,6 +202,7 @@ int alloc_state(Ctx *c) {
     c->d = d;
     c->xa = d.x;
     ++c->layout_gen;
+    ++c->state_gen;
     if (int rc = alloc_scratch(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
@@ -412,6 +441,7 @@ int ensure_room(Ctx *c) {
     c->tbase = nbase;
     c->tcap = ncap;
     ++c->layout_gen;
+    ++c->state_gen;
     if (int rc = alloc_scratch(c)) return rc;
     return upload_tables(c);
 }
@@ -516,11 +546,13 @@ int velocity_impl(Ctx *c, int do_ab) {
         HIPCHK(c, nvk_src64(c->st, c->d, R, c->rec64.p));
         Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
         if (fp32) {
-            HIPCHK(c, nvk_src32(c->st, c->d, R, c->rec32.p, c->tinfo.p));
+            if (int rc = order_sources(c, false)) return rc;
+            ++c->order_age;
+            HIPCHK(c, nvk_src32(c->st, c->d, R, c->operm.p, c->rec32.p, c->tinfo.p));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
             HIPCHK(c, hipEventRecord(c->e2, c->st));
-            HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
+            HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
                                 c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 1;
@@ -622,10 +654,10 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
         HIPCHK(c, nv_fill_int(st, c->queue.p, 0, 1));
         HIPCHK(c, nvk_src64(st, d, R, c->rec64.p));
         if (fp32) {
-            HIPCHK(c, nvk_src32(st, d, R, c->rec32.p, c->tinfo.p));
+            HIPCHK(c, nvk_src32(st, d, R, c->operm.p, c->rec32.p, c->tinfo.p));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
-            HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
+            HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
                                 c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             mode = 1;
         } else {
@@ -728,6 +760,11 @@ int step_recorded(Ctx *c, int nsteps) {
         Ctx::StepGraph &g = c->sg[par][recon ? 1 : 0];
         if (g.gen != c->layout_gen)
             if (int rc = record_steps(c)) return rc;
+        const bool fp32 = c->prm.velocity == NV_VEL_BS && c->prm.precision == NV_PREC_FP32C;
+        if (fp32) {
+            if (int rc = order_sources(c, false)) return rc;
+            ++c->order_age;
+        }
         HIPCHK(c, hipGraphLaunch(g.e, c->st));
         c->itime += 1;
         swap_x(c);
@@ -1059,6 +1096,9 @@ int nv_step(void *h, int32_t nsteps) {
         nsteps = 0;
     }
     for (int k = 0; k < nsteps && rc == NV_OK; ++k) {
+        // room for this step's insertions first, as the recorded step does,
+        // so both make the same layout moves at the same points
+        if ((rc = ensure_room(c))) break;
         c->itime += 1;
         if ((rc = nv_ghostp(c))) break;
         if (use_tree && (rc = nv_construct_tree(c))) break;
@@ -1192,7 +1232,10 @@ int nv_autotune(void *h, const char *cache_path) {
         if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, false, &nitems)) return rc;
         HIPCHK(c, c->queue.reserve(1));
         HIPCHK(c, nvk_src64(c->st, c->d, c->rows, c->rec64.p));
-        if (fp32) HIPCHK(c, nvk_src32(c->st, c->d, c->rows, c->rec32.p, c->tinfo.p));
+        if (fp32) {
+            if (int rc = order_sources(c, false)) return rc;
+            HIPCHK(c, nvk_src32(c->st, c->d, c->rows, c->operm.p, c->rec32.p, c->tinfo.p));
+        }
         Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
         const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
         float tot = 0.f;
@@ -1201,7 +1244,7 @@ int nv_autotune(void *h, const char *cache_path) {
             HIPCHK(c, hipEventRecord(c->e2, c->st));
             if (fp32) {
                 const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
-                HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p,
+                HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
                                     c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
             } else {
                 const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));

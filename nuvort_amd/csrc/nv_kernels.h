@@ -31,9 +31,13 @@ struct DCtl {
 hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec);
 hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
                     const double *rec, const Shifts *sh, double kappa, double *part);
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, void *rec, void *tiles);
+// spatial order of the FP32 source records: perm (in v0) and its inverse
+hipError_t nvk_order(hipStream_t st, DState s, Tangles tg, int rows, const double *box, unsigned long long *k0,
+                     int *v0, unsigned long long *k1, int *v1, int *iperm, int *hist, int *scan_tmp);
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles);
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
-                     const void *rec, const void *tiles, const double *rec64, const Shifts *sh, double kappa,
+                     const void *rec, const void *tiles, const int *iperm, const double *rec64, const Shifts *sh,
+                     double kappa,
                      float rn2, double *part, int tpt, int kb);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl);
