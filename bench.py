@@ -253,6 +253,125 @@ def dense(args):
         os.remove(TUNE_CACHE)
 
 
+def oracle_tree_velocity(st, p, ranges, threads):
+    """The oracle's tree velocity (reference tree build and walk) for the
+    rows in `ranges` [(first0, count), ...] (0-based first rows)."""
+    from oracle import port
+    L = port.lib()
+    ps = port.PortState(st)
+    pr = port.params_struct(p, threads)
+    L.nvo_ghostp(ps.ref(), C.byref(pr))
+    tr = port.PortTree()
+    rc = L.nvo_tree_build(ps.ref(), C.byref(pr), tr.ref())
+    if rc != 0:
+        raise RuntimeError("oracle tree build failed: %d" % rc)
+    out = []
+    for first0, count in ranges:
+        u = np.zeros((count, 3))
+        L.nvo_velocity(ps.ref(), C.byref(pr), tr.ref(), first0 + 1, count, u.ctypes.data_as(port._dp))
+        out.append(u)
+    tr.free()
+    return np.concatenate(out)
+
+
+def tree1m(args):
+    """Random rings (radii U[0.01, 0.03] cm, delta 5e-4 cm, seed 1239) until
+    1M points in a periodic 0.2 cm box; Barnes-Hut theta 0.4 (octree rebuilt
+    every step), uniform normal flow 0.5 cm/s along x with mutual friction
+    alpha = 0.206, alpha' = 0.0083, dt 5e-6 s, reconnection and re-meshing
+    every step (stale-tree close search).  Velocities of a sample of targets
+    are checked against the oracle's direct FP64 sum (the tree's own
+    approximation error) and, for the FP32 walk, against the FP64 walk."""
+    from nuvort_amd import tangles
+    from nuvort_amd.engine import FilamentSolver
+    from nuvort_amd.runfile import RunParams
+    st = tangles.random_rings(1239, 0, 0.01, 0.03, 0.2, 5e-4, target_points=1000000)
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.2, 0.2, 0.2), velocity="Tree", tree_theta=0.4,
+                  normal_velocity="xflow", norm_vel_xflow=0.5, alpha=(0.206, 0.0083), recon_shots=1,
+                  hip_precision=args.tree_precision)
+    log("tree1m: %d points, walk %s" % (st.active, args.tree_precision))
+    s = FilamentSolver(p)
+    s.set_state(st)
+    s.step(1)
+    log("warm-up step done")
+    per, stats = [], []
+    s0 = s.stats()
+    t_all = 0.0
+    for k in range(args.steps if args.steps < 100 else 5):
+        a = s.stats()
+        t0 = time.perf_counter()
+        s.step(1)
+        dt_ = time.perf_counter() - t0
+        b = s.stats()
+        t_all += dt_
+        per.append(dt_ * 1e3)
+        stats.append({"step_ms": dt_ * 1e3, "tree_build_ms": b["ms_tree_build"], "close_ms": b["ms_close"],
+                      "walk_ms": b["walk_ms_total"] - a["walk_ms_total"],
+                      "walk_nodes": b["walk_nodes_total"] - a["walk_nodes_total"]})
+        log("step: %.1f ms (build %.1f, walk %.1f, close %.1f)" % (dt_ * 1e3, b["ms_tree_build"],
+                                                                   stats[-1]["walk_ms"], b["ms_close"]))
+    s1 = s.stats()
+    nsteps = len(per)
+    pu = s1["point_updates"] - s0["point_updates"]
+    cur = s.get_state()
+    s.close()
+    # velocities on the current state: this walk, the FP64 walk, and the oracle's direct sum on a sample
+    def tree_velocity(prec):
+        q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+        q.hip_precision = prec
+        v = FilamentSolver(q)
+        v.set_state(cur)
+        v.ghostp()
+        v.construct_tree()
+        v.velocity()
+        g = v.get_state()
+        v.close()
+        return g
+    g = tree_velocity(args.tree_precision)
+    g64 = tree_velocity("fp64") if args.tree_precision != "fp64" else g
+    acc = {}
+    if args.tree_precision != "fp64":
+        act = g64.infront != 0
+        acc["walk_vs_fp64_walk"] = rel_err_stats(g64.u_sup[act], g.u_sup[act])
+    # the oracle's tree (the reference's algorithm) on 8 blocks of 512 rows
+    n_all = cur.pcount
+    ranges = [(int(b * (n_all - 512) / 7), 512) for b in range(8)]
+    rows = np.concatenate([np.arange(a, a + c) for a, c in ranges])
+    log("oracle tree velocity on %d targets" % len(rows))
+    uot = oracle_tree_velocity(cur, p, ranges, args.cpu_threads)
+    keep = cur.infront[rows] != 0
+    acc["fp64_walk_vs_oracle_tree_sample"] = rel_err_stats(uot[keep], g64.u[rows][keep])
+    if args.tree_precision != "fp64":
+        acc["walk_vs_oracle_tree_sample"] = rel_err_stats(uot[keep], g.u[rows][keep])
+    sample = 1024
+    log("oracle direct sum on %d targets" % sample)
+    pd = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+    pd.velocity, pd.tree_theta = "BS", 0.0
+    secs, idx, ucpu = cpu_baseline_parallel(cur, pd, sample, args.cpu_threads)
+    # the tree's own approximation error (theta 0.4, the reference's monopole
+    # nodes) against the direct sum
+    acc["tree_vs_direct_fp64_oracle_sample"] = rel_err_stats(ucpu, g.u[idx])
+    walk_nodes = np.mean([x["walk_nodes"] for x in stats])
+    walk_ms = np.mean([x["walk_ms"] for x in stats])
+    n = st.active
+    rec = 56 if args.tree_precision == "fp32c" else 80
+    walk_bytes = walk_nodes * rec + n * 27 * 12 + n * 12
+    line = {
+        "metric": METRIC, "value": pu / t_all, "unit": "point-updates/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 1, "ms_per_step": t_all / nsteps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32c+f64" if args.tree_precision == "fp32c" else "f64", "data": "synthetic",
+        "config": {"workload": "counterflow tangle: random rings to 1M points in a periodic 0.2 cm box, delta "
+                               "5e-4 cm, Barnes-Hut theta 0.4 rebuilt every step, v_n = 0.5 cm/s along x, "
+                               "alpha 0.206 / 0.0083, reconnection and re-meshing every step",
+                   "points": n, "seed": 1239, "dt_s": 5e-6, "walk": args.tree_precision, "parallelism": "1 GPU"},
+        "per_step": stats, "accuracy": acc,
+        "walk": {"ms": walk_ms, "node_reads": walk_nodes, "record_bytes": rec,
+                 "algorithmic_GBps": walk_bytes / (walk_ms * 1e-3) / 1e9, "hbm_peak_GBps": 8000.0},
+        "oracle_sample": {"targets": sample, "seconds": secs, "threads": args.cpu_threads},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,9 +381,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and tree comparison runs")
-    ap.add_argument("--workload", choices=("rings64", "dense460k"), default="rings64",
+    ap.add_argument("--workload", choices=("rings64", "dense460k", "tree1m"), default="rings64",
                     help="rings64: the headline step (default); dense460k: the dense 800-ring tangle "
-                         "(467k points), relaxed 50 steps, FP32C error statistics against FP64")
+                         "(467k points), relaxed 50 steps, FP32C error statistics against FP64; tree1m: "
+                         "1M points in a 0.2 cm box, Barnes-Hut theta 0.4, counterflow mutual friction")
+    ap.add_argument("--tree-precision", choices=("fp64", "fp32c"), default="fp32c",
+                    help="tree1m: walk arithmetic (FP32 far field + FP64 near field, or FP64)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "for scaling HBM byte counters) and exit")
@@ -289,6 +411,9 @@ def main():
     nvlib.load()
     if args.workload == "dense460k":
         dense(args)
+        return
+    if args.workload == "tree1m":
+        tree1m(args)
         return
     st, p = workload()
     n = st.active

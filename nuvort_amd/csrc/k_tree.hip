@@ -40,6 +40,16 @@ struct __align__(16) WNode {
     int pcount, leaf_infront, skip, pad;
 };
 
+// Compact walk record of the FP32 walk: FP64 centroid and circulation, the
+// opening radius squared (FP32, rounded up), and one word: the skip index of
+// an internal node, or minus one minus the leaf's infront-at-build link.
+struct __align__(8) Node56 {
+    double cent[3];
+    double circ[3];
+    float r2;
+    int word;
+};
+
 __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
     return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
 }
@@ -262,6 +272,159 @@ __global__ void k_zero_counters(unsigned long long *c) {
 // ------------------------------------------------------------ walks
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// WNode -> Node56.  The reference accepts a node when pcount == 1 or
+// width / (dist - geo) < theta (width / dist when dist <= geo or without the
+// correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
+// geo) that is dist > geo + width / theta (width / theta without the
+// correction), compared here on squares.
+__global__ void k_pack56(const WNode *__restrict__ walk, int nwalk, double theta, int corr, Node56 *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nwalk) return;
+    const WNode w = walk[q];
+    Node56 o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o.cent[k] = w.cent[k];
+        o.circ[k] = w.circ[k];
+    }
+    const double r = (corr ? w.geo : 0.0) + w.width / theta;
+    o.r2 = (float)(r * r) * (1.0f + 2.0e-7f);
+    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? w.leaf_infront : 0) - 1 : w.skip;
+    out[q] = o;
+}
+
+__device__ __forceinline__ void kahan_add32(float &sum, float &comp, float v) {
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// The reference's node term (src/tree.f90:297-307) in FP64, a = dist**2.
+__device__ __forceinline__ void tree_term64(double v0, double v1, double v2, const double c[3], double dd,
+                                            double kappa, double nr[3], unsigned long long &nev) {
+    const double dist = sqrt(dd);
+    const double a = dist * dist;
+    const double b = 2.0 * (v0 * c[0] + v1 * c[1] + v2 * c[2]);
+    const double cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    const double D = 4.0 * a * cc - b * b;
+    if (!(D < NV_EPS)) {
+        const double den = (2.0 * NV_PI) * D;
+        const double fac = (2.0 * cc + b) / sqrt(a + b + cc) - (b / sqrt(a));
+        nr[0] = nr[0] + ((v1 * c[2] - v2 * c[1]) * kappa / den) * fac;
+        nr[1] = nr[1] + ((v2 * c[0] - v0 * c[2]) * kappa / den) * fac;
+        nr[2] = nr[2] + ((v0 * c[1] - v1 * c[0]) * kappa / den) * fac;
+        ++nev;
+    }
+}
+
+// The walk of k_walk_vel with FP32 interactions: displacements in FP64, the
+// node term in the well-conditioned form (as the direct sum's far field),
+// Kahan-compensated FP32 accumulation.  The reference's FP64 term is kept for
+// leaves closer than r_near and for nodes whose pseudo-segment (centroid to
+// centroid + circulation) passes close by the target, where the FP32 form's
+// r1 r2 + R1.R2 cancels.  Same traversal, exclusions and singularity test.
+template <int BT>
+__global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
+                                                   const Node56 *__restrict__ walk, int nwalk, double rn2,
+                                                   double *__restrict__ part, DCtl *ctl,
+                                                   unsigned long long *counters) {
+    const int r = blockIdx.x * BT + threadIdx.x;
+    const int m = blockIdx.y;
+    const bool act = r < n && s.infront[r] != 0;
+    double xi[3] = {0, 0, 0};
+    int beh = -5, me = -5;
+    if (act) {
+        xi[0] = s.x[r]; xi[1] = s.x[s.cap + r]; xi[2] = s.x[2 * s.cap + r];
+        beh = s.behind[r];
+        me = r + 1;
+    }
+    const double kappa = p.kappa;
+    float f[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
+    double nr[3] = {0.0, 0.0, 0.0};
+    unsigned long long nev = 0, nvis = 0;
+    bool sing = false;
+    const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+    const bool img0 = (m == 0);
+    int lskip = act ? 0 : 0x7fffffff;
+    int cur = 0;
+    while (cur < nwalk) {
+        cur = wave_uniform(cur);
+        const Node56 nd = walk[cur];
+        const bool leaf = nd.word < 0;
+        bool open = false;
+        if (cur >= lskip) {
+            ++nvis;
+            const double v0 = ((nd.cent[0] + s0) - xi[0]);
+            const double v1 = ((nd.cent[1] + s1) - xi[1]);
+            const double v2 = ((nd.cent[2] + s2) - xi[2]);
+            const double dd = v0 * v0 + v1 * v1 + v2 * v2;
+            if (leaf || dd > (double)nd.r2) {
+                lskip = leaf ? cur + 1 : nd.word;
+                bool use = true;
+                bool near = false;
+                int j = 0;
+                if (leaf) {
+                    const int li = -nd.word - 1;
+                    j = li > 0 ? s.behind[li - 1] : 0;
+                    if (img0 && (j == me || j == beh)) use = false;
+                    if (use && sqrt(dd) < NV_EPS) { sing = true; use = false; }
+                    near = dd < rn2 && j > 0;
+                }
+                if (use && near) {
+                    const int q = j - 1;
+                    double c[3];
+                    for (int k = 0; k < 3; ++k) c[k] = s.gi[k * s.cap + q] - s.x[k * s.cap + q];
+                    tree_term64(v0, v1, v2, c, dd, kappa, nr, nev);
+                } else if (use) {
+                    const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
+                    const float e0 = (float)nd.circ[0], e1 = (float)nd.circ[1], e2 = (float)nd.circ[2];
+                    const float x0 = d1 * e2 - d2 * e1, x1 = d2 * e0 - d0 * e2, x2 = d0 * e1 - d1 * e0;
+                    const float r1sq = (float)dd;
+                    const float de = d0 * e0 + d1 * e1 + d2 * e2;
+                    const float r2sq = r1sq + 2.0f * de + (e0 * e0 + e1 * e1 + e2 * e2);
+                    const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+                    const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+                    const float r12 = (r1sq * inv1) * (r2sq * inv2);
+                    const float den = r12 + (r1sq + de);
+                    if (den < 0.0625f * r12) {
+                        tree_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
+                    } else if (!(4.0f * (x0 * x0 + x1 * x1 + x2 * x2) < (float)NV_EPS)) {
+                        // (the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon)
+                        const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
+                        kahan_add32(f[0], kc[0], x0 * fac);
+                        kahan_add32(f[1], kc[1], x1 * fac);
+                        kahan_add32(f[2], kc[2], x2 * fac);
+                        ++nev;
+                    }
+                }
+            } else {
+                open = true;
+            }
+        }
+        cur = __any(open) ? cur + 1 : (leaf ? cur + 1 : nd.word);
+    }
+    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
+    if (r < npad) {
+        const double k4 = kappa / (4.0 * NV_PI);
+        double *pp = part + (size_t)m * 6 * npad;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double acc = ((double)f[k] - (double)kc[k]) * k4 + nr[k];
+            pp[(img0 ? k : 3 + k) * npad + r] = acc;
+            pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nev += __shfl_down(nev, off);
+        nvis += __shfl_down(nvis, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counters[0], nev);
+        atomicAdd(&counters[1], nvis);
+    }
+}
+
 // one (target block, image) per workgroup: blockIdx.y = image; partial block
 // [images][6][npad], image 0 in the unshifted half, images 1..26 in the
 // image half, so k_finish's chunk-ordered reduction rebuilds the reference's
@@ -420,6 +583,8 @@ struct TreeBuf {
     int flag_cap = 0;
     WNode *walk = nullptr;
     int walk_cap = 0;
+    Node56 *walk56 = nullptr;            // FP32 walk records, packed per walk
+    int walk56_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
     std::vector<int> lvl;                // level start offsets
     unsigned long long *counters = nullptr;
@@ -458,6 +623,7 @@ void nv_tree_free(TreeBuf *t) {
     if (!t) return;
     hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
     hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
+    hipFree(t->walk56);
     hipFree(t->counters);
     hipHostFree(t->h_int);
     free_nodes(t->nd);
@@ -592,14 +758,33 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
 }
 
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
-                          int precision, double *part, DCtl *ctl, nv_stats *stats) {
-    (void)precision;
+                          int precision, double rn2, double *part, DCtl *ctl, nv_stats *stats) {
     (void)stats;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
-    hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad, t->walk,
-                       t->nwalk, part, ctl, t->counters);
+    // the FP32 walk's single opening radius needs width / theta > geo
+    const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
+    if (fp32) {
+        if (t->nwalk > t->walk56_cap) {
+            hipFree(t->walk56);
+            t->walk56 = nullptr;
+            t->walk56_cap = std::max(t->nwalk, 2 * t->walk56_cap);
+            TCHK(hipMalloc(&t->walk56, sizeof(Node56) * (size_t)t->walk56_cap));
+        }
+        if (t->nwalk > 0)
+            hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, t->walk, t->nwalk,
+                               p.tree_theta, p.extra_corr, t->walk56);
+        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad,
+                           t->walk56, t->nwalk, rn2, part, ctl, t->counters);
+    } else {
+        hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad, t->walk,
+                           t->nwalk, part, ctl, t->counters);
+    }
     TCHK(hipGetLastError());
     return NV_OK;
+}
+
+int nv_tree_walk_record_bytes(int precision, double theta) {
+    return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node56) : (int)sizeof(WNode);
 }
 
 // interaction and node-visit counters of the last velocity walk (synchronises)

@@ -531,8 +531,9 @@ int velocity_impl(Ctx *c, int do_ab) {
         HIPCHK(c, hipMemcpyAsync(c->d_S.p, &nimg, sizeof(int), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipMemcpyAsync(c->d_poff.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipEventRecord(c->e2, c->st));
+        const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
         int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision,
-                                       c->part.p, c->ctl, &c->stats);
+                                       rn * rn, c->part.p, c->ctl, &c->stats);
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
         HIPCHK(c, hipEventRecord(c->e3, c->st));
         timed_walk = true;

@@ -10,8 +10,11 @@ void nv_tree_free(TreeBuf *t);
 int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl, nv_stats *stats);
 // tree_walk (src/tree.f90:250-316) for every point and image: writes the
 // unshifted and image sums to part[0..6)[npad] (the layout k_finish reads)
+// (precision FP32C: FP32 node terms, FP64 leaves within sqrt(rn2))
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
-                          int precision, double *part, DCtl *ctl, nv_stats *stats);
+                          int precision, double rn2, double *part, DCtl *ctl, nv_stats *stats);
+// bytes of one walk record read by the velocity walk of that precision
+int nv_tree_walk_record_bytes(int precision, double theta);
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats);
 // pclose_tree / closest_tree (src/tree.f90:190-244) over the (stale) tree
 int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats);
