@@ -144,7 +144,7 @@ typedef struct {
     int32_t kahan_block;       /* FP32C terms between compensated adds */
     int32_t items_per_cu;      /* direct-sum work items per CU (sets the chunk size) */
     int32_t tuned;             /* 0 defaults, 1 timed by nv_autotune, 2 read from its cache */
-    double tune_ms;            /* direct-sum launch time of the chosen variant when tuned */
+    double tune_ms;            /* direct-sum launch + chunk reduction time of the chosen variant */
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */

@@ -483,31 +483,58 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
             if (__any(lane)) need |= 1u << q;
         }
         if (!__syncthreads_or(need != 0)) continue;
+        // empty rows get coordinates no distance can come near the bound from
         for (int q = tid; q < T; q += 256) {
             const int jl = t * T + q, j = base + jl;
             const bool v = jl < n && s.infront[j] != 0;
-            tile[q * 4 + 0] = v ? s.x[j] : 0.0;
-            tile[q * 4 + 1] = v ? s.x[s.cap + j] : 0.0;
-            tile[q * 4 + 2] = v ? s.x[2 * s.cap + j] : 0.0;
-            tile[q * 4 + 3] = v ? 1.0 : 0.0;
+            tile[q * 4 + 0] = v ? s.x[j] : 1.0e150;
+            tile[q * 4 + 1] = v ? s.x[s.cap + j] : 1.0e150;
+            tile[q * 4 + 2] = v ? s.x[2 * s.cap + j] : 1.0e150;
         }
         __syncthreads();
-        if (!act) continue;
+        const int row0 = base + t * T;
         for (int q = 0; q < 4; ++q) {
             if (!(need & (1u << q))) continue;
-            for (int jj = q * 64; jj < q * 64 + 64; ++jj) {
-                const double *r = &tile[jj * 4];
-                if (r[3] == 0.0) continue;
-                const int j = base + t * T + jj;
-                if (j == i || j == fi || j == bi) continue;
-                const double d0 = x0 - r[0], d1 = x1 - r[1], d2 = x2 - r[2];
-                const double dd = d0 * d0 + d1 * d1 + d2 * d2;
-                if (dd < bound) {
-                    const double dist = sqrt(dd);
-                    if (dist < best) {
-                        best = dist;
-                        bound = dist * dist * (1.0 + 8e-16);
-                        bj = j;
+            const int s0 = row0 + q * 64;
+            // rows this lane excludes (itself, infront, behind) in the segment?
+            const bool ex = act && ((unsigned)(i - s0) < 64u || (unsigned)(fi - s0) < 64u ||
+                                    (unsigned)(bi - s0) < 64u);
+            if (!act) continue;
+            if (__any(ex)) {
+                for (int jj = q * 64; jj < q * 64 + 64; ++jj) {
+                    const double *r = &tile[jj * 4];
+                    const int j = row0 + jj;
+                    if (j == i || j == fi || j == bi) continue;
+                    const double d0 = x0 - r[0], d1 = x1 - r[1], d2 = x2 - r[2];
+                    const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+                    if (dd < bound) {
+                        const double dist = sqrt(dd);
+                        if (dist < best) {
+                            best = dist;
+                            bound = dist * dist * (1.0 + 8e-16);
+                            bj = j;
+                        }
+                    }
+                }
+            } else {
+                // no exclusion in the segment: a fused filter (within 1e-15 of
+                // the reference's sum of squares), the exact sum for candidates
+                double fbound = bound * (1.0 + 1e-15);
+                for (int jj = q * 64; jj < q * 64 + 64; ++jj) {
+                    const double *r = &tile[jj * 4];
+                    const double d0 = x0 - r[0], d1 = x1 - r[1], d2 = x2 - r[2];
+                    const double ddf = fma(d0, d0, fma(d1, d1, d2 * d2));
+                    if (ddf < fbound) {
+                        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+                        if (dd < bound) {
+                            const double dist = sqrt(dd);
+                            if (dist < best) {
+                                best = dist;
+                                bound = dist * dist * (1.0 + 8e-16);
+                                fbound = bound * (1.0 + 1e-15);
+                                bj = row0 + jj;
+                            }
+                        }
                     }
                 }
             }

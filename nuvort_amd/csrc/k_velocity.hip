@@ -484,7 +484,32 @@ __global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *_
     finish_point(s, p, r, ub0, ubs, ctl->t, do_ab);
 }
 
+// Reads every chunk partial of the active rows the way k_finish does, and
+// writes nothing unless a sum is NaN: the reduction's cost, for the
+// autotuner (the partial volume grows with the chunk count).
+__global__ void k_reduce_probe(DState s, Tangles tg, int rows, const double *__restrict__ part,
+                               double *__restrict__ sink) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows || s.infront[r] == 0) return;
+    const int kk = tg.row2t[r];
+    const int capk = tg.cap[kk], loc = r - tg.base[kk], S = tg.S[kk];
+    const double *pb = part + tg.poff[kk];
+    double acc = 0.0;
+    for (int c = 0; c < S; ++c) {
+        const double *pp = pb + (size_t)c * 6 * capk;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc += pp[k * capk + loc];
+    }
+    if (acc != acc) sink[0] = acc;
+}
+
 }  // namespace
+
+hipError_t nvk_reduce_probe(hipStream_t st, DState s, Tangles tg, int rows, const double *part, double *sink) {
+    if (rows > 0)
+        hipLaunchKernelGGL(k_reduce_probe, dim3((rows + 255) / 256), dim3(256), 0, st, s, tg, rows, part, sink);
+    return hipGetLastError();
+}
 
 // ------------------------------------------------------------------ launch
 size_t nvk_rec32_bytes() { return sizeof(Rec32); }

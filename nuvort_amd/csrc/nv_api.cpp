@@ -1239,6 +1239,7 @@ int nv_autotune(void *h, const char *cache_path) {
         }
         Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
         const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+        HIPCHK(c, c->pd.reserve(1));
         float tot = 0.f;
         for (int r = 0; r < warm + reps; ++r) {
             HIPCHK(c, nv_fill_int(c->st, c->queue.p, 0, 1));
@@ -1252,6 +1253,8 @@ int nv_autotune(void *h, const char *cache_path) {
                 HIPCHK(c, nvk_bs64(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec64.p, &c->sh,
                                    c->phys.kappa, c->part.p));
             }
+            // and the chunk reduction that follows (its volume grows with the chunk count)
+            HIPCHK(c, nvk_reduce_probe(c->st, c->d, tg, c->rows, c->part.p, c->pd.p));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             HIPCHK(c, hipEventSynchronize(c->e3));
             float ms = 0.f;

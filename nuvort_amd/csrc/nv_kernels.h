@@ -41,6 +41,7 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
                      float rn2, double *part, int tpt, int kb);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl);
+hipError_t nvk_reduce_probe(hipStream_t st, DState s, Tangles tg, int rows, const double *part, double *sink);
 size_t nvk_rec32_bytes();
 size_t nvk_tileinfo_bytes();
 int nvk_occupancy_bs64();
