@@ -14,6 +14,8 @@
 #include "nv_device.h"
 #include "nv_kernels.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int DIRTY_MAX = 8192;
@@ -566,6 +568,44 @@ __global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, c
     if (bj >= 0) s.closest[i] = bj + 1;
 }
 
+// ------------------------------------------------------- step begin / end
+// src/run.f90 loop head of a recorded step: itime advances, the per-step
+// counters and the work queue start at zero.
+__global__ void k_step_begin(DCtl *ctl, int *tnact, int K, int *queue) {
+    for (int k = threadIdx.x; k < K; k += blockDim.x) tnact[k] = 0;
+    if (threadIdx.x == 0) {
+        ctl->itime = ctl->itime + 1;
+        ctl->nvel = 0;
+        ctl->nactive = 0;
+        queue[0] = 0;
+    }
+}
+
+// enforce_boundary (src/boundary.f90:91-138), then t = t + dt, then the NaN
+// test of the wrapped positions (src/general.f90:172-191), in one launch
+__global__ void k_step_end(DState s, Phys p, int n, DCtl *ctl, double dt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) {
+        ctl->t = ctl->t + dt;
+        ctl->nvel_total += (unsigned long long)ctl->nvel;
+    }
+    if (r >= n) return;
+    double x[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = s.x[k * s.cap + r];
+    if (s.infront[r] != 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (!p.periodic[k]) continue;
+            const double L = p.box[k];
+            if (x[k] > (L / 2.0)) x[k] = x[k] - L;
+            else if (x[k] < (-L / 2.0)) x[k] = x[k] + L;
+            s.x[k * s.cap + r] = x[k];
+        }
+    }
+    if (isnan(x[0]) || isnan(x[1]) || isnan(x[2])) atomicOr(&ctl->status, NVK_NAN);
+}
+
 // ------------------------------------------------------------ relayout
 // Copy every row of the old layout to its tangle's new range; links (1-based
 // global rows, 0 = none) move with their tangle.
@@ -681,5 +721,14 @@ hipError_t nvk_relayout(hipStream_t st, DState o, DState nu, const int *row2t, c
                         const int *base_new, int rows_old) {
     if (rows_old > 0)
         hipLaunchKernelGGL(k_relayout, GRID(rows_old), 0, st, o, nu, row2t, base_old, base_new, rows_old);
+    return hipGetLastError();
+}
+
+hipError_t nvk_step_begin(hipStream_t st, DCtl *ctl, int *tnact, int K, int *queue) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, ctl, tnact, K, queue);
+    return hipGetLastError();
+}
+hipError_t nvk_step_end(hipStream_t st, DState s, Phys p, int n, DCtl *ctl, double dt) {
+    hipLaunchKernelGGL(k_step_end, dim3((std::max(n, 1) + 255) / 256), dim3(256), 0, st, s, p, n, ctl, dt);
     return hipGetLastError();
 }

@@ -50,7 +50,13 @@ struct Ctx {
     // tangle layout: host mirrors and device copies
     int K = 0, rows = 0;             // rows = sum of tangle capacities = the state arrays' stride
     std::vector<int> tbase, tcap, tn, trecon, tremove;
-    DevBuf<int> d_tbase, d_tcap, d_tn, d_trecon, d_tremove, d_tnact, d_row2t;
+    DevBuf<int> d_tbase, d_tcap, d_row2t;
+    // per-tangle counters, one device block [pcount | recon | remove | nact]
+    // so that a step's host read is one copy into pinned memory
+    DevBuf<int> d_tcnt;
+    int *p_tn = nullptr, *p_trecon = nullptr, *p_tremove = nullptr, *p_tnact = nullptr;
+    int *h_tcnt = nullptr;           // pinned mirror of the first three
+    int h_tcnt_cap = 0;
     DState d{};
     DCtl *ctl = nullptr;             // device
     DCtl *ctl_h = nullptr;           // pinned host mirror
@@ -113,10 +119,10 @@ Tangles tangles(Ctx *c, const int *S, const long long *poff) {
     g.K = c->K;
     g.base = c->d_tbase.p;
     g.cap = c->d_tcap.p;
-    g.n = c->d_tn.p;
-    g.recon = c->d_trecon.p;
-    g.remove = c->d_tremove.p;
-    g.nact = c->d_tnact.p;
+    g.n = c->p_tn;
+    g.recon = c->p_trecon;
+    g.remove = c->p_tremove;
+    g.nact = c->p_tnact;
     g.row2t = c->d_row2t.p;
     g.S = S;
     g.poff = poff;
@@ -213,16 +219,24 @@ int upload_tables(Ctx *c) {
     const int K = c->K;
     HIPCHK(c, c->d_tbase.reserve(K));
     HIPCHK(c, c->d_tcap.reserve(K));
-    HIPCHK(c, c->d_tn.reserve(K));
-    HIPCHK(c, c->d_trecon.reserve(K));
-    HIPCHK(c, c->d_tremove.reserve(K));
-    HIPCHK(c, c->d_tnact.reserve(K));
+    HIPCHK(c, c->d_tcnt.reserve((size_t)4 * K));
+    c->p_tn = c->d_tcnt.p;
+    c->p_trecon = c->d_tcnt.p + K;
+    c->p_tremove = c->d_tcnt.p + 2 * K;
+    c->p_tnact = c->d_tcnt.p + 3 * K;
+    if (c->h_tcnt_cap < 3 * K) {
+        if (c->h_tcnt) (void)hipHostFree(c->h_tcnt);
+        c->h_tcnt = nullptr;
+        c->h_tcnt_cap = 0;
+        HIPCHK(c, hipHostMalloc((void **)&c->h_tcnt, sizeof(int) * 3 * (size_t)K, hipHostMallocDefault));
+        c->h_tcnt_cap = 3 * K;
+    }
     HIPCHK(c, c->d_row2t.reserve(c->rows));
     HIPCHK(c, hipMemcpy(c->d_tbase.p, c->tbase.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tcap.p, c->tcap.data(), sizeof(int) * K, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_tn.p, c->tn.data(), sizeof(int) * K, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_trecon.p, c->trecon.data(), sizeof(int) * K, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_tremove.p, c->tremove.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->p_tn, c->tn.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->p_trecon, c->trecon.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->p_tremove, c->tremove.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     std::vector<int> r2t(c->rows);
     for (int k = 0; k < K; ++k) std::fill(r2t.begin() + c->tbase[k], r2t.begin() + c->tbase[k] + c->tcap[k], k);
     HIPCHK(c, hipMemcpy(c->d_row2t.p, r2t.data(), sizeof(int) * c->rows, hipMemcpyHostToDevice));
@@ -231,12 +245,15 @@ int upload_tables(Ctx *c) {
 
 int read_ctl(Ctx *c) {
     HIPCHK(c, hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(DCtl), hipMemcpyDeviceToHost, c->st));
-    if (c->K > 0) {
-        HIPCHK(c, hipMemcpyAsync(c->tn.data(), c->d_tn.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->trecon.data(), c->d_trecon.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->tremove.data(), c->d_tremove.p, sizeof(int) * c->K, hipMemcpyDeviceToHost, c->st));
-    }
+    const int K = c->K;
+    if (K > 0)
+        HIPCHK(c, hipMemcpyAsync(c->h_tcnt, c->d_tcnt.p, sizeof(int) * 3 * (size_t)K, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (K > 0) {
+        std::copy(c->h_tcnt, c->h_tcnt + K, c->tn.begin());
+        std::copy(c->h_tcnt + K, c->h_tcnt + 2 * K, c->trecon.begin());
+        std::copy(c->h_tcnt + 2 * K, c->h_tcnt + 3 * K, c->tremove.begin());
+    }
     int s = c->ctl_h->status;
     if (s & NVK_NAN) return fail(c, NV_ERR_NAN, "run.x: there is a NAN in the (filament) f%x array");
     if (s & NVK_ZERO_LENGTH) return fail(c, NV_ERR_ZERO_LENGTH, "line.mod:pinsert: vortex line length is 0, run over");
@@ -646,13 +663,12 @@ int build_close_items(Ctx *c, bool cap_rows, int *nitems_out) {
 int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
     hipStream_t st = c->st;
     const int R = c->rows;
-    HIPCHK(c, nvk_clock_begin_step(st, c->ctl));
+    HIPCHK(c, nvk_step_begin(st, c->ctl, c->p_tnact, c->K, c->queue.p));
     HIPCHK(c, nvk_ghosts(st, d, c->phys, R));
     int mode = 2;
     Tangles tv = tangles(c, c->d_S.p, c->d_poff.p);
     if (c->prm.velocity == NV_VEL_BS) {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
-        HIPCHK(c, nv_fill_int(st, c->queue.p, 0, 1));
         HIPCHK(c, nvk_src64(st, d, R, c->rec64.p));
         if (fp32) {
             HIPCHK(c, nvk_src32(st, d, R, c->operm.p, c->rec32.p, c->tinfo.p));
@@ -668,13 +684,10 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
             mode = 0;
         }
     }
-    HIPCHK(c, nv_fill_int(st, &c->ctl->nvel, 0, 1));
     HIPCHK(c, nvk_finish(st, d, c->phys, tv, R, c->part.p, mode, 1, c->ctl));
     std::swap(d.x, d.xn);
     Tangles tg = tangles(c, nullptr, nullptr);
-    // pinsert
-    HIPCHK(c, nv_fill_int(st, &c->ctl->nactive, 0, 1));
-    HIPCHK(c, nv_fill_int(st, c->d_tnact.p, 0, c->K));
+    // pinsert (its counters were cleared by nvk_step_begin)
     HIPCHK(c, nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
     HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
     HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
@@ -689,9 +702,7 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
         HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
         HIPCHK(c, nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl));
     }
-    HIPCHK(c, nvk_wrap(st, d, c->phys, R));
-    HIPCHK(c, nvk_clock_end_step(st, c->ctl, c->prm.dt));
-    HIPCHK(c, nvk_nan(st, d, R, c->ctl));
+    HIPCHK(c, nvk_step_end(st, d, c->phys, R, c->ctl, c->prm.dt));
     return NV_OK;
 }
 
@@ -717,6 +728,7 @@ int record_steps(Ctx *c) {
     } else {
         HIPCHK(c, c->d_S.reserve(c->K));
         HIPCHK(c, c->d_poff.reserve(c->K));
+        HIPCHK(c, c->queue.reserve(1));
     }
     if (int rc = build_close_items(c, true, &npitems)) return rc;
     DState d0 = c->d;
@@ -888,7 +900,8 @@ int nv_destroy(void *h) {
     for (auto &row : c->sg)
         for (auto &g : row) drop_graph(g);
     free_state(c);
-    DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tn, &c->d_trecon, &c->d_tremove, &c->d_tnact, &c->d_row2t,
+    if (c->h_tcnt) (void)hipHostFree(c->h_tcnt);
+    DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tcnt, &c->d_row2t,
                          &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp, &c->pj, &c->d_S, &c->d_pS, &c->queue};
     for (auto *b : ib) b->release();
     c->rec64.release(); c->tinfo.release(); c->sph.release(); c->part.release(); c->pd.release(); c->rec32.release();
@@ -996,7 +1009,7 @@ int nv_pinsert(void *h) {
     Tangles tg = tangles(c, nullptr, nullptr);
     c->ctl_h->nactive = 0;
     HIPCHK(c, hipMemcpyAsync(&c->ctl->nactive, &c->ctl_h->nactive, sizeof(int), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, nv_fill_int(c->st, c->d_tnact.p, 0, c->K));
+    HIPCHK(c, nv_fill_int(c->st, c->p_tnact, 0, c->K));
     HIPCHK(c, nvk_pinsert_pre(c->st, c->d, c->phys, tg, c->rows, c->cand.p, c->empty.p, c->ctl));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
     if (int rc = compact(c, c->empty, c->lempty, &c->ctl->nempty)) return rc;

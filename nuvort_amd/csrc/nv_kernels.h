@@ -56,6 +56,9 @@ hipError_t nvk_relayout(hipStream_t st, DState o, DState nu, const int *row2t, c
 hipError_t nvk_clock_set(hipStream_t st, DCtl *ctl, double t, int itime);
 hipError_t nvk_clock_begin_step(hipStream_t st, DCtl *ctl);
 hipError_t nvk_clock_end_step(hipStream_t st, DCtl *ctl, double dt);
+// recorded-step head (itime, counters, queue) and tail (wrap, t, NaN test)
+hipError_t nvk_step_begin(hipStream_t st, DCtl *ctl, int *tnact, int K, int *queue);
+hipError_t nvk_step_end(hipStream_t st, DState s, Phys p, int n, DCtl *ctl, double dt);
 
 // boundary / topology / close search (k_topology.hip)
 hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n);
