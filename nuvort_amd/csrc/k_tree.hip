@@ -348,10 +348,14 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
+    Node56 nd = nwalk > 0 ? walk[0] : Node56{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
-        const Node56 nd = walk[cur];
         const bool leaf = nd.word < 0;
+        // both possible successors are fetched while this node is evaluated
+        const int n1 = cur + 1, n2 = leaf ? cur + 1 : nd.word;
+        const Node56 a = walk[n1 < nwalk ? n1 : cur];
+        const Node56 b = walk[n2 < nwalk ? n2 : cur];
         bool open = false;
         if (cur >= lskip) {
             ++nvis;
@@ -402,7 +406,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                 open = true;
             }
         }
-        cur = __any(open) ? cur + 1 : (leaf ? cur + 1 : nd.word);
+        const bool down = __any(open);
+        cur = down ? n1 : n2;
+        nd = down ? a : b;
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     if (r < npad) {
@@ -452,9 +458,13 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
+    WNode nd = nwalk > 0 ? walk[0] : WNode{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
-        const WNode nd = walk[cur];
+        // both possible successors are fetched while this node is evaluated
+        const int n1 = cur + 1, n2 = nd.skip;
+        const WNode a = walk[n1 < nwalk ? n1 : cur];
+        const WNode b = walk[n2 < nwalk ? n2 : cur];
         bool open = false;
         if (cur >= lskip) {
             ++nvis;
@@ -498,7 +508,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
                 open = true;
             }
         }
-        cur = __any(open) ? cur + 1 : nd.skip;
+        const bool down = __any(open);
+        cur = down ? n1 : n2;
+        nd = down ? a : b;
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     if (r < npad) {
@@ -537,9 +549,13 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const WNode 
     int closest = act ? s.closest[r] : 0;
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
+    WNode nd = nwalk > 0 ? walk[0] : WNode{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
-        const WNode nd = walk[cur];
+        // both possible successors are fetched while this node is evaluated
+        const int n1 = cur + 1, n2 = nd.skip;
+        const WNode a = walk[n1 < nwalk ? n1 : cur];
+        const WNode b = walk[n2 < nwalk ? n2 : cur];
         bool open = false;
         if (cur >= lskip) {
             const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
@@ -560,7 +576,9 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const WNode 
                 open = true;
             }
         }
-        cur = __any(open) ? cur + 1 : nd.skip;
+        const bool down = __any(open);
+        cur = down ? n1 : n2;
+        nd = down ? a : b;
     }
     if (act) {
         s.closest[r] = closest;
