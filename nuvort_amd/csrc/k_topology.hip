@@ -38,6 +38,13 @@ __device__ __forceinline__ void clear_row(const DState &s, int r) {
     s.closestd[r] = 0.0;
 }
 
+// the periodic image of neighbour coordinate g seen from coordinate xi, as
+// get_ghost_p does per component (src/boundary.f90:44-83)
+__device__ __forceinline__ void wrap_ghost(double xi, double L, double &g) {
+    if ((xi - g) > (L / 2.0)) g = g + L;
+    else if ((xi - g) < (-L / 2.0)) g = g - L;
+}
+
 __device__ __forceinline__ void store_ghosts(const DState &s, const Phys &p, int r) {
     double gi[3], gb[3];
     ghost_of(s, p, r, gi, gb);
@@ -200,6 +207,20 @@ __global__ void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand, 
         }
         const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
         if (!(disti > p.delta)) continue;
+        // every other input of the insertion, loaded together: the three
+        // ghost refreshes below are computed from these values, as
+        // store_ghosts (src/boundary.f90:26-84) would from the updated state
+        const int fi = s.infront[r] - 1, bb = s.behind[r] - 1;
+        const int ffi = s.infront[fi] - 1;
+        double xf[3], xff[3], xb[3], ur[3], uf[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            xf[k] = s.x[k * cap + fi];
+            xff[k] = s.x[k * cap + ffi];
+            xb[k] = s.x[k * cap + bb];
+            ur[k] = s.u[k * cap + r];
+            uf[k] = s.u[k * cap + fi];
+        }
         int slot;
         if (ei < e1) {
             slot = empty[ei++];
@@ -232,21 +253,43 @@ __global__ void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand, 
 #pragma unroll
             for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
         }
-        const int fi = s.infront[r] - 1;
+        // the new point, its links, and the ghosts of the new point, of r and
+        // of its old infront fi from the positions now in place (fi's
+        // infront and r's behind are unchanged by this insertion)
+        double gs_i[3], gs_b[3], gr_i[3], gr_b[3], gf_i[3], gf_b[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gs_i[k] = xf[k]; gs_b[k] = x[k];
+            gr_i[k] = xn[k]; gr_b[k] = xb[k];
+            gf_i[k] = xff[k]; gf_b[k] = xn[k];
+            if (!p.periodic[k]) continue;
+            const double L = p.box[k];
+            wrap_ghost(xn[k], L, gs_i[k]);
+            wrap_ghost(xn[k], L, gs_b[k]);
+            wrap_ghost(x[k], L, gr_i[k]);
+            wrap_ghost(x[k], L, gr_b[k]);
+            wrap_ghost(xf[k], L, gf_i[k]);
+            wrap_ghost(xf[k], L, gf_b[k]);
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             s.x[k * cap + slot] = xn[k];
-            s.u[k * cap + slot] = 0.5 * (s.u[k * cap + r] + s.u[k * cap + fi]);
+            s.u[k * cap + slot] = 0.5 * (ur[k] + uf[k]);
             s.u1[k * cap + slot] = 0.0;
             s.u2[k * cap + slot] = 0.0;
+            s.gi[k * cap + slot] = gs_i[k];
+            s.gb[k * cap + slot] = gs_b[k];
+            // fi before r, the reference's refresh order (they coincide only
+            // for a one-point loop)
+            s.gi[k * cap + fi] = gf_i[k];
+            s.gb[k * cap + fi] = gf_b[k];
+            s.gi[k * cap + r] = gr_i[k];
+            s.gb[k * cap + r] = gr_b[k];
         }
         s.behind[slot] = r + 1;
-        s.infront[slot] = s.infront[r];
-        store_ghosts(s, p, slot);
+        s.infront[slot] = fi + 1;
         s.behind[fi] = slot + 1;
-        store_ghosts(s, p, fi);
         s.infront[r] = slot + 1;
-        store_ghosts(s, p, r);
         // rows the reference loop still has to reach whose inputs changed
         if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
         if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
