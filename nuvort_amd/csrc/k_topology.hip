@@ -168,133 +168,223 @@ __global__ void k_pinsert_pre(DState s, Phys p, Tangles tg, int n, int *cand, in
         c = disti > p.delta ? 1 : 0;
     }
     if (r < n) cand[r] = c;
-    // active points per tangle for the zero-length stop (src/line.f90:25-27);
-    // tangles are 256-row aligned, so a wavefront lies in one tangle
-    const unsigned long long m = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && m) {
-        atomicAdd(&ctl->nactive, (int)__popcll(m));
-        atomicAdd(&tg.nact[tg.row2t[r]], (int)__popcll(m));
+}
+
+// Inputs of one insertion at row r (src/line.f90:30-91): r's position and
+// ghosts, its links, its infront's position, infront and velocity, its
+// behind's position, and its own velocity.
+struct InsIn {
+    double x[3], gi[3], gb[3], xf[3], xff[3], xb[3], ur[3], uf[3];
+    int fi, bb, act;
+};
+
+__device__ __forceinline__ void load_insertion(const DState &s, int r, InsIn &in) {
+    const int cap = s.cap;
+    in.act = s.infront[r] != 0;
+    in.fi = in.bb = r;
+    if (!in.act) return;
+    in.fi = s.infront[r] - 1;
+    in.bb = s.behind[r] - 1;
+    const int ffi = s.infront[in.fi] - 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        in.x[k] = s.x[k * cap + r];
+        in.gi[k] = s.gi[k * cap + r];
+        in.gb[k] = s.gb[k * cap + r];
+        in.xf[k] = s.x[k * cap + in.fi];
+        in.xff[k] = s.x[k * cap + ffi];
+        in.xb[k] = s.x[k * cap + in.bb];
+        in.ur[k] = s.u[k * cap + r];
+        in.uf[k] = s.u[k * cap + in.fi];
     }
 }
 
-__global__ void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand, const int *empty, DCtl *ctl) {
-    if (threadIdx.x != 0) return;
+// One visit of the reference loop at row r (src/line.f90:30-91) from its
+// inputs: returns the slot used (or -1 when r does not insert); -2 when the
+// tangle's rows are exhausted.
+__device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in, const int *empty, int &ei,
+                         int e1, int &nl, int base, int tcap) {
+    const int cap = s.cap;
+    if (!in.act) return -1;
+    const double disti = dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]);
+    if (!(disti > p.delta)) return -1;
+    int slot;
+    if (ei < e1) {
+        slot = empty[ei++];
+    } else {
+        if (nl >= tcap) return -2;
+        slot = base + nl;
+        ++nl;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s.usup[k * cap + slot] = 0.0;
+            s.umf[k * cap + slot] = 0.0;
+        }
+        clear_row(s, slot);
+    }
+    double fd[3], fdd[3];
+    derivs(in.x, in.gi, in.gb, fd, fdd);
+    double curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
+    double xn[3];
+    if (curv > 1e-5) {
+        curv = 1.0 / curv;
+        if (curv * curv - 0.25 * (disti * disti) > 0.0) {
+            const double rr = (sqrt(curv * curv - 0.25 * (disti * disti)) - curv) * curv;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (in.x[k] + in.gi[k]) + rr * fdd[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (in.x[k] + in.gi[k]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (in.x[k] + in.gi[k]);
+    }
+    // the new point, its links, and the ghosts of the new point, of r and of
+    // its old infront fi from the positions now in place, as store_ghosts
+    // (src/boundary.f90:26-84) computes them (fi's infront and r's behind are
+    // unchanged by this insertion)
+    const int fi = in.fi;
+    double gs_i[3], gs_b[3], gr_i[3], gr_b[3], gf_i[3], gf_b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gs_i[k] = in.xf[k]; gs_b[k] = in.x[k];
+        gr_i[k] = xn[k]; gr_b[k] = in.xb[k];
+        gf_i[k] = in.xff[k]; gf_b[k] = xn[k];
+        if (!p.periodic[k]) continue;
+        const double L = p.box[k];
+        wrap_ghost(xn[k], L, gs_i[k]);
+        wrap_ghost(xn[k], L, gs_b[k]);
+        wrap_ghost(in.x[k], L, gr_i[k]);
+        wrap_ghost(in.x[k], L, gr_b[k]);
+        wrap_ghost(in.xf[k], L, gf_i[k]);
+        wrap_ghost(in.xf[k], L, gf_b[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.x[k * cap + slot] = xn[k];
+        s.u[k * cap + slot] = 0.5 * (in.ur[k] + in.uf[k]);
+        s.u1[k * cap + slot] = 0.0;
+        s.u2[k * cap + slot] = 0.0;
+        s.gi[k * cap + slot] = gs_i[k];
+        s.gb[k * cap + slot] = gs_b[k];
+        // fi before r, the reference's refresh order (they coincide only for
+        // a one-point loop)
+        s.gi[k * cap + fi] = gf_i[k];
+        s.gb[k * cap + fi] = gf_b[k];
+        s.gi[k * cap + r] = gr_i[k];
+        s.gb[k * cap + r] = gr_b[k];
+    }
+    s.behind[slot] = r + 1;
+    s.infront[slot] = fi + 1;
+    s.behind[fi] = slot + 1;
+    s.infront[r] = slot + 1;
+    return slot;
+}
+
+constexpr int INS_CHUNK = 128;
+
+// One workgroup per tangle.  Candidates are taken INS_CHUNK at a time: every
+// thread loads one candidate's inputs into LDS, then thread 0 makes the
+// reference's visits in row order (candidates merged with the dirty queue).
+// A staged candidate whose inputs an earlier insertion changed -- it is the
+// old infront of an inserting row (its behind and ghosts), or its infront
+// inserted (its infront's infront) -- is reloaded from memory.
+__global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand,
+                                                              const int *empty, DCtl *ctl) {
     __shared__ Dirty d;
-    d.hd = d.nd = 0;
+    __shared__ InsIn st[INS_CHUNK];
+    __shared__ int strow[INS_CHUNK];
+    __shared__ unsigned char stale[INS_CHUNK];
+    __shared__ int nl_sh, stop_sh;
+    const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk], tcap = tg.cap[tk];
-    int nl = tg.n[tk];                     // local pcount
-    const int n_old = base + nl;           // rows the reference loop visits end here
-    // this tangle's slices of the candidate and empty-slot lists
+    const int n_old = base + tg.n[tk];     // rows the reference loop visits end here
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tcap);
     const int e0 = lower_row(empty, ctl->nempty, base), e1 = lower_row(empty, ctl->nempty, base + tcap);
-    // src/line.f90:25-27: no active point in this tangle
-    if (tg.nact[tk] == 0) { atomicOr(&ctl->status, NVK_ZERO_LENGTH); return; }
+    // src/line.f90:25-27: no active point in this tangle (active points =
+    // pcount less the empty slots below it)
+    if (tg.n[tk] - (e1 - e0) == 0) {
+        if (tid == 0) atomicOr(&ctl->status, NVK_ZERO_LENGTH);
+        return;
+    }
     const int *cl = cand + c0;
     const int ncand = c1 - c0;
-    const int cap = s.cap;
-    int ci = 0, ei = e0;
-    while (true) {
-        const int r = next_row(cl, ncand, ci, d);
-        if (r == 0x7fffffff) break;
-        if (s.infront[r] == 0) continue;
-        double x[3], gi[3], gb[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            x[k] = s.x[k * cap + r];
-            gi[k] = s.gi[k * cap + r];
-            gb[k] = s.gb[k * cap + r];
-        }
-        const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
-        if (!(disti > p.delta)) continue;
-        // every other input of the insertion, loaded together: the three
-        // ghost refreshes below are computed from these values, as
-        // store_ghosts (src/boundary.f90:26-84) would from the updated state
-        const int fi = s.infront[r] - 1, bb = s.behind[r] - 1;
-        const int ffi = s.infront[fi] - 1;
-        double xf[3], xff[3], xb[3], ur[3], uf[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            xf[k] = s.x[k * cap + fi];
-            xff[k] = s.x[k * cap + ffi];
-            xb[k] = s.x[k * cap + bb];
-            ur[k] = s.u[k * cap + r];
-            uf[k] = s.u[k * cap + fi];
-        }
-        int slot;
-        if (ei < e1) {
-            slot = empty[ei++];
-        } else {
-            if (nl >= tcap) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
-            slot = base + nl;
-            ++nl;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                s.usup[k * cap + slot] = 0.0;
-                s.umf[k * cap + slot] = 0.0;
-            }
-            clear_row(s, slot);
-        }
-        double fd[3], fdd[3];
-        derivs(x, gi, gb, fd, fdd);
-        double curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
-        double xn[3];
-        if (curv > 1e-5) {
-            curv = 1.0 / curv;
-            if (curv * curv - 0.25 * (disti * disti) > 0.0) {
-                const double rr = (sqrt(curv * curv - 0.25 * (disti * disti)) - curv) * curv;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]) + rr * fdd[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
-        }
-        // the new point, its links, and the ghosts of the new point, of r and
-        // of its old infront fi from the positions now in place (fi's
-        // infront and r's behind are unchanged by this insertion)
-        double gs_i[3], gs_b[3], gr_i[3], gr_b[3], gf_i[3], gf_b[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            gs_i[k] = xf[k]; gs_b[k] = x[k];
-            gr_i[k] = xn[k]; gr_b[k] = xb[k];
-            gf_i[k] = xff[k]; gf_b[k] = xn[k];
-            if (!p.periodic[k]) continue;
-            const double L = p.box[k];
-            wrap_ghost(xn[k], L, gs_i[k]);
-            wrap_ghost(xn[k], L, gs_b[k]);
-            wrap_ghost(x[k], L, gr_i[k]);
-            wrap_ghost(x[k], L, gr_b[k]);
-            wrap_ghost(xf[k], L, gf_i[k]);
-            wrap_ghost(xf[k], L, gf_b[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            s.x[k * cap + slot] = xn[k];
-            s.u[k * cap + slot] = 0.5 * (ur[k] + uf[k]);
-            s.u1[k * cap + slot] = 0.0;
-            s.u2[k * cap + slot] = 0.0;
-            s.gi[k * cap + slot] = gs_i[k];
-            s.gb[k * cap + slot] = gs_b[k];
-            // fi before r, the reference's refresh order (they coincide only
-            // for a one-point loop)
-            s.gi[k * cap + fi] = gf_i[k];
-            s.gb[k * cap + fi] = gf_b[k];
-            s.gi[k * cap + r] = gr_i[k];
-            s.gb[k * cap + r] = gr_b[k];
-        }
-        s.behind[slot] = r + 1;
-        s.infront[slot] = fi + 1;
-        s.behind[fi] = slot + 1;
-        s.infront[r] = slot + 1;
-        // rows the reference loop still has to reach whose inputs changed
-        if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
-        if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+    if (tid == 0) {
+        d.hd = d.nd = 0;
+        nl_sh = tg.n[tk];
+        stop_sh = 0;
     }
-    tg.n[tk] = nl;
+    int ei = e0;   // used by thread 0 only
+    for (int cs = 0; cs <= ncand; cs += INS_CHUNK) {
+        const int m = min(INS_CHUNK, ncand - cs);
+        __syncthreads();
+        if (tid < m) {
+            const int r = cl[cs + tid];
+            strow[tid] = r;
+            stale[tid] = 0;
+            load_insertion(s, r, st[tid]);
+        }
+        __syncthreads();
+        if (tid == 0 && !stop_sh) {
+            // rows below the next chunk's first candidate are this chunk's
+            const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
+            int nl = nl_sh;
+            int k = 0;
+            while (true) {
+                const int rc = k < m ? strow[k] : 0x7fffffff;
+                const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+                const int r = rc < rd ? rc : rd;
+                if (r == 0x7fffffff || r >= limit) break;
+                InsIn in;
+                const bool staged = (rc == r) && !stale[k];
+                if (rc == r) ++k;
+                if (rd == r) ++d.hd;
+                if (staged) in = st[k - 1];
+                else load_insertion(s, r, in);
+                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap);
+                if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
+                if (slot < 0) continue;
+                const int fi = in.fi, bb = in.bb;
+                // staged rows after r whose inputs changed
+                for (int q = k; q < m; ++q) {
+                    const int rq = strow[q];
+                    if (rq > fi && rq > bb) break;
+                    if (rq == fi || rq == bb) stale[q] = 1;
+                }
+                // rows the reference loop still has to reach whose inputs changed
+                if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
+                if (slot > r && slot < n_old && !dirty_push(d, slot)) {
+                    atomicOr(&ctl->status, NVK_CAPACITY);
+                    stop_sh = 1;
+                    break;
+                }
+            }
+            nl_sh = nl;
+        }
+        if (cs + m >= ncand) {
+            // the last chunk: dirty rows beyond it are visited from memory
+            __syncthreads();
+            if (tid == 0 && !stop_sh) {
+                int nl = nl_sh;
+                while (d.hd < d.nd) {
+                    const int r = d.v[d.hd++];
+                    InsIn in;
+                    load_insertion(s, r, in);
+                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap);
+                    if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+                    if (slot < 0) continue;
+                    if (in.fi > r && in.fi < n_old && !dirty_push(d, in.fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+                    if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+                }
+                nl_sh = nl;
+            }
+            break;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) tg.n[tk] = nl_sh;
 }
 
 // ----------------------------------------------------------------- precon
@@ -324,9 +414,123 @@ __device__ __forceinline__ double dist_rows(const DState &s, const double *a, in
     return dist_gen(s.x[ra], s.x[s.cap + ra], s.x[2 * s.cap + ra], b[rb], b[s.cap + rb], b[2 * s.cap + rb]);
 }
 
-__global__ void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl, nv_recon_event *ev,
-                                int ev_cap) {
-    if (threadIdx.x != 0) return;
+// Inputs of one precon visit at row r (src/reconnection.f90:18-54).
+struct ReconIn {
+    double x[3], gi[3], gb[3];       // r
+    double xj[3], gij[3], gbj[3];    // its closest point rj
+    double xpi[3], xpjb[3];          // infront(r) and behind(rj)
+    double cd;
+    int act, pari, behr, j, actj, parjb;
+};
+
+__device__ __forceinline__ void load_recon(const DState &s, int r, ReconIn &in) {
+    const int cap = s.cap;
+    in.act = s.infront[r] != 0;
+    in.pari = s.infront[r];
+    in.behr = s.behind[r];
+    in.cd = s.closestd[r];
+    in.j = s.closest[r];
+    in.actj = 0;
+    in.parjb = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        in.x[k] = s.x[k * cap + r];
+        in.gi[k] = s.gi[k * cap + r];
+        in.gb[k] = s.gb[k * cap + r];
+    }
+    if (in.j <= 0 || !in.act) return;
+    const int rj = in.j - 1;
+    in.actj = s.infront[rj] != 0;
+    in.parjb = s.behind[rj];
+    const int pi = in.pari > 0 ? in.pari - 1 : r, pjb = in.parjb > 0 ? in.parjb - 1 : rj;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        in.xj[k] = s.x[k * cap + rj];
+        in.gij[k] = s.gi[k * cap + rj];
+        in.gbj[k] = s.gb[k * cap + rj];
+        in.xpi[k] = s.x[k * cap + pi];
+        in.xpjb[k] = s.x[k * cap + pjb];
+    }
+}
+
+// src/general.f90:106-117 norm_tanf from a point's position and ghosts
+__device__ __forceinline__ void norm_tanf_of(const double x[3], const double gi[3], const double gb[3], double t[3]) {
+    double fdd[3];
+    derivs(x, gi, gb, t, fdd);
+    const double length = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = t[k] / length;
+}
+
+// rows whose links, validity or contents a reconnection changed since the
+// chunk was staged: open-addressing set in LDS
+constexpr int MOD_SLOTS = 4096;
+
+struct ModSet {
+    int v[MOD_SLOTS];
+    int n;
+};
+
+__device__ __forceinline__ bool mod_has(const ModSet &m, int r) {
+    unsigned h = ((unsigned)r * 2654435761u) & (MOD_SLOTS - 1);
+    for (int q = 0; q < MOD_SLOTS; ++q) {
+        const int v = m.v[h];
+        if (v == r) return true;
+        if (v == -1) return false;
+        h = (h + 1) & (MOD_SLOTS - 1);
+    }
+    return false;
+}
+
+// false once the set is half full (the caller then reloads every visit)
+__device__ __forceinline__ bool mod_add(ModSet &m, int r) {
+    if (m.n >= MOD_SLOTS / 2) return false;
+    unsigned h = ((unsigned)r * 2654435761u) & (MOD_SLOTS - 1);
+    while (m.v[h] != -1) {
+        if (m.v[h] == r) return true;
+        h = (h + 1) & (MOD_SLOTS - 1);
+    }
+    m.v[h] = r;
+    ++m.n;
+    return true;
+}
+
+// loop_killer (above) that also records the rows it clears
+__device__ bool loop_killer_rec(const DState &s, int particle, int n, ModSet &m) {
+    int counter = 1, next = particle;
+    for (int it = 1; it <= n; ++it) {
+        next = s.infront[next] - 1;
+        if (next == particle) break;
+        if (next < 0) return true;
+        counter = counter + 1;
+        if (counter >= 4) return true;
+    }
+    bool ok = true;
+    if (counter < 4) {
+        next = particle;
+        for (int it = 1; it <= n; ++it) {
+            int store_next = s.infront[next] - 1;
+            clear_row(s, next);
+            ok = mod_add(m, next) && ok;
+            next = store_next;
+            if (next == particle || next < 0) break;
+        }
+    }
+    return ok;
+}
+
+constexpr int REC_CHUNK = 128;
+
+// One workgroup per tangle: candidates staged REC_CHUNK at a time (every
+// thread loads one), thread 0 visits them in row order.  A staged visit is
+// reloaded from memory when a reconnection earlier in the chunk changed any
+// row it read (r, its closest, infront(r), behind(closest)).
+__global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
+                                                             nv_recon_event *ev, int ev_cap) {
+    __shared__ ReconIn st[REC_CHUNK];
+    __shared__ int strow[REC_CHUNK];
+    __shared__ ModSet mod;
+    const int tid = threadIdx.x;
     const double t = ctl->t;
     const int itime = ctl->itime;
     const int tk = blockIdx.x;
@@ -334,46 +538,77 @@ __global__ void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, D
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
     const int n = tg.n[tk];
     const int cap = s.cap;
-    for (int ci = c0; ci < c1; ++ci) {
-        const int r = cand[ci];
-        if (s.infront[r] == 0) continue;
-        const int pari = s.infront[r];
-        const double cd = s.closestd[r];
-        if (!((cd < p.delta / 2.0) && (cd > NV_EPS))) continue;
-        const int j = s.closest[r];
-        if (j == pari) continue;
-        if (j == s.behind[r]) continue;
-        if (j == 0) continue;
-        const int rj = j - 1;
-        if (s.infront[rj] == 0) continue;
-        const int parjb = s.behind[rj];
-        double t1[3], t2[3];
-        norm_tanf(s, r, t1);
-        norm_tanf(s, rj, t2);
-        const double dot_val = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
-        if (dot_val > 0.9) continue;
-        const double l_before = dist_rows(s, s.x, r, s.gi, r) + dist_rows(s, s.x, r, s.gb, r) +
-                                dist_rows(s, s.x, rj, s.gi, rj) + dist_rows(s, s.x, rj, s.gb, rj);
-        const double l_after = dist_rows(s, s.x, r, s.gb, r) + dist_rows(s, s.x, r, s.x, rj) +
-                               dist_rows(s, s.x, rj, s.gi, rj) + dist_rows(s, s.x, pari - 1, s.x, parjb - 1);
-        if (l_after <= l_before) {
-            tg.recon[tk] += 1;
-            const int q = atomicAdd(&ctl->nevents, 1);
-            if (q < ev_cap) {
-                nv_recon_event e;
-                e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) e.mid[k] = 0.5 * (s.x[k * cap + r] + s.x[k * cap + rj]);
-                e.angle = acos(dot_val);
-                e.dl = l_before - l_after;
-                ev[q] = e;
+    for (int cs = c0; cs < c1; cs += REC_CHUNK) {
+        const int m = min(REC_CHUNK, c1 - cs);
+        __syncthreads();
+        for (int q = tid; q < MOD_SLOTS; q += REC_CHUNK) mod.v[q] = -1;
+        if (tid == 0) mod.n = 0;
+        if (tid < m) {
+            strow[tid] = cand[cs + tid];
+            load_recon(s, strow[tid], st[tid]);
+        }
+        __syncthreads();
+        if (tid != 0) continue;
+        bool tracked = true;   // the modified set is complete
+        for (int k = 0; k < m; ++k) {
+            const int r = strow[k];
+            ReconIn in = st[k];
+            bool fresh = tracked && !mod_has(mod, r);
+            if (fresh && in.act && in.j > 0) {
+                const int rj = in.j - 1;
+                fresh = !mod_has(mod, rj) && (in.pari <= 0 || !mod_has(mod, in.pari - 1)) &&
+                        (in.parjb <= 0 || !mod_has(mod, in.parjb - 1));
             }
-            s.infront[parjb - 1] = pari;
-            s.behind[pari - 1] = parjb;
-            s.infront[r] = j;
-            s.behind[rj] = r + 1;
-            loop_killer(s, pari - 1, n);
-            loop_killer(s, r, n);
+            if (!fresh) load_recon(s, r, in);
+            if (!in.act) continue;
+            const int pari = in.pari;
+            const double cd = in.cd;
+            if (!((cd < p.delta / 2.0) && (cd > NV_EPS))) continue;
+            const int j = in.j;
+            if (j == pari) continue;
+            if (j == in.behr) continue;
+            if (j == 0) continue;
+            const int rj = j - 1;
+            if (!in.actj) continue;
+            const int parjb = in.parjb;
+            double t1[3], t2[3];
+            norm_tanf_of(in.x, in.gi, in.gb, t1);
+            norm_tanf_of(in.xj, in.gij, in.gbj, t2);
+            const double dot_val = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
+            if (dot_val > 0.9) continue;
+            const double l_before =
+                dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]) +
+                dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
+                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
+                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gbj[0], in.gbj[1], in.gbj[2]);
+            const double l_after =
+                dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
+                dist_gen(in.x[0], in.x[1], in.x[2], in.xj[0], in.xj[1], in.xj[2]) +
+                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
+                dist_gen(in.xpi[0], in.xpi[1], in.xpi[2], in.xpjb[0], in.xpjb[1], in.xpjb[2]);
+            if (l_after <= l_before) {
+                tg.recon[tk] += 1;
+                const int q = atomicAdd(&ctl->nevents, 1);
+                if (q < ev_cap) {
+                    nv_recon_event e;
+                    e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
+#pragma unroll
+                    for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
+                    e.angle = acos(dot_val);
+                    e.dl = l_before - l_after;
+                    ev[q] = e;
+                }
+                s.infront[parjb - 1] = pari;
+                s.behind[pari - 1] = parjb;
+                s.infront[r] = j;
+                s.behind[rj] = r + 1;
+                tracked = mod_add(mod, parjb - 1) && tracked;
+                tracked = mod_add(mod, pari - 1) && tracked;
+                tracked = mod_add(mod, r) && tracked;
+                tracked = mod_add(mod, rj) && tracked;
+                tracked = loop_killer_rec(s, pari - 1, n, mod) && tracked;
+                tracked = loop_killer_rec(s, r, n, mod) && tracked;
+            }
         }
     }
 }
@@ -716,7 +951,7 @@ hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, Tangles tg, int n, 
 }
 hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, const int *empty,
                               DCtl *ctl) {
-    hipLaunchKernelGGL(k_pinsert_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, empty, ctl);
+    hipLaunchKernelGGL(k_pinsert_serial, dim3(tg.K), dim3(INS_CHUNK), 0, st, s, p, tg, cand, empty, ctl);
     return hipGetLastError();
 }
 hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
@@ -725,7 +960,7 @@ hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
 }
 hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
                              nv_recon_event *ev, int ev_cap) {
-    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl, ev, ev_cap);
+    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(REC_CHUNK), 0, st, s, p, tg, cand, ctl, ev, ev_cap);
     return hipGetLastError();
 }
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
