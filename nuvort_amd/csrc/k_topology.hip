@@ -199,11 +199,17 @@ __device__ __forceinline__ void load_insertion(const DState &s, int r, InsIn &in
     }
 }
 
+// What an insertion leaves behind for later visits: the new point and the
+// refreshed infront ghosts of the new point and of r's old infront.
+struct InsOut {
+    double xn[3], gs_i[3], gf_i[3], gf_b[3];
+};
+
 // One visit of the reference loop at row r (src/line.f90:30-91) from its
 // inputs: returns the slot used (or -1 when r does not insert); -2 when the
 // tangle's rows are exhausted.
 __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in, const int *empty, int &ei,
-                         int e1, int &nl, int base, int tcap) {
+                         int e1, int &nl, int base, int tcap, InsOut &out) {
     const int cap = s.cap;
     if (!in.act) return -1;
     const double disti = dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]);
@@ -279,23 +285,63 @@ __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in,
     s.infront[slot] = fi + 1;
     s.behind[fi] = slot + 1;
     s.infront[r] = slot + 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        out.xn[k] = xn[k];
+        out.gs_i[k] = gs_i[k];
+        out.gf_i[k] = gf_i[k];
+        out.gf_b[k] = gf_b[k];
+    }
     return slot;
+}
+
+// After an insertion at r (old infront fi, behind bb, new point in slot):
+// the staged inputs of later candidates fi and bb take the new state in
+// place (fi: ghosts and behind; bb: its infront's infront is the new
+// point), and a row the reference loop will revisit is queued only when
+// that visit inserts -- a revisit tests |x - ghost_infront| > delta, and the
+// ghost it will see is the one just computed (a row's infront ghost changes
+// only when the row itself inserts).
+__device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn &in, int slot, const InsOut &o,
+                             InsIn *st, const int *strow, int k, int m, Dirty &d, int n_old) {
+    const int fi = in.fi, bb = in.bb;
+    for (int q = k; q < m; ++q) {
+        const int rq = strow[q];
+        if (rq > fi && rq > bb) break;
+        if (rq == fi) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                st[q].gi[c] = o.gf_i[c];
+                st[q].gb[c] = o.gf_b[c];
+                st[q].xb[c] = o.xn[c];
+            }
+            st[q].bb = slot;
+        }
+        if (rq == bb) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) st[q].xff[c] = o.xn[c];
+        }
+    }
+    if (fi > r && fi < n_old &&
+        dist_gen(in.xf[0], in.xf[1], in.xf[2], o.gf_i[0], o.gf_i[1], o.gf_i[2]) > p.delta && !dirty_push(d, fi))
+        return false;
+    if (slot > r && slot < n_old &&
+        dist_gen(o.xn[0], o.xn[1], o.xn[2], o.gs_i[0], o.gs_i[1], o.gs_i[2]) > p.delta && !dirty_push(d, slot))
+        return false;
+    return true;
 }
 
 constexpr int INS_CHUNK = 128;
 
 // One workgroup per tangle.  Candidates are taken INS_CHUNK at a time: every
 // thread loads one candidate's inputs into LDS, then thread 0 makes the
-// reference's visits in row order (candidates merged with the dirty queue).
-// A staged candidate whose inputs an earlier insertion changed -- it is the
-// old infront of an inserting row (its behind and ghosts), or its infront
-// inserted (its infront's infront) -- is reloaded from memory.
+// reference's visits in row order (candidates merged with the dirty queue);
+// after_insert keeps the staged inputs current.
 __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand,
                                                               const int *empty, DCtl *ctl) {
     __shared__ Dirty d;
     __shared__ InsIn st[INS_CHUNK];
     __shared__ int strow[INS_CHUNK];
-    __shared__ unsigned char stale[INS_CHUNK];
     __shared__ int nl_sh, stop_sh;
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
@@ -323,7 +369,6 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         if (tid < m) {
             const int r = cl[cs + tid];
             strow[tid] = r;
-            stale[tid] = 0;
             load_insertion(s, r, st[tid]);
         }
         __syncthreads();
@@ -338,24 +383,16 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 const int r = rc < rd ? rc : rd;
                 if (r == 0x7fffffff || r >= limit) break;
                 InsIn in;
-                const bool staged = (rc == r) && !stale[k];
+                const bool staged = rc == r;
                 if (rc == r) ++k;
                 if (rd == r) ++d.hd;
                 if (staged) in = st[k - 1];
                 else load_insertion(s, r, in);
-                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap);
+                InsOut o;
+                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o);
                 if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
                 if (slot < 0) continue;
-                const int fi = in.fi, bb = in.bb;
-                // staged rows after r whose inputs changed
-                for (int q = k; q < m; ++q) {
-                    const int rq = strow[q];
-                    if (rq > fi && rq > bb) break;
-                    if (rq == fi || rq == bb) stale[q] = 1;
-                }
-                // rows the reference loop still has to reach whose inputs changed
-                if (fi > r && fi < n_old && !dirty_push(d, fi)) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
-                if (slot > r && slot < n_old && !dirty_push(d, slot)) {
+                if (!after_insert(s, p, r, in, slot, o, st, strow, k, m, d, n_old)) {
                     atomicOr(&ctl->status, NVK_CAPACITY);
                     stop_sh = 1;
                     break;
@@ -372,11 +409,14 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                     const int r = d.v[d.hd++];
                     InsIn in;
                     load_insertion(s, r, in);
-                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap);
+                    InsOut o;
+                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o);
                     if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
                     if (slot < 0) continue;
-                    if (in.fi > r && in.fi < n_old && !dirty_push(d, in.fi)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
-                    if (slot > r && slot < n_old && !dirty_push(d, slot)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+                    if (!after_insert(s, p, r, in, slot, o, st, strow, 0, 0, d, n_old)) {
+                        atomicOr(&ctl->status, NVK_CAPACITY);
+                        break;
+                    }
                 }
                 nl_sh = nl;
             }
@@ -421,6 +461,9 @@ struct ReconIn {
     double xpi[3], xpjb[3];          // infront(r) and behind(rj)
     double cd;
     int act, pari, behr, j, actj, parjb;
+    // the visit's tests, evaluated on these inputs
+    double dot, lb, la;
+    int go;
 };
 
 __device__ __forceinline__ void load_recon(const DState &s, int r, ReconIn &in) {
@@ -519,6 +562,33 @@ __device__ bool loop_killer_rec(const DState &s, int particle, int n, ModSet &m)
     return ok;
 }
 
+// src/reconnection.f90:20-43: the tests of one visit; go = reconnect
+__device__ void recon_tests(const Phys &p, ReconIn &in) {
+    in.go = 0;
+    if (!in.act) return;
+    const double cd = in.cd;
+    if (!((cd < p.delta / 2.0) && (cd > NV_EPS))) return;
+    const int j = in.j;
+    if (j == in.pari) return;
+    if (j == in.behr) return;
+    if (j == 0) return;
+    if (!in.actj) return;
+    double t1[3], t2[3];
+    norm_tanf_of(in.x, in.gi, in.gb, t1);
+    norm_tanf_of(in.xj, in.gij, in.gbj, t2);
+    in.dot = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
+    if (in.dot > 0.9) return;
+    in.lb = dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]) +
+            dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gbj[0], in.gbj[1], in.gbj[2]);
+    in.la = dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
+            dist_gen(in.x[0], in.x[1], in.x[2], in.xj[0], in.xj[1], in.xj[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
+            dist_gen(in.xpi[0], in.xpi[1], in.xpi[2], in.xpjb[0], in.xpjb[1], in.xpjb[2]);
+    in.go = in.la <= in.lb;
+}
+
 constexpr int REC_CHUNK = 128;
 
 // One workgroup per tangle: candidates staged REC_CHUNK at a time (every
@@ -546,6 +616,7 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
         if (tid < m) {
             strow[tid] = cand[cs + tid];
             load_recon(s, strow[tid], st[tid]);
+            recon_tests(p, st[tid]);
         }
         __syncthreads();
         if (tid != 0) continue;
@@ -559,34 +630,14 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
                 fresh = !mod_has(mod, rj) && (in.pari <= 0 || !mod_has(mod, in.pari - 1)) &&
                         (in.parjb <= 0 || !mod_has(mod, in.parjb - 1));
             }
-            if (!fresh) load_recon(s, r, in);
-            if (!in.act) continue;
-            const int pari = in.pari;
-            const double cd = in.cd;
-            if (!((cd < p.delta / 2.0) && (cd > NV_EPS))) continue;
-            const int j = in.j;
-            if (j == pari) continue;
-            if (j == in.behr) continue;
-            if (j == 0) continue;
-            const int rj = j - 1;
-            if (!in.actj) continue;
-            const int parjb = in.parjb;
-            double t1[3], t2[3];
-            norm_tanf_of(in.x, in.gi, in.gb, t1);
-            norm_tanf_of(in.xj, in.gij, in.gbj, t2);
-            const double dot_val = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
-            if (dot_val > 0.9) continue;
-            const double l_before =
-                dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]) +
-                dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
-                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
-                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gbj[0], in.gbj[1], in.gbj[2]);
-            const double l_after =
-                dist_gen(in.x[0], in.x[1], in.x[2], in.gb[0], in.gb[1], in.gb[2]) +
-                dist_gen(in.x[0], in.x[1], in.x[2], in.xj[0], in.xj[1], in.xj[2]) +
-                dist_gen(in.xj[0], in.xj[1], in.xj[2], in.gij[0], in.gij[1], in.gij[2]) +
-                dist_gen(in.xpi[0], in.xpi[1], in.xpi[2], in.xpjb[0], in.xpjb[1], in.xpjb[2]);
-            if (l_after <= l_before) {
+            if (!fresh) {
+                load_recon(s, r, in);
+                recon_tests(p, in);
+            }
+            if (!in.go) continue;
+            const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
+            const double dot_val = in.dot, l_before = in.lb, l_after = in.la;
+            {
                 tg.recon[tk] += 1;
                 const int q = atomicAdd(&ctl->nevents, 1);
                 if (q < ev_cap) {
