@@ -24,6 +24,7 @@
 // recursive walk would, in the same order, while all node reads are
 // wave-uniform (scalar loads).
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "nv_tree.h"
@@ -318,6 +319,25 @@ __device__ __forceinline__ void tree_term64(double v0, double v1, double v2, con
     }
 }
 
+// The node term for the FP32 walk's FP64 cases (near leaves, ill-conditioned
+// far nodes): the reference's expression (src/tree.f90:297-307) with
+// a = dist**2 taken as dd, both inverse distances from rsqrt and one
+// division, accurate to a few FP64 ulps (the FP32 walk is not bit-exact).
+__device__ __forceinline__ void near_term64(double v0, double v1, double v2, const double c[3], double dd,
+                                            double kappa, double nr[3], unsigned long long &nev) {
+    const double b = 2.0 * (v0 * c[0] + v1 * c[1] + v2 * c[2]);
+    const double cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    const double D = 4.0 * dd * cc - b * b;
+    if (!(D < NV_EPS)) {
+        const double fac = (2.0 * cc + b) * rsqrt(dd + b + cc) - b * rsqrt(dd);
+        const double sc = kappa * fac / ((2.0 * NV_PI) * D);
+        nr[0] = nr[0] + (v1 * c[2] - v2 * c[1]) * sc;
+        nr[1] = nr[1] + (v2 * c[0] - v0 * c[2]) * sc;
+        nr[2] = nr[2] + (v0 * c[1] - v1 * c[0]) * sc;
+        ++nev;
+    }
+}
+
 // The walk of k_walk_vel with FP32 interactions: displacements in FP64, the
 // node term in the well-conditioned form (as the direct sum's far field),
 // Kahan-compensated FP32 accumulation.  The reference's FP64 term is kept for
@@ -326,10 +346,12 @@ __device__ __forceinline__ void tree_term64(double v0, double v1, double v2, con
 // r1 r2 + R1.R2 cancels.  Same traversal, exclusions and singularity test.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
+                                                   const int *__restrict__ order, int nord,
                                                    const Node56 *__restrict__ walk, int nwalk, double rn2,
-                                                   double *__restrict__ part, DCtl *ctl,
+                                                   double sing2, double *__restrict__ part, DCtl *ctl,
                                                    unsigned long long *counters) {
-    const int r = blockIdx.x * BT + threadIdx.x;
+    const int q = blockIdx.x * BT + threadIdx.x;
+    const int r = q < nord ? order[q] : q;
     const int m = blockIdx.y;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
@@ -342,7 +364,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     const double kappa = p.kappa;
     float f[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
     double nr[3] = {0.0, 0.0, 0.0};
-    unsigned long long nev = 0, nvis = 0;
+    unsigned long long nev = 0, nvis = 0, nit = 0;
     bool sing = false;
     const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
     const bool img0 = (m == 0);
@@ -351,6 +373,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     Node56 nd = nwalk > 0 ? walk[0] : Node56{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
+        ++nit;
         const bool leaf = nd.word < 0;
         // both possible successors are fetched while this node is evaluated
         const int n1 = cur + 1, n2 = leaf ? cur + 1 : nd.word;
@@ -372,15 +395,12 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                     const int li = -nd.word - 1;
                     j = li > 0 ? s.behind[li - 1] : 0;
                     if (img0 && (j == me || j == beh)) use = false;
-                    if (use && sqrt(dd) < NV_EPS) { sing = true; use = false; }
+                    // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
+                    if (use && dd < sing2) { sing = true; use = false; }
                     near = dd < rn2 && j > 0;
                 }
-                if (use && near) {
-                    const int q = j - 1;
-                    double c[3];
-                    for (int k = 0; k < 3; ++k) c[k] = s.gi[k * s.cap + q] - s.x[k * s.cap + q];
-                    tree_term64(v0, v1, v2, c, dd, kappa, nr, nev);
-                } else if (use) {
+                bool need64 = use && near;
+                if (use && !near) {
                     const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
                     const float e0 = (float)nd.circ[0], e1 = (float)nd.circ[1], e2 = (float)nd.circ[2];
                     const float x0 = d1 * e2 - d2 * e1, x1 = d2 * e0 - d0 * e2, x2 = d0 * e1 - d1 * e0;
@@ -392,7 +412,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                     const float r12 = (r1sq * inv1) * (r2sq * inv2);
                     const float den = r12 + (r1sq + de);
                     if (den < 0.0625f * r12) {
-                        tree_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
+                        need64 = true;
                     } else if (!(4.0f * (x0 * x0 + x1 * x1 + x2 * x2) < (float)NV_EPS)) {
                         // (the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon)
                         const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
@@ -401,6 +421,16 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                         kahan_add32(f[2], kc[2], x2 * fac);
                         ++nev;
                     }
+                }
+                if (need64) {
+                    // a near leaf's own segment (ghosti - x of its source row), or
+                    // the node's pseudo-segment
+                    double c[3] = {nd.circ[0], nd.circ[1], nd.circ[2]};
+                    if (near) {
+                        const int q = j - 1;
+                        for (int k = 0; k < 3; ++k) c[k] = s.gi[k * s.cap + q] - s.x[k * s.cap + q];
+                    }
+                    near_term64(v0, v1, v2, c, dd, kappa, nr, nev);
                 }
             } else {
                 open = true;
@@ -411,7 +441,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         nd = down ? a : b;
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
-    if (r < npad) {
+    if (r < n) {
         const double k4 = kappa / (4.0 * NV_PI);
         double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
@@ -428,6 +458,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev);
         atomicAdd(&counters[1], nvis);
+#ifdef NV_WALK_PROBE
+        atomicAdd(&counters[2], nit);
+#endif
     }
 }
 
@@ -437,9 +470,11 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
 // two sums (src/timestep.f90:107-115)
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
+                                                 const int *__restrict__ order, int nord,
                                                  const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
                                                  DCtl *ctl, unsigned long long *counters) {
-    const int r = blockIdx.x * BT + threadIdx.x;
+    const int q = blockIdx.x * BT + threadIdx.x;
+    const int r = q < nord ? order[q] : q;
     const int m = blockIdx.y;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
@@ -452,7 +487,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     const double kappa = p.kappa, theta_c = p.tree_theta;
     const int corr = p.extra_corr;
     double acc[3] = {0, 0, 0};
-    unsigned long long nev = 0, nvis = 0;
+    unsigned long long nev = 0, nvis = 0, nit = 0;
     bool sing = false;
     const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
     const bool img0 = (m == 0);
@@ -461,6 +496,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     WNode nd = nwalk > 0 ? walk[0] : WNode{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
+        ++nit;
         // both possible successors are fetched while this node is evaluated
         const int n1 = cur + 1, n2 = nd.skip;
         const WNode a = walk[n1 < nwalk ? n1 : cur];
@@ -513,7 +549,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
         nd = down ? a : b;
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
-    if (r < npad) {
+    if (r < n) {
         double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -529,12 +565,37 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev);
         atomicAdd(&counters[1], nvis);
+#ifdef NV_WALK_PROBE
+        atomicAdd(&counters[2], nit);
+#endif
     }
 }
 
+// 30-bit Morton keys of the current positions (1024 cells per box edge),
+// inactive rows last: the target order of the close search, which runs
+// after insertions the tree build did not see.
+__global__ void k_target_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    vals[r] = r;
+    unsigned code = 0xffffffffu;
+    if (s.infront[r] != 0) {
+        unsigned c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int q = (int)floor((s.x[a * s.cap + r] / bsize + 0.5) * 1024.0);
+            c[a] = (unsigned)(q < 0 ? 0 : q > 1023 ? 1023 : q);
+        }
+        code = nv_spread10(c[0]) | (nv_spread10(c[1]) << 1) | (nv_spread10(c[2]) << 2);
+    }
+    keys[r] = code;
+}
+
 template <int BT>
-__global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const WNode *__restrict__ walk, int nwalk) {
-    const int r = blockIdx.x * BT + threadIdx.x;
+__global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
+                                                   const WNode *__restrict__ walk, int nwalk) {
+    const int q = blockIdx.x * BT + threadIdx.x;
+    const int r = q < nord ? order[q] : q;
     if (r < n) s.closestd[r] = 100.0;   // set for every slot (src/tree.f90:196)
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
@@ -604,6 +665,7 @@ struct TreeBuf {
     Node56 *walk56 = nullptr;            // FP32 walk records, packed per walk
     int walk56_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
+    int built_rows = 0;                  // rows the last build sorted into vals
     std::vector<int> lvl;                // level start offsets
     unsigned long long *counters = nullptr;
     int *h_int = nullptr;                // pinned scratch
@@ -713,6 +775,7 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
     TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
     t->vals = t->v0;
+    t->built_rows = n;
     TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     const int nact = t->h_int[0];
@@ -775,10 +838,40 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     return NV_OK;
 }
 
+// The least double d with sqrt(d) >= e: for d >= 0, sqrt(d) < e <=> d < it
+// (a correctly rounded square root is monotone).
+static double sqrt_threshold(double e) {
+    double lo = 0.0, hi = e * e * 4.0;   // sqrt(lo) < e <= sqrt(hi)
+    while (true) {
+        const double mid = lo + (hi - lo) / 2.0;
+        if (mid <= lo || mid >= hi) break;
+        (std::sqrt(mid) < e ? lo : hi) = mid;
+    }
+    while (std::sqrt(lo) >= e) lo = std::nextafter(lo, 0.0);
+    double d = lo;
+    while (std::sqrt(d) < e) d = std::nextafter(d, 1.0);
+    return d;
+}
+
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
                           int precision, double rn2, double *part, DCtl *ctl, nv_stats *stats) {
     (void)stats;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
+    if (n <= 0) return NV_OK;
+    // Targets are taken in the tree's Morton order (the build's sorted rows,
+    // active first), so that the 64 lanes of a wave are close together and
+    // their traversals overlap; rows the build did not see follow in row
+    // order.  Each lane's sequence of accepted nodes does not depend on its
+    // wave-mates, so the order changes no result.  Rows >= n hold no target
+    // and k_finish reads no partial there.
+#ifdef NV_WALK_ROWORDER
+    const int *order = nullptr;
+    const int nord = 0;
+#else
+    const int *order = t->vals;
+    const int nord = t->vals && t->built_rows <= n ? t->built_rows : 0;
+#endif
+    const int nblk = (n + 255) / 256;
     // the FP32 walk's single opening radius needs width / theta > geo
     const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
     if (fp32) {
@@ -791,11 +884,11 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, t->walk, t->nwalk,
                                p.tree_theta, p.extra_corr, t->walk56);
-        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad,
-                           t->walk56, t->nwalk, rn2, part, ctl, t->counters);
+        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
+                           t->walk56, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters);
     } else {
-        hipLaunchKernelGGL(k_walk_vel<256>, dim3(npad / 256, sh.n), dim3(256), 0, st, d, p, sh, n, npad, t->walk,
-                           t->nwalk, part, ctl, t->counters);
+        hipLaunchKernelGGL(k_walk_vel<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
+                           t->walk, t->nwalk, part, ctl, t->counters);
     }
     TCHK(hipGetLastError());
     return NV_OK;
@@ -807,18 +900,41 @@ int nv_tree_walk_record_bytes(int precision, double theta) {
 
 // interaction and node-visit counters of the last velocity walk (synchronises)
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats) {
-    unsigned long long h[2];
-    TCHK(hipMemcpyAsync(h, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    unsigned long long h[3];
+    TCHK(hipMemcpyAsync(h, t->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
+#ifdef NV_WALK_PROBE
+    fprintf(stderr, "walk: lane visits %llu, wave iterations %llu (x64 = %.2f lane visits)\n", h[1], h[2],
+            (double)h[2] * 64.0 / (double)(h[1] ? h[1] : 1));
+#endif
     stats->tree_evals = (int64_t)h[0];
     stats->tree_nodes_visited = (int64_t)h[1];
     return NV_OK;
 }
 
-int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats) {
+int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n, nv_stats *stats) {
     (void)stats;
     if (n <= 0) return NV_OK;
-    hipLaunchKernelGGL(k_walk_close<256>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, t->walk, t->nwalk);
+    // Targets in the Morton order of their current positions.  The build's
+    // order holds points inserted since the build (at its end, or among the
+    // inactive rows), whose waves would be scattered over the whole tangle;
+    // the sort replaces it (the build's keys and rows are not needed after
+    // the build; the velocity walk accepts any order of the rows).
+#ifdef NV_WALK_ROWORDER
+    const int *order = nullptr;
+    const int nord = 0;
+#else
+    if (int rc = ensure_rows(t, n)) return rc;
+    hipLaunchKernelGGL(k_target_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
+    TCHK(hipGetLastError());
+    TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 32, t->hist, t->tmp));
+    t->vals = t->v0;
+    t->built_rows = n;
+    const int *order = t->vals;
+    const int nord = n;
+#endif
+    hipLaunchKernelGGL(k_walk_close<256>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, order, nord, t->walk,
+                       t->nwalk);
     TCHK(hipGetLastError());
     return NV_OK;
 }

@@ -122,15 +122,6 @@ struct __align__(16) Rec32 {
 // box), then the inactive rows in row order; key = tangle << 32 | code.
 // Any order is correct (records carry their row); a compact tile keeps the
 // FP32 positions small relative to its origin and makes more tiles far.
-__device__ __forceinline__ unsigned spread10(unsigned v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __global__ void k_order_keys(DState s, Tangles tg, int rows, double bx, double by, double bz,
                              unsigned long long *__restrict__ keys, int *__restrict__ vals) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,7 +137,7 @@ __global__ void k_order_keys(DState s, Tangles tg, int rows, double bx, double b
             const int q = (int)floor(u * 1024.0);
             c[a] = (unsigned)(q < 0 ? 0 : q > 1023 ? 1023 : q);
         }
-        code = spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2);
+        code = nv_spread10(c[0]) | (nv_spread10(c[1]) << 1) | (nv_spread10(c[2]) << 2);
     } else {
         code = 0x80000000u | (unsigned)(r - tg.base[k]);
     }

@@ -1040,7 +1040,8 @@ int nv_pclose_tree(void *h) {
     if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pclose_tree without construct_tree");
     if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
-    int rc = nv_tree_walk_close(c->tree, c->st, c->d, c->tn[0], &c->stats);
+    const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
+    int rc = nv_tree_walk_close(c->tree, c->st, c->d, bsize, c->tn[0], &c->stats);
     if (rc) return fail(c, NV_ERR_HIP, "tree close search launch failed");
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

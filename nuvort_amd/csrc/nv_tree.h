@@ -17,6 +17,6 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
 int nv_tree_walk_record_bytes(int precision, double theta);
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats);
 // pclose_tree / closest_tree (src/tree.f90:190-244) over the (stale) tree
-int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, int n, nv_stats *stats);
+int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n, nv_stats *stats);
 // every node, depth-first, empty ones included
 int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *n);

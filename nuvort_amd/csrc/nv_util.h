@@ -13,3 +13,13 @@ int nv_radix_blocks(int n);
 int nv_radix_hist_ints(int n);
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
                                int bits, int *hist, int *scan_tmp);
+
+// 10-bit Morton spreading: bit b of v moves to bit 3b
+__device__ __forceinline__ unsigned nv_spread10(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
