@@ -38,12 +38,14 @@ struct __align__(16) WNode {
     double cent[3];
     double circ[3];
     double width, geo;
-    int pcount, leaf_infront, skip, pad;
+    int pcount, leaf_infront, skip;
+    int leaf_j;   // a leaf's behind(leaf_infront) when the walk runs (k_leaf_links)
 };
 
 // Compact walk record of the FP32 walk: FP64 centroid and circulation, the
 // opening radius squared (FP32, rounded up), and one word: the skip index of
-// an internal node, or minus one minus the leaf's infront-at-build link.
+// an internal node, or minus one minus a leaf's j = behind(leaf_infront),
+// resolved when the record is packed, right before the walk.
 struct __align__(8) Node56 {
     double cent[3];
     double circ[3];
@@ -262,8 +264,18 @@ __global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk
     w.pcount = pc;
     w.leaf_infront = li;
     w.skip = my + nd.size[q];
-    w.pad = 0;
+    w.leaf_j = 0;
     walk[my] = w;
+}
+
+// j = behind(leaf_infront) of every leaf record, read once before a walk
+// (links do not change while it runs), so that the walk loop has no load
+// that depends on the node it evaluates
+__global__ void k_leaf_links(DState s, WNode *__restrict__ walk, int nwalk) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nwalk) return;
+    const int pc = walk[q].pcount, li = walk[q].leaf_infront;
+    if (pc == 1) walk[q].leaf_j = li > 0 ? s.behind[li - 1] : 0;
 }
 
 __global__ void k_zero_counters(unsigned long long *c) {
@@ -278,7 +290,8 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
 // geo) that is dist > geo + width / theta (width / theta without the
 // correction), compared here on squares.
-__global__ void k_pack56(const WNode *__restrict__ walk, int nwalk, double theta, int corr, Node56 *__restrict__ out) {
+__global__ void k_pack56(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
+                         Node56 *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nwalk) return;
     const WNode w = walk[q];
@@ -290,7 +303,7 @@ __global__ void k_pack56(const WNode *__restrict__ walk, int nwalk, double theta
     }
     const double r = (corr ? w.geo : 0.0) + w.width / theta;
     o.r2 = (float)(r * r) * (1.0f + 2.0e-7f);
-    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? w.leaf_infront : 0) - 1 : w.skip;
+    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
     out[q] = o;
 }
 
@@ -392,8 +405,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                 bool near = false;
                 int j = 0;
                 if (leaf) {
-                    const int li = -nd.word - 1;
-                    j = li > 0 ? s.behind[li - 1] : 0;
+                    j = -nd.word - 1;
                     if (img0 && (j == me || j == beh)) use = false;
                     // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
                     if (use && dd < sing2) { sing = true; use = false; }
@@ -423,14 +435,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                     }
                 }
                 if (need64) {
-                    // a near leaf's own segment (ghosti - x of its source row), or
-                    // the node's pseudo-segment
-                    double c[3] = {nd.circ[0], nd.circ[1], nd.circ[2]};
-                    if (near) {
-                        const int q = j - 1;
-                        for (int k = 0; k < 3; ++k) c[k] = s.gi[k * s.cap + q] - s.x[k * s.cap + q];
-                    }
-                    near_term64(v0, v1, v2, c, dd, kappa, nr, nev);
+                    near_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
                 }
             } else {
                 open = true;
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
                 lskip = nd.skip;
                 bool use = true;
                 if (nd.pcount == 1) {
-                    const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
+                    const int j = nd.leaf_j;
                     if (img0 && (j == me || j == beh)) use = false;
                     if (use && dist < NV_EPS) { sing = true; use = false; }
                 }
@@ -625,7 +630,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
             if (nd.pcount == 1 || theta < .5) {
                 lskip = nd.skip;
                 if (dist < best) {
-                    const int j = nd.leaf_infront > 0 ? s.behind[nd.leaf_infront - 1] : 0;
+                    const int j = nd.leaf_j;
                     if (nd.pcount > 1) {
                         closest = 0;
                     } else if ((me != j) && (inf != j) && (beh != j)) {
@@ -882,11 +887,13 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
             TCHK(hipMalloc(&t->walk56, sizeof(Node56) * (size_t)t->walk56_cap));
         }
         if (t->nwalk > 0)
-            hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, t->walk, t->nwalk,
+            hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
                                p.tree_theta, p.extra_corr, t->walk56);
         hipLaunchKernelGGL(k_walk_vel32<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
                            t->walk56, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters);
     } else {
+        if (t->nwalk > 0)
+            hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
         hipLaunchKernelGGL(k_walk_vel<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
                            t->walk, t->nwalk, part, ctl, t->counters);
     }
@@ -933,6 +940,8 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
     const int *order = t->vals;
     const int nord = n;
 #endif
+    if (t->nwalk > 0)
+        hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
     hipLaunchKernelGGL(k_walk_close<256>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, order, nord, t->walk,
                        t->nwalk);
     TCHK(hipGetLastError());
