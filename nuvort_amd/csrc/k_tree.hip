@@ -357,15 +357,28 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 // leaves closer than r_near and for nodes whose pseudo-segment (centroid to
 // centroid + circulation) passes close by the target, where the FP32 form's
 // r1 r2 + R1.R2 cancels.  Same traversal, exclusions and singularity test.
+//
+// Persistent: a grid the size of the resident waves, each wave taking items
+// (image, block of 64 targets) from a queue, the unshifted image's (longest)
+// walks first, so that no SIMD idles while long walks finish.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
-                                                   const int *__restrict__ order, int nord,
-                                                   const Node56 *__restrict__ walk, int nwalk, double rn2,
-                                                   double sing2, double *__restrict__ part, DCtl *ctl,
-                                                   unsigned long long *counters) {
-    const int q = blockIdx.x * BT + threadIdx.x;
+                                                   const int *__restrict__ order, int nord, int nb64,
+                                                   int nitems, const Node56 *__restrict__ walk, int nwalk,
+                                                   double rn2, double sing2, double *__restrict__ part, DCtl *ctl,
+                                                   unsigned long long *counters, int *queue) {
+    const int lane = threadIdx.x & 63;
+    const double kappa = p.kappa;
+    unsigned long long nev = 0, nvis = 0, nit = 0;
+    bool sing = false;
+    for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = wave_uniform(it);   // every lane is active here: the first is lane 0
+    if (it >= nitems) break;
+    const int m = it / nb64;
+    const int q = (it - m * nb64) * 64 + lane;
     const int r = q < nord ? order[q] : q;
-    const int m = blockIdx.y;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
     int beh = -5, me = -5;
@@ -374,11 +387,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         beh = s.behind[r];
         me = r + 1;
     }
-    const double kappa = p.kappa;
     float f[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
     double nr[3] = {0.0, 0.0, 0.0};
-    unsigned long long nev = 0, nvis = 0, nit = 0;
-    bool sing = false;
     const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
@@ -445,7 +455,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
-    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     if (r < n) {
         const double k4 = kappa / (4.0 * NV_PI);
         double *pp = part + (size_t)m * 6 * npad;
@@ -456,6 +465,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
             pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
         }
     }
+    }
+    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     for (int off = 32; off > 0; off >>= 1) {
         nev += __shfl_down(nev, off);
         nvis += __shfl_down(nvis, off);
@@ -469,18 +480,26 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     }
 }
 
-// one (target block, image) per workgroup: blockIdx.y = image; partial block
-// [images][6][npad], image 0 in the unshifted half, images 1..26 in the
+// Persistent like k_walk_vel32: items (image, block of 64 targets); partial
+// block [images][6][npad], image 0 in the unshifted half, images 1..26 in the
 // image half, so k_finish's chunk-ordered reduction rebuilds the reference's
 // two sums (src/timestep.f90:107-115)
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
-                                                 const int *__restrict__ order, int nord,
+                                                 const int *__restrict__ order, int nord, int nb64, int nitems,
                                                  const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
-                                                 DCtl *ctl, unsigned long long *counters) {
-    const int q = blockIdx.x * BT + threadIdx.x;
+                                                 DCtl *ctl, unsigned long long *counters, int *queue) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long nev = 0, nvis = 0, nit = 0;
+    bool sing = false;
+    for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = wave_uniform(it);   // every lane is active here: the first is lane 0
+    if (it >= nitems) break;
+    const int m = it / nb64;
+    const int q = (it - m * nb64) * 64 + lane;
     const int r = q < nord ? order[q] : q;
-    const int m = blockIdx.y;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
     int beh = -5, me = -5;
@@ -492,8 +511,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     const double kappa = p.kappa, theta_c = p.tree_theta;
     const int corr = p.extra_corr;
     double acc[3] = {0, 0, 0};
-    unsigned long long nev = 0, nvis = 0, nit = 0;
-    bool sing = false;
     const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
@@ -553,7 +570,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
-    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     if (r < n) {
         double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
@@ -562,6 +578,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
             pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
         }
     }
+    }
+    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
     // wave-level counter partials
     for (int off = 32; off > 0; off >>= 1) {
         nev += __shfl_down(nev, off);
@@ -598,8 +616,15 @@ __global__ void k_target_keys(DState s, int n, double bsize, unsigned long long 
 
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
-                                                   const WNode *__restrict__ walk, int nwalk) {
-    const int q = blockIdx.x * BT + threadIdx.x;
+                                                   int nitems, const WNode *__restrict__ walk, int nwalk,
+                                                   int *queue) {
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = wave_uniform(it);   // every lane is active here: the first is lane 0
+    if (it >= nitems) break;
+    const int q = it * 64 + lane;
     const int r = q < nord ? order[q] : q;
     if (r < n) s.closestd[r] = 100.0;   // set for every slot (src/tree.f90:196)
     const bool act = r < n && s.infront[r] != 0;
@@ -650,6 +675,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         s.closest[r] = closest;
         s.closestd[r] = best;
     }
+    }
 }
 
 }  // namespace
@@ -671,6 +697,7 @@ struct TreeBuf {
     int walk56_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
     int built_rows = 0;                  // rows the last build sorted into vals
+    int grid32 = 0, grid64 = 0, gridc = 0;   // resident workgroups of the persistent walks
     std::vector<int> lvl;                // level start offsets
     unsigned long long *counters = nullptr;
     int *h_int = nullptr;                // pinned scratch
@@ -858,6 +885,17 @@ static double sqrt_threshold(double e) {
     return d;
 }
 
+// workgroups of 256 lanes resident on the device at once for kernel k
+template <typename K>
+static int resident_blocks(K k) {
+    int per_cu = 0, ncu = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess)
+        return 0;
+    return std::max(1, per_cu) * std::max(1, ncu);
+}
+
 int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int n, int npad,
                           int precision, double rn2, double *part, DCtl *ctl, nv_stats *stats) {
     (void)stats;
@@ -876,7 +914,7 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     const int *order = t->vals;
     const int nord = t->vals && t->built_rows <= n ? t->built_rows : 0;
 #endif
-    const int nblk = (n + 255) / 256;
+    const int nb64 = (n + 63) / 64, nitems = nb64 * sh.n;
     // the FP32 walk's single opening radius needs width / theta > geo
     const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
     if (fp32) {
@@ -889,13 +927,18 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
                                p.tree_theta, p.extra_corr, t->walk56);
-        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           t->walk56, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters);
+        if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256>)) == 0) return NV_ERR_HIP;
+        const int grid = std::min(t->grid32, (nitems + 3) / 4);
+        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord, nb64,
+                           nitems, t->walk56, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters,
+                           (int *)(t->counters + 3));
     } else {
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
-        hipLaunchKernelGGL(k_walk_vel<256>, dim3(nblk, sh.n), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           t->walk, t->nwalk, part, ctl, t->counters);
+        if (t->grid64 == 0 && (t->grid64 = resident_blocks(k_walk_vel<256>)) == 0) return NV_ERR_HIP;
+        const int grid = std::min(t->grid64, (nitems + 3) / 4);
+        hipLaunchKernelGGL(k_walk_vel<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord, nb64,
+                           nitems, t->walk, t->nwalk, part, ctl, t->counters, (int *)(t->counters + 3));
     }
     TCHK(hipGetLastError());
     return NV_OK;
@@ -942,8 +985,11 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
 #endif
     if (t->nwalk > 0)
         hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
-    hipLaunchKernelGGL(k_walk_close<256>, dim3((n + 255) / 256), dim3(256), 0, st, d, n, order, nord, t->walk,
-                       t->nwalk);
+    if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
+    const int nitems = (n + 63) / 64;
+    TCHK(nv_fill_int(st, (int *)(t->counters + 3), 0, 1));   // the queue (the walk counters stay)
+    hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, order,
+                       nord, nitems, t->walk, t->nwalk, (int *)(t->counters + 3));
     TCHK(hipGetLastError());
     return NV_OK;
 }
