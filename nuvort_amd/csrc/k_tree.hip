@@ -42,13 +42,16 @@ struct __align__(16) WNode {
     int leaf_j;   // a leaf's behind(leaf_infront) when the walk runs (k_leaf_links)
 };
 
-// Compact walk record of the FP32 walk: FP64 centroid and circulation, the
-// opening radius squared (FP32, rounded up), and one word: the skip index of
-// an internal node, or minus one minus a leaf's j = behind(leaf_infront),
-// resolved when the record is packed, right before the walk.
-struct __align__(8) Node56 {
+// Walk record of the FP32 walk: FP64 centroid and circulation (the FP64
+// cases), the circulation in FP32 with its squared length (the FP32 term),
+// the opening radius squared (FP32, rounded up), and one word: the skip
+// index of an internal node, or minus one minus a leaf's j = behind(
+// leaf_infront), resolved when the record is packed, right before the walk.
+struct __align__(8) Node72 {
     double cent[3];
     double circ[3];
+    float e[3];
+    float ee;
     float r2;
     int word;
 };
@@ -285,17 +288,21 @@ __global__ void k_zero_counters(unsigned long long *c) {
 // ------------------------------------------------------------ walks
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// WNode -> Node56.  The reference accepts a node when pcount == 1 or
+// WNode -> Node72.  The reference accepts a node when pcount == 1 or
 // width / (dist - geo) < theta (width / dist when dist <= geo or without the
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
 // geo) that is dist > geo + width / theta (width / theta without the
 // correction), compared here on squares.
-__global__ void k_pack56(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
-                         Node56 *__restrict__ out) {
+__global__ void k_pack72(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
+                         Node72 *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nwalk) return;
+    if (q > nwalk) return;
+    if (q == nwalk) {   // the spare record
+        out[q] = Node72{};
+        return;
+    }
     const WNode w = walk[q];
-    Node56 o;
+    Node72 o;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         o.cent[k] = w.cent[k];
@@ -304,14 +311,10 @@ __global__ void k_pack56(DState s, const WNode *__restrict__ walk, int nwalk, do
     const double r = (corr ? w.geo : 0.0) + w.width / theta;
     o.r2 = (float)(r * r) * (1.0f + 2.0e-7f);
     o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.e[k] = (float)w.circ[k];
+    o.ee = __fmaf_rn(o.e[0], o.e[0], __fmaf_rn(o.e[1], o.e[1], o.e[2] * o.e[2]));
     out[q] = o;
-}
-
-__device__ __forceinline__ void kahan_add32(float &sum, float &comp, float v) {
-    const float y = v - comp;
-    const float t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
 }
 
 // The reference's node term (src/tree.f90:297-307) in FP64, a = dist**2.
@@ -352,8 +355,10 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 }
 
 // The walk of k_walk_vel with FP32 interactions: displacements in FP64, the
-// node term in the well-conditioned form (as the direct sum's far field),
-// Kahan-compensated FP32 accumulation.  The reference's FP64 term is kept for
+// node term in FP32 in the well-conditioned form (as the direct sum's far
+// field), the terms accumulated in FP64 (on this hardware an FP64 add issues
+// at the unpacked FP32 rate: two instructions per term and component, half
+// of a Kahan update).  The reference's FP64 term is kept for
 // leaves closer than r_near and for nodes whose pseudo-segment (centroid to
 // centroid + circulation) passes close by the target, where the FP32 form's
 // r1 r2 + R1.R2 cancels.  Same traversal, exclusions and singularity test.
@@ -364,7 +369,7 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
                                                    const int *__restrict__ order, int nord, int nb64,
-                                                   int nitems, const Node56 *__restrict__ walk, int nwalk,
+                                                   int nitems, const Node72 *__restrict__ walk, int nwalk,
                                                    double rn2, double sing2, double *__restrict__ part, DCtl *ctl,
                                                    unsigned long long *counters, int *queue) {
     const int lane = threadIdx.x & 63;
@@ -387,69 +392,78 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         beh = s.behind[r];
         me = r + 1;
     }
-    float f[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
-    double nr[3] = {0.0, 0.0, 0.0};
-    const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
+    // the shifted target: displacements are centroid - xs (in the
+    // unshifted image exactly the reference's cent - x)
+    const double xs0 = xi[0] - sh.s[m][0], xs1 = xi[1] - sh.s[m][1], xs2 = xi[2] - sh.s[m][2];
+    double fa[3] = {0.0, 0.0, 0.0};   // FP32 terms, accumulated in FP64
+    double nr[3] = {0.0, 0.0, 0.0};   // FP64 terms
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
-    Node56 nd = nwalk > 0 ? walk[0] : Node56{};
+    Node72 nd = nwalk > 0 ? walk[0] : Node72{};
+    // Control flow on wave-uniform conditions only (the node is uniform):
+    // lane conditions become selects, so that the scalar unit, which the
+    // four SIMDs of a CU share, does not spend the walk on exec masks.
     while (cur < nwalk) {
         cur = wave_uniform(cur);
         ++nit;
-        const bool leaf = nd.word < 0;
+        const int w = nd.word;
+        const bool leaf = w < 0;
         // both possible successors are fetched while this node is evaluated
-        const int n1 = cur + 1, n2 = leaf ? cur + 1 : nd.word;
-        const Node56 a = walk[n1 < nwalk ? n1 : cur];
-        const Node56 b = walk[n2 < nwalk ? n2 : cur];
-        bool open = false;
-        if (cur >= lskip) {
-            ++nvis;
-            const double v0 = ((nd.cent[0] + s0) - xi[0]);
-            const double v1 = ((nd.cent[1] + s1) - xi[1]);
-            const double v2 = ((nd.cent[2] + s2) - xi[2]);
-            const double dd = v0 * v0 + v1 * v1 + v2 * v2;
-            if (leaf || dd > (double)nd.r2) {
-                lskip = leaf ? cur + 1 : nd.word;
-                bool use = true;
-                bool near = false;
-                int j = 0;
-                if (leaf) {
-                    j = -nd.word - 1;
-                    if (img0 && (j == me || j == beh)) use = false;
-                    // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
-                    if (use && dd < sing2) { sing = true; use = false; }
-                    near = dd < rn2 && j > 0;
-                }
-                bool need64 = use && near;
-                if (use && !near) {
-                    const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
-                    const float e0 = (float)nd.circ[0], e1 = (float)nd.circ[1], e2 = (float)nd.circ[2];
-                    const float x0 = d1 * e2 - d2 * e1, x1 = d2 * e0 - d0 * e2, x2 = d0 * e1 - d1 * e0;
-                    const float r1sq = (float)dd;
-                    const float de = d0 * e0 + d1 * e1 + d2 * e2;
-                    const float r2sq = r1sq + 2.0f * de + (e0 * e0 + e1 * e1 + e2 * e2);
-                    const float inv1 = __builtin_amdgcn_rsqf(r1sq);
-                    const float inv2 = __builtin_amdgcn_rsqf(r2sq);
-                    const float r12 = (r1sq * inv1) * (r2sq * inv2);
-                    const float den = r12 + (r1sq + de);
-                    if (den < 0.0625f * r12) {
-                        need64 = true;
-                    } else if (!(4.0f * (x0 * x0 + x1 * x1 + x2 * x2) < (float)NV_EPS)) {
-                        // (the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon)
-                        const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
-                        kahan_add32(f[0], kc[0], x0 * fac);
-                        kahan_add32(f[1], kc[1], x1 * fac);
-                        kahan_add32(f[2], kc[2], x2 * fac);
-                        ++nev;
-                    }
-                }
-                if (need64) {
-                    near_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
-                }
-            } else {
-                open = true;
-            }
+        // (the records end with one spare, so neither index needs a clamp)
+        const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
+        const Node72 a = walk[n1];
+        const Node72 b = walk[n2];
+        const bool vis = cur >= lskip;
+        nvis += (unsigned long long)__popcll(__ballot(vis));
+        const double v0 = nd.cent[0] - xs0, v1 = nd.cent[1] - xs1, v2 = nd.cent[2] - xs2;
+        const double dd = __fma_rn(v0, v0, __fma_rn(v1, v1, v2 * v2));
+        const bool acc = vis && (leaf || dd > (double)nd.r2);
+        const bool open = vis && !acc;
+        lskip = acc ? n2 : lskip;   // a leaf's successor, or the node's skip
+        bool use = acc, near = false;
+        if (leaf) {
+            const int j = -w - 1;
+            const bool excl = img0 && (j == me || j == beh);
+            // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
+            const bool sg = acc && !excl && dd < sing2;
+            sing = sing || sg;
+            use = acc && !excl && !sg;
+            near = use && j > 0 && dd < rn2;
+        }
+        const bool far = use && !near;
+        bool need64 = use && near;
+        if (__any(far)) {
+            const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
+            const float e0 = nd.e[0], e1 = nd.e[1], e2 = nd.e[2];
+            const float x0 = __fmaf_rn(d1, e2, -(d2 * e1));
+            const float x1 = __fmaf_rn(d2, e0, -(d0 * e2));
+            const float x2 = __fmaf_rn(d0, e1, -(d1 * e0));
+            const float r1sq = (float)dd;
+            const float de = __fmaf_rn(d0, e0, __fmaf_rn(d1, e1, d2 * e2));
+            const float r2sq = __fmaf_rn(2.0f, de, r1sq) + nd.ee;
+            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+            const float r12 = (r1sq * inv1) * (r2sq * inv2);
+            const float den = r12 + (r1sq + de);
+            // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
+            // the pseudo-segment ending close to the target (r2 < r1 / 4, where
+            // the FP32 r2^2 can even round to <= 0); NaN goes there too
+            const bool ill = !(den >= 0.0625f * r12) || !(r2sq >= 0.0625f * r1sq);
+            // the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon
+            // (x 4 is exact: |d x e|^2 < epsilon / 4)
+            const float xx = __fmaf_rn(x0, x0, __fmaf_rn(x1, x1, x2 * x2));
+            const bool ok = far && !ill && !(xx < 0.25f * (float)NV_EPS);
+            // x is finite for every lane, so x * 0 = 0 where the term is not taken
+            const float fac = ok ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
+            fa[0] += (double)(x0 * fac);
+            fa[1] += (double)(x1 * fac);
+            fa[2] += (double)(x2 * fac);
+            nev += ok ? 1u : 0u;
+            need64 = need64 || (far && ill);
+        }
+        if (__any(need64)) {
+            if (need64) near_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
         }
         const bool down = __any(open);
         cur = down ? n1 : n2;
@@ -460,17 +474,15 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double acc = ((double)f[k] - (double)kc[k]) * k4 + nr[k];
+            const double acc = fa[k] * k4 + nr[k];
             pp[(img0 ? k : 3 + k) * npad + r] = acc;
             pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
         }
     }
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
-    for (int off = 32; off > 0; off >>= 1) {
-        nev += __shfl_down(nev, off);
-        nvis += __shfl_down(nvis, off);
-    }
+    // nvis counts the visits of the whole wave already
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev);
         atomicAdd(&counters[1], nvis);
@@ -693,8 +705,8 @@ struct TreeBuf {
     int flag_cap = 0;
     WNode *walk = nullptr;
     int walk_cap = 0;
-    Node56 *walk56 = nullptr;            // FP32 walk records, packed per walk
-    int walk56_cap = 0;
+    Node72 *walk72 = nullptr;            // FP32 walk records, packed per walk
+    int walk72_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
     int built_rows = 0;                  // rows the last build sorted into vals
     int grid32 = 0, grid64 = 0, gridc = 0;   // resident workgroups of the persistent walks
@@ -735,7 +747,7 @@ void nv_tree_free(TreeBuf *t) {
     if (!t) return;
     hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
     hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
-    hipFree(t->walk56);
+    hipFree(t->walk72);
     hipFree(t->counters);
     hipHostFree(t->h_int);
     free_nodes(t->nd);
@@ -918,19 +930,20 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     // the FP32 walk's single opening radius needs width / theta > geo
     const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
     if (fp32) {
-        if (t->nwalk > t->walk56_cap) {
-            hipFree(t->walk56);
-            t->walk56 = nullptr;
-            t->walk56_cap = std::max(t->nwalk, 2 * t->walk56_cap);
-            TCHK(hipMalloc(&t->walk56, sizeof(Node56) * (size_t)t->walk56_cap));
+        // one spare record past the last: the walk fetches a successor index
+        // up to nwalk without a clamp
+        if (t->nwalk + 1 > t->walk72_cap) {
+            hipFree(t->walk72);
+            t->walk72 = nullptr;
+            t->walk72_cap = std::max(t->nwalk + 1, 2 * t->walk72_cap);
+            TCHK(hipMalloc(&t->walk72, sizeof(Node72) * (size_t)t->walk72_cap));
         }
-        if (t->nwalk > 0)
-            hipLaunchKernelGGL(k_pack56, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
-                               p.tree_theta, p.extra_corr, t->walk56);
+        hipLaunchKernelGGL(k_pack72, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
+                           p.tree_theta, p.extra_corr, t->walk72);
         if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256>)) == 0) return NV_ERR_HIP;
         const int grid = std::min(t->grid32, (nitems + 3) / 4);
         hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord, nb64,
-                           nitems, t->walk56, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters,
+                           nitems, t->walk72, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters,
                            (int *)(t->counters + 3));
     } else {
         if (t->nwalk > 0)
@@ -945,7 +958,7 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
 }
 
 int nv_tree_walk_record_bytes(int precision, double theta) {
-    return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node56) : (int)sizeof(WNode);
+    return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node72) : (int)sizeof(WNode);
 }
 
 // interaction and node-visit counters of the last velocity walk (synchronises)
