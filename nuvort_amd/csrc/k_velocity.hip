@@ -111,10 +111,13 @@ __global__ __launch_bounds__(TB) void k_bs64(DState s, Tangles tg, const Item *_
 }
 
 // FP32 source records, one per row: position relative to its tile's origin
-// (the first active row of the tile, FP64), e in FP32, c = |e|^2, row.
+// (the first active row of the tile, FP64), e in FP32, c = |e|^2, row, and
+// xe = x x e (from the FP64 position and segment), with which the far-tile
+// loop forms sum (x - t) x e fac as sum x x e fac - t x sum e fac.
 struct __align__(16) Rec32 {
     float x0, x1, x2, e0, e1, e2, c;
     int j;
+    float xe0, xe1, xe2, pad;
 };
 
 // Spatial order of the FP32 source records.  Within each tangle, active
@@ -194,17 +197,23 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
             x[k] = ld(s.x, s.cap, k, r);
             e[k] = ld(s.gi, s.cap, k, r) - x[k];
         }
-        v.x0 = (float)(x[0] - o[0]); v.x1 = (float)(x[1] - o[1]); v.x2 = (float)(x[2] - o[2]);
+        const double xr[3] = {x[0] - o[0], x[1] - o[1], x[2] - o[2]};
+        v.x0 = (float)xr[0]; v.x1 = (float)xr[1]; v.x2 = (float)xr[2];
         v.e0 = (float)e[0]; v.e1 = (float)e[1]; v.e2 = (float)e[2];
         v.c = (float)(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
         v.j = r;
+        v.xe0 = (float)(xr[1] * e[2] - xr[2] * e[1]);
+        v.xe1 = (float)(xr[2] * e[0] - xr[0] * e[2]);
+        v.xe2 = (float)(xr[0] * e[1] - xr[1] * e[0]);
     } else {
         // a neutral record: far away, zero segment, never matched
         v.x0 = v.x1 = v.x2 = 1.0e18f;
         v.e0 = v.e1 = v.e2 = 0.0f;
         v.c = 0.0f;
         v.j = -1;
+        v.xe0 = v.xe1 = v.xe2 = 0.0f;
     }
+    v.pad = 0.0f;
     if (q0 < rows) rec[q0] = v;
     // bounding box of the active sources, then its circumscribed sphere
     float lo[3], hi[3];
@@ -339,11 +348,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
             }
             if (__all(far)) {
                 // far tile for the whole wave: no pair needs the near-field
-                // test or an exclusion, every term is FP32
+                // test or an exclusion, every term is FP32.  With d = x - t,
+                // sum d x e fac = sum (x x e) fac - t x sum e fac: six FMAs
+                // per pair instead of the cross product's nine operations,
+                // and one cross product per target and block of KB sources.
                 for (int jb = 0; jb < TS; jb += KB) {
-                    float a[TPT][3];
+                    float A[TPT][3], B[TPT][3];
 #pragma unroll
-                    for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
+                    for (int q = 0; q < TPT; ++q) {
+                        A[q][0] = A[q][1] = A[q][2] = 0.f;
+                        B[q][0] = B[q][1] = B[q][2] = 0.f;
+                    }
 #pragma unroll
                     for (int jj = 0; jj < KB; ++jj) {
                         const Rec32 r = tile[jb + jj];
@@ -357,16 +372,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
                             const float inv2 = __builtin_amdgcn_rsqf(r2sq);
                             const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
                             const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
-                            a[q][0] = fmaf(fmaf(d1, r.e2, -(d2 * r.e1)), fac, a[q][0]);
-                            a[q][1] = fmaf(fmaf(d2, r.e0, -(d0 * r.e2)), fac, a[q][1]);
-                            a[q][2] = fmaf(fmaf(d0, r.e1, -(d1 * r.e0)), fac, a[q][2]);
+                            A[q][0] = fmaf(r.xe0, fac, A[q][0]);
+                            A[q][1] = fmaf(r.xe1, fac, A[q][1]);
+                            A[q][2] = fmaf(r.xe2, fac, A[q][2]);
+                            B[q][0] = fmaf(r.e0, fac, B[q][0]);
+                            B[q][1] = fmaf(r.e1, fac, B[q][1]);
+                            B[q][2] = fmaf(r.e2, fac, B[q][2]);
                         }
                     }
 #pragma unroll
                     for (int q = 0; q < TPT; ++q) {
-                        kahan_add(f[q][0], k[q][0], a[q][0]);
-                        kahan_add(f[q][1], k[q][1], a[q][1]);
-                        kahan_add(f[q][2], k[q][2], a[q][2]);
+                        const float* t = tt[q];
+                        kahan_add(f[q][0], k[q][0], A[q][0] - fmaf(t[1], B[q][2], -(t[2] * B[q][1])));
+                        kahan_add(f[q][1], k[q][1], A[q][1] - fmaf(t[2], B[q][0], -(t[0] * B[q][2])));
+                        kahan_add(f[q][2], k[q][2], A[q][2] - fmaf(t[0], B[q][1], -(t[1] * B[q][0])));
                     }
                 }
                 continue;
