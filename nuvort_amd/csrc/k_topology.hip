@@ -209,14 +209,16 @@ struct InsOut {
 // inputs: returns the slot used (or -1 when r does not insert); -2 when the
 // tangle's rows are exhausted.
 __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in, const int *empty, int &ei,
-                         int e1, int &nl, int base, int tcap, InsOut &out) {
+                         int e1, int &nl, int base, int tcap, InsOut &out, const int *est, int es0, int es1) {
     const int cap = s.cap;
     if (!in.act) return -1;
     const double disti = dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]);
     if (!(disti > p.delta)) return -1;
     int slot;
     if (ei < e1) {
-        slot = empty[ei++];
+        // the empty list's entries [es0, es1) are staged in est
+        slot = (ei >= es0 && ei < es1) ? est[ei - es0] : empty[ei];
+        ++ei;
     } else {
         if (nl >= tcap) return -2;
         slot = base + nl;
@@ -295,6 +297,16 @@ __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in,
     return slot;
 }
 
+// First index q in [lo, hi) with v[q] >= key (v ascending), hi if none.
+__device__ __forceinline__ int lower_index(const int *v, int lo, int hi, int key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // After an insertion at r (old infront fi, behind bb, new point in slot):
 // the staged inputs of later candidates fi and bb take the new state in
 // place (fi: ghosts and behind; bb: its infront's infront is the new
@@ -305,22 +317,22 @@ __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in,
 __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn &in, int slot, const InsOut &o,
                              InsIn *st, const int *strow, int k, int m, Dirty &d, int n_old) {
     const int fi = in.fi, bb = in.bb;
-    for (int q = k; q < m; ++q) {
-        const int rq = strow[q];
-        if (rq > fi && rq > bb) break;
-        if (rq == fi) {
+    // the staged rows are ascending: find fi and bb among the rows not yet
+    // visited by bisection (either can be far from r, e.g. a new slot)
+    const int qf = lower_index(strow, k, m, fi);
+    if (qf < m && strow[qf] == fi) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                st[q].gi[c] = o.gf_i[c];
-                st[q].gb[c] = o.gf_b[c];
-                st[q].xb[c] = o.xn[c];
-            }
-            st[q].bb = slot;
+        for (int c = 0; c < 3; ++c) {
+            st[qf].gi[c] = o.gf_i[c];
+            st[qf].gb[c] = o.gf_b[c];
+            st[qf].xb[c] = o.xn[c];
         }
-        if (rq == bb) {
+        st[qf].bb = slot;
+    }
+    const int qb = lower_index(strow, k, m, bb);
+    if (qb < m && strow[qb] == bb) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) st[q].xff[c] = o.xn[c];
-        }
+        for (int c = 0; c < 3; ++c) st[qb].xff[c] = o.xn[c];
     }
     if (fi > r && fi < n_old &&
         dist_gen(in.xf[0], in.xf[1], in.xf[2], o.gf_i[0], o.gf_i[1], o.gf_i[2]) > p.delta && !dirty_push(d, fi))
@@ -342,7 +354,8 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
     __shared__ Dirty d;
     __shared__ InsIn st[INS_CHUNK];
     __shared__ int strow[INS_CHUNK];
-    __shared__ int nl_sh, stop_sh;
+    __shared__ int nl_sh, stop_sh, ei_sh;
+    __shared__ int est[INS_CHUNK];   // the next entries of the empty list
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk], tcap = tg.cap[tk];
@@ -361,16 +374,19 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         d.hd = d.nd = 0;
         nl_sh = tg.n[tk];
         stop_sh = 0;
+        ei_sh = e0;
     }
     int ei = e0;   // used by thread 0 only
     for (int cs = 0; cs <= ncand; cs += INS_CHUNK) {
         const int m = min(INS_CHUNK, ncand - cs);
         __syncthreads();
+        const int es0 = ei_sh, es1 = min(e1, es0 + INS_CHUNK);
         if (tid < m) {
             const int r = cl[cs + tid];
             strow[tid] = r;
             load_insertion(s, r, st[tid]);
         }
+        if (es0 + tid < es1) est[tid] = empty[es0 + tid];
         __syncthreads();
         if (tid == 0 && !stop_sh) {
             // rows below the next chunk's first candidate are this chunk's
@@ -389,7 +405,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 if (staged) in = st[k - 1];
                 else load_insertion(s, r, in);
                 InsOut o;
-                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o);
+                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                 if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
                 if (slot < 0) continue;
                 if (!after_insert(s, p, r, in, slot, o, st, strow, k, m, d, n_old)) {
@@ -399,6 +415,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 }
             }
             nl_sh = nl;
+            ei_sh = ei;
         }
         if (cs + m >= ncand) {
             // the last chunk: dirty rows beyond it are visited from memory
@@ -410,7 +427,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                     InsIn in;
                     load_insertion(s, r, in);
                     InsOut o;
-                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o);
+                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                     if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
                     if (slot < 0) continue;
                     if (!after_insert(s, p, r, in, slot, o, st, strow, 0, 0, d, n_old)) {
@@ -590,16 +607,63 @@ __device__ void recon_tests(const Phys &p, ReconIn &in) {
 }
 
 constexpr int REC_CHUNK = 128;
+constexpr int REC_PARALLEL_MAX = 8;
 
 // One workgroup per tangle: candidates staged REC_CHUNK at a time (every
 // thread loads one), thread 0 visits them in row order.  A staged visit is
 // reloaded from memory when a reconnection earlier in the chunk changed any
 // row it read (r, its closest, infront(r), behind(closest)).
+// One visit of the reference's precon loop (src/reconnection.f90:11-61) at
+// staged index kk, by the serial thread: from the staged inputs unless one
+// of the visit's rows was modified (then from memory), and the reconnection
+// when the tests pass.  Returns whether the modified set is still complete.
+__device__ bool precon_visit(const DState &s, const Phys &p, int *recon, int tk, int base, int n, int kk,
+                             const int *strow, const ReconIn *st, ModSet &mod, bool tracked, DCtl *ctl,
+                             nv_recon_event *ev, int ev_cap, int itime, double t) {
+    const int r = strow[kk];
+    ReconIn in = st[kk];
+    bool fresh = tracked && (mod.n == 0 || !mod_has(mod, r));
+    if (fresh && mod.n > 0 && in.act && in.j > 0)
+        fresh = !mod_has(mod, in.j - 1) && (in.pari <= 0 || !mod_has(mod, in.pari - 1)) &&
+                (in.parjb <= 0 || !mod_has(mod, in.parjb - 1));
+    if (!fresh) {
+        load_recon(s, r, in);
+        recon_tests(p, in);
+    }
+    if (!in.go) return tracked;
+    const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
+    recon[tk] += 1;
+    const int q = atomicAdd(&ctl->nevents, 1);
+    if (q < ev_cap) {
+        nv_recon_event e;
+        e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
+        e.angle = acos(in.dot);
+        e.dl = in.lb - in.la;
+        ev[q] = e;
+    }
+    s.infront[parjb - 1] = pari;
+    s.behind[pari - 1] = parjb;
+    s.infront[r] = j;
+    s.behind[rj] = r + 1;
+    tracked = mod_add(mod, parjb - 1) && tracked;
+    tracked = mod_add(mod, pari - 1) && tracked;
+    tracked = mod_add(mod, r) && tracked;
+    tracked = mod_add(mod, rj) && tracked;
+    tracked = loop_killer_rec(s, pari - 1, n, mod) && tracked;
+    tracked = loop_killer_rec(s, r, n, mod) && tracked;
+    return tracked;
+}
+
 __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
                                                              nv_recon_event *ev, int ev_cap) {
     __shared__ ReconIn st[REC_CHUNK];
     __shared__ int strow[REC_CHUNK];
     __shared__ ModSet mod;
+    __shared__ int nk[2];          // the next visit that needs the serial thread
+    __shared__ int tracked_sh;     // the modified set is complete
+    __shared__ int ngo;            // staged visits whose tests say "reconnect"
     const int tid = threadIdx.x;
     const double t = ctl->t;
     const int itime = ctl->itime;
@@ -607,59 +671,62 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
     const int n = tg.n[tk];
-    const int cap = s.cap;
     for (int cs = c0; cs < c1; cs += REC_CHUNK) {
         const int m = min(REC_CHUNK, c1 - cs);
         __syncthreads();
         for (int q = tid; q < MOD_SLOTS; q += REC_CHUNK) mod.v[q] = -1;
-        if (tid == 0) mod.n = 0;
+        if (tid == 0) {
+            mod.n = 0;
+            tracked_sh = 1;
+            nk[0] = nk[1] = m;
+            ngo = 0;
+        }
+        __syncthreads();
         if (tid < m) {
             strow[tid] = cand[cs + tid];
             load_recon(s, strow[tid], st[tid]);
             recon_tests(p, st[tid]);
+            if (st[tid].go) atomicAdd(&ngo, 1);
         }
         __syncthreads();
-        if (tid != 0) continue;
-        bool tracked = true;   // the modified set is complete
-        for (int k = 0; k < m; ++k) {
-            const int r = strow[k];
-            ReconIn in = st[k];
-            bool fresh = tracked && !mod_has(mod, r);
-            if (fresh && in.act && in.j > 0) {
-                const int rj = in.j - 1;
-                fresh = !mod_has(mod, rj) && (in.pari <= 0 || !mod_has(mod, in.pari - 1)) &&
-                        (in.parjb <= 0 || !mod_has(mod, in.parjb - 1));
+        if (ngo > REC_PARALLEL_MAX) {
+            // many reconnections: the serial thread visits every candidate
+            if (tid == 0) {
+                bool tracked = true;
+                for (int kk = 0; kk < m; ++kk)
+                    tracked = precon_visit(s, p, tg.recon, tk, base, n, kk, strow, st, mod, tracked, ctl, ev, ev_cap,
+                                           itime, t);
             }
-            if (!fresh) {
-                load_recon(s, r, in);
-                recon_tests(p, in);
-            }
-            if (!in.go) continue;
-            const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
-            const double dot_val = in.dot, l_before = in.lb, l_after = in.la;
-            {
-                tg.recon[tk] += 1;
-                const int q = atomicAdd(&ctl->nevents, 1);
-                if (q < ev_cap) {
-                    nv_recon_event e;
-                    e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
-#pragma unroll
-                    for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
-                    e.angle = acos(dot_val);
-                    e.dl = l_before - l_after;
-                    ev[q] = e;
+            continue;
+        }
+        // The reference visits the candidates in order.  A visit whose staged
+        // tests say "no reconnection" and none of whose rows a reconnection of
+        // this chunk has modified changes nothing: the block finds the next
+        // visit that is not such a one, and the serial thread runs it.
+        int k = 0, par = 0;
+        while (true) {
+            if (tid >= k && tid < m) {
+                const ReconIn &in = st[tid];
+                bool need = in.go;
+                if (!need && mod.n > 0) {
+                    need = !tracked_sh || mod_has(mod, strow[tid]) ||
+                           (in.act && in.j > 0 &&
+                            (mod_has(mod, in.j - 1) || (in.pari > 0 && mod_has(mod, in.pari - 1)) ||
+                             (in.parjb > 0 && mod_has(mod, in.parjb - 1))));
                 }
-                s.infront[parjb - 1] = pari;
-                s.behind[pari - 1] = parjb;
-                s.infront[r] = j;
-                s.behind[rj] = r + 1;
-                tracked = mod_add(mod, parjb - 1) && tracked;
-                tracked = mod_add(mod, pari - 1) && tracked;
-                tracked = mod_add(mod, r) && tracked;
-                tracked = mod_add(mod, rj) && tracked;
-                tracked = loop_killer_rec(s, pari - 1, n, mod) && tracked;
-                tracked = loop_killer_rec(s, r, n, mod) && tracked;
+                if (need) atomicMin(&nk[par], tid);
             }
+            __syncthreads();
+            const int kk = nk[par];
+            if (kk >= m) break;
+            if (tid == 0) {
+                nk[par ^ 1] = m;
+                tracked_sh = precon_visit(s, p, tg.recon, tk, base, n, kk, strow, st, mod, tracked_sh != 0, ctl, ev,
+                                          ev_cap, itime, t) ? 1 : 0;
+            }
+            k = kk + 1;
+            par ^= 1;
+            __syncthreads();
         }
     }
 }
@@ -676,35 +743,95 @@ __global__ void k_premove_pre(DState s, Phys p, int n, int *cand) {
     cand[r] = c;
 }
 
-__global__ void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
-    if (threadIdx.x != 0) return;
+// The inputs of one premove visit (src/line.f90:108-113) of row r: its
+// infront inf, that one's infront ii, and the removal test.
+struct RmIn {
+    int inf, ii, go;
+};
+
+__device__ __forceinline__ void load_rm(const DState &s, const Phys &p, int r, RmIn &in) {
+    in.inf = in.ii = -1;
+    in.go = 0;
+    if (s.infront[r] == 0) return;
+    in.inf = s.infront[r] - 1;
+    in.ii = s.infront[in.inf] - 1;
+    if (in.ii < 0) return;
+    in.go = dist_rows(s, s.x, r, s.x, in.ii) < .99 * p.delta;
+}
+
+constexpr int RM_CHUNK = 256;
+
+// The reference's loop over the candidates in row order (merged with the
+// dirty rows), RM_CHUNK candidates at a time: the workgroup loads every
+// candidate's inputs and test at once; the serial thread uses a staged visit
+// unless one of its rows (r, inf, ii) was modified by a removal of this chunk,
+// and then reloads it.
+__global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
     __shared__ Dirty d;
-    d.hd = d.nd = 0;
+    __shared__ RmIn st[RM_CHUNK];
+    __shared__ int strow[RM_CHUNK];
+    __shared__ ModSet mod;
+    __shared__ int stop_sh;
+    const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
     const int *cl = cand + c0;
     const int ncand = c1 - c0;
     const int n = tg.n[tk];
-    int ci = 0;
-    while (true) {
-        const int r = next_row(cl, ncand, ci, d);
-        if (r == 0x7fffffff) break;
-        if (s.infront[r] == 0) continue;
-        const int inf = s.infront[r] - 1;
-        const int ii = s.infront[inf] - 1;
-        if (ii < 0) continue;
-        const double distii = dist_rows(s, s.x, r, s.x, ii);
-        if (distii < .99 * p.delta) {
-            const int tinfront = ii;
+    if (tid == 0) {
+        d.hd = d.nd = 0;
+        stop_sh = 0;
+    }
+    for (int cs = 0; cs < ncand; cs += RM_CHUNK) {
+        const int m = min(RM_CHUNK, ncand - cs);
+        __syncthreads();
+        for (int q = tid; q < MOD_SLOTS; q += RM_CHUNK) mod.v[q] = -1;
+        if (tid == 0) mod.n = 0;
+        if (tid < m) {
+            strow[tid] = cl[cs + tid];
+            load_rm(s, p, strow[tid], st[tid]);
+        }
+        __syncthreads();
+        if (tid != 0 || stop_sh) continue;
+        // rows below the next chunk's first candidate are this chunk's; the
+        // last chunk also takes every dirty row left
+        const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
+        bool tracked = true;   // the modified set is complete
+        int k = 0;
+        while (true) {
+            const int rc = k < m ? strow[k] : 0x7fffffff;
+            const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+            const int r = rc < rd ? rc : rd;
+            if (r == 0x7fffffff || r >= limit) break;
+            const bool staged = rc == r;
+            if (staged) ++k;
+            if (rd == r) ++d.hd;
+            RmIn in;
+            bool fresh = false;
+            if (staged) {
+                in = st[k - 1];
+                fresh = tracked && (mod.n == 0 || (!mod_has(mod, r) && (in.inf < 0 || !mod_has(mod, in.inf)) &&
+                                                   (in.ii < 0 || !mod_has(mod, in.ii))));
+            }
+            if (!fresh) load_rm(s, p, r, in);
+            if (!in.go) continue;
+            const int inf = in.inf, tinfront = in.ii;
             s.behind[tinfront] = r + 1;
             s.infront[r] = tinfront + 1;
             clear_row(s, inf);
-            loop_killer(s, r, n);
+            tracked = mod_add(mod, tinfront) && tracked;
+            tracked = mod_add(mod, r) && tracked;
+            tracked = mod_add(mod, inf) && tracked;
+            tracked = loop_killer_rec(s, r, n, mod) && tracked;
             tg.remove[tk] += 1;
             if (s.infront[r] != 0) {
                 const int b = s.behind[r] - 1;
-                if (b > r && !dirty_push(d, b)) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
+                if (b > r && !dirty_push(d, b)) {
+                    atomicOr(&ctl->status, NVK_CAPACITY);
+                    stop_sh = 1;
+                    break;
+                }
             }
         }
     }
@@ -1019,7 +1146,7 @@ hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
     return hipGetLastError();
 }
 hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
-    hipLaunchKernelGGL(k_premove_serial, dim3(tg.K), dim3(64), 0, st, s, p, tg, cand, ctl);
+    hipLaunchKernelGGL(k_premove_serial, dim3(tg.K), dim3(RM_CHUNK), 0, st, s, p, tg, cand, ctl);
     return hipGetLastError();
 }
 size_t nvk_sphere_bytes() { return sizeof(Sphere64); }
