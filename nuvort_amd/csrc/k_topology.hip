@@ -205,31 +205,23 @@ struct InsOut {
     double xn[3], gs_i[3], gf_i[3], gf_b[3];
 };
 
-// One visit of the reference loop at row r (src/line.f90:30-91) from its
-// inputs: returns the slot used (or -1 when r does not insert); -2 when the
-// tangle's rows are exhausted.
-__device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in, const int *empty, int &ei,
-                         int e1, int &nl, int base, int tcap, InsOut &out, const int *est, int es0, int es1) {
-    const int cap = s.cap;
-    if (!in.act) return -1;
+// The arithmetic of one visit (src/line.f90:30-91), which depends on the
+// visit's inputs only: whether r inserts, the new point, its velocity, and
+// the ghosts of the new point, of r and of r's old infront fi from the
+// positions the insertion leaves in place, as store_ghosts
+// (src/boundary.f90:26-84) computes them (fi's infront and r's behind are
+// unchanged by the insertion).
+struct InsPlan {
+    double xn[3], un[3], gs_i[3], gs_b[3], gr_i[3], gr_b[3], gf_i[3], gf_b[3];
+    int go;
+};
+
+__device__ void plan_insertion(const Phys &p, const InsIn &in, InsPlan &pl) {
+    pl.go = 0;
+    if (!in.act) return;
     const double disti = dist_gen(in.x[0], in.x[1], in.x[2], in.gi[0], in.gi[1], in.gi[2]);
-    if (!(disti > p.delta)) return -1;
-    int slot;
-    if (ei < e1) {
-        // the empty list's entries [es0, es1) are staged in est
-        slot = (ei >= es0 && ei < es1) ? est[ei - es0] : empty[ei];
-        ++ei;
-    } else {
-        if (nl >= tcap) return -2;
-        slot = base + nl;
-        ++nl;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            s.usup[k * cap + slot] = 0.0;
-            s.umf[k * cap + slot] = 0.0;
-        }
-        clear_row(s, slot);
-    }
+    if (!(disti > p.delta)) return;
+    pl.go = 1;
     double fd[3], fdd[3];
     derivs(in.x, in.gi, in.gb, fd, fdd);
     double curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
@@ -248,40 +240,62 @@ __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in,
 #pragma unroll
         for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (in.x[k] + in.gi[k]);
     }
-    // the new point, its links, and the ghosts of the new point, of r and of
-    // its old infront fi from the positions now in place, as store_ghosts
-    // (src/boundary.f90:26-84) computes them (fi's infront and r's behind are
-    // unchanged by this insertion)
-    const int fi = in.fi;
-    double gs_i[3], gs_b[3], gr_i[3], gr_b[3], gf_i[3], gf_b[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        gs_i[k] = in.xf[k]; gs_b[k] = in.x[k];
-        gr_i[k] = xn[k]; gr_b[k] = in.xb[k];
-        gf_i[k] = in.xff[k]; gf_b[k] = xn[k];
+        pl.xn[k] = xn[k];
+        pl.un[k] = 0.5 * (in.ur[k] + in.uf[k]);
+        pl.gs_i[k] = in.xf[k]; pl.gs_b[k] = in.x[k];
+        pl.gr_i[k] = xn[k]; pl.gr_b[k] = in.xb[k];
+        pl.gf_i[k] = in.xff[k]; pl.gf_b[k] = xn[k];
         if (!p.periodic[k]) continue;
         const double L = p.box[k];
-        wrap_ghost(xn[k], L, gs_i[k]);
-        wrap_ghost(xn[k], L, gs_b[k]);
-        wrap_ghost(in.x[k], L, gr_i[k]);
-        wrap_ghost(in.x[k], L, gr_b[k]);
-        wrap_ghost(in.xf[k], L, gf_i[k]);
-        wrap_ghost(in.xf[k], L, gf_b[k]);
+        wrap_ghost(xn[k], L, pl.gs_i[k]);
+        wrap_ghost(xn[k], L, pl.gs_b[k]);
+        wrap_ghost(in.x[k], L, pl.gr_i[k]);
+        wrap_ghost(in.x[k], L, pl.gr_b[k]);
+        wrap_ghost(in.xf[k], L, pl.gf_i[k]);
+        wrap_ghost(in.xf[k], L, pl.gf_b[k]);
     }
+}
+
+// One visit of the reference loop at row r from its plan: returns the slot
+// used (or -1 when r does not insert); -2 when the tangle's rows are
+// exhausted.
+__device__ int insert_at(const DState &s, int r, const InsIn &in, const InsPlan &pl, const int *empty, int &ei,
+                         int e1, int &nl, int base, int tcap, InsOut &out, const int *est, int es0, int es1) {
+    const int cap = s.cap;
+    if (!pl.go) return -1;
+    int slot;
+    if (ei < e1) {
+        // the empty list's entries [es0, es1) are staged in est
+        slot = (ei >= es0 && ei < es1) ? est[ei - es0] : empty[ei];
+        ++ei;
+    } else {
+        if (nl >= tcap) return -2;
+        slot = base + nl;
+        ++nl;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s.usup[k * cap + slot] = 0.0;
+            s.umf[k * cap + slot] = 0.0;
+        }
+        clear_row(s, slot);
+    }
+    const int fi = in.fi;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        s.x[k * cap + slot] = xn[k];
-        s.u[k * cap + slot] = 0.5 * (in.ur[k] + in.uf[k]);
+        s.x[k * cap + slot] = pl.xn[k];
+        s.u[k * cap + slot] = pl.un[k];
         s.u1[k * cap + slot] = 0.0;
         s.u2[k * cap + slot] = 0.0;
-        s.gi[k * cap + slot] = gs_i[k];
-        s.gb[k * cap + slot] = gs_b[k];
+        s.gi[k * cap + slot] = pl.gs_i[k];
+        s.gb[k * cap + slot] = pl.gs_b[k];
         // fi before r, the reference's refresh order (they coincide only for
         // a one-point loop)
-        s.gi[k * cap + fi] = gf_i[k];
-        s.gb[k * cap + fi] = gf_b[k];
-        s.gi[k * cap + r] = gr_i[k];
-        s.gb[k * cap + r] = gr_b[k];
+        s.gi[k * cap + fi] = pl.gf_i[k];
+        s.gb[k * cap + fi] = pl.gf_b[k];
+        s.gi[k * cap + r] = pl.gr_i[k];
+        s.gb[k * cap + r] = pl.gr_b[k];
     }
     s.behind[slot] = r + 1;
     s.infront[slot] = fi + 1;
@@ -289,10 +303,10 @@ __device__ int insert_at(const DState &s, const Phys &p, int r, const InsIn &in,
     s.infront[r] = slot + 1;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        out.xn[k] = xn[k];
-        out.gs_i[k] = gs_i[k];
-        out.gf_i[k] = gf_i[k];
-        out.gf_b[k] = gf_b[k];
+        out.xn[k] = pl.xn[k];
+        out.gs_i[k] = pl.gs_i[k];
+        out.gf_i[k] = pl.gf_i[k];
+        out.gf_b[k] = pl.gf_b[k];
     }
     return slot;
 }
@@ -315,12 +329,14 @@ __device__ __forceinline__ int lower_index(const int *v, int lo, int hi, int key
 // ghost it will see is the one just computed (a row's infront ghost changes
 // only when the row itself inserts).
 __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn &in, int slot, const InsOut &o,
-                             InsIn *st, const int *strow, int k, int m, Dirty &d, int n_old) {
+                             InsIn *st, unsigned char *stale, const int *strow, int k, int m, Dirty &d, int n_old) {
     const int fi = in.fi, bb = in.bb;
-    // the staged rows are ascending: find fi and bb among the rows not yet
-    // visited by bisection (either can be far from r, e.g. a new slot)
-    const int qf = lower_index(strow, k, m, fi);
+    // the staged rows are ascending and those not yet visited are above r:
+    // find fi and bb among them (usually fi is the next one; either can be
+    // far from r, e.g. a new slot); their plans are then stale
+    const int qf = fi <= r ? m : (k < m && strow[k] == fi) ? k : lower_index(strow, k, m, fi);
     if (qf < m && strow[qf] == fi) {
+        stale[qf] = 1;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             st[qf].gi[c] = o.gf_i[c];
@@ -329,8 +345,9 @@ __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn 
         }
         st[qf].bb = slot;
     }
-    const int qb = lower_index(strow, k, m, bb);
+    const int qb = bb <= r ? m : lower_index(strow, k, m, bb);
     if (qb < m && strow[qb] == bb) {
+        stale[qb] = 1;
 #pragma unroll
         for (int c = 0; c < 3; ++c) st[qb].xff[c] = o.xn[c];
     }
@@ -353,6 +370,8 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                                                               const int *empty, DCtl *ctl) {
     __shared__ Dirty d;
     __shared__ InsIn st[INS_CHUNK];
+    __shared__ InsPlan pl[INS_CHUNK];
+    __shared__ unsigned char stale[INS_CHUNK];   // the staged inputs changed after the plan
     __shared__ int strow[INS_CHUNK];
     __shared__ int nl_sh, stop_sh, ei_sh;
     __shared__ int est[INS_CHUNK];   // the next entries of the empty list
@@ -385,6 +404,8 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
             const int r = cl[cs + tid];
             strow[tid] = r;
             load_insertion(s, r, st[tid]);
+            plan_insertion(p, st[tid], pl[tid]);
+            stale[tid] = 0;
         }
         if (es0 + tid < es1) est[tid] = empty[es0 + tid];
         __syncthreads();
@@ -399,16 +420,23 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 const int r = rc < rd ? rc : rd;
                 if (r == 0x7fffffff || r >= limit) break;
                 InsIn in;
+                InsPlan plan;
                 const bool staged = rc == r;
                 if (rc == r) ++k;
                 if (rd == r) ++d.hd;
-                if (staged) in = st[k - 1];
-                else load_insertion(s, r, in);
+                if (staged) {
+                    in = st[k - 1];
+                    if (stale[k - 1]) plan_insertion(p, in, plan);
+                    else plan = pl[k - 1];
+                } else {
+                    load_insertion(s, r, in);
+                    plan_insertion(p, in, plan);
+                }
                 InsOut o;
-                const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
+                const int slot = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                 if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
                 if (slot < 0) continue;
-                if (!after_insert(s, p, r, in, slot, o, st, strow, k, m, d, n_old)) {
+                if (!after_insert(s, p, r, in, slot, o, st, stale, strow, k, m, d, n_old)) {
                     atomicOr(&ctl->status, NVK_CAPACITY);
                     stop_sh = 1;
                     break;
@@ -425,12 +453,14 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 while (d.hd < d.nd) {
                     const int r = d.v[d.hd++];
                     InsIn in;
+                    InsPlan plan;
                     load_insertion(s, r, in);
+                    plan_insertion(p, in, plan);
                     InsOut o;
-                    const int slot = insert_at(s, p, r, in, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
+                    const int slot = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                     if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
                     if (slot < 0) continue;
-                    if (!after_insert(s, p, r, in, slot, o, st, strow, 0, 0, d, n_old)) {
+                    if (!after_insert(s, p, r, in, slot, o, st, stale, strow, 0, 0, d, n_old)) {
                         atomicOr(&ctl->status, NVK_CAPACITY);
                         break;
                     }
