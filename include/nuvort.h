@@ -81,7 +81,7 @@ typedef struct {
     int32_t close_search;        /* NV_CLOSE_*: AUTO = tree when tree_theta > 0 */
     int32_t exact_order;         /* 1: sum each point's terms in the reference's order (one
                                     chunk per point, bit-exact FP64, slower); 0: chunked */
-    int32_t capacity;            /* initial device rows (0 = 2 x pcount) */
+    int32_t capacity;            /* initial device rows (0 = 3 x pcount + 1024; at least 2 x pcount + 256) */
     int32_t targets_per_lane;    /* FP32C direct sum: points per lane, 1 or 2 (0 = tuned) */
     int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
                                     adds, 1 (every term), 2, 4, 8 or 16 (0 = default 8) */

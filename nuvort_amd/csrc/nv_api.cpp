@@ -297,8 +297,12 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
     c->tremove.assign(K, 0);
     long rows = 0;
     for (int k = 0; k < K; ++k) {
+        // the rows a move to a wider layout would give (3 x pcount + margin),
+        // or the caller's count, never fewer than one step of insertions needs
+        const long deflt = 3L * pcounts[k] + (K == 1 ? 1024 : 256);
         const long want = caps ? (long)(*caps)[k]
-                               : std::max<long>(2L * pcounts[k] + (K == 1 ? 1024 : 256), (long)c->prm.capacity);
+                               : c->prm.capacity > 0 ? std::max<long>(2L * pcounts[k] + 256, (long)c->prm.capacity)
+                                                     : deflt;
         c->tbase[k] = (int)rows;
         c->tcap[k] = round256(std::max<long>(want, 256));
         rows += c->tcap[k];

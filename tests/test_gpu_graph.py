@@ -25,11 +25,11 @@ def params(**kw):
     return p
 
 
-def run(p, st, launch, steps, batch=False, between=None):
+def run(p, st, launch, steps, batch=False, between=None, capacity=0):
     from nuvort_amd.engine import FilamentSolver
     q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
     q.hip_step_launch = launch
-    s = FilamentSolver(q)
+    s = FilamentSolver(q, capacity=capacity)
     if batch:
         s.set_state_batch(st)
     else:
@@ -96,15 +96,17 @@ def concat(a, b):
 def test_recorded_step_after_layout_change():
     # the reconnecting tangle plus rings with segments of 1.9 delta: those
     # insert a point on every segment in step 1, so step 2 outgrows the
-    # initial rows and the state moves to a wider layout between two
-    # replays; the reconnections of step 1 stay in the log
+    # initial rows (2 x pcount + 1024, below the default) and the state moves
+    # to a wider layout between two replays; the reconnections of step 1 stay
+    # in the log
     g = np.random.default_rng(11)
     loops = [tangles.ring(g.uniform(-0.015, 0.015, 3), 0.01, g.normal(size=3), 5e-4, chord=1.9)
              for _ in range(10)]
     st = concat(dense(), tangles._assemble(loops, (0.03, 0.03, 0.03)))
     p = params(recon_shots=1, alpha=(0.206, 0.0083), normal_velocity="xflow", norm_vel_xflow=0.5)
-    ref = run(p, st, "eager", [5])
-    got = run(p, st, "graph", [5])
+    cap = 2 * st.pcount + 1024
+    ref = run(p, st, "eager", [5], capacity=cap)
+    got = run(p, st, "graph", [5], capacity=cap)
     assert ref[0][0].pcount > st.pcount + 600
     assert len(ref[1]) > 0
     assert_same(ref, got)
