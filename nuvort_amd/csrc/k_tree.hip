@@ -626,10 +626,54 @@ __global__ void k_target_keys(DState s, int n, double bsize, unsigned long long 
     keys[r] = code;
 }
 
+// Walk record of the close search: FP64 centroid, the bounds on dist^2
+// that decide the reference's opening test width / dist < 0.5 without its
+// division (below lo the node is opened, above hi accepted; between them the
+// test is evaluated as the reference does), and one word: the skip index,
+// or minus one minus a leaf's j = behind(leaf_infront) resolved when packed.
+struct CNode {
+    double cent[3];
+    double lo, hi;
+    int word, pad;
+};
+
+// With d = RN(sqrt(dd)) and q = RN(w / d): q < 0.5 <=> w / d < 0.5 - 2^-55
+// (0.5 is a power of two); dd > 4 w^2 (1 + 2^-46) makes d > 2 w (1 + 2^-48),
+// so q < 0.5, and dd < 4 w^2 (1 - 2^-46) makes d < 2 w, so q >= 0.5.  The
+// factors are far above the rounding of dd, of 4 w^2 and of the bounds.
+constexpr double CLOSE_BAND = 0x1p-46;
+
+__global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk, CNode *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > nwalk) return;
+    if (q == nwalk) {   // the spare record
+        out[q] = CNode{};
+        return;
+    }
+    const WNode w = walk[q];
+    CNode o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.cent[k] = w.cent[k];
+    const double w4 = 4.0 * (w.width * w.width);
+    o.lo = w4 * (1.0 - CLOSE_BAND);
+    o.hi = w4 * (1.0 + CLOSE_BAND);
+    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
+    o.pad = 0;
+    out[q] = o;
+}
+
+// src/tree.f90:190-244 pclose_tree on the stale tree: each lane accepts the
+// nodes the reference's recursion accepts (leaves, or width / dist < 0.5);
+// an accepted node closer than the best leaf so far sets closest = 0 (an
+// internal node) or, not being i, infront(i) or behind(i), becomes the best
+// (a leaf).  dist < best is decided from dd against best^2 as the opening
+// test is, and sqrt runs only where it cannot be: the stored closestd is the
+// reference's sqrt.  Control flow on wave-uniform conditions only, as in
+// k_walk_vel32.  Persistent over 64-target items.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
-                                                   int nitems, const WNode *__restrict__ walk, int nwalk,
-                                                   int *queue) {
+                                                   int nitems, const CNode *__restrict__ walk,
+                                                   const WNode *__restrict__ full, int nwalk, int *queue) {
     const int lane = threadIdx.x & 63;
     for (;;) {
     int it = 0;
@@ -649,35 +693,56 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         me = r + 1;
     }
     double best = 100.0;
+    double blo = best * best * (1.0 - CLOSE_BAND), bhi = best * best * (1.0 + CLOSE_BAND);
     int closest = act ? s.closest[r] : 0;
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
-    WNode nd = nwalk > 0 ? walk[0] : WNode{};
+    CNode nd = nwalk > 0 ? walk[0] : CNode{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
+        const int w = nd.word;
+        const bool leaf = w < 0;
         // both possible successors are fetched while this node is evaluated
-        const int n1 = cur + 1, n2 = nd.skip;
-        const WNode a = walk[n1 < nwalk ? n1 : cur];
-        const WNode b = walk[n2 < nwalk ? n2 : cur];
-        bool open = false;
-        if (cur >= lskip) {
-            const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
-            const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-            const double theta = nd.width / dist;
-            if (nd.pcount == 1 || theta < .5) {
-                lskip = nd.skip;
-                if (dist < best) {
-                    const int j = nd.leaf_j;
-                    if (nd.pcount > 1) {
-                        closest = 0;
-                    } else if ((me != j) && (inf != j) && (beh != j)) {
-                        closest = j;
-                        best = dist;
-                    }
-                }
-            } else {
-                open = true;
+        // (the records end with one spare, so neither index needs a clamp)
+        const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
+        const CNode a = walk[n1];
+        const CNode b = walk[n2];
+        const bool vis = cur >= lskip;
+        const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
+        const double dd = e0 * e0 + e1 * e1 + e2 * e2;
+        bool acc = vis;
+        if (!leaf) {
+            bool yes = dd > nd.hi;
+            const bool band = vis && !yes && !(dd < nd.lo);
+            if (__any(band)) {
+                const double width = full[cur].width;
+                if (band) yes = width / sqrt(dd) < .5;
             }
+            acc = vis && yes;
+        }
+        const bool open = vis && !acc;
+        lskip = acc ? n2 : lskip;
+        // dist < best: sure below blo, excluded above bhi
+        const bool sure = acc && dd < blo;
+        const bool maybe = acc && !sure && dd < bhi;
+        if (leaf) {
+            const int j = -w - 1;
+            const bool cand = (sure || maybe) && (me != j) && (inf != j) && (beh != j);
+            if (__any(cand)) {
+                const double dist = sqrt(dd);
+                if (cand && dist < best) {
+                    closest = j;
+                    best = dist;
+                    blo = best * best * (1.0 - CLOSE_BAND);
+                    bhi = best * best * (1.0 + CLOSE_BAND);
+                }
+            }
+        } else {
+            bool lt = sure;
+            if (__any(maybe)) {
+                if (maybe) lt = sqrt(dd) < best;
+            }
+            closest = lt ? 0 : closest;
         }
         const bool down = __any(open);
         cur = down ? n1 : n2;
@@ -707,6 +772,8 @@ struct TreeBuf {
     int walk_cap = 0;
     Node72 *walk72 = nullptr;            // FP32 walk records, packed per walk
     int walk72_cap = 0;
+    CNode *walkc = nullptr;              // close-search records, packed per search
+    int walkc_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
     int built_rows = 0;                  // rows the last build sorted into vals
     int grid32 = 0, grid64 = 0, gridc = 0;   // resident workgroups of the persistent walks
@@ -748,6 +815,7 @@ void nv_tree_free(TreeBuf *t) {
     hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
     hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
     hipFree(t->walk72);
+    hipFree(t->walkc);
     hipFree(t->counters);
     hipHostFree(t->h_int);
     free_nodes(t->nd);
@@ -919,13 +987,8 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     // order.  Each lane's sequence of accepted nodes does not depend on its
     // wave-mates, so the order changes no result.  Rows >= n hold no target
     // and k_finish reads no partial there.
-#ifdef NV_WALK_ROWORDER
-    const int *order = nullptr;
-    const int nord = 0;
-#else
     const int *order = t->vals;
     const int nord = t->vals && t->built_rows <= n ? t->built_rows : 0;
-#endif
     const int nb64 = (n + 63) / 64, nitems = nb64 * sh.n;
     // the FP32 walk's single opening radius needs width / theta > geo
     const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
@@ -983,26 +1046,26 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
     // inactive rows), whose waves would be scattered over the whole tangle;
     // the sort replaces it (the build's keys and rows are not needed after
     // the build; the velocity walk accepts any order of the rows).
-#ifdef NV_WALK_ROWORDER
-    const int *order = nullptr;
-    const int nord = 0;
-#else
     if (int rc = ensure_rows(t, n)) return rc;
     hipLaunchKernelGGL(k_target_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
     TCHK(hipGetLastError());
     TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 32, t->hist, t->tmp));
     t->vals = t->v0;
     t->built_rows = n;
-    const int *order = t->vals;
-    const int nord = n;
-#endif
-    if (t->nwalk > 0)
-        hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
+    // close-search records, one spare past the last
+    if (t->nwalk + 1 > t->walkc_cap) {
+        hipFree(t->walkc);
+        t->walkc = nullptr;
+        t->walkc_cap = std::max(t->nwalk + 1, 2 * t->walkc_cap);
+        TCHK(hipMalloc(&t->walkc, sizeof(CNode) * (size_t)t->walkc_cap));
+    }
+    hipLaunchKernelGGL(k_pack_close, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
+                       t->walkc);
     if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
     const int nitems = (n + 63) / 64;
     TCHK(nv_fill_int(st, (int *)(t->counters + 3), 0, 1));   // the queue (the walk counters stay)
-    hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, order,
-                       nord, nitems, t->walk, t->nwalk, (int *)(t->counters + 3));
+    hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
+                       n, nitems, t->walkc, t->walk, t->nwalk, (int *)(t->counters + 3));
     TCHK(hipGetLastError());
     return NV_OK;
 }
