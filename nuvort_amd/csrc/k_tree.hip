@@ -288,6 +288,41 @@ __global__ void k_zero_counters(unsigned long long *c) {
 // ------------------------------------------------------------ walks
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Work queues of the persistent walks, one per XCD.  The nb blocks of 64
+// targets (in Morton order) are split into XCDS contiguous ranges; queue k
+// hands out (image, block) items of range k, the images in order (the
+// unshifted one, the longest, first).  A workgroup takes items from queue
+// blockIdx.x % XCDS (workgroups are dispatched round-robin over the XCDs, so
+// that is its own XCD, whose L2 then holds one region's nodes), then from the
+// others in turn, so every queue drains whatever the mapping.
+constexpr int XCDS = 8;
+
+struct WalkQueues {
+    int *cnt;   // XCDS counters, zero at launch
+    int nb;     // blocks of 64 targets
+    int nimg;   // images per block
+};
+
+// The next item of the calling wave (every lane active): false once all
+// queues are empty.  h: the queues this wave has emptied.
+__device__ __forceinline__ bool next_walk_item(const WalkQueues &wq, int &h, int &m, int &tb) {
+    const int home = blockIdx.x % XCDS;
+    while (h < XCDS) {
+        const int k = (home + h) % XCDS;
+        const int b0 = (int)((long)wq.nb * k / XCDS), len = (int)((long)wq.nb * (k + 1) / XCDS) - b0;
+        int it = 0;
+        if ((threadIdx.x & 63) == 0) it = atomicAdd(&wq.cnt[k], 1);
+        it = wave_uniform(it);   // lane 0 is the first active lane
+        if (it < len * wq.nimg) {
+            m = it / len;
+            tb = b0 + (it - m * len);
+            return true;
+        }
+        ++h;
+    }
+    return false;
+}
+
 // WNode -> Node72.  The reference accepts a node when pcount == 1 or
 // width / (dist - geo) < theta (width / dist when dist <= geo or without the
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
@@ -368,21 +403,19 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 // walks first, so that no SIMD idles while long walks finish.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
-                                                   const int *__restrict__ order, int nord, int nb64,
-                                                   int nitems, const Node72 *__restrict__ walk, int nwalk,
+                                                   const int *__restrict__ order, int nord, WalkQueues wq,
+                                                   const Node72 *__restrict__ walk, int nwalk,
                                                    double rn2, double sing2, double *__restrict__ part, DCtl *ctl,
-                                                   unsigned long long *counters, int *queue) {
+                                                   unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
+    int h = 0;   // queues emptied
     const double kappa = p.kappa;
     unsigned long long nev = 0, nvis = 0, nit = 0;
     bool sing = false;
     for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(queue, 1);
-    it = wave_uniform(it);   // every lane is active here: the first is lane 0
-    if (it >= nitems) break;
-    const int m = it / nb64;
-    const int q = (it - m * nb64) * 64 + lane;
+    int m, tb;
+    if (!next_walk_item(wq, h, m, tb)) break;
+    const int q = tb * 64 + lane;
     const int r = q < nord ? order[q] : q;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
@@ -498,19 +531,17 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
 // two sums (src/timestep.f90:107-115)
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
-                                                 const int *__restrict__ order, int nord, int nb64, int nitems,
+                                                 const int *__restrict__ order, int nord, WalkQueues wq,
                                                  const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
-                                                 DCtl *ctl, unsigned long long *counters, int *queue) {
+                                                 DCtl *ctl, unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
+    int h = 0;   // queues emptied
     unsigned long long nev = 0, nvis = 0, nit = 0;
     bool sing = false;
     for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(queue, 1);
-    it = wave_uniform(it);   // every lane is active here: the first is lane 0
-    if (it >= nitems) break;
-    const int m = it / nb64;
-    const int q = (it - m * nb64) * 64 + lane;
+    int m, tb;
+    if (!next_walk_item(wq, h, m, tb)) break;
+    const int q = tb * 64 + lane;
     const int r = q < nord ? order[q] : q;
     const bool act = r < n && s.infront[r] != 0;
     double xi[3] = {0, 0, 0};
@@ -672,15 +703,15 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk
 // k_walk_vel32.  Persistent over 64-target items.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
-                                                   int nitems, const CNode *__restrict__ walk,
-                                                   const WNode *__restrict__ full, int nwalk, int *queue) {
+                                                   WalkQueues wq, const CNode *__restrict__ walk,
+                                                   const WNode *__restrict__ full, int nwalk,
+                                                   unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
+    int h = 0;   // queues emptied
     for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(queue, 1);
-    it = wave_uniform(it);   // every lane is active here: the first is lane 0
-    if (it >= nitems) break;
-    const int q = it * 64 + lane;
+    int m, tb;
+    if (!next_walk_item(wq, h, m, tb)) break;
+    const int q = tb * 64 + lane;
     const int r = q < nord ? order[q] : q;
     if (r < n) s.closestd[r] = 100.0;   // set for every slot (src/tree.f90:196)
     const bool act = r < n && s.infront[r] != 0;
@@ -708,6 +739,10 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         const CNode a = walk[n1];
         const CNode b = walk[n2];
         const bool vis = cur >= lskip;
+#ifdef NV_WALK_PROBE
+        if ((threadIdx.x & 63) == 0) atomicAdd(&counters[2], 1ull);
+        atomicAdd(&counters[3], vis ? 1ull : 0ull);
+#endif
         const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
         const double dd = e0 * e0 + e1 * e1 + e2 * e2;
         bool acc = vis;
@@ -779,6 +814,7 @@ struct TreeBuf {
     int grid32 = 0, grid64 = 0, gridc = 0;   // resident workgroups of the persistent walks
     std::vector<int> lvl;                // level start offsets
     unsigned long long *counters = nullptr;
+    int *wq = nullptr;                   // the walks' per-XCD work queues
     int *h_int = nullptr;                // pinned scratch
     const int *vals = nullptr;           // sorted rows of the last build
 };
@@ -792,6 +828,7 @@ struct TreeBuf {
 TreeBuf *nv_tree_new() {
     TreeBuf *t = new TreeBuf();
     if (hipMalloc(&t->nact, sizeof(int)) != hipSuccess || hipMalloc(&t->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&t->wq, XCDS * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void **)&t->h_int, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         delete t;
         return nullptr;
@@ -817,6 +854,7 @@ void nv_tree_free(TreeBuf *t) {
     hipFree(t->walk72);
     hipFree(t->walkc);
     hipFree(t->counters);
+    hipFree(t->wq);
     hipHostFree(t->h_int);
     free_nodes(t->nd);
     delete t;
@@ -1005,16 +1043,18 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
                            p.tree_theta, p.extra_corr, t->walk72);
         if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256>)) == 0) return NV_ERR_HIP;
         const int grid = std::min(t->grid32, (nitems + 3) / 4);
-        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord, nb64,
-                           nitems, t->walk72, t->nwalk, rn2, sqrt_threshold(NV_EPS), part, ctl, t->counters,
-                           (int *)(t->counters + 3));
+        TCHK(nv_fill_int(st, t->wq, 0, XCDS));
+        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
+                           WalkQueues{t->wq, nb64, sh.n}, t->walk72, t->nwalk, rn2, sqrt_threshold(NV_EPS), part,
+                           ctl, t->counters);
     } else {
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
         if (t->grid64 == 0 && (t->grid64 = resident_blocks(k_walk_vel<256>)) == 0) return NV_ERR_HIP;
         const int grid = std::min(t->grid64, (nitems + 3) / 4);
-        hipLaunchKernelGGL(k_walk_vel<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord, nb64,
-                           nitems, t->walk, t->nwalk, part, ctl, t->counters, (int *)(t->counters + 3));
+        TCHK(nv_fill_int(st, t->wq, 0, XCDS));
+        hipLaunchKernelGGL(k_walk_vel<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
+                           WalkQueues{t->wq, nb64, sh.n}, t->walk, t->nwalk, part, ctl, t->counters);
     }
     TCHK(hipGetLastError());
     return NV_OK;
@@ -1063,9 +1103,22 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
                        t->walkc);
     if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
     const int nitems = (n + 63) / 64;
-    TCHK(nv_fill_int(st, (int *)(t->counters + 3), 0, 1));   // the queue (the walk counters stay)
+    TCHK(nv_fill_int(st, t->wq, 0, XCDS));
     hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
-                       n, nitems, t->walkc, t->walk, t->nwalk, (int *)(t->counters + 3));
+                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, t->counters);
+#ifdef NV_WALK_PROBE
+    {
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
+        TCHK(nv_fill_int(st, t->wq, 0, XCDS));
+        hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n,
+                           t->vals, n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, t->counters);
+        unsigned long long h[4];
+        TCHK(hipMemcpyAsync(h, t->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        TCHK(hipStreamSynchronize(st));
+        fprintf(stderr, "close: items %d, wave iterations %llu, lane visits %llu (wave its x64 / visits %.2f)\n",
+                nitems, h[2], h[3], (double)h[2] * 64.0 / (double)(h[3] ? h[3] : 1));
+    }
+#endif
     TCHK(hipGetLastError());
     return NV_OK;
 }
