@@ -84,7 +84,7 @@ typedef struct {
     int32_t capacity;            /* initial device rows (0 = 3 x pcount + 1024; at least 2 x pcount + 256) */
     int32_t targets_per_lane;    /* FP32C direct sum: points per lane, 1 or 2 (0 = tuned) */
     int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
-                                    adds, 1 (every term), 2, 4, 8 or 16 (0 = default 8) */
+                                    adds, 1 (every term), 2, 4, 8, 16 or 32 (0 = default 8) */
     int32_t step_launch;         /* nv_step: 0 = record one step's launches in a HIP graph and
                                     replay it every step (direct sum and LIA; the octree step
                                     launches eagerly), 1 = eager launches */

@@ -545,6 +545,7 @@ int nvk_occupancy_bs32c(int tpt, int kb) {
             case 2: return occ_bs32c<2, 2>();
             case 4: return occ_bs32c<2, 4>();
             case 16: return occ_bs32c<2, 16>();
+            case 32: return occ_bs32c<2, 32>();
             default: return occ_bs32c<2, 8>();
         }
     }
@@ -553,6 +554,7 @@ int nvk_occupancy_bs32c(int tpt, int kb) {
         case 2: return occ_bs32c<1, 2>();
         case 4: return occ_bs32c<1, 4>();
         case 16: return occ_bs32c<1, 16>();
+        case 32: return occ_bs32c<1, 32>();
         default: return occ_bs32c<1, 8>();
     }
 }
@@ -602,12 +604,14 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
     if (tpt == 2) {
         if (kb == 8) NV_BS32(2, 8);
         else if (kb == 16) NV_BS32(2, 16);
+        else if (kb == 32) NV_BS32(2, 32);
         else if (kb == 4) NV_BS32(2, 4);
         else if (kb == 2) NV_BS32(2, 2);
         else NV_BS32(2, 1);
     } else {
         if (kb == 8) NV_BS32(1, 8);
         else if (kb == 16) NV_BS32(1, 16);
+        else if (kb == 32) NV_BS32(1, 32);
         else if (kb == 4) NV_BS32(1, 4);
         else if (kb == 2) NV_BS32(1, 2);
         else NV_BS32(1, 1);

@@ -874,7 +874,7 @@ int nv_create(const nv_params *params, void **out) {
     // default: compensated add every 8 terms (measured: same error as every
     // term on the 16k-point ring tangle, 1.27x faster)
     c->kb = (p.kahan_block == 1 || p.kahan_block == 2 || p.kahan_block == 4 || p.kahan_block == 8 ||
-             p.kahan_block == 16) ? p.kahan_block : 8;
+             p.kahan_block == 16 || p.kahan_block == 32) ? p.kahan_block : 8;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
@@ -1234,7 +1234,7 @@ int nv_autotune(void *h, const char *cache_path) {
                 }
             }
             std::fclose(f);
-            if (hit && (ht == 1 || ht == 2) && (hk == 1 || hk == 2 || hk == 4 || hk == 8 || hk == 16) && hi >= 1 &&
+            if (hit && (ht == 1 || ht == 2) && (hk == 1 || hk == 2 || hk == 4 || hk == 8 || hk == 16 || hk == 32) && hi >= 1 &&
                 hi <= 256) {
                 apply(ht, hk, hi);
                 c->stats.tuned = 2;
@@ -1306,7 +1306,7 @@ int nv_autotune(void *h, const char *cache_path) {
         for (int t : {1, 2})
             if (int rc = consider(t, bk, bi)) return rc;
     if (free_kb)
-        for (int k : {4, 8, 16})
+        for (int k : {4, 8, 16, 32})
             if (int rc = consider(bt, k, bi)) return rc;
     apply(bt, bk, bi);
     c->stats.tuned = 1;
