@@ -311,6 +311,11 @@ __device__ int insert_at(const DState &s, int r, const InsIn &in, const InsPlan 
     return slot;
 }
 
+__device__ __forceinline__ double dist2_gen(const double a[3], const double b[3]) {
+    const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
 // First index q in [lo, hi) with v[q] >= key (v ascending), hi if none.
 __device__ __forceinline__ int lower_index(const int *v, int lo, int hi, int key) {
     while (lo < hi) {
@@ -351,11 +356,15 @@ __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn 
 #pragma unroll
         for (int c = 0; c < 3; ++c) st[qb].xff[c] = o.xn[c];
     }
-    if (fi > r && fi < n_old &&
-        dist_gen(in.xf[0], in.xf[1], in.xf[2], o.gf_i[0], o.gf_i[1], o.gf_i[2]) > p.delta && !dirty_push(d, fi))
+    // A row the loop has still to reach is queued when its visit may insert;
+    // a staged candidate is visited anyway.  The test is on squared
+    // distances with a margin, so it queues a superset of the rows that
+    // insert (a visit that does not insert changes nothing).
+    const double lim2 = p.delta * p.delta * (1.0 - 1.0e-9);
+    if (fi > r && fi < n_old && !(qf < m && strow[qf] == fi) &&
+        dist2_gen(in.xf, o.gf_i) > lim2 && !dirty_push(d, fi))
         return false;
-    if (slot > r && slot < n_old &&
-        dist_gen(o.xn[0], o.xn[1], o.xn[2], o.gs_i[0], o.gs_i[1], o.gs_i[2]) > p.delta && !dirty_push(d, slot))
+    if (slot > r && slot < n_old && dist2_gen(o.xn, o.gs_i) > lim2 && !dirty_push(d, slot))
         return false;
     return true;
 }
