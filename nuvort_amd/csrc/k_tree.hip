@@ -62,13 +62,16 @@ __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
 
 __global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals, int *nact) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = r < n && s.infront[r] != 0;
+    // active points counted per wave, one atomic each
+    const unsigned long long bal = __ballot(act);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(nact, (int)__popcll(bal));
     if (r >= n) return;
     vals[r] = r;
-    if (s.infront[r] == 0) {
+    if (!act) {
         keys[r] = ~0ull;
         return;
     }
-    atomicAdd(nact, 1);
     const double x[3] = {s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r]};
     double pos[3] = {-bsize / 2.0, -bsize / 2.0, -bsize / 2.0};
     double w = bsize;
@@ -132,11 +135,29 @@ __device__ __forceinline__ int lower_digit(const unsigned long long *keys, int l
     return lo;
 }
 
-// children of the split nodes of level `level` (nodes [L0, L1)) into L1 + 8*off
-__global__ void k_children(Nodes nd, int L0, int L1, const int *flags, const int *off,
-                           const unsigned long long *keys, int level) {
+// child c of split node q, its keys [lo, hi)
+__device__ __forceinline__ void write_child(const Nodes &nd, int q, int base, int c, int lo, int hi) {
+    const double w = nd.w[q];
+    const double p0 = nd.pos[q], p1 = nd.pos[nd.ncap + q], p2 = nd.pos[2 * nd.ncap + q];
+    const int ch = base + c;
+    const int xb = c & 1, zb = (c >> 1) & 1, yb = (c >> 2) & 1;
+    nd.pos[ch] = xb ? p0 + (w / 2) : p0;
+    nd.pos[nd.ncap + ch] = yb ? p1 + (w / 2) : p1;
+    nd.pos[2 * nd.ncap + ch] = zb ? p2 + (w / 2) : p2;
+    nd.w[ch] = w / 2.0;
+    nd.beg[ch] = lo;
+    nd.cnt[ch] = hi - lo;
+    nd.pcount[ch] = hi - lo;
+    nd.parent[ch] = q;
+    nd.first[ch] = -1;
+    nd.dropb[ch] = hi;
+}
+
+// children of the split nodes (pcount > 1) of level `level` (nodes [L0,
+// L1)) into L1 + 8 * off, one thread per node
+__global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsigned long long *keys, int level) {
     int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L1 || !flags[q - L0]) return;
+    if (q >= L1 || nd.pcount[q] <= 1) return;
     const int base = L1 + 8 * off[q - L0];
     nd.first[q] = base;
     const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
@@ -144,21 +165,39 @@ __global__ void k_children(Nodes nd, int L0, int L1, const int *flags, const int
     bnd[0] = b0;
     for (int d = 1; d <= 8; ++d) bnd[d] = lower_digit(keys, bnd[d - 1], b1, level + 1, d);
     nd.dropb[q] = bnd[8];
-    const double w = nd.w[q];
-    const double p0 = nd.pos[q], p1 = nd.pos[nd.ncap + q], p2 = nd.pos[2 * nd.ncap + q];
-    for (int c = 0; c < 8; ++c) {
-        const int ch = base + c;
-        const int xb = c & 1, zb = (c >> 1) & 1, yb = (c >> 2) & 1;
-        nd.pos[ch] = xb ? p0 + (w / 2) : p0;
-        nd.pos[nd.ncap + ch] = yb ? p1 + (w / 2) : p1;
-        nd.pos[2 * nd.ncap + ch] = zb ? p2 + (w / 2) : p2;
-        nd.w[ch] = w / 2.0;
-        nd.beg[ch] = bnd[c];
-        nd.cnt[ch] = bnd[c + 1] - bnd[c];
-        nd.pcount[ch] = bnd[c + 1] - bnd[c];
-        nd.parent[ch] = q;
-        nd.first[ch] = -1;
-        nd.dropb[ch] = bnd[c + 1];
+    for (int c = 0; c < 8; ++c) write_child(nd, q, base, c, bnd[c], bnd[c + 1]);
+}
+
+// The same for a level of few nodes with long key ranges, one workgroup of
+// 256 per node: 32 lanes find each of the 8 child boundaries by a 32-way
+// search (a node's keys are ascending in the next level's digit).
+__global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off,
+                                                       const unsigned long long *keys, int level) {
+    __shared__ int bnd[9];
+    const int q = L0 + blockIdx.x;
+    if (q >= L1 || nd.pcount[q] <= 1) return;
+    const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
+    const int d = 1 + (threadIdx.x >> 5), li = threadIdx.x & 31, sh = threadIdx.x & 32;
+    // first position in [b0, b1) with digit >= d
+    int lo = b0, hi = b1;
+    while (hi > lo) {
+        const int step = (hi - lo + 31) / 32;
+        const int pos = lo + li * step;
+        const bool below = pos < hi && digit_at(keys[pos], level + 1) < d;
+        const int c = __popcll((__ballot(below) >> sh) & 0xffffffffull);
+        if (c == 0) { hi = lo; break; }
+        const int nlo = lo + (c - 1) * step + 1, nhi = min(hi, lo + c * step);
+        lo = nlo;
+        hi = nhi;
+    }
+    if (li == 0) bnd[d] = lo;
+    if (threadIdx.x == 0) bnd[0] = b0;
+    __syncthreads();
+    const int base = L1 + 8 * off[q - L0];
+    if (threadIdx.x < 8) write_child(nd, q, base, threadIdx.x, bnd[threadIdx.x], bnd[threadIdx.x + 1]);
+    if (threadIdx.x == 0) {
+        nd.first[q] = base;
+        nd.dropb[q] = bnd[8];
     }
 }
 
@@ -940,22 +979,25 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
         const int L0 = t->lvl[level], L1 = t->lvl[level + 1], m = L1 - L0;
         if (m == 0) break;
         if (int rc = ensure_flags(t, m)) return rc;
-        hipLaunchKernelGGL(k_split_flags, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->flags,
+        // split flags, scanned in place into the children offsets
+        hipLaunchKernelGGL(k_split_flags, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->offs,
                            level == LEVELS ? 1 : 0, ctl);
         TCHK(hipGetLastError());
-        TCHK(hipMemcpyAsync(t->offs, t->flags, sizeof(int) * m, hipMemcpyDeviceToDevice, st));
         TCHK(nv_exclusive_scan(st, t->offs, m, t->scan_tmp));
         TCHK(hipMemcpyAsync(t->h_int, t->offs + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-        TCHK(hipMemcpyAsync(t->h_int + 1, t->flags + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+        TCHK(hipMemcpyAsync(t->h_int + 1, t->nd.pcount + (L1 - 1), sizeof(int), hipMemcpyDeviceToHost, st));
         TCHK(hipStreamSynchronize(st));
-        const int nsplit = t->h_int[0] + t->h_int[1];
+        const int nsplit = t->h_int[0] + (t->h_int[1] > 1 ? 1 : 0);
         if (nsplit == 0 || level == LEVELS) {
             if (nsplit && level == LEVELS) return NV_ERR_TREE_DEPTH;
             break;
         }
         if (int rc = grow_nodes(t, st, L1 + 8 * nsplit, L1)) return rc;
-        hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->flags, t->offs,
-                           t->k0, level);
+        if (m <= 4096 && nsplit * 64 <= t->nact_h)   // few nodes holding many keys each
+            hipLaunchKernelGGL(k_children_wide, dim3(m), dim3(256), 0, st, t->nd, L0, L1, t->offs, t->k0, level);
+        else
+            hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->offs, t->k0,
+                               level);
         TCHK(hipGetLastError());
         t->lvl.push_back(L1 + 8 * nsplit);
         ++level;
