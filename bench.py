@@ -354,7 +354,7 @@ def tree1m(args):
     walk_nodes = np.mean([x["walk_nodes"] for x in stats])
     walk_ms = np.mean([x["walk_ms"] for x in stats])
     n = st.active
-    rec = 72 if args.tree_precision == "fp32c" else 80
+    rec = 48 if args.tree_precision == "fp32c" else 80
     walk_bytes = walk_nodes * rec + n * 27 * 12 + n * 12
     line = {
         "metric": METRIC, "value": pu / t_all, "unit": "point-updates/s", "n_gpus": 1, "steps": nsteps,

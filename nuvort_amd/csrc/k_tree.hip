@@ -24,6 +24,7 @@
 // recursive walk would, in the same order, while all node reads are
 // wave-uniform (scalar loads).
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <vector>
 
@@ -42,19 +43,33 @@ struct __align__(16) WNode {
     int leaf_j;   // a leaf's behind(leaf_infront) when the walk runs (k_leaf_links)
 };
 
-// Walk record of the FP32 walk: FP64 centroid and circulation (the FP64
-// cases), the circulation in FP32 with its squared length (the FP32 term),
-// the opening radius squared (FP32, rounded up), and one word: the skip
-// index of an internal node, or minus one minus a leaf's j = behind(
-// leaf_infront), resolved when the record is packed, right before the walk.
-struct __align__(8) Node72 {
+// Walk record of the FP32 walk: FP64 centroid, the circulation in FP32 with
+// its squared length (the FP32 term), the opening radius squared (FP32,
+// rounded up), and one word: the skip index of an internal node, or minus
+// one minus a leaf's j = behind(leaf_infront), resolved when the record is
+// packed, right before the walk.  The FP64 circulation, which only the FP64
+// cases read, is in a side array (three doubles per record), so that the
+// records a wave keeps in flight fit in scalar registers.
+struct __align__(8) Node48 {
     double cent[3];
-    double circ[3];
+    float r2;
+    int word;
     float e[3];
     float ee;
+};
+// The part of a record the walk fetches ahead for both possible successors,
+// and the part the FP32 term reads for the current node.
+struct __align__(8) NodeHot {
+    double cent[3];
     float r2;
     int word;
 };
+struct __align__(16) NodeCold {
+    float e[3];
+    float ee;
+};
+static_assert(sizeof(Node48) == 48 && sizeof(NodeHot) == 32 && offsetof(Node48, e) == sizeof(NodeHot),
+              "the hot part leads the record");
 
 __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
     return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
@@ -362,25 +377,25 @@ __device__ __forceinline__ bool next_walk_item(const WalkQueues &wq, int &h, int
     return false;
 }
 
-// WNode -> Node72.  The reference accepts a node when pcount == 1 or
+// WNode -> Node48.  The reference accepts a node when pcount == 1 or
 // width / (dist - geo) < theta (width / dist when dist <= geo or without the
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
 // geo) that is dist > geo + width / theta (width / theta without the
 // correction), compared here on squares.
-__global__ void k_pack72(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
-                         Node72 *__restrict__ out) {
+__global__ void k_pack48(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
+                         Node48 *__restrict__ out, double *__restrict__ circ64) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > nwalk) return;
     if (q == nwalk) {   // the spare record
-        out[q] = Node72{};
+        out[q] = Node48{};
         return;
     }
     const WNode w = walk[q];
-    Node72 o;
+    Node48 o;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         o.cent[k] = w.cent[k];
-        o.circ[k] = w.circ[k];
+        circ64[3 * q + k] = w.circ[k];
     }
     const double r = (corr ? w.geo : 0.0) + w.width / theta;
     o.r2 = (float)(r * r) * (1.0f + 2.0e-7f);
@@ -443,7 +458,8 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
                                                    const int *__restrict__ order, int nord, WalkQueues wq,
-                                                   const Node72 *__restrict__ walk, int nwalk,
+                                                   const Node48 *__restrict__ walk,
+                                                   const double *__restrict__ circ64, int nwalk,
                                                    double rn2, double sing2, double *__restrict__ part, DCtl *ctl,
                                                    unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
@@ -472,7 +488,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     const bool img0 = (m == 0);
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
-    Node72 nd = nwalk > 0 ? walk[0] : Node72{};
+    const char *wb = (const char *)walk;
+    NodeHot nd = *(const NodeHot *)wb;
     // Control flow on wave-uniform conditions only (the node is uniform):
     // lane conditions become selects, so that the scalar unit, which the
     // four SIMDs of a CU share, does not spend the walk on exec masks.
@@ -484,8 +501,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         // both possible successors are fetched while this node is evaluated
         // (the records end with one spare, so neither index needs a clamp)
         const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
-        const Node72 a = walk[n1];
-        const Node72 b = walk[n2];
+        const NodeHot a = *(const NodeHot *)(wb + (size_t)n1 * sizeof(Node48));
+        const NodeHot b = *(const NodeHot *)(wb + (size_t)n2 * sizeof(Node48));
         const bool vis = cur >= lskip;
         nvis += (unsigned long long)__popcll(__ballot(vis));
         const double v0 = nd.cent[0] - xs0, v1 = nd.cent[1] - xs1, v2 = nd.cent[2] - xs2;
@@ -506,14 +523,15 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         const bool far = use && !near;
         bool need64 = use && near;
         if (__any(far)) {
+            const NodeCold cd = *(const NodeCold *)(wb + (size_t)cur * sizeof(Node48) + sizeof(NodeHot));
             const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
-            const float e0 = nd.e[0], e1 = nd.e[1], e2 = nd.e[2];
+            const float e0 = cd.e[0], e1 = cd.e[1], e2 = cd.e[2];
             const float x0 = __fmaf_rn(d1, e2, -(d2 * e1));
             const float x1 = __fmaf_rn(d2, e0, -(d0 * e2));
             const float x2 = __fmaf_rn(d0, e1, -(d1 * e0));
             const float r1sq = (float)dd;
             const float de = __fmaf_rn(d0, e0, __fmaf_rn(d1, e1, d2 * e2));
-            const float r2sq = __fmaf_rn(2.0f, de, r1sq) + nd.ee;
+            const float r2sq = __fmaf_rn(2.0f, de, r1sq) + cd.ee;
             const float inv1 = __builtin_amdgcn_rsqf(r1sq);
             const float inv2 = __builtin_amdgcn_rsqf(r2sq);
             const float r12 = (r1sq * inv1) * (r2sq * inv2);
@@ -535,7 +553,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
             need64 = need64 || (far && ill);
         }
         if (__any(need64)) {
-            if (need64) near_term64(v0, v1, v2, nd.circ, dd, kappa, nr, nev);
+            const double c64[3] = {circ64[3 * cur], circ64[3 * cur + 1], circ64[3 * cur + 2]};
+            if (need64) near_term64(v0, v1, v2, c64, dd, kappa, nr, nev);
         }
         const bool down = __any(open);
         cur = down ? n1 : n2;
@@ -844,8 +863,9 @@ struct TreeBuf {
     int flag_cap = 0;
     WNode *walk = nullptr;
     int walk_cap = 0;
-    Node72 *walk72 = nullptr;            // FP32 walk records, packed per walk
-    int walk72_cap = 0;
+    Node48 *walk48 = nullptr;            // FP32 walk records, packed per walk
+    double *circ64 = nullptr;            // their FP64 circulations
+    int walk48_cap = 0;
     CNode *walkc = nullptr;              // close-search records, packed per search
     int walkc_cap = 0;
     int nwalk = 0, nnodes = 0, nact_h = 0;
@@ -890,7 +910,8 @@ void nv_tree_free(TreeBuf *t) {
     if (!t) return;
     hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
     hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
-    hipFree(t->walk72);
+    hipFree(t->walk48);
+    hipFree(t->circ64);
     hipFree(t->walkc);
     hipFree(t->counters);
     hipFree(t->wq);
@@ -1075,19 +1096,22 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     if (fp32) {
         // one spare record past the last: the walk fetches a successor index
         // up to nwalk without a clamp
-        if (t->nwalk + 1 > t->walk72_cap) {
-            hipFree(t->walk72);
-            t->walk72 = nullptr;
-            t->walk72_cap = std::max(t->nwalk + 1, 2 * t->walk72_cap);
-            TCHK(hipMalloc(&t->walk72, sizeof(Node72) * (size_t)t->walk72_cap));
+        if (t->nwalk + 1 > t->walk48_cap) {
+            hipFree(t->walk48);
+            t->walk48 = nullptr;
+            t->walk48_cap = std::max(t->nwalk + 1, 2 * t->walk48_cap);
+            TCHK(hipMalloc(&t->walk48, sizeof(Node48) * (size_t)t->walk48_cap));
+            hipFree(t->circ64);
+            t->circ64 = nullptr;
+            TCHK(hipMalloc(&t->circ64, 3 * sizeof(double) * (size_t)t->walk48_cap));
         }
-        hipLaunchKernelGGL(k_pack72, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
-                           p.tree_theta, p.extra_corr, t->walk72);
+        hipLaunchKernelGGL(k_pack48, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
+                           p.tree_theta, p.extra_corr, t->walk48, t->circ64);
         if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256>)) == 0) return NV_ERR_HIP;
         const int grid = std::min(t->grid32, (nitems + 3) / 4);
         TCHK(nv_fill_int(st, t->wq, 0, XCDS));
         hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           WalkQueues{t->wq, nb64, sh.n}, t->walk72, t->nwalk, rn2, sqrt_threshold(NV_EPS), part,
+                           WalkQueues{t->wq, nb64, sh.n}, t->walk48, t->circ64, t->nwalk, rn2, sqrt_threshold(NV_EPS), part,
                            ctl, t->counters);
     } else {
         if (t->nwalk > 0)
@@ -1103,7 +1127,7 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
 }
 
 int nv_tree_walk_record_bytes(int precision, double theta) {
-    return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node72) : (int)sizeof(WNode);
+    return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node48) : (int)sizeof(WNode);
 }
 
 // interaction and node-visit counters of the last velocity walk (synchronises)
