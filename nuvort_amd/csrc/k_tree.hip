@@ -465,7 +465,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     const double kappa = p.kappa;
-    unsigned long long nev = 0, nvis = 0, nit = 0;
+    unsigned long long nev = 0, nvis = 0;
     bool sing = false;
     for (;;) {
     int m, tb;
@@ -495,7 +495,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     // four SIMDs of a CU share, does not spend the walk on exec masks.
     while (cur < nwalk) {
         cur = wave_uniform(cur);
-        ++nit;
         const int w = nd.word;
         const bool leaf = w < 0;
         // both possible successors are fetched while this node is evaluated
@@ -577,9 +576,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev);
         atomicAdd(&counters[1], nvis);
-#ifdef NV_WALK_PROBE
-        atomicAdd(&counters[2], nit);
-#endif
     }
 }
 
@@ -594,7 +590,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
                                                  DCtl *ctl, unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
-    unsigned long long nev = 0, nvis = 0, nit = 0;
+    unsigned long long nev = 0, nvis = 0;
     bool sing = false;
     for (;;) {
     int m, tb;
@@ -619,7 +615,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     WNode nd = nwalk > 0 ? walk[0] : WNode{};
     while (cur < nwalk) {
         cur = wave_uniform(cur);
-        ++nit;
         // both possible successors are fetched while this node is evaluated
         const int n1 = cur + 1, n2 = nd.skip;
         const WNode a = walk[n1 < nwalk ? n1 : cur];
@@ -689,9 +684,6 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev);
         atomicAdd(&counters[1], nvis);
-#ifdef NV_WALK_PROBE
-        atomicAdd(&counters[2], nit);
-#endif
     }
 }
 
@@ -762,8 +754,7 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
                                                    WalkQueues wq, const CNode *__restrict__ walk,
-                                                   const WNode *__restrict__ full, int nwalk,
-                                                   unsigned long long *counters) {
+                                                   const WNode *__restrict__ full, int nwalk) {
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     for (;;) {
@@ -797,10 +788,6 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         const CNode a = walk[n1];
         const CNode b = walk[n2];
         const bool vis = cur >= lskip;
-#ifdef NV_WALK_PROBE
-        if ((threadIdx.x & 63) == 0) atomicAdd(&counters[2], 1ull);
-        atomicAdd(&counters[3], vis ? 1ull : 0ull);
-#endif
         const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
         const double dd = e0 * e0 + e1 * e1 + e2 * e2;
         bool acc = vis;
@@ -1132,13 +1119,9 @@ int nv_tree_walk_record_bytes(int precision, double theta) {
 
 // interaction and node-visit counters of the last velocity walk (synchronises)
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats) {
-    unsigned long long h[3];
-    TCHK(hipMemcpyAsync(h, t->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    unsigned long long h[2];
+    TCHK(hipMemcpyAsync(h, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
-#ifdef NV_WALK_PROBE
-    fprintf(stderr, "walk: lane visits %llu, wave iterations %llu (x64 = %.2f lane visits)\n", h[1], h[2],
-            (double)h[2] * 64.0 / (double)(h[1] ? h[1] : 1));
-#endif
     stats->tree_evals = (int64_t)h[0];
     stats->tree_nodes_visited = (int64_t)h[1];
     return NV_OK;
@@ -1171,20 +1154,7 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
     const int nitems = (n + 63) / 64;
     TCHK(nv_fill_int(st, t->wq, 0, XCDS));
     hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
-                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, t->counters);
-#ifdef NV_WALK_PROBE
-    {
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
-        TCHK(nv_fill_int(st, t->wq, 0, XCDS));
-        hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n,
-                           t->vals, n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, t->counters);
-        unsigned long long h[4];
-        TCHK(hipMemcpyAsync(h, t->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        TCHK(hipStreamSynchronize(st));
-        fprintf(stderr, "close: items %d, wave iterations %llu, lane visits %llu (wave its x64 / visits %.2f)\n",
-                nitems, h[2], h[3], (double)h[2] * 64.0 / (double)(h[3] ? h[3] : 1));
-    }
-#endif
+                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk);
     TCHK(hipGetLastError());
     return NV_OK;
 }
