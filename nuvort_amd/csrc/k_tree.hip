@@ -502,6 +502,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
         const NodeHot a = *(const NodeHot *)(wb + (size_t)n1 * sizeof(Node48));
         const NodeHot b = *(const NodeHot *)(wb + (size_t)n2 * sizeof(Node48));
+        // (this file is compiled without machine sinking, which would move
+        // these loads down to their use at the end of the iteration)
         const bool vis = cur >= lskip;
         nvis += (unsigned long long)__popcll(__ballot(vis));
         const double v0 = nd.cent[0] - xs0, v1 = nd.cent[1] - xs1, v2 = nd.cent[2] - xs2;

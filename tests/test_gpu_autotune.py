@@ -30,7 +30,7 @@ def test_autotune_keeps_state_and_accuracy(tmp_path):
     cache = str(tmp_path / "tune.txt")
     t = s.autotune(cache)
     assert t["tuned"] == 1 and t["tune_ms"] > 0
-    assert t["targets_per_lane"] in (1, 2) and t["kahan_block"] in (4, 8, 16) and t["items_per_cu"] in (8, 16, 32)
+    assert t["targets_per_lane"] in (1, 2) and t["kahan_block"] in (4, 8, 16, 32) and t["items_per_cu"] in (8, 16, 32)
     after = s.get_state()
     for f in FIELDS:
         assert np.array_equal(getattr(before, f), getattr(after, f)), f
