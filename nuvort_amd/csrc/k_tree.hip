@@ -75,15 +75,11 @@ __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
     return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
 }
 
-__global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals, int *nact) {
+__global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = r < n && s.infront[r] != 0;
-    // active points counted per wave, one atomic each
-    const unsigned long long bal = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(nact, (int)__popcll(bal));
     if (r >= n) return;
     vals[r] = r;
-    if (!act) {
+    if (s.infront[r] == 0) {
         keys[r] = ~0ull;
         return;
     }
@@ -113,6 +109,15 @@ __global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, 
         w = cw;
     }
     keys[r] = key;
+}
+
+// The number of active points, from the sorted keys: an active key's top
+// nibble is at most 8, so active keys sort before the inactive ~0 ones. One
+// thread finds the boundary and writes it; *nact was cleared beforehand.
+__global__ void k_count_active(const unsigned long long *__restrict__ keys, int n, int *nact) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    if (keys[r] != ~0ull && (r + 1 == n || keys[r + 1] == ~0ull)) *nact = r + 1;
 }
 
 // ------------------------------------------------------------ nodes
@@ -969,10 +974,12 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     if (int rc = ensure_rows(t, n)) return rc;
     const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
     TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0, t->nact);
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
     TCHK(hipGetLastError());
     // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
     TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
+    TCHK(hipGetLastError());
     t->vals = t->v0;
     t->built_rows = n;
     TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));

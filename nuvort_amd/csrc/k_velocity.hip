@@ -449,12 +449,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
 // Reduce chunk partials in chunk order and finish every point.
 // mode 0: partials hold (acc0, accs) of k_bs64 or of the tree walk;
 // mode 1: (far, near) of k_bs32c; mode 2: no partials (LIA).
-__global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode,
-                         int do_ab, DCtl *ctl) {
+__global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows,
+                                                const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < rows && s.infront[r] != 0;
+    // active points counted per workgroup, one atomic each: atomics on one
+    // address are serialised in L2, one per wave cost ~12 ns each
+    __shared__ int wcount[4];
     const unsigned long long msk = __ballot(act);
-    if ((threadIdx.x & 63) == 0 && msk) atomicAdd(&ctl->nvel, (int)__popcll(msk));
+    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = (int)__popcll(msk);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int c = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        if (c) atomicAdd(&ctl->nvel, c);
+    }
     if (r >= rows) return;
     if (!act) {
         if (do_ab) {
@@ -469,6 +477,9 @@ __global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *_
         const int capk = tg.cap[kk], loc = r - tg.base[kk], S = tg.S[kk];
         const double *pb = part + tg.poff[kk];
         if (mode == 0) {
+            // unrolled so that the loads of several chunks are in flight; the
+            // adds stay in chunk order
+#pragma unroll 3
             for (int c = 0; c < S; ++c) {
                 const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll
@@ -479,6 +490,7 @@ __global__ void k_finish(DState s, Phys p, Tangles tg, int rows, const double *_
             }
         } else {
             double far[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
+#pragma unroll 3
             for (int c = 0; c < S; ++c) {
                 const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll
