@@ -145,6 +145,9 @@ typedef struct {
     int32_t items_per_cu;      /* direct-sum work items per CU (sets the chunk size) */
     int32_t tuned;             /* 0 defaults, 1 timed by nv_autotune, 2 read from its cache */
     double tune_ms;            /* direct-sum launch + chunk reduction time of the chosen variant */
+    int32_t tree_builds;       /* octree builds */
+    int32_t tree_sync_builds;  /* of which ran the level loop with a host round trip per level (the
+                                  first build, and a tree that outgrew the previous build's levels) */
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */

@@ -124,10 +124,30 @@ __global__ void k_count_active(const unsigned long long *__restrict__ keys, int 
 struct Nodes {
     int *beg, *cnt, *pcount, *dropb, *first, *parent, *minrow, *size, *dfs, *leafinf;
     double *pos, *w, *sum, *circ;   // pos/sum/circ: [3][ncap]
+    double *rad;                     // bound on |centroid of a descendant or point - centroid|
     int ncap;
 };
 
-__global__ void k_root(Nodes nd, DState s, int n, double bsize, const int *nact) {
+// Level bookkeeping of a build whose level loop runs without host round
+// trips: each level's kernels read their node range here, and the launch
+// bounds come from the previous build's level sizes.
+struct LevelCtl {
+    int lvl[LEVELS + 3];    // level starts: level l holds nodes [lvl[l], lvl[l + 1])
+    int wide[LEVELS + 1];   // level's children made by k_children_wide
+    int overflow;           // a level outgrew its launch bound or the node arrays
+    int depth;              // a node of the deepest level holds more than one point
+    int nact;
+    int nwalk;              // non-empty nodes, the depth-first walk's length
+};
+
+__global__ void k_root(Nodes nd, DState s, int n, double bsize, const int *nact, LevelCtl *lc) {
+    if (lc) {
+        lc->lvl[0] = 0;
+        lc->lvl[1] = 1;
+        lc->overflow = 0;
+        lc->depth = 0;
+        lc->nact = *nact;
+    }
     nd.beg[0] = 0;
     nd.cnt[0] = *nact;
     nd.pcount[0] = n;   // the root holds every slot (src/tree.f90:48)
@@ -144,6 +164,44 @@ __global__ void k_split_flags(Nodes nd, int L0, int L1, int *flags, int at_max, 
     const int sp = nd.pcount[q] > 1 ? 1 : 0;
     flags[q - L0] = sp;
     if (sp && at_max) atomicOr(&ctl->status, NVK_TREE_DEPTH);
+}
+
+// k_split_flags for a level whose range is on the device: flags[0, bound),
+// zero past the level's end so that a scan over the bound is the level's
+__global__ void k_split_flags_d(Nodes nd, const LevelCtl *lc, int level, int bound, int *flags, DCtl *ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bound) return;
+    const int L0 = lc->lvl[level], m = lc->lvl[level + 1] - L0;
+    int sp = 0;
+    if (i < m) {
+        sp = nd.pcount[L0 + i] > 1 ? 1 : 0;
+        if (sp && level == LEVELS) atomicOr(&ctl->status, NVK_TREE_DEPTH);
+    }
+    flags[i] = sp;
+}
+
+// after a level's scan: the next level's start, the children kernel, and
+// whether the level fit its bounds (one thread)
+__global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wgrid, const int *offs) {
+    const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1], m = L1 - L0;
+    int next = L1, wide = 0;
+    if (m > bound) {
+        lc->overflow = 1;
+    } else if (m > 0 && !lc->overflow) {
+        const int nsplit = offs[m - 1] + (nd.pcount[L1 - 1] > 1 ? 1 : 0);
+        if (nsplit && level == LEVELS) {
+            lc->depth = 1;
+        } else if (nsplit) {
+            if ((long long)L1 + 8ll * nsplit > (long long)nd.ncap) {
+                lc->overflow = 1;
+            } else {
+                next = L1 + 8 * nsplit;
+                wide = (m <= wgrid && m <= 4096 && nsplit * 64 <= lc->nact) ? 1 : 0;
+            }
+        }
+    }
+    lc->wide[level] = wide;
+    lc->lvl[level + 2] = next;
 }
 
 __device__ __forceinline__ int lower_digit(const unsigned long long *keys, int lo, int hi, int level, int d) {
@@ -175,8 +233,8 @@ __device__ __forceinline__ void write_child(const Nodes &nd, int q, int base, in
 
 // children of the split nodes (pcount > 1) of level `level` (nodes [L0,
 // L1)) into L1 + 8 * off, one thread per node
-__global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsigned long long *keys, int level) {
-    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int L1, const int *off,
+                                            const unsigned long long *keys, int level) {
     if (q >= L1 || nd.pcount[q] <= 1) return;
     const int base = L1 + 8 * off[q - L0];
     nd.first[q] = base;
@@ -188,13 +246,23 @@ __global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsig
     for (int c = 0; c < 8; ++c) write_child(nd, q, base, c, bnd[c], bnd[c + 1]);
 }
 
+__global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsigned long long *keys, int level) {
+    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, keys, level);
+}
+
+__global__ void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const unsigned long long *keys,
+                             int level) {
+    if (lc->overflow || lc->wide[level]) return;
+    const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
+    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, keys, level);
+}
+
 // The same for a level of few nodes with long key ranges, one workgroup of
 // 256 per node: 32 lanes find each of the 8 child boundaries by a 32-way
 // search (a node's keys are ascending in the next level's digit).
-__global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off,
-                                                       const unsigned long long *keys, int level) {
+__device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0, int L1, const int *off,
+                                                 const unsigned long long *keys, int level) {
     __shared__ int bnd[9];
-    const int q = L0 + blockIdx.x;
     if (q >= L1 || nd.pcount[q] <= 1) return;
     const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
     const int d = 1 + (threadIdx.x >> 5), li = threadIdx.x & 31, sh = threadIdx.x & 32;
@@ -221,10 +289,25 @@ __global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1,
     }
 }
 
+__global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off,
+                                                       const unsigned long long *keys, int level) {
+    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, keys, level);
+}
+
+__global__ __launch_bounds__(256) void k_children_wide_d(Nodes nd, const LevelCtl *lc, const int *off,
+                                                         const unsigned long long *keys, int level) {
+    if (lc->overflow || !lc->wide[level]) return;
+    const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
+    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, keys, level);
+}
+
 // moments, minimum row and non-empty subtree size, children before parents
-__global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals) {
-    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L1) return;
+__device__ __forceinline__ double dist3(double a0, double a1, double a2, double b0, double b1, double b2) {
+    const double d0 = a0 - b0, d1 = a1 - b1, d2 = a2 - b2;
+    return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+__device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, const int *vals, int nact) {
     const int cap = s.cap, nc = nd.ncap;
     double sm[3] = {0.0, 0.0, 0.0}, ci[3] = {0.0, 0.0, 0.0};
     int mr = 0x7fffffff, size;
@@ -276,6 +359,29 @@ __global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals) {
             mr = min(mr, p);
         }
     }
+    // The centroid k_down gives this node, and a bound on the distance from
+    // it to every centroid and point below it (the triangle inequality over
+    // the children, plus the points kept at this node), inflated by 2^-40
+    // to cover the rounding of the norms: the close search's exact pruning.
+    const double cntd = q == 0 ? (double)nact : (double)nd.pcount[q];
+    const double c0 = sm[0] / cntd, c1 = sm[1] / cntd, c2 = sm[2] / cntd;
+    double rad = 0.0;
+    int p0 = nd.beg[q];
+    if (nd.first[q] >= 0) {
+        for (int c = 0; c < 8; ++c) {
+            const int ch = nd.first[q] + c;
+            const double pcc = (double)nd.pcount[ch];
+            if (nd.pcount[ch] > 0)
+                rad = fmax(rad, dist3(nd.sum[ch] / pcc, nd.sum[nc + ch] / pcc, nd.sum[2 * nc + ch] / pcc, c0, c1, c2) +
+                                    nd.rad[ch]);
+        }
+        p0 = nd.dropb[q];
+    }
+    for (int j = p0; j < nd.beg[q] + nd.cnt[q]; ++j) {
+        const int pt = vals[j];
+        rad = fmax(rad, dist3(s.x[pt], s.x[cap + pt], s.x[2 * cap + pt], c0, c1, c2));
+    }
+    nd.rad[q] = rad * (1.0 + 0x1p-40);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         nd.sum[k * nc + q] = sm[k];
@@ -285,9 +391,22 @@ __global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals) {
     nd.size[q] = size;
 }
 
+__global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals, int nact) {
+    const int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < L1) up_of(nd, s, q, vals, nact);
+}
+
+__global__ void k_up_d(Nodes nd, DState s, const LevelCtl *lc, int level, const int *vals) {
+    if (lc->overflow) return;
+    const int q = lc->lvl[level] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < lc->lvl[level + 1]) up_of(nd, s, q, vals, lc->nact);
+}
+
+__global__ void k_build_tail(const Nodes nd, LevelCtl *lc) { lc->nwalk = lc->overflow ? 0 : nd.size[0]; }
+
 // depth-first positions of the children of level-`L` nodes and the walk
 // records of the level-`L` nodes themselves
-__global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk) {
+__global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk, float *wrad) {
     int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= L1) return;
     const int nc = nd.ncap;
@@ -328,6 +447,11 @@ __global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk
     w.skip = my + nd.size[q];
     w.leaf_j = 0;
     walk[my] = w;
+    // the node radius as a float rounded up
+    const double rd = nd.rad[q];
+    float rf = (float)rd;
+    if ((double)rf < rd) rf = nextafterf(rf, INFINITY);
+    wrad[my] = rf;
 }
 
 // j = behind(leaf_infront) of every leaf record, read once before a walk
@@ -722,7 +846,8 @@ __global__ void k_target_keys(DState s, int n, double bsize, unsigned long long 
 struct CNode {
     double cent[3];
     double lo, hi;
-    int word, pad;
+    int word;
+    float rad;   // every centroid and point below lies within rad of cent
 };
 
 // With d = RN(sqrt(dd)) and q = RN(w / d): q < 0.5 <=> w / d < 0.5 - 2^-55
@@ -731,7 +856,8 @@ struct CNode {
 // factors are far above the rounding of dd, of 4 w^2 and of the bounds.
 constexpr double CLOSE_BAND = 0x1p-46;
 
-__global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk, CNode *__restrict__ out) {
+__global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const float *__restrict__ wrad, int nwalk,
+                             CNode *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > nwalk) return;
     if (q == nwalk) {   // the spare record
@@ -746,7 +872,7 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk
     o.lo = w4 * (1.0 - CLOSE_BAND);
     o.hi = w4 * (1.0 + CLOSE_BAND);
     o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
-    o.pad = 0;
+    o.rad = wrad[q];
     out[q] = o;
 }
 
@@ -761,7 +887,7 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, int nwalk
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
                                                    WalkQueues wq, const CNode *__restrict__ walk,
-                                                   const WNode *__restrict__ full, int nwalk) {
+                                                   const WNode *__restrict__ full, int nwalk, double pm) {
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     for (;;) {
@@ -781,6 +907,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
     }
     double best = 100.0;
     double blo = best * best * (1.0 - CLOSE_BAND), bhi = best * best * (1.0 + CLOSE_BAND);
+    double bpm = best + pm;
     int closest = act ? s.closest[r] : 0;
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
@@ -798,6 +925,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
         const double dd = e0 * e0 + e1 * e1 + e2 * e2;
         bool acc = vis;
+        bool prune = false;
         if (!leaf) {
             bool yes = dd > nd.hi;
             const bool band = vis && !yes && !(dd < nd.lo);
@@ -806,9 +934,16 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
                 if (band) yes = width / sqrt(dd) < .5;
             }
             acc = vis && yes;
+            // Exact pruning: every centroid and point below lies within rad
+            // of cent, so a lane farther than best + rad (and a margin far
+            // above the rounding of the distances) finds nothing there with
+            // dist < best, and dist < best is the only test that changes
+            // its closest or best: the subtree is skipped with no effect.
+            const double thr = bpm + (double)nd.rad;
+            prune = vis && !acc && dd > thr * thr * (1.0 + 0x1p-30);
         }
-        const bool open = vis && !acc;
-        lskip = acc ? n2 : lskip;
+        const bool open = vis && !acc && !prune;
+        lskip = (acc || prune) ? n2 : lskip;
         // dist < best: sure below blo, excluded above bhi
         const bool sure = acc && dd < blo;
         const bool maybe = acc && !sure && dd < bhi;
@@ -822,6 +957,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
                     best = dist;
                     blo = best * best * (1.0 - CLOSE_BAND);
                     bhi = best * best * (1.0 + CLOSE_BAND);
+                    bpm = best + pm;
                 }
             }
         } else {
@@ -856,6 +992,7 @@ struct TreeBuf {
     int *flags = nullptr, *offs = nullptr, *scan_tmp = nullptr;
     int flag_cap = 0;
     WNode *walk = nullptr;
+    float *wrad = nullptr;               // node radii in walk order (close-search pruning)
     int walk_cap = 0;
     Node48 *walk48 = nullptr;            // FP32 walk records, packed per walk
     double *circ64 = nullptr;            // their FP64 circulations
@@ -870,6 +1007,8 @@ struct TreeBuf {
     int *wq = nullptr;                   // the walks' per-XCD work queues
     int *h_int = nullptr;                // pinned scratch
     const int *vals = nullptr;           // sorted rows of the last build
+    LevelCtl *lc = nullptr, *lc_h = nullptr;   // level loop control, device and pinned copy
+    int sync_builds = 0;                 // builds that ran the synchronous level loop
 };
 
 #define TCHK(x)                                     \
@@ -882,7 +1021,9 @@ TreeBuf *nv_tree_new() {
     TreeBuf *t = new TreeBuf();
     if (hipMalloc(&t->nact, sizeof(int)) != hipSuccess || hipMalloc(&t->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&t->wq, XCDS * sizeof(int)) != hipSuccess ||
-        hipHostMalloc((void **)&t->h_int, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&t->h_int, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&t->lc, sizeof(LevelCtl)) != hipSuccess ||
+        hipHostMalloc((void **)&t->lc_h, sizeof(LevelCtl), hipHostMallocDefault) != hipSuccess) {
         delete t;
         return nullptr;
     }
@@ -894,7 +1035,7 @@ static void free_nodes(Nodes &nd) {
                   &nd.leafinf};
     for (int **p : ip)
         if (*p) { hipFree(*p); *p = nullptr; }
-    double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ};
+    double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ, &nd.rad};
     for (double **p : dp)
         if (*p) { hipFree(*p); *p = nullptr; }
     nd.ncap = 0;
@@ -904,12 +1045,15 @@ void nv_tree_free(TreeBuf *t) {
     if (!t) return;
     hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
     hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
+    hipFree(t->wrad);
     hipFree(t->walk48);
     hipFree(t->circ64);
     hipFree(t->walkc);
     hipFree(t->counters);
     hipFree(t->wq);
     hipHostFree(t->h_int);
+    hipFree(t->lc);
+    hipHostFree(t->lc_h);
     free_nodes(t->nd);
     delete t;
 }
@@ -928,6 +1072,7 @@ static int grow_nodes(TreeBuf *t, hipStream_t st, int want, int keep) {
     }
     TCHK(hipMalloc(&n.w, sizeof(double) * (size_t)ncap));
     if (o.w && keep) TCHK(hipMemcpyAsync(n.w, o.w, sizeof(double) * keep, hipMemcpyDeviceToDevice, st));
+    TCHK(hipMalloc(&n.rad, sizeof(double) * (size_t)ncap));
     double **dp[] = {&n.pos, &n.sum, &n.circ};
     double *od[] = {o.pos, o.sum, o.circ};
     for (int q = 0; q < 3; ++q) {
@@ -969,25 +1114,15 @@ static int ensure_flags(TreeBuf *t, int m) {
     return NV_OK;
 }
 
-int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl, nv_stats *stats) {
-    if (n <= 0) return NV_ERR_ZERO_LENGTH;
-    if (int rc = ensure_rows(t, n)) return rc;
-    const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
-    TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
-    TCHK(hipGetLastError());
-    // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
-    TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
-    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
-    TCHK(hipGetLastError());
-    t->vals = t->v0;
-    t->built_rows = n;
+// Level loop with a host round trip per level: the level sizes decide the
+// launches. Leaves t->lvl and t->nwalk; nodes are moments-complete.
+static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
     TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     const int nact = t->h_int[0];
     t->nact_h = nact;
     if (int rc = grow_nodes(t, st, std::max(1024, 4 * nact + 64), 0)) return rc;
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact);
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, (LevelCtl *)nullptr);
     TCHK(hipGetLastError());
     t->lvl.assign(1, 0);
     t->lvl.push_back(1);
@@ -1020,30 +1155,128 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
         ++level;
     }
     const int nlev = (int)t->lvl.size() - 1;
-    t->nnodes = t->lvl.back();
     for (int lv = nlev - 1; lv >= 0; --lv) {
         const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
         if (L1 > L0)
-            hipLaunchKernelGGL(k_up, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->vals);
+            hipLaunchKernelGGL(k_up, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->vals, nact);
         TCHK(hipGetLastError());
     }
     TCHK(hipMemcpyAsync(t->h_int, t->nd.size, sizeof(int), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     t->nwalk = t->h_int[0];
+    return NV_OK;
+}
+
+constexpr int BUILD_RETRY = -1000;
+
+// The same level loop with no host round trip until its end: every level is
+// launched on a bound taken from the previous build's level sizes (one more
+// level than it had), the kernels take their node range from LevelCtl, and a
+// level that outgrows its bound, the node arrays or the levels launched makes
+// the build return BUILD_RETRY, after which the caller runs the synchronous
+// loop from the root again. Both loops run the same kernels on the same
+// ranges, so the trees are identical.
+static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
+    const int prev = (int)t->lvl.size() - 1;   // levels of the previous build
+    if (prev < 1 || !t->lc) return BUILD_RETRY;
+    const int top = std::min(prev, LEVELS);    // deepest level launched
+    int bound[LEVELS + 1], wgrid[LEVELS + 1];
+    long long total = 1;
+    int fmax = 1;
+    for (int l = 0; l <= top; ++l) {
+        const int hint = l < prev ? t->lvl[l + 1] - t->lvl[l] : 128;
+        bound[l] = (hint + hint / 8 + 256 + 255) / 256 * 256;
+        wgrid[l] = hint <= 4096 ? std::min(4096, bound[l]) : 0;
+        total += bound[l];
+        fmax = std::max(fmax, bound[l]);
+    }
+    // children of the deepest launched level land past its bound
+    total += 8ll * bound[top];
+    if (total > (1ll << 30)) return BUILD_RETRY;
+    if (int rc = grow_nodes(t, st, (int)total, 0)) return rc;
+    if (int rc = ensure_flags(t, fmax)) return rc;
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, t->lc);
+    TCHK(hipGetLastError());
+    for (int l = 0; l <= top; ++l) {
+        hipLaunchKernelGGL(k_split_flags_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc, l,
+                           bound[l], t->offs, ctl);
+        TCHK(hipGetLastError());
+        TCHK(nv_exclusive_scan(st, t->offs, bound[l], t->scan_tmp));
+        hipLaunchKernelGGL(k_level_end, dim3(1), dim3(1), 0, st, t->nd, t->lc, l, bound[l], wgrid[l],
+                           (const int *)t->offs);
+        TCHK(hipGetLastError());
+        if (wgrid[l] > 0)
+            hipLaunchKernelGGL(k_children_wide_d, dim3(wgrid[l]), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
+                               (const int *)t->offs, (const unsigned long long *)t->k0, l);
+        hipLaunchKernelGGL(k_children_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
+                           (const int *)t->offs, (const unsigned long long *)t->k0, l);
+        TCHK(hipGetLastError());
+    }
+    for (int l = top; l >= 0; --l) {
+        hipLaunchKernelGGL(k_up_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, d, (const LevelCtl *)t->lc, l,
+                           t->vals);
+        TCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_build_tail, dim3(1), dim3(1), 0, st, t->nd, t->lc);
+    TCHK(hipGetLastError());
+    TCHK(hipMemcpyAsync(t->lc_h, t->lc, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    const LevelCtl &h = *t->lc_h;
+    if (h.depth) return NV_ERR_TREE_DEPTH;
+    // a tree deeper than the levels launched has children past the last one
+    if (h.overflow || h.lvl[top + 2] > h.lvl[top + 1]) return BUILD_RETRY;
+    t->nact_h = h.nact;
+    int nlev = 0;
+    while (nlev <= top && h.lvl[nlev + 1] > h.lvl[nlev]) ++nlev;
+    t->lvl.assign(h.lvl, h.lvl + nlev + 1);
+    t->nwalk = h.nwalk;
+    return NV_OK;
+}
+
+int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl, nv_stats *stats) {
+    if (n <= 0) return NV_ERR_ZERO_LENGTH;
+    if (int rc = ensure_rows(t, n)) return rc;
+    const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+    TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
+    TCHK(hipGetLastError());
+    // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
+    TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
+    TCHK(hipGetLastError());
+    t->vals = t->v0;
+    t->built_rows = n;
+    int rc = build_levels_async(t, st, d, n, bsize, ctl);
+    if (rc == BUILD_RETRY) {
+        ++t->sync_builds;
+        rc = build_levels_sync(t, st, d, n, bsize, ctl);
+    }
+    if (rc) {
+        t->lvl.clear();   // no hints from a failed build
+        return rc;
+    }
+    const int nlev = (int)t->lvl.size() - 1;
+    t->nnodes = t->lvl.back();
     if (t->nwalk > t->walk_cap) {
         hipFree(t->walk);
+        hipFree(t->wrad);
+        t->walk = nullptr;
+        t->wrad = nullptr;
         t->walk_cap = std::max(t->nwalk, 2 * t->walk_cap);
         TCHK(hipMalloc(&t->walk, sizeof(WNode) * (size_t)t->walk_cap));
+        TCHK(hipMalloc(&t->wrad, sizeof(float) * (size_t)t->walk_cap));
     }
     for (int lv = 0; lv < nlev; ++lv) {
         const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
         if (L1 > L0)
-            hipLaunchKernelGGL(k_down, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, nact,
-                               t->walk);
+            hipLaunchKernelGGL(k_down, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->nact_h,
+                               t->walk, t->wrad);
         TCHK(hipGetLastError());
     }
     stats->tree_nodes = t->nnodes;
     stats->tree_levels = nlev;
+    stats->tree_builds += 1;
+    stats->tree_sync_builds = t->sync_builds;
     return NV_OK;
 }
 
@@ -1157,13 +1390,13 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
         t->walkc_cap = std::max(t->nwalk + 1, 2 * t->walkc_cap);
         TCHK(hipMalloc(&t->walkc, sizeof(CNode) * (size_t)t->walkc_cap));
     }
-    hipLaunchKernelGGL(k_pack_close, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
-                       t->walkc);
+    hipLaunchKernelGGL(k_pack_close, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk,
+                       (const float *)t->wrad, t->nwalk, t->walkc);
     if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
     const int nitems = (n + 63) / 64;
     TCHK(nv_fill_int(st, t->wq, 0, XCDS));
     hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
-                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk);
+                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, bsize * 0x1p-40);
     TCHK(hipGetLastError());
     return NV_OK;
 }
