@@ -135,6 +135,7 @@ struct LevelCtl {
     int lvl[LEVELS + 3];    // level starts: level l holds nodes [lvl[l], lvl[l + 1])
     int wide[LEVELS + 1];   // level's children made by k_children_wide
     int overflow;           // a level outgrew its launch bound or the node arrays
+    int ovf_level;          // the first such level: the levels above it are complete
     int depth;              // a node of the deepest level holds more than one point
     int nact;
     int nwalk;              // non-empty nodes, the depth-first walk's length
@@ -145,6 +146,7 @@ __global__ void k_root(Nodes nd, DState s, int n, double bsize, const int *nact,
         lc->lvl[0] = 0;
         lc->lvl[1] = 1;
         lc->overflow = 0;
+        lc->ovf_level = LEVELS + 1;
         lc->depth = 0;
         lc->nact = *nact;
     }
@@ -185,8 +187,9 @@ __global__ void k_split_flags_d(Nodes nd, const LevelCtl *lc, int level, int bou
 __global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wgrid, const int *offs) {
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1], m = L1 - L0;
     int next = L1, wide = 0;
-    if (m > bound) {
+    if (m > bound && !lc->overflow) {
         lc->overflow = 1;
+        lc->ovf_level = level;
     } else if (m > 0 && !lc->overflow) {
         const int nsplit = offs[m - 1] + (nd.pcount[L1 - 1] > 1 ? 1 : 0);
         if (nsplit && level == LEVELS) {
@@ -194,6 +197,7 @@ __global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wg
         } else if (nsplit) {
             if ((long long)L1 + 8ll * nsplit > (long long)nd.ncap) {
                 lc->overflow = 1;
+                lc->ovf_level = level;
             } else {
                 next = L1 + 8 * nsplit;
                 wide = (m <= wgrid && m <= 4096 && nsplit * 64 <= lc->nact) ? 1 : 0;
@@ -272,7 +276,7 @@ __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0,
         const int step = (hi - lo + 31) / 32;
         const int pos = lo + li * step;
         const bool below = pos < hi && digit_at(keys[pos], level + 1) < d;
-        const int c = __popcll((__ballot(below) >> sh) & 0xffffffffull);
+        const int c = __popcll((wave_ballot(below) >> sh) & 0xffffffffull);
         if (c == 0) { hi = lo; break; }
         const int nlo = lo + (c - 1) * step + 1, nhi = min(hi, lo + c * step);
         lo = nlo;
@@ -594,7 +598,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     const double kappa = p.kappa;
-    unsigned long long nev = 0, nvis = 0;
+    unsigned long long nev = 0, nvis = 0;   // nev: FP64 terms, per lane
+    unsigned long long nevw = 0;            // FP32 terms, per wave
     bool sing = false;
     for (;;) {
     int m, tb;
@@ -615,6 +620,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     double fa[3] = {0.0, 0.0, 0.0};   // FP32 terms, accumulated in FP64
     double nr[3] = {0.0, 0.0, 0.0};   // FP64 terms
     const bool img0 = (m == 0);
+    const unsigned long long img0m = img0 ? ~0ull : 0ull;
     int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
     const char *wb = (const char *)walk;
@@ -633,26 +639,40 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         const NodeHot b = *(const NodeHot *)(wb + (size_t)n2 * sizeof(Node48));
         // (this file is compiled without machine sinking, which would move
         // these loads down to their use at the end of the iteration)
-        const bool vis = cur >= lskip;
-        nvis += (unsigned long long)__popcll(__ballot(vis));
+        // Every lane condition is a 64-bit mask built from ballots of raw
+        // compares (each beside its compare), combined on the scalar unit;
+        // a per-lane select reads its mask back with inverse_ballot, which
+        // costs nothing. Bools combined per lane would be kept twice, as lane
+        // masks and as ballots, and a ballot of such a bool is rebuilt in
+        // vector registers.
+        const unsigned long long mvis = wave_ballot(cur >= lskip);
+        nvis += (unsigned long long)__popcll(mvis);
         const double v0 = nd.cent[0] - xs0, v1 = nd.cent[1] - xs1, v2 = nd.cent[2] - xs2;
         const double dd = __fma_rn(v0, v0, __fma_rn(v1, v1, v2 * v2));
-        const bool acc = vis && (leaf || dd > (double)nd.r2);
-        const bool open = vis && !acc;
-        lskip = acc ? n2 : lskip;   // a leaf's successor, or the node's skip
-        bool use = acc, near = false;
+        const unsigned long long mgt = wave_ballot(dd > (double)nd.r2);
+        unsigned long long macc, muse, mnear;
+        bool down;
         if (leaf) {
             const int j = -w - 1;
-            const bool excl = img0 && (j == me || j == beh);
+            const unsigned long long mexcl = (wave_ballot(j == me) | wave_ballot(j == beh)) & img0m;
             // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
-            const bool sg = acc && !excl && dd < sing2;
-            sing = sing || sg;
-            use = acc && !excl && !sg;
-            near = use && j > 0 && dd < rn2;
+            const unsigned long long msg = mvis & ~mexcl & wave_ballot(dd < sing2);
+            const unsigned long long mrn = wave_ballot(dd < rn2);
+            sing = sing || msg != 0ull;
+            macc = mvis;
+            muse = mvis & ~mexcl & ~msg;
+            mnear = muse & mrn & (j > 0 ? ~0ull : 0ull);
+            down = false;
+        } else {
+            macc = mvis & mgt;
+            muse = macc;
+            mnear = 0ull;
+            down = (mvis & ~mgt) != 0ull;   // a lane opens the node
         }
-        const bool far = use && !near;
-        bool need64 = use && near;
-        if (__any(far)) {
+        lskip = lane_in(macc) ? n2 : lskip;   // a leaf's successor, or the node's skip
+        const unsigned long long mfar = muse & ~mnear;
+        unsigned long long mneed = mnear;
+        if (mfar != 0ull) {
             const NodeCold cd = *(const NodeCold *)(wb + (size_t)cur * sizeof(Node48) + sizeof(NodeHot));
             const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
             const float e0 = cd.e[0], e1 = cd.e[1], e2 = cd.e[2];
@@ -669,24 +689,24 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
             // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
             // the pseudo-segment ending close to the target (r2 < r1 / 4, where
             // the FP32 r2^2 can even round to <= 0); NaN goes there too
-            const bool ill = !(den >= 0.0625f * r12) || !(r2sq >= 0.0625f * r1sq);
+            const unsigned long long mill =
+                wave_ballot(!(den >= 0.0625f * r12)) | wave_ballot(!(r2sq >= 0.0625f * r1sq));
             // the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon
             // (x 4 is exact: |d x e|^2 < epsilon / 4)
             const float xx = __fmaf_rn(x0, x0, __fmaf_rn(x1, x1, x2 * x2));
-            const bool ok = far && !ill && !(xx < 0.25f * (float)NV_EPS);
+            const unsigned long long mok = mfar & ~mill & ~wave_ballot(xx < 0.25f * (float)NV_EPS);
             // x is finite for every lane, so x * 0 = 0 where the term is not taken
-            const float fac = ok ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
+            const float fac = lane_in(mok) ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
             fa[0] += (double)(x0 * fac);
             fa[1] += (double)(x1 * fac);
             fa[2] += (double)(x2 * fac);
-            nev += ok ? 1u : 0u;
-            need64 = need64 || (far && ill);
+            nevw += (unsigned long long)__popcll(mok);
+            mneed |= mfar & mill;
         }
-        if (__any(need64)) {
+        if (mneed != 0ull) {
             const double c64[3] = {circ64[3 * cur], circ64[3 * cur + 1], circ64[3 * cur + 2]};
-            if (need64) near_term64(v0, v1, v2, c64, dd, kappa, nr, nev);
+            if (lane_in(mneed)) near_term64(v0, v1, v2, c64, dd, kappa, nr, nev);
         }
-        const bool down = __any(open);
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
@@ -701,11 +721,11 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         }
     }
     }
-    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
+    if (sing && lane == 0) atomicOr(&ctl->status, NVK_SINGULAR);
     // nvis counts the visits of the whole wave already
     for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&counters[0], nev);
+        atomicAdd(&counters[0], nev + nevw);
         atomicAdd(&counters[1], nvis);
     }
 }
@@ -793,7 +813,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
                 open = true;
             }
         }
-        const bool down = __any(open);
+        const bool down = wave_any(open);
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
@@ -929,7 +949,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         if (!leaf) {
             bool yes = dd > nd.hi;
             const bool band = vis && !yes && !(dd < nd.lo);
-            if (__any(band)) {
+            if (wave_any(band)) {
                 const double width = full[cur].width;
                 if (band) yes = width / sqrt(dd) < .5;
             }
@@ -950,7 +970,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         if (leaf) {
             const int j = -w - 1;
             const bool cand = (sure || maybe) && (me != j) && (inf != j) && (beh != j);
-            if (__any(cand)) {
+            if (wave_any(cand)) {
                 const double dist = sqrt(dd);
                 if (cand && dist < best) {
                     closest = j;
@@ -962,12 +982,12 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
             }
         } else {
             bool lt = sure;
-            if (__any(maybe)) {
+            if (wave_any(maybe)) {
                 if (maybe) lt = sqrt(dd) < best;
             }
             closest = lt ? 0 : closest;
         }
-        const bool down = __any(open);
+        const bool down = wave_any(open);
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
@@ -1116,17 +1136,10 @@ static int ensure_flags(TreeBuf *t, int m) {
 
 // Level loop with a host round trip per level: the level sizes decide the
 // launches. Leaves t->lvl and t->nwalk; nodes are moments-complete.
-static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
-    TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
-    TCHK(hipStreamSynchronize(st));
-    const int nact = t->h_int[0];
-    t->nact_h = nact;
-    if (int rc = grow_nodes(t, st, std::max(1024, 4 * nact + 64), 0)) return rc;
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, (LevelCtl *)nullptr);
-    TCHK(hipGetLastError());
-    t->lvl.assign(1, 0);
-    t->lvl.push_back(1);
-    int level = 0;
+// From level `level` on, t->lvl holding the starts of levels 0 .. level + 1
+// and the nodes of the levels above complete (children included).
+static int level_loop_sync(TreeBuf *t, hipStream_t st, DState d, DCtl *ctl, int level) {
+    const int nact = t->nact_h;
     while (true) {
         const int L0 = t->lvl[level], L1 = t->lvl[level + 1], m = L1 - L0;
         if (m == 0) break;
@@ -1165,6 +1178,18 @@ static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double
     TCHK(hipStreamSynchronize(st));
     t->nwalk = t->h_int[0];
     return NV_OK;
+}
+
+static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
+    TCHK(hipMemcpyAsync(t->h_int, t->nact, sizeof(int), hipMemcpyDeviceToHost, st));
+    TCHK(hipStreamSynchronize(st));
+    t->nact_h = t->h_int[0];
+    if (int rc = grow_nodes(t, st, std::max(1024, 4 * t->nact_h + 64), 0)) return rc;
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, (LevelCtl *)nullptr);
+    TCHK(hipGetLastError());
+    t->lvl.assign(1, 0);
+    t->lvl.push_back(1);
+    return level_loop_sync(t, st, d, ctl, 0);
 }
 
 constexpr int BUILD_RETRY = -1000;
@@ -1223,9 +1248,16 @@ static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, doubl
     TCHK(hipStreamSynchronize(st));
     const LevelCtl &h = *t->lc_h;
     if (h.depth) return NV_ERR_TREE_DEPTH;
-    // a tree deeper than the levels launched has children past the last one
-    if (h.overflow || h.lvl[top + 2] > h.lvl[top + 1]) return BUILD_RETRY;
     t->nact_h = h.nact;
+    // A level that outgrew its bound, or a tree deeper than the levels
+    // launched (children past the last one): the levels above are complete,
+    // and the synchronous loop goes on from there.
+    const int resume = h.overflow ? h.ovf_level : h.lvl[top + 2] > h.lvl[top + 1] ? top + 1 : -1;
+    if (resume >= 0) {
+        t->lvl.assign(h.lvl, h.lvl + resume + 2);
+        ++t->sync_builds;
+        return level_loop_sync(t, st, d, ctl, resume);
+    }
     int nlev = 0;
     while (nlev <= top && h.lvl[nlev + 1] > h.lvl[nlev]) ++nlev;
     t->lvl.assign(h.lvl, h.lvl + nlev + 1);

@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
         if (mode == 0) {
             // unrolled so that the loads of several chunks are in flight; the
             // adds stay in chunk order
-#pragma unroll 3
+#pragma unroll 9
             for (int c = 0; c < S; ++c) {
                 const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
             }
         } else {
             double far[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
-#pragma unroll 3
+#pragma unroll 9
             for (int c = 0; c < S; ++c) {
                 const double *pp = pb + (size_t)c * 6 * capk;
 #pragma unroll

@@ -14,6 +14,16 @@ int nv_radix_hist_ints(int n);
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
                                int bits, int *hist, int *scan_tmp);
 
+// Wave votes on a lane condition. HIP's __ballot and __any take an int, so
+// a condition already held as a lane mask is first turned into 0/1 in a
+// vector register and compared again (two vector instructions); these read
+// the mask directly.
+__device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
+// this lane's bit of a wave-uniform mask, as a lane condition (no instruction:
+// the mask is used as the execution or select mask directly)
+__device__ __forceinline__ bool lane_in(unsigned long long m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 // 10-bit Morton spreading: bit b of v moves to bit 3b
 __device__ __forceinline__ unsigned nv_spread10(unsigned v) {
     v &= 0x3ffu;
