@@ -941,38 +941,46 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
         const CNode a = walk[n1];
         const CNode b = walk[n2];
-        const bool vis = cur >= lskip;
+        // Lane conditions as 64-bit masks from ballots of raw compares, as
+        // in the velocity walk; per-lane updates read them with lane_in.
+        const unsigned long long mvis = wave_ballot(cur >= lskip);
         const double e0 = xi[0] - nd.cent[0], e1 = xi[1] - nd.cent[1], e2 = xi[2] - nd.cent[2];
         const double dd = e0 * e0 + e1 * e1 + e2 * e2;
-        bool acc = vis;
-        bool prune = false;
+        // dist < best: sure below blo, excluded above bhi
+        const unsigned long long mlo = wave_ballot(dd < blo), mhi = wave_ballot(dd < bhi);
+        bool down;
         if (!leaf) {
-            bool yes = dd > nd.hi;
-            const bool band = vis && !yes && !(dd < nd.lo);
-            if (wave_any(band)) {
+            // accepted (width / dist < 0.5): sure above hi, excluded below
+            // lo, decided by the reference's division in between
+            const unsigned long long mfar = wave_ballot(dd > nd.hi), mnear = wave_ballot(dd < nd.lo);
+            const unsigned long long mband = mvis & ~mfar & ~mnear;
+            unsigned long long myes = mfar;
+            if (mband != 0ull) {
                 const double width = full[cur].width;
-                if (band) yes = width / sqrt(dd) < .5;
+                myes |= mband & wave_ballot(width / sqrt(dd) < .5);
             }
-            acc = vis && yes;
+            const unsigned long long macc = mvis & myes;
             // Exact pruning: every centroid and point below lies within rad
             // of cent, so a lane farther than best + rad (and a margin far
             // above the rounding of the distances) finds nothing there with
             // dist < best, and dist < best is the only test that changes
             // its closest or best: the subtree is skipped with no effect.
             const double thr = bpm + (double)nd.rad;
-            prune = vis && !acc && dd > thr * thr * (1.0 + 0x1p-30);
-        }
-        const bool open = vis && !acc && !prune;
-        lskip = (acc || prune) ? n2 : lskip;
-        // dist < best: sure below blo, excluded above bhi
-        const bool sure = acc && dd < blo;
-        const bool maybe = acc && !sure && dd < bhi;
-        if (leaf) {
+            const unsigned long long mprune = mvis & ~macc & wave_ballot(dd > thr * thr * (1.0 + 0x1p-30));
+            // an accepted node closer than the best leaf so far
+            unsigned long long mlt = macc & mlo;
+            const unsigned long long mmaybe = macc & ~mlo & mhi;
+            if (mmaybe != 0ull) mlt |= mmaybe & wave_ballot(sqrt(dd) < best);
+            closest = lane_in(mlt) ? 0 : closest;
+            lskip = lane_in(macc | mprune) ? n2 : lskip;
+            down = (mvis & ~macc & ~mprune) != 0ull;
+        } else {
             const int j = -w - 1;
-            const bool cand = (sure || maybe) && (me != j) && (inf != j) && (beh != j);
-            if (wave_any(cand)) {
+            const unsigned long long mcand =
+                mvis & mhi & ~(wave_ballot(me == j) | wave_ballot(inf == j) | wave_ballot(beh == j));
+            if (mcand != 0ull) {
                 const double dist = sqrt(dd);
-                if (cand && dist < best) {
+                if (lane_in(mcand & wave_ballot(dist < best))) {
                     closest = j;
                     best = dist;
                     blo = best * best * (1.0 - CLOSE_BAND);
@@ -980,14 +988,9 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
                     bpm = best + pm;
                 }
             }
-        } else {
-            bool lt = sure;
-            if (wave_any(maybe)) {
-                if (maybe) lt = sqrt(dd) < best;
-            }
-            closest = lt ? 0 : closest;
+            lskip = lane_in(mvis) ? n2 : lskip;
+            down = false;
         }
-        const bool down = wave_any(open);
         cur = down ? n1 : n2;
         nd = down ? a : b;
     }
