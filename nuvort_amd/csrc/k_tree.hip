@@ -125,6 +125,7 @@ struct Nodes {
     int *beg, *cnt, *pcount, *dropb, *first, *parent, *minrow, *size, *dfs, *leafinf;
     double *pos, *w, *sum, *circ;   // pos/sum/circ: [3][ncap]
     double *rad;                     // bound on |centroid of a descendant or point - centroid|
+    double *cent;                    // [3][ncap] centroid: sum / count (the root: / active points)
     int ncap;
 };
 
@@ -184,14 +185,15 @@ __global__ void k_split_flags_d(Nodes nd, const LevelCtl *lc, int level, int bou
 
 // after a level's scan: the next level's start, the children kernel, and
 // whether the level fit its bounds (one thread)
-__global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wgrid, const int *offs) {
+__global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wgrid, const int *offs,
+                            const int *tsum) {
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1], m = L1 - L0;
     int next = L1, wide = 0;
     if (m > bound && !lc->overflow) {
         lc->overflow = 1;
         lc->ovf_level = level;
     } else if (m > 0 && !lc->overflow) {
-        const int nsplit = offs[m - 1] + (nd.pcount[L1 - 1] > 1 ? 1 : 0);
+        const int nsplit = offs[m - 1] + tsum[(m - 1) / NV_SCAN_TILE] + (nd.pcount[L1 - 1] > 1 ? 1 : 0);
         if (nsplit && level == LEVELS) {
             lc->depth = 1;
         } else if (nsplit) {
@@ -237,10 +239,16 @@ __device__ __forceinline__ void write_child(const Nodes &nd, int q, int base, in
 
 // children of the split nodes (pcount > 1) of level `level` (nodes [L0,
 // L1)) into L1 + 8 * off, one thread per node
+// off[]: the split flags' exclusive scan, or its tile scan with the tile
+// prefixes in tsum[] (the one-launch scan)
+__device__ __forceinline__ int scanned(const int *off, const int *tsum, int i) {
+    return off[i] + (tsum ? tsum[i / NV_SCAN_TILE] : 0);
+}
+
 __device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int L1, const int *off,
-                                            const unsigned long long *keys, int level) {
+                                            const int *tsum, const unsigned long long *keys, int level) {
     if (q >= L1 || nd.pcount[q] <= 1) return;
-    const int base = L1 + 8 * off[q - L0];
+    const int base = L1 + 8 * scanned(off, tsum, q - L0);
     nd.first[q] = base;
     const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
     int bnd[9];
@@ -251,21 +259,21 @@ __device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int 
 }
 
 __global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsigned long long *keys, int level) {
-    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, keys, level);
+    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, nullptr, keys, level);
 }
 
-__global__ void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const unsigned long long *keys,
-                             int level) {
+__global__ void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const int *tsum,
+                             const unsigned long long *keys, int level) {
     if (lc->overflow || lc->wide[level]) return;
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
-    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, keys, level);
+    children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, tsum, keys, level);
 }
 
 // The same for a level of few nodes with long key ranges, one workgroup of
 // 256 per node: 32 lanes find each of the 8 child boundaries by a 32-way
 // search (a node's keys are ascending in the next level's digit).
 __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0, int L1, const int *off,
-                                                 const unsigned long long *keys, int level) {
+                                                 const int *tsum, const unsigned long long *keys, int level) {
     __shared__ int bnd[9];
     if (q >= L1 || nd.pcount[q] <= 1) return;
     const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
@@ -285,7 +293,7 @@ __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0,
     if (li == 0) bnd[d] = lo;
     if (threadIdx.x == 0) bnd[0] = b0;
     __syncthreads();
-    const int base = L1 + 8 * off[q - L0];
+    const int base = L1 + 8 * scanned(off, tsum, q - L0);
     if (threadIdx.x < 8) write_child(nd, q, base, threadIdx.x, bnd[threadIdx.x], bnd[threadIdx.x + 1]);
     if (threadIdx.x == 0) {
         nd.first[q] = base;
@@ -295,14 +303,14 @@ __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0,
 
 __global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off,
                                                        const unsigned long long *keys, int level) {
-    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, keys, level);
+    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, nullptr, keys, level);
 }
 
 __global__ __launch_bounds__(256) void k_children_wide_d(Nodes nd, const LevelCtl *lc, const int *off,
-                                                         const unsigned long long *keys, int level) {
+                                                         const int *tsum, const unsigned long long *keys, int level) {
     if (lc->overflow || !lc->wide[level]) return;
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
-    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, keys, level);
+    children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, tsum, keys, level);
 }
 
 // moments, minimum row and non-empty subtree size, children before parents
@@ -367,25 +375,6 @@ __device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, c
     // it to every centroid and point below it (the triangle inequality over
     // the children, plus the points kept at this node), inflated by 2^-40
     // to cover the rounding of the norms: the close search's exact pruning.
-    const double cntd = q == 0 ? (double)nact : (double)nd.pcount[q];
-    const double c0 = sm[0] / cntd, c1 = sm[1] / cntd, c2 = sm[2] / cntd;
-    double rad = 0.0;
-    int p0 = nd.beg[q];
-    if (nd.first[q] >= 0) {
-        for (int c = 0; c < 8; ++c) {
-            const int ch = nd.first[q] + c;
-            const double pcc = (double)nd.pcount[ch];
-            if (nd.pcount[ch] > 0)
-                rad = fmax(rad, dist3(nd.sum[ch] / pcc, nd.sum[nc + ch] / pcc, nd.sum[2 * nc + ch] / pcc, c0, c1, c2) +
-                                    nd.rad[ch]);
-        }
-        p0 = nd.dropb[q];
-    }
-    for (int j = p0; j < nd.beg[q] + nd.cnt[q]; ++j) {
-        const int pt = vals[j];
-        rad = fmax(rad, dist3(s.x[pt], s.x[cap + pt], s.x[2 * cap + pt], c0, c1, c2));
-    }
-    nd.rad[q] = rad * (1.0 + 0x1p-40);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         nd.sum[k * nc + q] = sm[k];
@@ -393,6 +382,30 @@ __device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, c
     }
     nd.minrow[q] = mr;
     nd.size[q] = size;
+    // empty nodes: no centroid (nothing reads it)
+    if (nd.pcount[q] == 0) return;
+    const double cntd = q == 0 ? (double)nact : (double)nd.pcount[q];
+    const double c0 = sm[0] / cntd, c1 = sm[1] / cntd, c2 = sm[2] / cntd;
+    double rad = 0.0;
+    int p0 = nd.beg[q];
+    if (nd.first[q] >= 0) {
+        for (int c = 0; c < 8; ++c) {
+            const int ch = nd.first[q] + c;
+            if (nd.pcount[ch] > 0)
+                rad = fmax(rad, dist3(nd.cent[ch], nd.cent[nc + ch], nd.cent[2 * nc + ch], c0, c1, c2) + nd.rad[ch]);
+        }
+        p0 = nd.dropb[q];
+    }
+    // a one-point leaf is its point (c = (0 + x) / 1 = x): radius 0
+    if (!(nd.first[q] < 0 && nd.cnt[q] == 1 && nd.pcount[q] == 1))
+        for (int j = p0; j < nd.beg[q] + nd.cnt[q]; ++j) {
+            const int pt = vals[j];
+            rad = fmax(rad, dist3(s.x[pt], s.x[cap + pt], s.x[2 * cap + pt], c0, c1, c2));
+        }
+    nd.rad[q] = rad * (1.0 + 0x1p-40);
+    nd.cent[q] = c0;
+    nd.cent[nc + q] = c1;
+    nd.cent[2 * nc + q] = c2;
 }
 
 __global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals, int nact) {
@@ -410,7 +423,7 @@ __global__ void k_build_tail(const Nodes nd, LevelCtl *lc) { lc->nwalk = lc->ove
 
 // depth-first positions of the children of level-`L` nodes and the walk
 // records of the level-`L` nodes themselves
-__global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk, float *wrad) {
+__global__ void k_down(Nodes nd, DState s, int L0, int L1, WNode *walk, float *wrad) {
     int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= L1) return;
     const int nc = nd.ncap;
@@ -433,11 +446,10 @@ __global__ void k_down(Nodes nd, DState s, int L0, int L1, int nact, WNode *walk
         }
     }
     WNode w;
-    const double cntd = (q == 0) ? (double)nact : (double)pc;
-    double cent[3];
+    double cent[3];   // sum / count, from k_up
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        cent[k] = nd.sum[k * nc + q] / cntd;
+        cent[k] = nd.cent[k * nc + q];
         w.cent[k] = cent[k];
         w.circ[k] = (q == 0) ? 0.0 : nd.circ[k * nc + q];
     }
@@ -1058,7 +1070,7 @@ static void free_nodes(Nodes &nd) {
                   &nd.leafinf};
     for (int **p : ip)
         if (*p) { hipFree(*p); *p = nullptr; }
-    double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ, &nd.rad};
+    double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ, &nd.rad, &nd.cent};
     for (double **p : dp)
         if (*p) { hipFree(*p); *p = nullptr; }
     nd.ncap = 0;
@@ -1096,6 +1108,7 @@ static int grow_nodes(TreeBuf *t, hipStream_t st, int want, int keep) {
     TCHK(hipMalloc(&n.w, sizeof(double) * (size_t)ncap));
     if (o.w && keep) TCHK(hipMemcpyAsync(n.w, o.w, sizeof(double) * keep, hipMemcpyDeviceToDevice, st));
     TCHK(hipMalloc(&n.rad, sizeof(double) * (size_t)ncap));
+    TCHK(hipMalloc(&n.cent, sizeof(double) * 3 * (size_t)ncap));
     double **dp[] = {&n.pos, &n.sum, &n.circ};
     double *od[] = {o.pos, o.sum, o.circ};
     for (int q = 0; q < 3; ++q) {
@@ -1229,15 +1242,15 @@ static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, doubl
         hipLaunchKernelGGL(k_split_flags_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc, l,
                            bound[l], t->offs, ctl);
         TCHK(hipGetLastError());
-        TCHK(nv_exclusive_scan(st, t->offs, bound[l], t->scan_tmp));
+        TCHK(nv_scan_tiles_sums(st, t->offs, bound[l], t->scan_tmp));
         hipLaunchKernelGGL(k_level_end, dim3(1), dim3(1), 0, st, t->nd, t->lc, l, bound[l], wgrid[l],
-                           (const int *)t->offs);
+                           (const int *)t->offs, (const int *)t->scan_tmp);
         TCHK(hipGetLastError());
         if (wgrid[l] > 0)
             hipLaunchKernelGGL(k_children_wide_d, dim3(wgrid[l]), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
-                               (const int *)t->offs, (const unsigned long long *)t->k0, l);
+                               (const int *)t->offs, (const int *)t->scan_tmp, (const unsigned long long *)t->k0, l);
         hipLaunchKernelGGL(k_children_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
-                           (const int *)t->offs, (const unsigned long long *)t->k0, l);
+                           (const int *)t->offs, (const int *)t->scan_tmp, (const unsigned long long *)t->k0, l);
         TCHK(hipGetLastError());
     }
     for (int l = top; l >= 0; --l) {
@@ -1304,8 +1317,8 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     for (int lv = 0; lv < nlev; ++lv) {
         const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
         if (L1 > L0)
-            hipLaunchKernelGGL(k_down, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->nact_h,
-                               t->walk, t->wrad);
+            hipLaunchKernelGGL(k_down, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->walk,
+                               t->wrad);
         TCHK(hipGetLastError());
     }
     stats->tree_nodes = t->nnodes;

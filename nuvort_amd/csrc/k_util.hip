@@ -47,6 +47,31 @@ __global__ void k_scan_tiles(int *data, int n, int *sums) {
     if (threadIdx.x == 0 && sums) sums[blockIdx.x] = total;
 }
 
+// Exclusive scan, in place, of the nb tile totals k_scan_tiles wrote (one
+// workgroup): afterwards the global exclusive prefix of item i of the tiled
+// array is data[i] + sums[i / SCAN_TILE].
+__global__ void k_scan_sums(int *sums, int nb) {
+    __shared__ int lds[SCAN_BT];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += SCAN_TILE) {
+        const int bb = b0 + threadIdx.x * SCAN_IPT;
+        int w[SCAN_IPT], loc = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; ++k) {
+            w[k] = bb + k < nb ? sums[bb + k] : 0;
+            loc += w[k];
+        }
+        int tot;
+        int o = block_exclusive_scan(loc, lds, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; ++k) {
+            if (bb + k < nb) sums[bb + k] = o;
+            o += w[k];
+        }
+        carry += tot;
+    }
+}
+
 __global__ void k_scan_add(int *data, int n, const int *sums) {
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_IPT;
     const int add = sums[blockIdx.x];
@@ -109,11 +134,11 @@ __global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int 
 }
 
 __global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, int shift, const int *hist,
-                             int nblk, unsigned long long *okeys, int *ovals) {
+                             const int *tsum, int nblk, unsigned long long *okeys, int *ovals) {
     __shared__ int running[256];
     __shared__ int wcnt[RS_BT / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    running[tid] = hist[tid * nblk + blockIdx.x];
+    running[tid] = hist[tid * nblk + blockIdx.x] + tsum[(tid * nblk + blockIdx.x) / SCAN_TILE];
     const int base = blockIdx.x * RS_TILE;
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (int k = 0; k < RS_IPT; ++k) {
@@ -200,6 +225,20 @@ int nv_compact_tmp_ints(int n) { return nv_scan_tmp_ints((n + SCAN_TILE - 1) / S
 
 int nv_radix_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
+// Two-launch scan of data[0, n): each tile scanned in place, then the tile
+// totals (ceil(n / NV_SCAN_TILE) ints in sums) scanned by one workgroup;
+// afterwards the global exclusive prefix of item i is
+// data[i] + sums[i / NV_SCAN_TILE].
+hipError_t nv_scan_tiles_sums(hipStream_t st, int *data, int n, int *sums) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(SCAN_BT), 0, st, data, n, sums);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BT), 0, st, sums, nb);
+    return hipGetLastError();
+}
+
 int nv_radix_hist_ints(int n) { return 256 * nv_radix_blocks(n); }
 
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
@@ -208,12 +247,16 @@ hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, 
     const int nblk = nv_radix_blocks(n);
     unsigned long long *ka = k0, *kb = k1;
     int *va = v0, *vb = v1;
+    // histogram, tile scan, scan of the tile totals (in scan_tmp), scatter
+    // adding the tile prefix: four launches per 8-bit pass
+    const int nh = 256 * nblk;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, hist, nblk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if ((e = nv_exclusive_scan(st, hist, 256 * nblk, scan_tmp)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, hist, nblk, kb, vb);
+        if ((e = nv_scan_tiles_sums(st, hist, nh, scan_tmp)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, (const int *)hist,
+                           (const int *)scan_tmp, nblk, kb, vb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         std::swap(ka, kb);
         std::swap(va, vb);

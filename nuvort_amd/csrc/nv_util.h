@@ -10,6 +10,8 @@ hipError_t nv_fill_int(hipStream_t st, int *p, int v, int n);
 // keys (a multiple of 16, so that the result ends in k0/v0); hist holds
 // nv_radix_hist_ints(n) ints, scan_tmp nv_scan_tmp_ints(nv_radix_hist_ints(n))
 int nv_radix_blocks(int n);
+constexpr int NV_SCAN_TILE = 1024;
+hipError_t nv_scan_tiles_sums(hipStream_t st, int *data, int n, int *sums);
 int nv_radix_hist_ints(int n);
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
                                int bits, int *hist, int *scan_tmp);
