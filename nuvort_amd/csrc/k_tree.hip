@@ -71,6 +71,21 @@ struct __align__(16) NodeCold {
 static_assert(sizeof(Node48) == 48 && sizeof(NodeHot) == 32 && offsetof(Node48, e) == sizeof(NodeHot),
               "the hot part leads the record");
 
+// c ? a : b for a wave-uniform c, selected as four 64-bit words: every field
+// stays in scalar registers (selected field by field, the FP32 opening radius
+// is moved to a vector register and selected there, three vector
+// instructions per walk iteration)
+__device__ __forceinline__ NodeHot pick_hot(bool c, const NodeHot &a, const NodeHot &b) {
+    unsigned long long ua[4], ub[4], uo[4];
+    __builtin_memcpy(ua, &a, sizeof(NodeHot));
+    __builtin_memcpy(ub, &b, sizeof(NodeHot));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) uo[k] = c ? ua[k] : ub[k];
+    NodeHot o;
+    __builtin_memcpy(&o, uo, sizeof(NodeHot));
+    return o;
+}
+
 __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
     return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
 }
@@ -720,7 +735,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
             if (lane_in(mneed)) near_term64(v0, v1, v2, c64, dd, kappa, nr, nev);
         }
         cur = down ? n1 : n2;
-        nd = down ? a : b;
+        nd = pick_hot(down, a, b);
     }
     if (r < n) {
         const double k4 = kappa / (4.0 * NV_PI);
