@@ -615,7 +615,10 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 // Persistent: a grid the size of the resident waves, each wave taking items
 // (image, block of 64 targets) from a queue, the unshifted image's (longest)
 // walks first, so that no SIMD idles while long walks finish.
-template <int BT>
+// targets per lane of the FP32 walk
+constexpr int WALK_TPT = 2;
+
+template <int BT, int TPT>
 __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
                                                    const int *__restrict__ order, int nord, WalkQueues wq,
                                                    const Node48 *__restrict__ walk,
@@ -631,24 +634,37 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     for (;;) {
     int m, tb;
     if (!next_walk_item(wq, h, m, tb)) break;
-    const int q = tb * 64 + lane;
-    const int r = q < nord ? order[q] : q;
-    const bool act = r < n && s.infront[r] != 0;
-    double xi[3] = {0, 0, 0};
-    int beh = -5, me = -5;
-    if (act) {
-        xi[0] = s.x[r]; xi[1] = s.x[s.cap + r]; xi[2] = s.x[2 * s.cap + r];
-        beh = s.behind[r];
-        me = r + 1;
+    // TPT targets per lane: those of TPT consecutive blocks of 64 (in the
+    // Morton order), which share the wave's traversal and record reads
+    int r[TPT], beh[TPT], me[TPT], lskip[TPT];
+    double xs[TPT][3];
+    double fa[TPT][3];   // FP32 terms, accumulated in FP64
+    double nr[TPT][3];   // FP64 terms
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int q = (tb * TPT + t) * 64 + lane;
+        r[t] = q < nord ? order[q] : q;
+        const bool act = r[t] < n && s.infront[r[t]] != 0;
+        double xi[3] = {0, 0, 0};
+        beh[t] = -5;
+        me[t] = -5;
+        if (act) {
+            xi[0] = s.x[r[t]]; xi[1] = s.x[s.cap + r[t]]; xi[2] = s.x[2 * s.cap + r[t]];
+            beh[t] = s.behind[r[t]];
+            me[t] = r[t] + 1;
+        }
+        // the shifted target: displacements are centroid - xs (in the
+        // unshifted image exactly the reference's cent - x)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            xs[t][k] = xi[k] - sh.s[m][k];
+            fa[t][k] = 0.0;
+            nr[t][k] = 0.0;
+        }
+        lskip[t] = act ? 0 : 0x7fffffff;
     }
-    // the shifted target: displacements are centroid - xs (in the
-    // unshifted image exactly the reference's cent - x)
-    const double xs0 = xi[0] - sh.s[m][0], xs1 = xi[1] - sh.s[m][1], xs2 = xi[2] - sh.s[m][2];
-    double fa[3] = {0.0, 0.0, 0.0};   // FP32 terms, accumulated in FP64
-    double nr[3] = {0.0, 0.0, 0.0};   // FP64 terms
     const bool img0 = (m == 0);
     const unsigned long long img0m = img0 ? ~0ull : 0ull;
-    int lskip = act ? 0 : 0x7fffffff;
     int cur = 0;
     const char *wb = (const char *)walk;
     NodeHot nd = *(const NodeHot *)wb;
@@ -672,79 +688,97 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         // costs nothing. Bools combined per lane would be kept twice, as lane
         // masks and as ballots, and a ballot of such a bool is rebuilt in
         // vector registers.
-        const unsigned long long mvis = wave_ballot(cur >= lskip);
-        nvis += (unsigned long long)__popcll(mvis);
-        const double v0 = nd.cent[0] - xs0, v1 = nd.cent[1] - xs1, v2 = nd.cent[2] - xs2;
-        const double dd = __fma_rn(v0, v0, __fma_rn(v1, v1, v2 * v2));
-        const unsigned long long mgt = wave_ballot(dd > (double)nd.r2);
-        unsigned long long macc, muse, mnear;
-        bool down;
-        if (leaf) {
-            const int j = -w - 1;
-            const unsigned long long mexcl = (wave_ballot(j == me) | wave_ballot(j == beh)) & img0m;
-            // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
-            const unsigned long long msg = mvis & ~mexcl & wave_ballot(dd < sing2);
-            const unsigned long long mrn = wave_ballot(dd < rn2);
-            sing = sing || msg != 0ull;
-            macc = mvis;
-            muse = mvis & ~mexcl & ~msg;
-            mnear = muse & mrn & (j > 0 ? ~0ull : 0ull);
-            down = false;
-        } else {
-            macc = mvis & mgt;
-            muse = macc;
-            mnear = 0ull;
-            down = (mvis & ~mgt) != 0ull;   // a lane opens the node
+        unsigned long long mfar[TPT], mneed[TPT];
+        double v[TPT][3], dd[TPT];
+        bool down = false;
+        unsigned long long anyfar = 0ull, anyneed = 0ull;
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) {
+            const unsigned long long mvis = wave_ballot(cur >= lskip[t]);
+            nvis += (unsigned long long)__popcll(mvis);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[t][k] = nd.cent[k] - xs[t][k];
+            dd[t] = __fma_rn(v[t][0], v[t][0], __fma_rn(v[t][1], v[t][1], v[t][2] * v[t][2]));
+            const unsigned long long mgt = wave_ballot(dd[t] > (double)nd.r2);
+            unsigned long long macc, muse, mnear;
+            if (leaf) {
+                const int j = -w - 1;
+                const unsigned long long mexcl = (wave_ballot(j == me[t]) | wave_ballot(j == beh[t])) & img0m;
+                // dd < sing2 <=> sqrt(dd) < epsilon (sqrt is correctly rounded)
+                const unsigned long long msg = mvis & ~mexcl & wave_ballot(dd[t] < sing2);
+                const unsigned long long mrn = wave_ballot(dd[t] < rn2);
+                sing = sing || msg != 0ull;
+                macc = mvis;
+                muse = mvis & ~mexcl & ~msg;
+                mnear = muse & mrn & (j > 0 ? ~0ull : 0ull);
+            } else {
+                macc = mvis & mgt;
+                muse = macc;
+                mnear = 0ull;
+                down = down || (mvis & ~mgt) != 0ull;   // a lane opens the node
+            }
+            lskip[t] = lane_in(macc) ? n2 : lskip[t];   // a leaf's successor, or the node's skip
+            mfar[t] = muse & ~mnear;
+            mneed[t] = mnear;
+            anyfar |= mfar[t];
         }
-        lskip = lane_in(macc) ? n2 : lskip;   // a leaf's successor, or the node's skip
-        const unsigned long long mfar = muse & ~mnear;
-        unsigned long long mneed = mnear;
-        if (mfar != 0ull) {
+        if (anyfar != 0ull) {
             const NodeCold cd = *(const NodeCold *)(wb + (size_t)cur * sizeof(Node48) + sizeof(NodeHot));
-            const float d0 = (float)v0, d1 = (float)v1, d2 = (float)v2;
             const float e0 = cd.e[0], e1 = cd.e[1], e2 = cd.e[2];
-            const float x0 = __fmaf_rn(d1, e2, -(d2 * e1));
-            const float x1 = __fmaf_rn(d2, e0, -(d0 * e2));
-            const float x2 = __fmaf_rn(d0, e1, -(d1 * e0));
-            const float r1sq = (float)dd;
-            const float de = __fmaf_rn(d0, e0, __fmaf_rn(d1, e1, d2 * e2));
-            const float r2sq = __fmaf_rn(2.0f, de, r1sq) + cd.ee;
-            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
-            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
-            const float r12 = (r1sq * inv1) * (r2sq * inv2);
-            const float den = r12 + (r1sq + de);
-            // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
-            // the pseudo-segment ending close to the target (r2 < r1 / 4, where
-            // the FP32 r2^2 can even round to <= 0); NaN goes there too
-            const unsigned long long mill =
-                wave_ballot(!(den >= 0.0625f * r12)) | wave_ballot(!(r2sq >= 0.0625f * r1sq));
-            // the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon
-            // (x 4 is exact: |d x e|^2 < epsilon / 4)
-            const float xx = __fmaf_rn(x0, x0, __fmaf_rn(x1, x1, x2 * x2));
-            const unsigned long long mok = mfar & ~mill & ~wave_ballot(xx < 0.25f * (float)NV_EPS);
-            // x is finite for every lane, so x * 0 = 0 where the term is not taken
-            const float fac = lane_in(mok) ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
-            fa[0] += (double)(x0 * fac);
-            fa[1] += (double)(x1 * fac);
-            fa[2] += (double)(x2 * fac);
-            nevw += (unsigned long long)__popcll(mok);
-            mneed |= mfar & mill;
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                if (TPT > 1 && mfar[t] == 0ull) continue;
+                const float d0 = (float)v[t][0], d1 = (float)v[t][1], d2 = (float)v[t][2];
+                const float x0 = __fmaf_rn(d1, e2, -(d2 * e1));
+                const float x1 = __fmaf_rn(d2, e0, -(d0 * e2));
+                const float x2 = __fmaf_rn(d0, e1, -(d1 * e0));
+                const float r1sq = (float)dd[t];
+                const float de = __fmaf_rn(d0, e0, __fmaf_rn(d1, e1, d2 * e2));
+                const float r2sq = __fmaf_rn(2.0f, de, r1sq) + cd.ee;
+                const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+                const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+                const float r12 = (r1sq * inv1) * (r2sq * inv2);
+                const float den = r12 + (r1sq + de);
+                // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
+                // the pseudo-segment ending close to the target (r2 < r1 / 4, where
+                // the FP32 r2^2 can even round to <= 0); NaN goes there too
+                const unsigned long long mill =
+                    wave_ballot(!(den >= 0.0625f * r12)) | wave_ballot(!(r2sq >= 0.0625f * r1sq));
+                // the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon
+                // (x 4 is exact: |d x e|^2 < epsilon / 4)
+                const float xx = __fmaf_rn(x0, x0, __fmaf_rn(x1, x1, x2 * x2));
+                const unsigned long long mok = mfar[t] & ~mill & ~wave_ballot(xx < 0.25f * (float)NV_EPS);
+                // x is finite for every lane, so x * 0 = 0 where the term is not taken
+                const float fac = lane_in(mok) ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
+                fa[t][0] += (double)(x0 * fac);
+                fa[t][1] += (double)(x1 * fac);
+                fa[t][2] += (double)(x2 * fac);
+                nevw += (unsigned long long)__popcll(mok);
+                mneed[t] |= mfar[t] & mill;
+            }
         }
-        if (mneed != 0ull) {
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) anyneed |= mneed[t];
+        if (anyneed != 0ull) {
             const double c64[3] = {circ64[3 * cur], circ64[3 * cur + 1], circ64[3 * cur + 2]};
-            if (lane_in(mneed)) near_term64(v0, v1, v2, c64, dd, kappa, nr, nev);
+#pragma unroll
+            for (int t = 0; t < TPT; ++t)
+                if (lane_in(mneed[t])) near_term64(v[t][0], v[t][1], v[t][2], c64, dd[t], kappa, nr[t], nev);
         }
         cur = down ? n1 : n2;
         nd = pick_hot(down, a, b);
     }
-    if (r < n) {
-        const double k4 = kappa / (4.0 * NV_PI);
-        double *pp = part + (size_t)m * 6 * npad;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double acc = fa[k] * k4 + nr[k];
-            pp[(img0 ? k : 3 + k) * npad + r] = acc;
-            pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
+    for (int t = 0; t < TPT; ++t) {
+        if (r[t] < n) {
+            const double k4 = kappa / (4.0 * NV_PI);
+            double *pp = part + (size_t)m * 6 * npad;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double acc = fa[t][k] * k4 + nr[t][k];
+                pp[(img0 ? k : 3 + k) * npad + r[t]] = acc;
+                pp[(img0 ? 3 + k : k) * npad + r[t]] = 0.0;
+            }
         }
     }
     }
@@ -1399,12 +1433,14 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         }
         hipLaunchKernelGGL(k_pack48, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
                            p.tree_theta, p.extra_corr, t->walk48, t->circ64);
-        if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256>)) == 0) return NV_ERR_HIP;
-        const int grid = std::min(t->grid32, (nitems + 3) / 4);
+        // items: (image, WALK_TPT blocks of 64 targets)
+        const int nbt = (n + 64 * WALK_TPT - 1) / (64 * WALK_TPT), nit32 = nbt * sh.n;
+        if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256, WALK_TPT>)) == 0) return NV_ERR_HIP;
+        const int grid = std::min(t->grid32, (nit32 + 3) / 4);
         TCHK(nv_fill_int(st, t->wq, 0, XCDS));
-        hipLaunchKernelGGL(k_walk_vel32<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           WalkQueues{t->wq, nb64, sh.n}, t->walk48, t->circ64, t->nwalk, rn2, sqrt_threshold(NV_EPS), part,
-                           ctl, t->counters);
+        hipLaunchKernelGGL((k_walk_vel32<256, WALK_TPT>), dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
+                           WalkQueues{t->wq, nbt, sh.n}, t->walk48, t->circ64, t->nwalk, rn2, sqrt_threshold(NV_EPS),
+                           part, ctl, t->counters);
     } else {
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
