@@ -772,13 +772,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
     for (int t = 0; t < TPT; ++t) {
         if (r[t] < n) {
             const double k4 = kappa / (4.0 * NV_PI);
-            double *pp = part + (size_t)m * 6 * npad;
+            double *pp = part + (size_t)m * 3 * npad;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const double acc = fa[t][k] * k4 + nr[t][k];
-                pp[(img0 ? k : 3 + k) * npad + r[t]] = acc;
-                pp[(img0 ? 3 + k : k) * npad + r[t]] = 0.0;
-            }
+            for (int k = 0; k < 3; ++k) pp[k * npad + r[t]] = fa[t][k] * k4 + nr[t][k];
         }
     }
     }
@@ -792,9 +788,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
 }
 
 // Persistent like k_walk_vel32: items (image, block of 64 targets); partial
-// block [images][6][npad], image 0 in the unshifted half, images 1..26 in the
-// image half, so k_finish's chunk-ordered reduction rebuilds the reference's
-// two sums (src/timestep.f90:107-115)
+// block [images][3][npad], from which k_finish (mode 3) rebuilds the
+// reference's two sums, image 0 and the images in order
+// (src/timestep.f90:107-115)
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
                                                  const int *__restrict__ order, int nord, WalkQueues wq,
@@ -879,12 +875,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
         nd = down ? a : b;
     }
     if (r < n) {
-        double *pp = part + (size_t)m * 6 * npad;
+        double *pp = part + (size_t)m * 3 * npad;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            pp[(img0 ? k : 3 + k) * npad + r] = acc[k];
-            pp[(img0 ? 3 + k : k) * npad + r] = 0.0;
-        }
+        for (int k = 0; k < 3; ++k) pp[k * npad + r] = acc[k];
     }
     }
     if (sing) atomicOr(&ctl->status, NVK_SINGULAR);

@@ -447,8 +447,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
 }
 
 // Reduce chunk partials in chunk order and finish every point.
-// mode 0: partials hold (acc0, accs) of k_bs64 or of the tree walk;
-// mode 1: (far, near) of k_bs32c; mode 2: no partials (LIA).
+// mode 0: partials hold (acc0, accs) of k_bs64; mode 1: (far, near) of
+// k_bs32c; mode 2: no partials (LIA); mode 3: the tree walks' [image][3][rows],
+// image 0 the unshifted sum, the others the images' (added in image order,
+// the same additions of the same values as mode 0 with the zero halves).
 __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows,
                                                 const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -476,7 +478,16 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
         const int kk = tg.row2t[r];
         const int capk = tg.cap[kk], loc = r - tg.base[kk], S = tg.S[kk];
         const double *pb = part + tg.poff[kk];
-        if (mode == 0) {
+        if (mode == 3) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ub0[k] = 0.0 + pb[k * capk + loc];
+#pragma unroll 9
+            for (int c = 1; c < S; ++c) {
+                const double *pp = pb + (size_t)c * 3 * capk;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ubs[k] = ubs[k] + pp[k * capk + loc];
+            }
+        } else if (mode == 0) {
             // unrolled so that the loads of several chunks are in flight; the
             // adds stay in chunk order
 #pragma unroll 9

@@ -542,13 +542,13 @@ int velocity_impl(Ctx *c, int do_ab) {
         HIPCHK(c, c->d_poff.reserve(c->K));
     } else if (c->prm.velocity == NV_VEL_TREE) {
         if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pmotion: tree velocity without construct_tree");
-        // one chunk per image: partial block [images][6][rows] at offset 0
+        // one chunk per image: partial block [images][3][rows] at offset 0
         const int nimg = c->sh.n;
         const long long zero = 0;
         ++c->layout_gen;   // rewrites d_S / d_poff that recorded steps read
         HIPCHK(c, c->d_S.reserve(1));
         HIPCHK(c, c->d_poff.reserve(1));
-        HIPCHK(c, c->part.reserve((size_t)nimg * 6 * R));
+        HIPCHK(c, c->part.reserve((size_t)nimg * 3 * R));
         HIPCHK(c, hipMemcpyAsync(c->d_S.p, &nimg, sizeof(int), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipMemcpyAsync(c->d_poff.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipEventRecord(c->e2, c->st));
@@ -558,7 +558,7 @@ int velocity_impl(Ctx *c, int do_ab) {
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
         HIPCHK(c, hipEventRecord(c->e3, c->st));
         timed_walk = true;
-        mode = 0;
+        mode = 3;
     } else {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
         int nitems = 0;
