@@ -355,7 +355,9 @@ def tree1m(args):
     walk_ms = np.mean([x["walk_ms"] for x in stats])
     n = st.active
     rec = 48 if args.tree_precision == "fp32c" else 80
-    walk_bytes = walk_nodes * rec + n * 27 * 12 + n * 12
+    # node records read, plus per target and image its FP64 position read
+    # and its 3-double partial written
+    walk_bytes = walk_nodes * rec + n * 27 * 48
     line = {
         "metric": METRIC, "value": pu / t_all, "unit": "point-updates/s", "n_gpus": 1, "steps": nsteps,
         "warmup": 1, "ms_per_step": t_all / nsteps * 1e3, "higher_is_better": True, "scaling": "weak",
