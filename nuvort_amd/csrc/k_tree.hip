@@ -71,18 +71,20 @@ struct __align__(16) NodeCold {
 static_assert(sizeof(Node48) == 48 && sizeof(NodeHot) == 32 && offsetof(Node48, e) == sizeof(NodeHot),
               "the hot part leads the record");
 
-// c ? a : b for a wave-uniform c, selected as four 64-bit words: every field
-// stays in scalar registers (selected field by field, the FP32 opening radius
-// is moved to a vector register and selected there, three vector
-// instructions per walk iteration)
-__device__ __forceinline__ NodeHot pick_hot(bool c, const NodeHot &a, const NodeHot &b) {
-    unsigned long long ua[4], ub[4], uo[4];
-    __builtin_memcpy(ua, &a, sizeof(NodeHot));
-    __builtin_memcpy(ub, &b, sizeof(NodeHot));
+// c ? a : b for a wave-uniform c, selected as 64-bit words: every field
+// stays in scalar registers (selected field by field, a record's FP32 field
+// is moved to a vector register and selected there)
+template <typename R>
+__device__ __forceinline__ R pick64(bool c, const R &a, const R &b) {
+    static_assert(sizeof(R) % 8 == 0, "whole 64-bit words");
+    constexpr int NW = sizeof(R) / 8;
+    unsigned long long ua[NW], ub[NW], uo[NW];
+    __builtin_memcpy(ua, &a, sizeof(R));
+    __builtin_memcpy(ub, &b, sizeof(R));
 #pragma unroll
-    for (int k = 0; k < 4; ++k) uo[k] = c ? ua[k] : ub[k];
-    NodeHot o;
-    __builtin_memcpy(&o, uo, sizeof(NodeHot));
+    for (int k = 0; k < NW; ++k) uo[k] = c ? ua[k] : ub[k];
+    R o;
+    __builtin_memcpy(&o, uo, sizeof(R));
     return o;
 }
 
@@ -766,7 +768,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                 if (lane_in(mneed[t])) near_term64(v[t][0], v[t][1], v[t][2], c64, dd[t], kappa, nr[t], nev);
         }
         cur = down ? n1 : n2;
-        nd = pick_hot(down, a, b);
+        nd = pick64(down, a, b);
     }
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
@@ -912,17 +914,22 @@ __global__ void k_target_keys(DState s, int n, double bsize, unsigned long long 
     keys[r] = code;
 }
 
-// Walk record of the close search: FP64 centroid, the bounds on dist^2
-// that decide the reference's opening test width / dist < 0.5 without its
-// division (below lo the node is opened, above hi accepted; between them the
-// test is evaluated as the reference does), and one word: the skip index,
-// or minus one minus a leaf's j = behind(leaf_infront) resolved when packed.
+// Walk record of the close search (32 bytes): FP64 centroid, one word and
+// the node radius. The word is minus one minus a leaf's j =
+// behind(leaf_infront), resolved when packed, or for an internal node the
+// skip index with the node's level in bits 27-30 (internal nodes are at
+// levels 0-15; skip < 2^27). The level gives the width, bsize 2^-level
+// exactly, and so the bounds on dist^2 that decide the reference's opening
+// test width / dist < 0.5 without its division (below lo the node is
+// opened, above hi accepted; between them the test is evaluated as the
+// reference does).
 struct CNode {
     double cent[3];
-    double lo, hi;
     int word;
     float rad;   // every centroid and point below lies within rad of cent
 };
+constexpr int CLOSE_SKIP_BITS = 27;
+
 
 // With d = RN(sqrt(dd)) and q = RN(w / d): q < 0.5 <=> w / d < 0.5 - 2^-55
 // (0.5 is a power of two); dd > 4 w^2 (1 + 2^-46) makes d > 2 w (1 + 2^-48),
@@ -931,7 +938,7 @@ struct CNode {
 constexpr double CLOSE_BAND = 0x1p-46;
 
 __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const float *__restrict__ wrad, int nwalk,
-                             CNode *__restrict__ out) {
+                             double bsize, CNode *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > nwalk) return;
     if (q == nwalk) {   // the spare record
@@ -942,10 +949,10 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const flo
     CNode o;
 #pragma unroll
     for (int k = 0; k < 3; ++k) o.cent[k] = w.cent[k];
-    const double w4 = 4.0 * (w.width * w.width);
-    o.lo = w4 * (1.0 - CLOSE_BAND);
-    o.hi = w4 * (1.0 + CLOSE_BAND);
-    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
+    // width = bsize 2^-level exactly (the build halves it level by level)
+    const int level = ilogb(bsize / w.width);
+    o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1
+                           : w.skip | (level << CLOSE_SKIP_BITS);
     o.rad = wrad[q];
     out[q] = o;
 }
@@ -961,7 +968,8 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const flo
 template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
                                                    WalkQueues wq, const CNode *__restrict__ walk,
-                                                   const WNode *__restrict__ full, int nwalk, double pm) {
+                                                   const WNode *__restrict__ full, int nwalk, double pm,
+                                                   double b4) {
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     for (;;) {
@@ -992,7 +1000,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         const bool leaf = w < 0;
         // both possible successors are fetched while this node is evaluated
         // (the records end with one spare, so neither index needs a clamp)
-        const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
+        const int n1 = cur + 1, n2 = leaf ? cur + 1 : (w & ((1 << CLOSE_SKIP_BITS) - 1));
         const CNode a = walk[n1];
         const CNode b = walk[n2];
         // Lane conditions as 64-bit masks from ballots of raw compares, as
@@ -1006,7 +1014,11 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
         if (!leaf) {
             // accepted (width / dist < 0.5): sure above hi, excluded below
             // lo, decided by the reference's division in between
-            const unsigned long long mfar = wave_ballot(dd > nd.hi), mnear = wave_ballot(dd < nd.lo);
+            // 4 width^2 = 4 RN(bsize^2) 2^-2 level exactly, as k_pack_close and
+            // the reference's width give it
+            const double w4 = ldexp(b4, -2 * (w >> CLOSE_SKIP_BITS));
+            const double lo = w4 * (1.0 - CLOSE_BAND), hi = w4 * (1.0 + CLOSE_BAND);
+            const unsigned long long mfar = wave_ballot(dd > hi), mnear = wave_ballot(dd < lo);
             const unsigned long long mband = mvis & ~mfar & ~mnear;
             unsigned long long myes = mfar;
             if (mband != 0ull) {
@@ -1046,7 +1058,7 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
             down = false;
         }
         cur = down ? n1 : n2;
-        nd = down ? a : b;
+        nd = pick64(down, a, b);
     }
     if (act) {
         s.closest[r] = closest;
@@ -1482,13 +1494,15 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
         t->walkc_cap = std::max(t->nwalk + 1, 2 * t->walkc_cap);
         TCHK(hipMalloc(&t->walkc, sizeof(CNode) * (size_t)t->walkc_cap));
     }
+    if (t->nwalk >= (1 << CLOSE_SKIP_BITS)) return NV_ERR_UNSUPPORTED;   // the skip index field
     hipLaunchKernelGGL(k_pack_close, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk,
-                       (const float *)t->wrad, t->nwalk, t->walkc);
+                       (const float *)t->wrad, t->nwalk, bsize, t->walkc);
     if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
     const int nitems = (n + 63) / 64;
     TCHK(nv_fill_int(st, t->wq, 0, XCDS));
     hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
-                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, bsize * 0x1p-40);
+                       n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, bsize * 0x1p-40,
+                       4.0 * (bsize * bsize));
     TCHK(hipGetLastError());
     return NV_OK;
 }
