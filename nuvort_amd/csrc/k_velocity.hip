@@ -156,9 +156,13 @@ __global__ void k_order_inverse(const int *__restrict__ perm, int rows, int *__r
 // Per source tile: the FP64 origin its FP32 records are relative to, and
 // a sphere (FP32, relative to the origin) holding its active sources;
 // rad < 0 marks a tile without active sources.
+constexpr int SUB = 16;     // sources per sub-block of a tile
 struct TileInfo {
     double o[3];
     float c[3], rad;
+    // bounding sphere of each sub-block of SUB sources (centre xyz, radius;
+    // radius -1 where the sub-block holds no active source)
+    float sub[TS / SUB][4];
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -219,6 +223,35 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
     float lo[3], hi[3];
     lo[0] = act ? v.x0 : 3.0e38f; lo[1] = act ? v.x1 : 3.0e38f; lo[2] = act ? v.x2 : 3.0e38f;
     hi[0] = act ? v.x0 : -3.0e38f; hi[1] = act ? v.x1 : -3.0e38f; hi[2] = act ? v.x2 : -3.0e38f;
+    {
+        // the same for each sub-block of SUB consecutive records
+        float sl[3], sh_[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sl[k] = lo[k];
+            sh_[k] = hi[k];
+            for (int off = SUB / 2; off > 0; off >>= 1) {
+                sl[k] = fminf(sl[k], __shfl_xor(sl[k], off));
+                sh_[k] = fmaxf(sh_[k], __shfl_xor(sh_[k], off));
+            }
+        }
+        if (q0 < rows && (threadIdx.x % SUB) == 0) {
+            float *o4 = tiles[blockIdx.x].sub[threadIdx.x / SUB];
+            if (sl[0] > sh_[0]) {
+                o4[0] = o4[1] = o4[2] = 0.f;
+                o4[3] = -1.f;
+            } else {
+                float d2 = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    o4[k] = 0.5f * (sl[k] + sh_[k]);
+                    const float h = sh_[k] - sl[k];
+                    d2 += h * h;
+                }
+                o4[3] = 0.5f * sqrtf(d2) * 1.00001f;
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         lo[k] = wave_min(lo[k]);
@@ -237,7 +270,7 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
                 lo[k] = fminf(lo[k], red[q][k]);
                 hi[k] = fmaxf(hi[k], red[q][3 + k]);
             }
-        TileInfo ti;
+        TileInfo &ti = tiles[blockIdx.x];
         ti.o[0] = o[0]; ti.o[1] = o[1]; ti.o[2] = o[2];
         if (lo[0] > hi[0]) {
             ti.c[0] = ti.c[1] = ti.c[2] = 0.f;
@@ -253,7 +286,6 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
             // half the diagonal, rounded up by a relative 1e-5
             ti.rad = 0.5f * sqrtf(d2) * 1.00001f;
         }
-        tiles[blockIdx.x] = ti;
     }
 }
 
@@ -262,6 +294,50 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
     float t = sum + y;
     comp = (t - sum) - y;
     sum = t;
+}
+
+// One block of KB sources, far from every target of the wave: no pair needs
+// the near-field test or an exclusion, every term is FP32.  With d = x - t,
+// sum d x e fac = sum (x x e) fac - t x sum e fac: six FMAs per pair instead
+// of the cross product's nine operations, and one cross product per target
+// and block.
+template <int TPT, int KB>
+__device__ __forceinline__ void far_block(const Rec32 *tile, int jb, const float (&tt)[TPT][3], float (&f)[TPT][3],
+                                          float (&k)[TPT][3]) {
+    float A[TPT][3], B[TPT][3];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        A[q][0] = A[q][1] = A[q][2] = 0.f;
+        B[q][0] = B[q][1] = B[q][2] = 0.f;
+    }
+#pragma unroll
+    for (int jj = 0; jj < KB; ++jj) {
+        const Rec32 r = tile[jb + jj];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
+            const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+            const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
+            const float r2sq = fmaf(2.0f, de, r1sq + r.c);
+            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
+            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
+            const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
+            const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
+            A[q][0] = fmaf(r.xe0, fac, A[q][0]);
+            A[q][1] = fmaf(r.xe1, fac, A[q][1]);
+            A[q][2] = fmaf(r.xe2, fac, A[q][2]);
+            B[q][0] = fmaf(r.e0, fac, B[q][0]);
+            B[q][1] = fmaf(r.e1, fac, B[q][1]);
+            B[q][2] = fmaf(r.e2, fac, B[q][2]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const float *t = tt[q];
+        kahan_add(f[q][0], k[q][0], A[q][0] - fmaf(t[1], B[q][2], -(t[2] * B[q][1])));
+        kahan_add(f[q][1], k[q][1], A[q][1] - fmaf(t[2], B[q][0], -(t[0] * B[q][2])));
+        kahan_add(f[q][2], k[q][2], A[q][2] - fmaf(t[0], B[q][1], -(t[1] * B[q][0])));
+    }
 }
 
 // FP32 compensated far field + FP64 near field.  Far pairs use the
@@ -326,7 +402,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
             for (int q = tid; q < TS; q += TB) tile[q] = rec[(size_t)gt * TS + q];
             __syncthreads();
             if (!any) continue;
-            const TileInfo ti = tiles[gt];
+            const TileInfo &ti = tiles[gt];
             if (ti.rad < 0.f) continue;     // no active source in this tile
             const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
             float tt[TPT][3];
@@ -348,49 +424,35 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
             }
             if (__all(far)) {
                 // far tile for the whole wave: no pair needs the near-field
-                // test or an exclusion, every term is FP32.  With d = x - t,
-                // sum d x e fac = sum (x x e) fac - t x sum e fac: six FMAs
-                // per pair instead of the cross product's nine operations,
-                // and one cross product per target and block of KB sources.
-                for (int jb = 0; jb < TS; jb += KB) {
-                    float A[TPT][3], B[TPT][3];
-#pragma unroll
-                    for (int q = 0; q < TPT; ++q) {
-                        A[q][0] = A[q][1] = A[q][2] = 0.f;
-                        B[q][0] = B[q][1] = B[q][2] = 0.f;
-                    }
-#pragma unroll
-                    for (int jj = 0; jj < KB; ++jj) {
-                        const Rec32 r = tile[jb + jj];
-#pragma unroll
-                        for (int q = 0; q < TPT; ++q) {
-                            const float d0 = r.x0 - tt[q][0], d1 = r.x1 - tt[q][1], d2 = r.x2 - tt[q][2];
-                            const float r1sq = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-                            const float de = fmaf(d0, r.e0, fmaf(d1, r.e1, d2 * r.e2));
-                            const float r2sq = fmaf(2.0f, de, r1sq + r.c);
-                            const float inv1 = __builtin_amdgcn_rsqf(r1sq);
-                            const float inv2 = __builtin_amdgcn_rsqf(r2sq);
-                            const float den = fmaf(r1sq * inv1, r2sq * inv2, r1sq + de);
-                            const float fac = (inv1 + inv2) * __builtin_amdgcn_rcpf(den);
-                            A[q][0] = fmaf(r.xe0, fac, A[q][0]);
-                            A[q][1] = fmaf(r.xe1, fac, A[q][1]);
-                            A[q][2] = fmaf(r.xe2, fac, A[q][2]);
-                            B[q][0] = fmaf(r.e0, fac, B[q][0]);
-                            B[q][1] = fmaf(r.e1, fac, B[q][1]);
-                            B[q][2] = fmaf(r.e2, fac, B[q][2]);
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < TPT; ++q) {
-                        const float* t = tt[q];
-                        kahan_add(f[q][0], k[q][0], A[q][0] - fmaf(t[1], B[q][2], -(t[2] * B[q][1])));
-                        kahan_add(f[q][1], k[q][1], A[q][1] - fmaf(t[2], B[q][0], -(t[0] * B[q][2])));
-                        kahan_add(f[q][2], k[q][2], A[q][2] - fmaf(t[0], B[q][1], -(t[1] * B[q][0])));
-                    }
-                }
+                // test or an exclusion, every term is FP32
+                for (int jb = 0; jb < TS; jb += KB) far_block<TPT, KB>(tile, jb, tt, f, k);
                 continue;
             }
             for (int jb = 0; jb < TS; jb += KB) {
+                // A block of KB sources that is far for the whole wave (its
+                // sub-block spheres tested as the tile's was) takes the far
+                // path: most blocks of a tile that is near some lane are far
+                // from every lane.
+                bool bfar = true;
+#pragma unroll
+                for (int sb = jb / SUB; sb * SUB < jb + KB; ++sb) {
+                    const float *si = ti.sub[sb];
+                    const float srad = si[3];
+                    if (srad < 0.f) continue;   // no active source
+#pragma unroll
+                    for (int q = 0; q < TPT; ++q) {
+                        const float c0 = tt[q][0] - si[0], c1 = tt[q][1] - si[1], c2 = tt[q][2] - si[2];
+                        const float dd = c0 * c0 + c1 * c1 + c2 * c2;
+                        const float mag = fabsf(tt[q][0]) + fabsf(tt[q][1]) + fabsf(tt[q][2]) + srad;
+                        const float lim = (srad + rnf) * 1.0001f + 1.0e-6f * mag;
+                        const bool fq = dd > lim * lim && (unsigned)(bpos[q] - (gt * TS + sb * SUB)) >= (unsigned)SUB;
+                        bfar = bfar && (fq || !act[q]);
+                    }
+                }
+                if (__all(bfar)) {
+                    far_block<TPT, KB>(tile, jb, tt, f, k);
+                    continue;
+                }
                 float a[TPT][3];
 #pragma unroll
                 for (int q = 0; q < TPT; ++q) a[q][0] = a[q][1] = a[q][2] = 0.f;
