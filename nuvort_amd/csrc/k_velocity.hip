@@ -513,8 +513,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
 // k_bs32c; mode 2: no partials (LIA); mode 3: the tree walks' [image][3][rows],
 // image 0 the unshifted sum, the others the images' (added in image order,
 // the same additions of the same values as mode 0 with the zero halves).
-__global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows,
-                                                const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
+__device__ __forceinline__ void finish_body(const DState &s, const Phys &p, const Tangles &tg, int rows,
+                                            const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < rows && s.infront[r] != 0;
     // active points counted per workgroup, one atomic each: atomics on one
@@ -577,6 +577,21 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
         }
     }
     finish_point(s, p, r, ub0, ubs, ctl->t, do_ab);
+}
+
+__global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part,
+                                                int mode, int do_ab, DCtl *ctl) {
+    finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
+}
+
+// The tree walks' layout (mode 3, 27 chunks of 3 doubles) at eight waves per
+// SIMD (64 VGPRs, no spills): a point streams about 0.9 KB through 36
+// component arrays, and more waves hide the memory latency (389 -> 298 us at
+// 1 M points against the compiler's 110 VGPRs and 4 waves). The chunked
+// direct-sum layouts (hundreds of chunks per point) are slower so.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_finish_tree(
+    DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
+    finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
 }
 
 // Reads every chunk partial of the active rows the way k_finish does, and
@@ -707,7 +722,11 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
 
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl) {
-    if (rows > 0)
+    if (rows <= 0) return hipSuccess;
+    if (mode == 3)
+        hipLaunchKernelGGL(k_finish_tree, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
+                           do_ab, ctl);
+    else
         hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode, do_ab,
                            ctl);
     return hipGetLastError();
