@@ -1463,13 +1463,25 @@ int nv_tree_walk_record_bytes(int precision, double theta) {
     return precision == NV_PREC_FP32C && theta <= 1.1 ? (int)sizeof(Node48) : (int)sizeof(WNode);
 }
 
-// interaction and node-visit counters of the last velocity walk (synchronises)
-int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats) {
-    unsigned long long h[2];
-    TCHK(hipMemcpyAsync(h, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    TCHK(hipStreamSynchronize(st));
+// queue a copy of the last velocity walk's interaction and node-visit
+// counters into pinned memory (ints 32..35 of h_int)
+int nv_tree_walk_counters_async(TreeBuf *t, hipStream_t st) {
+    TCHK(hipMemcpyAsync(t->h_int + 32, t->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    return NV_OK;
+}
+
+// the counters that copy brought, once the stream has passed it
+void nv_tree_walk_counters_take(const TreeBuf *t, nv_stats *stats) {
+    const unsigned long long *h = (const unsigned long long *)(t->h_int + 32);
     stats->tree_evals = (int64_t)h[0];
     stats->tree_nodes_visited = (int64_t)h[1];
+}
+
+// both of the above (synchronises)
+int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats) {
+    if (int rc = nv_tree_walk_counters_async(t, st)) return rc;
+    TCHK(hipStreamSynchronize(st));
+    nv_tree_walk_counters_take(t, stats);
     return NV_OK;
 }
 

@@ -74,6 +74,7 @@ struct Ctx {
     int itime = 0;
     nv_stats stats{};
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    hipEvent_t e4 = nullptr, e5 = nullptr, e6 = nullptr, e7 = nullptr;   // the tree step's build and close search
     std::string err;
     TreeBuf *tree = nullptr;
     bool tree_valid = false;
@@ -710,6 +711,116 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
     return NV_OK;
 }
 
+// pmotion on the octree (launches only): the walk's partials
+// [images][3][rows], then the AB update
+int enqueue_tree_pmotion(Ctx *c) {
+    hipStream_t st = c->st;
+    const int R = c->rows, nimg = c->sh.n;
+    ++c->layout_gen;   // rewrites d_S / d_poff that recorded steps read
+    HIPCHK(c, c->d_S.reserve(1));
+    HIPCHK(c, c->d_poff.reserve(1));
+    HIPCHK(c, c->part.reserve((size_t)nimg * 3 * R));
+    HIPCHK(c, hipEventRecord(c->e0, st));
+    HIPCHK(c, nv_fill_int(st, c->d_S.p, nimg, 1));
+    HIPCHK(c, nv_fill_int(st, (int *)c->d_poff.p, 0, 2));   // one long long
+    const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+    HIPCHK(c, hipEventRecord(c->e2, st));
+    if (nv_tree_walk_velocity(c->tree, st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision, rn * rn, c->part.p,
+                              c->ctl, &c->stats))
+        return fail(c, NV_ERR_HIP, "tree walk launch failed");
+    HIPCHK(c, hipEventRecord(c->e3, st));
+    if (int rc = nv_tree_walk_counters_async(c->tree, st)) return fail(c, rc, "tree counters");
+    HIPCHK(c, nv_fill_int(st, &c->ctl->nvel, 0, 1));
+    HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
+    HIPCHK(c, nvk_finish(st, c->d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p, 3, 1, c->ctl));
+    HIPCHK(c, hipEventRecord(c->e1, st));
+    swap_x(c);
+    return NV_OK;
+}
+
+// the figures of enqueue_tree_pmotion once the stream has passed it and
+// read_ctl has brought nvel
+void take_tree_pmotion_stats(Ctx *c) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->e0, c->e1);
+    c->stats.ms_velocity = ms;
+    (void)hipEventElapsedTime(&ms, c->e2, c->e3);
+    c->stats.walk_ms_total += ms;
+    c->stats.walk_launches += 1;
+    nv_tree_walk_counters_take(c->tree, &c->stats);
+    c->stats.walk_nodes_total += c->stats.tree_nodes_visited;
+    c->stats.point_updates += c->ctl_h->nvel;
+}
+
+// One pass of the src/run.f90 loop body on the octree path: the phases of
+// the eager calls (ghostp, construct_tree, pmotion, pinsert, the close
+// search, precon, premove, enforce_boundary, the NaN test) in their order,
+// with three host synchronisations instead of one per phase: the build's
+// read of its level sizes, the point count after pinsert (the close
+// search's target count) and the step's end.  An error that a phase after
+// pinsert raises on the device is reported at the step's end.
+int tree_step(Ctx *c) {
+    hipStream_t st = c->st;
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
+    if (int rc = ensure_room(c)) return rc;
+    c->itime += 1;
+    const int R = c->rows;
+    HIPCHK(c, nvk_ghosts(st, c->d, c->phys, R));
+    HIPCHK(c, hipEventRecord(c->e4, st));
+    int rc = nv_tree_build(c->tree, st, c->d, c->phys, c->tn[0], c->ctl, &c->stats);
+    if (rc == NV_ERR_TREE_DEPTH) return fail(c, rc, "octree deeper than NV_TREE_MAX_LEVELS (coincident points?)");
+    if (rc) return fail(c, rc, "tree build failed");
+    HIPCHK(c, hipEventRecord(c->e5, st));
+    c->tree_valid = true;
+    // with another velocity the octree serves the close search only
+    const bool walk = c->prm.velocity == NV_VEL_TREE;
+    if ((rc = walk ? enqueue_tree_pmotion(c) : velocity_impl(c, 1))) return rc;
+    // pinsert
+    Tangles tg = tangles(c, nullptr, nullptr);
+    HIPCHK(c, nv_fill_int(st, &c->ctl->nactive, 0, 1));
+    HIPCHK(c, nv_fill_int(st, c->p_tnact, 0, c->K));
+    HIPCHK(c, nvk_pinsert_pre(st, c->d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
+    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
+    HIPCHK(c, nvk_pinsert_serial(st, c->d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    const int rc_ins = read_ctl(c);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->e4, c->e5);
+    c->stats.ms_tree_build = ms;
+    if (walk) take_tree_pmotion_stats(c);
+    if (rc_ins) return rc_ins;
+    const bool search = c->itime % c->prm.recon_shots == 0;
+    const bool brute = c->prm.close_search == NV_CLOSE_BRUTE;
+    if (search) {
+        if (brute) {
+            if ((rc = nv_pclose(c))) return rc;
+        } else {
+            HIPCHK(c, hipEventRecord(c->e6, st));
+            const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
+            if (nv_tree_walk_close(c->tree, st, c->d, bsize, c->tn[0], &c->stats))
+                return fail(c, NV_ERR_HIP, "tree close search launch failed");
+            HIPCHK(c, hipEventRecord(c->e7, st));
+        }
+        HIPCHK(c, nvk_precon_pre(st, c->d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
+        HIPCHK(c, nvk_precon_serial(st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
+        HIPCHK(c, nvk_premove_pre(st, c->d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_premove_serial(st, c->d, c->phys, tg, c->lcand.p, c->ctl));
+    }
+    HIPCHK(c, nvk_wrap(st, c->d, c->phys, R));
+    c->t = c->t + c->prm.dt;
+    c->tree_valid = false;
+    HIPCHK(c, nvk_nan(st, c->d, R, c->ctl));
+    rc = read_ctl(c);
+    if (search && !brute) {
+        (void)hipEventElapsedTime(&ms, c->e6, c->e7);
+        c->stats.ms_close = ms;
+    }
+    return rc;
+}
+
 void drop_graph(Ctx::StepGraph &g) {
     if (g.e) (void)hipGraphExecDestroy(g.e);
     if (g.g) (void)hipGraphDestroy(g.g);
@@ -883,7 +994,9 @@ int nv_create(const nv_params *params, void **out) {
         hipHostMalloc((void **)&c->ctl_h, sizeof(DCtl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&c->ev, sizeof(nv_recon_event) * c->ev_cap) != hipSuccess || hipEventCreate(&c->e0) != hipSuccess ||
         hipEventCreate(&c->e1) != hipSuccess || hipEventCreate(&c->e2) != hipSuccess ||
-        hipEventCreate(&c->e3) != hipSuccess) {
+        hipEventCreate(&c->e3) != hipSuccess || hipEventCreate(&c->e4) != hipSuccess ||
+        hipEventCreate(&c->e5) != hipSuccess || hipEventCreate(&c->e6) != hipSuccess ||
+        hipEventCreate(&c->e7) != hipSuccess) {
         delete c;
         return NV_ERR_HIP;
     }
@@ -916,6 +1029,8 @@ int nv_destroy(void *h) {
     if (c->tree) nv_tree_free(c->tree);
     (void)hipEventDestroy(c->e0); (void)hipEventDestroy(c->e1);
     (void)hipEventDestroy(c->e2); (void)hipEventDestroy(c->e3);
+    (void)hipEventDestroy(c->e4); (void)hipEventDestroy(c->e5);
+    (void)hipEventDestroy(c->e6); (void)hipEventDestroy(c->e7);
     (void)hipStreamDestroy(c->st);
     delete c;
     return NV_OK;
@@ -1112,6 +1227,10 @@ int nv_step(void *h, int32_t nsteps) {
     int rc = NV_OK;
     if (!use_tree && c->prm.step_launch == 0 && c->K > 0 && nsteps > 0) {
         rc = step_recorded(c, nsteps);
+        nsteps = 0;
+    }
+    if (use_tree) {
+        for (int k = 0; k < nsteps && rc == NV_OK; ++k) rc = tree_step(c);
         nsteps = 0;
     }
     for (int k = 0; k < nsteps && rc == NV_OK; ++k) {

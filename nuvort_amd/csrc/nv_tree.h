@@ -16,6 +16,8 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
 // bytes of one walk record read by the velocity walk of that precision
 int nv_tree_walk_record_bytes(int precision, double theta);
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats);
+int nv_tree_walk_counters_async(TreeBuf *t, hipStream_t st);
+void nv_tree_walk_counters_take(const TreeBuf *t, nv_stats *stats);
 // pclose_tree / closest_tree (src/tree.f90:190-244) over the (stale) tree
 int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n, nv_stats *stats);
 // every node, depth-first, empty ones included
