@@ -254,32 +254,69 @@ __device__ __forceinline__ void write_child(const Nodes &nd, int q, int base, in
     nd.dropb[ch] = hi;
 }
 
+// Work on the split nodes among a wave's nodes (one per lane) with eight
+// lanes per node, so that reads and writes of the eight children (adjacent
+// nodes) are one child per lane, consecutive lanes on consecutive
+// children, instead of eight strided accesses per lane.  The split lanes'
+// nodes are listed in lane order and taken eight per round: group
+// g = lane / 8 gets the round's g-th node, lane c = lane % 8 its child c.
+// fn(q, c) runs on all eight lanes of a group that has a node, so it may
+// exchange values within the group.  Every lane of the wave must call this.
+constexpr int SPLIT8_WAVES = 4;   // waves per workgroup of the kernels that use it
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename F>
+__device__ __forceinline__ void for_split8(int q, bool split, F fn) {
+    __shared__ int lst[SPLIT8_WAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long m = wave_ballot(split);
+    if (m == 0ull) return;
+    const int k = __popcll(m);
+    if (split) lst[wv][__popcll(m & ((1ull << lane) - 1ull))] = q;
+    wave_lds_sync();
+    for (int r = 0; r < k; r += 8) {
+        const int i = r + (lane >> 3);
+        if (i < k) fn(lst[wv][i], lane & 7);
+    }
+    wave_lds_sync();
+}
+
 // children of the split nodes (pcount > 1) of level `level` (nodes [L0,
-// L1)) into L1 + 8 * off, one thread per node
+// L1)) into L1 + 8 * off, one thread per node and eight lanes per split node
 // off[]: the split flags' exclusive scan, or its tile scan with the tile
-// prefixes in tsum[] (the one-launch scan)
+// prefixes in tsum[] (the two-level scan)
 __device__ __forceinline__ int scanned(const int *off, const int *tsum, int i) {
     return off[i] + (tsum ? tsum[i / NV_SCAN_TILE] : 0);
 }
 
 __device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int L1, const int *off,
                                             const int *tsum, const unsigned long long *keys, int level) {
-    if (q >= L1 || nd.pcount[q] <= 1) return;
-    const int base = L1 + 8 * scanned(off, tsum, q - L0);
-    nd.first[q] = base;
-    const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
-    int bnd[9];
-    bnd[0] = b0;
-    for (int d = 1; d <= 8; ++d) bnd[d] = lower_digit(keys, bnd[d - 1], b1, level + 1, d);
-    nd.dropb[q] = bnd[8];
-    for (int c = 0; c < 8; ++c) write_child(nd, q, base, c, bnd[c], bnd[c + 1]);
+    for_split8(q, q < L1 && nd.pcount[q] > 1, [&](int p, int c) {
+        // lane c: the first key of p's range with digit >= c + 1 at the next
+        // level (the digits ascend over the range), the end of child c
+        const int base = L1 + 8 * scanned(off, tsum, p - L0);
+        const int b0 = nd.beg[p], b1 = nd.beg[p] + nd.cnt[p];
+        const int hi = lower_digit(keys, b0, b1, level + 1, c + 1);
+        const int lo = __shfl_up(hi, 1, 8);
+        if (c == 7) {
+            nd.first[p] = base;
+            nd.dropb[p] = hi;
+        }
+        write_child(nd, p, base, c, c == 0 ? b0 : lo, hi);
+    });
 }
 
-__global__ void k_children(Nodes nd, int L0, int L1, const int *off, const unsigned long long *keys, int level) {
+__global__ __launch_bounds__(256) void k_children(Nodes nd, int L0, int L1, const int *off,
+                                                  const unsigned long long *keys, int level) {
     children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, nullptr, keys, level);
 }
 
-__global__ void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const int *tsum,
+__global__ __launch_bounds__(256) void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const int *tsum,
                              const unsigned long long *keys, int level) {
     if (lc->overflow || lc->wide[level]) return;
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
