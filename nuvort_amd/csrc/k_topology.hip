@@ -783,19 +783,36 @@ __global__ void k_premove_pre(DState s, Phys p, int n, int *cand) {
 }
 
 // The inputs of one premove visit (src/line.f90:108-113) of row r: its
-// infront inf, that one's infront ii, and the removal test.
+// infront inf, that one's infront ii, and the removal test; for a removal
+// also r's behind and the next two rows of the loop after ii (iii, iv),
+// which are what loop_killer(r) and the revisit of behind(r) read.
 struct RmIn {
-    int inf, ii, go;
+    int inf, ii, go, beh, iii, iv;
 };
 
 __device__ __forceinline__ void load_rm(const DState &s, const Phys &p, int r, RmIn &in) {
-    in.inf = in.ii = -1;
+    in.inf = in.ii = in.beh = in.iii = in.iv = -1;
     in.go = 0;
     if (s.infront[r] == 0) return;
     in.inf = s.infront[r] - 1;
     in.ii = s.infront[in.inf] - 1;
     if (in.ii < 0) return;
     in.go = dist_rows(s, s.x, r, s.x, in.ii) < .99 * p.delta;
+    if (!in.go) return;
+    in.beh = s.behind[r] - 1;
+    in.iii = s.infront[in.ii] - 1;
+    if (in.iii >= 0) in.iv = s.infront[in.iii] - 1;
+}
+
+// loop_killer's count (src/reconnection.f90:155-170) of r's loop right after
+// the removal of r's infront, from the staged rows: the loop runs r -> ii
+// -> iii -> iv, and it has fewer than 4 points when one of those is r (an
+// empty row on the way ends the count with no clearing)
+__device__ __forceinline__ bool rm_loop_small(int r, const RmIn &in) {
+    if (in.ii == r) return true;
+    if (in.iii < 0) return false;
+    if (in.iii == r) return true;
+    return in.iv == r;
 }
 
 constexpr int RM_CHUNK = 256;
@@ -822,6 +839,7 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
         d.hd = d.nd = 0;
         stop_sh = 0;
     }
+    int nrm = 0;   // removals (the serial thread's)
     for (int cs = 0; cs < ncand; cs += RM_CHUNK) {
         const int m = min(RM_CHUNK, ncand - cs);
         __syncthreads();
@@ -849,9 +867,13 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
             RmIn in;
             bool fresh = false;
             if (staged) {
+                // staged rows are current unless a removal of this chunk
+                // changed r, inf, ii (the test, r's behind, ii's infront) or
+                // iii (its infront)
                 in = st[k - 1];
                 fresh = tracked && (mod.n == 0 || (!mod_has(mod, r) && (in.inf < 0 || !mod_has(mod, in.inf)) &&
-                                                   (in.ii < 0 || !mod_has(mod, in.ii))));
+                                                   (in.ii < 0 || !mod_has(mod, in.ii)) &&
+                                                   (in.iii < 0 || !mod_has(mod, in.iii))));
             }
             if (!fresh) load_rm(s, p, r, in);
             if (!in.go) continue;
@@ -862,10 +884,13 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
             tracked = mod_add(mod, tinfront) && tracked;
             tracked = mod_add(mod, r) && tracked;
             tracked = mod_add(mod, inf) && tracked;
-            tracked = loop_killer_rec(s, r, n, mod) && tracked;
-            tg.remove[tk] += 1;
-            if (s.infront[r] != 0) {
-                const int b = s.behind[r] - 1;
+            ++nrm;
+            // loop_killer(r): it clears r's loop when that has fewer than 4
+            // points, and otherwise changes nothing
+            const bool small = rm_loop_small(r, in);
+            if (small) tracked = loop_killer_rec(s, r, n, mod) && tracked;
+            if (!small) {
+                const int b = in.beh;
                 if (b > r && !dirty_push(d, b)) {
                     atomicOr(&ctl->status, NVK_CAPACITY);
                     stop_sh = 1;
@@ -874,6 +899,7 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
             }
         }
     }
+    if (tid == 0 && nrm) tg.remove[tk] += nrm;
 }
 
 // ------------------------------------------------------ brute close search
