@@ -187,7 +187,9 @@ int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.
 int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */
 
 /* nsteps iterations of the src/run.f90:229-270 loop body (diagnostics and
- * file output excluded), state kept on the device throughout. */
+ * file output excluded), state kept on the device throughout.  On the
+ * octree path a fatal condition raised by a phase after pinsert is returned
+ * at the end of the step in which it arose. */
 int nv_step(void *ctx, int32_t nsteps);
 
 /* outputs ---------------------------------------------------------------- */
