@@ -545,13 +545,12 @@ int velocity_impl(Ctx *c, int do_ab) {
         if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pmotion: tree velocity without construct_tree");
         // one chunk per image: partial block [images][3][rows] at offset 0
         const int nimg = c->sh.n;
-        const long long zero = 0;
         ++c->layout_gen;   // rewrites d_S / d_poff that recorded steps read
         HIPCHK(c, c->d_S.reserve(1));
         HIPCHK(c, c->d_poff.reserve(1));
         HIPCHK(c, c->part.reserve((size_t)nimg * 3 * R));
-        HIPCHK(c, hipMemcpyAsync(c->d_S.p, &nimg, sizeof(int), hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->d_poff.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, nv_fill_int(c->st, c->d_S.p, nimg, 1));
+        HIPCHK(c, nv_fill_int(c->st, (int *)c->d_poff.p, 0, 2));   // one long long
         HIPCHK(c, hipEventRecord(c->e2, c->st));
         const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
         int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision,
