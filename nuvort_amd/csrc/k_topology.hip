@@ -326,6 +326,18 @@ __device__ __forceinline__ int lower_index(const int *v, int lo, int hi, int key
     return lo;
 }
 
+// Index of row v among the staged rows not yet visited, strow[k, m)
+// (ascending), or m if it is not one of them: most rows looked up lie
+// before the next staged row or after the last, which two reads settle.
+__device__ __forceinline__ int staged_index(const int *strow, int k, int m, int v) {
+    if (k >= m) return m;
+    const int first = strow[k];
+    if (v <= first) return v == first ? k : m;
+    if (v > strow[m - 1]) return m;
+    const int q = lower_index(strow, k + 1, m, v);
+    return (q < m && strow[q] == v) ? q : m;
+}
+
 // After an insertion at r (old infront fi, behind bb, new point in slot):
 // the staged inputs of later candidates fi and bb take the new state in
 // place (fi: ghosts and behind; bb: its infront's infront is the new
@@ -339,7 +351,7 @@ __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn 
     // the staged rows are ascending and those not yet visited are above r:
     // find fi and bb among them (usually fi is the next one; either can be
     // far from r, e.g. a new slot); their plans are then stale
-    const int qf = fi <= r ? m : (k < m && strow[k] == fi) ? k : lower_index(strow, k, m, fi);
+    const int qf = fi <= r ? m : staged_index(strow, k, m, fi);
     if (qf < m && strow[qf] == fi) {
         stale[qf] = 1;
 #pragma unroll
@@ -350,7 +362,7 @@ __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn 
         }
         st[qf].bb = slot;
     }
-    const int qb = bb <= r ? m : lower_index(strow, k, m, bb);
+    const int qb = bb <= r ? m : staged_index(strow, k, m, bb);
     if (qb < m && strow[qb] == bb) {
         stale[qb] = 1;
 #pragma unroll
