@@ -445,10 +445,17 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 const bool staged = rc == r;
                 if (rc == r) ++k;
                 if (rd == r) ++d.hd;
-                if (staged) {
+                if (staged && stale[k - 1]) {
                     in = st[k - 1];
-                    if (stale[k - 1]) plan_insertion(p, in, plan);
-                    else plan = pl[k - 1];
+                    plan_insertion(p, in, plan);
+                } else if (staged) {
+                    // a planned visit: insert_at and after_insert read only
+                    // these of its inputs
+                    in.fi = st[k - 1].fi;
+                    in.bb = st[k - 1].bb;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) in.xf[c] = st[k - 1].xf[c];
+                    plan = pl[k - 1];
                 } else {
                     load_insertion(s, r, in);
                     plan_insertion(p, in, plan);
@@ -668,7 +675,7 @@ constexpr int REC_PARALLEL_MAX = 8;
 // staged index kk, by the serial thread: from the staged inputs unless one
 // of the visit's rows was modified (then from memory), and the reconnection
 // when the tests pass.  Returns whether the modified set is still complete.
-__device__ bool precon_visit(const DState &s, const Phys &p, int *recon, int tk, int base, int n, int kk,
+__device__ bool precon_visit(const DState &s, const Phys &p, int &nrec, int tk, int base, int n, int kk,
                              const int *strow, const ReconIn *st, ModSet &mod, bool tracked, DCtl *ctl,
                              nv_recon_event *ev, int ev_cap, int itime, double t) {
     const int r = strow[kk];
@@ -683,7 +690,7 @@ __device__ bool precon_visit(const DState &s, const Phys &p, int *recon, int tk,
     }
     if (!in.go) return tracked;
     const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
-    recon[tk] += 1;
+    ++nrec;
     const int q = atomicAdd(&ctl->nevents, 1);
     if (q < ev_cap) {
         nv_recon_event e;
@@ -722,6 +729,7 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
     const int n = tg.n[tk];
+    int nrec = 0;   // reconnections (the serial thread's)
     for (int cs = c0; cs < c1; cs += REC_CHUNK) {
         const int m = min(REC_CHUNK, c1 - cs);
         __syncthreads();
@@ -745,7 +753,7 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
             if (tid == 0) {
                 bool tracked = true;
                 for (int kk = 0; kk < m; ++kk)
-                    tracked = precon_visit(s, p, tg.recon, tk, base, n, kk, strow, st, mod, tracked, ctl, ev, ev_cap,
+                    tracked = precon_visit(s, p, nrec, tk, base, n, kk, strow, st, mod, tracked, ctl, ev, ev_cap,
                                            itime, t);
             }
             continue;
@@ -772,7 +780,7 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
             if (kk >= m) break;
             if (tid == 0) {
                 nk[par ^ 1] = m;
-                tracked_sh = precon_visit(s, p, tg.recon, tk, base, n, kk, strow, st, mod, tracked_sh != 0, ctl, ev,
+                tracked_sh = precon_visit(s, p, nrec, tk, base, n, kk, strow, st, mod, tracked_sh != 0, ctl, ev,
                                           ev_cap, itime, t) ? 1 : 0;
             }
             k = kk + 1;
@@ -780,6 +788,7 @@ __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, T
             __syncthreads();
         }
     }
+    if (tid == 0 && nrec) tg.recon[tk] += nrec;
 }
 
 // ---------------------------------------------------------------- premove
