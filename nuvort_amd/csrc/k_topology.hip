@@ -664,8 +664,8 @@ __device__ void recon_tests(const Phys &p, ReconIn &in) {
     in.go = in.la <= in.lb;
 }
 
-constexpr int REC_CHUNK = 128;
-constexpr int REC_PARALLEL_MAX = 8;
+constexpr int REC_CHUNK = 256;
+constexpr int REC_PARALLEL_MAX = 16;
 
 // One workgroup per tangle: candidates staged REC_CHUNK at a time (every
 // thread loads one), thread 0 visits them in row order.  A staged visit is
