@@ -810,10 +810,17 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         if (r[t] < n) {
+            // image 0 (the unshifted walk, near field included) in FP64, the
+            // other images' partials in FP32 after its block
             const double k4 = kappa / (4.0 * NV_PI);
-            double *pp = part + (size_t)m * 3 * npad;
+            if (m == 0) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pp[k * npad + r[t]] = fa[t][k] * k4 + nr[t][k];
+                for (int k = 0; k < 3; ++k) part[k * npad + r[t]] = fa[t][k] * k4 + nr[t][k];
+            } else {
+                float *pp = (float *)(part + (size_t)3 * npad) + (size_t)(m - 1) * 3 * npad;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pp[k * npad + r[t]] = (float)(fa[t][k] * k4 + nr[t][k]);
+            }
         }
     }
     }
@@ -1494,6 +1501,10 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     }
     TCHK(hipGetLastError());
     return NV_OK;
+}
+
+int nv_tree_walk_partial_mode(int precision, double theta) {
+    return precision == NV_PREC_FP32C && theta <= 1.1 ? 4 : 3;
 }
 
 int nv_tree_walk_record_bytes(int precision, double theta) {

@@ -512,7 +512,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
 // mode 0: partials hold (acc0, accs) of k_bs64; mode 1: (far, near) of
 // k_bs32c; mode 2: no partials (LIA); mode 3: the tree walks' [image][3][rows],
 // image 0 the unshifted sum, the others the images' (added in image order,
-// the same additions of the same values as mode 0 with the zero halves).
+// the same additions of the same values as mode 0 with the zero halves);
+// mode 4: the FP32 walk's, image 0 as in mode 3 and images 1.. as floats
+// ([image - 1][3][rows] after image 0's block).
 __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, const Tangles &tg, int rows,
                                             const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -549,6 +551,16 @@ __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, cons
 #pragma unroll
                 for (int k = 0; k < 3; ++k) ubs[k] = ubs[k] + pp[k * capk + loc];
             }
+        } else if (mode == 4) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ub0[k] = 0.0 + pb[k * capk + loc];
+            const float *pf = (const float *)(pb + (size_t)3 * capk);
+#pragma unroll 9
+            for (int c = 1; c < S; ++c) {
+                const float *pp = pf + (size_t)(c - 1) * 3 * capk;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ubs[k] = ubs[k] + (double)pp[k * capk + loc];
+            }
         } else if (mode == 0) {
             // unrolled so that the loads of several chunks are in flight; the
             // adds stay in chunk order
@@ -584,7 +596,7 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
     finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
 }
 
-// The tree walks' layout (mode 3, 27 chunks of 3 doubles) at eight waves per
+// The tree walks' layouts (modes 3 and 4, 27 chunks of 3 values) at eight waves per
 // SIMD (64 VGPRs, no spills): a point streams about 0.9 KB through 36
 // component arrays, and more waves hide the memory latency (389 -> 298 us at
 // 1 M points against the compiler's 110 VGPRs and 4 waves). The chunked
@@ -723,7 +735,7 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl) {
     if (rows <= 0) return hipSuccess;
-    if (mode == 3)
+    if (mode == 3 || mode == 4)
         hipLaunchKernelGGL(k_finish_tree, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
                            do_ab, ctl);
     else

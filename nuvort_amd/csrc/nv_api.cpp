@@ -558,7 +558,7 @@ int velocity_impl(Ctx *c, int do_ab) {
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
         HIPCHK(c, hipEventRecord(c->e3, c->st));
         timed_walk = true;
-        mode = 3;
+        mode = nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta);
     } else {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
         int nitems = 0;
@@ -731,7 +731,8 @@ int enqueue_tree_pmotion(Ctx *c) {
     if (int rc = nv_tree_walk_counters_async(c->tree, st)) return fail(c, rc, "tree counters");
     HIPCHK(c, nv_fill_int(st, &c->ctl->nvel, 0, 1));
     HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
-    HIPCHK(c, nvk_finish(st, c->d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p, 3, 1, c->ctl));
+    HIPCHK(c, nvk_finish(st, c->d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p,
+                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl));
     HIPCHK(c, hipEventRecord(c->e1, st));
     swap_x(c);
     return NV_OK;

@@ -15,6 +15,9 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
                           int precision, double rn2, double *part, DCtl *ctl, nv_stats *stats);
 // bytes of one walk record read by the velocity walk of that precision
 int nv_tree_walk_record_bytes(int precision, double theta);
+// the layout of its per-image partials, as nvk_finish's mode: 4 for the FP32
+// walk (images other than the unshifted one as floats), 3 for the FP64 walk
+int nv_tree_walk_partial_mode(int precision, double theta);
 int nv_tree_walk_counters(TreeBuf *t, hipStream_t st, nv_stats *stats);
 int nv_tree_walk_counters_async(TreeBuf *t, hipStream_t st);
 void nv_tree_walk_counters_take(const TreeBuf *t, nv_stats *stats);
