@@ -73,7 +73,7 @@ def test_parameter_checks(kw, code):
 
 def test_timestep_bound_matches_reference_formula():
     p = RunParams(delta=5e-4, corea=1.3e-8)
-    # BASELINE config 1 (dt = 1e-5) is above the bound and needs the override
+    # the 16.7k-point benchmark run's dt = 1e-5 is above the bound and needs the override
     assert 7.0e-6 < dt_max(p) < 7.4e-6
 
 
