@@ -5,7 +5,7 @@
  * is a set of argument-less module procedures that read and write module
  * globals (the point array f(:) of src/cdata.f90:31, run.in parameters of
  * src/cdata.f90:76-127, the octree vtree of src/tree.f90:34) and are called
- * in a fixed order from the main loop (src/run.f90:229-270).  Each entry
+ * in a fixed order from the main loop (src/run.f90:24-65).  Each entry
  * point below replaces one of those procedures; the citation on each says
  * which.  A caller (the Fortran shim nuvort_amd/fortran/nuvort_hip.f90, or
  * ctypes) keeps the reference's call sequence and only swaps the callee.
@@ -51,7 +51,11 @@ enum {
 
 enum { NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2 };
 enum { NV_BC_OPEN = 0, NV_BC_PERIODIC = 1 };
-enum { NV_NF_ZERO = 0, NV_NF_XFLOW = 1 };
+/* normal_velocity models of src/normal_fluid.f90:83-118 that have a closed
+ * form or use the noise table (the DNS fields laizetDNS / turbDNS need data
+ * files the reference does not ship) */
+enum { NV_NF_ZERO = 0, NV_NF_XFLOW = 1, NV_NF_NOISEY_XFLOW = 2, NV_NF_SHEAR_XFLOW = 3, NV_NF_YCHANNEL = 4,
+       NV_NF_CHANNEL = 5 };
 enum { NV_PREC_FP64 = 0, NV_PREC_FP32C = 1 };
 enum { NV_CLOSE_AUTO = 0, NV_CLOSE_BRUTE = 1, NV_CLOSE_TREE = 2 };
 
@@ -73,6 +77,7 @@ typedef struct {
     double super_velocity[3];    /* run.in super_velocity */
     double normal_fluid_cutoff;  /* run.in normal_fluid_cutoff (default 1e8) */
     double t_zero_normal_fluid;  /* run.in t_zero_normal_fluid (default 1e8) */
+    double norm_shear_omega;     /* run.in norm_shear_omega (default 0), shear_xflow */
     int32_t recon_shots;         /* run.in recon_shots (default 1) */
     int32_t overide_timestep_check; /* run.in overide_timestep_check */
     /* extensions (no reference counterpart) */
@@ -164,6 +169,12 @@ int nv_set_state(void *ctx, const nv_state *host);
 /* Download into caller arrays; on NV_ERR_CAPACITY host->pcount holds the rows needed. */
 int nv_get_state(void *ctx, nv_state *host);
 int nv_set_time(void *ctx, double t, int32_t itime);
+/* The noise table of normal_velocity noisey_xflow (replaces load_noise,
+ * src/normal_fluid.f90:156-162, which reads 100000 values from noise_file):
+ * the library copies the n values and restarts the table's counter at its
+ * first entry.  pmotion advances the counter by one per call and wraps it
+ * after the last entry (randomise_normal_fluid, src/normal_fluid.f90:340-347). */
+int nv_set_noise(void *ctx, int32_t n, const double *noise);
 /* Batches of independent tangles (no reference counterpart: the ensemble
  * form of the same path).  The arrays hold the tangles one after another,
  * tangle k with pcounts[k] rows and links local to it (1-based); every
@@ -172,7 +183,7 @@ int nv_set_time(void *ctx, double t, int32_t itime);
 int nv_set_state_batch(void *ctx, int32_t ntangles, const int32_t *pcounts, const nv_state *host);
 int nv_get_state_batch(void *ctx, int32_t ntangles, int32_t *pcounts, nv_state *host);
 
-/* phases (in the order src/run.f90:229-270 calls them) -------------------- */
+/* phases (in the order src/run.f90:24-65 calls them) -------------------- */
 int nv_ghostp(void *ctx);           /* src/boundary.f90:10-19 ghostp */
 int nv_construct_tree(void *ctx);   /* src/tree.f90:38-59 construct_tree */
 int nv_pmotion(void *ctx);          /* src/timestep.f90:16-45 pmotion */
@@ -183,10 +194,10 @@ int nv_pclose_tree(void *ctx);      /* src/tree.f90:190-203 pclose_tree */
 int nv_precon(void *ctx);           /* src/reconnection.f90:11-61 precon */
 int nv_premove(void *ctx);          /* src/line.f90:102-130 premove */
 int nv_enforce_boundary(void *ctx); /* src/boundary.f90:91-138 enforce_boundary */
-int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.f90:263-267 */
+int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.f90:58-62 */
 int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */
 
-/* nsteps iterations of the src/run.f90:229-270 loop body (diagnostics and
+/* nsteps iterations of the src/run.f90:24-65 loop body (diagnostics and
  * file output excluded), state kept on the device throughout.  On the
  * octree path a fatal condition raised by a phase after pinsert is returned
  * at the end of the step in which it arose. */

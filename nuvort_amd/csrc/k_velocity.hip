@@ -588,7 +588,8 @@ __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, cons
             for (int k = 0; k < 3; ++k) ub0[k] = nr[k] + far[k];
         }
     }
-    finish_point(s, p, r, ub0, ubs, ctl->t, do_ab);
+    const double noise = (p.nf == NV_NF_NOISEY_XFLOW && p.noise_n > 0) ? p.noise[ctl->noise_ctr] : 0.0;
+    finish_point(s, p, r, ub0, ubs, ctl->t, noise, do_ab);
 }
 
 __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part,
@@ -604,6 +605,14 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_finish_tree(
     DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
     finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
+}
+
+// src/normal_fluid.f90:340-347 randomise_normal_fluid: the noise counter
+// moves to the next value, back to the first after the last
+__global__ void k_noise_tick(DCtl *ctl, int n) {
+    int c = ctl->noise_ctr + 1;
+    if (c >= n) c = 0;
+    ctl->noise_ctr = c;
 }
 
 // Reads every chunk partial of the active rows the way k_finish does, and
@@ -734,12 +743,16 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
 
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl) {
-    if (rows <= 0) return hipSuccess;
-    if (mode == 3 || mode == 4)
-        hipLaunchKernelGGL(k_finish_tree, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
-                           do_ab, ctl);
-    else
-        hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode, do_ab,
-                           ctl);
+    if (rows > 0) {
+        if (mode == 3 || mode == 4)
+            hipLaunchKernelGGL(k_finish_tree, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
+                               do_ab, ctl);
+        else
+            hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
+                               do_ab, ctl);
+    }
+    // the end of pmotion advances the noise table (src/timestep.f90:44)
+    if (do_ab && p.nf == NV_NF_NOISEY_XFLOW && p.noise_n > 0)
+        hipLaunchKernelGGL(k_noise_tick, dim3(1), dim3(1), 0, st, ctl, p.noise_n);
     return hipGetLastError();
 }

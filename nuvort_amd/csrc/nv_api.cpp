@@ -1,5 +1,5 @@
 // Host side of the C-ABI declared in include/nuvort.h: the context, device
-// buffers, and one function per reference procedure (src/run.f90:229-270
+// buffers, and one function per reference procedure (src/run.f90:24-65
 // call order).  A context holds K >= 1 independent tangles (K = 1 for the
 // reference's single filament system); all work is launched on the
 // context's stream and every entry point synchronises before it returns.
@@ -64,6 +64,7 @@ struct Ctx {
     DevBuf<double> rec64, part, pd;
     DevBuf<char> rec32, tinfo, sph;
     DevBuf<int> pj;
+    DevBuf<double> noise;            // noisey_xflow table (nv_set_noise)
     // work lists of the last direct-sum and close-search launches
     DevBuf<Item> items, pitems;
     DevBuf<int> d_S, d_pS, queue;
@@ -358,7 +359,12 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
     HIPCHK(c, hipMemcpy(c->d.closestd, cd.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     c->t = hs->t;
     c->itime = hs->itime;
+    // a new point array; the noise table's counter is not part of it
+    // (module state of normal_fluid, src/normal_fluid.f90:31)
+    int noise_ctr = 0;
+    HIPCHK(c, hipMemcpy(&noise_ctr, &c->ctl->noise_ctr, sizeof(int), hipMemcpyDeviceToHost));
     std::memset(c->ctl_h, 0, sizeof(DCtl));
+    c->ctl_h->noise_ctr = noise_ctr;
     HIPCHK(c, hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice));
     return NV_OK;
 }
@@ -530,8 +536,15 @@ int64_t pair_evals(const Ctx *c, int64_t nv, const std::vector<int> &tn) {
     return pe;
 }
 
+// noisey_xflow reads its table in every pmotion that applies friction
+bool noise_missing(const Ctx *c) {
+    return c->prm.normal_velocity == NV_NF_NOISEY_XFLOW && !c->noise.p &&
+           (std::fabs(c->prm.alpha[0]) > NV_EPS || std::fabs(c->prm.alpha[1]) > NV_EPS);
+}
+
 int velocity_impl(Ctx *c, int do_ab) {
     if (c->K == 0) return NV_OK;
+    if (noise_missing(c)) return fail(c, NV_ERR_STATE, "normal_velocity noisey_xflow without a noise table (nv_set_noise)");
     const int R = c->rows;
     HIPCHK(c, hipEventRecord(c->e0, c->st));
     int mode = 0;
@@ -919,6 +932,9 @@ int set_phys(Ctx *c) {
     f.nf = p.normal_velocity; f.xflow = p.norm_vel_xflow;
     f.alpha[0] = p.alpha[0]; f.alpha[1] = p.alpha[1];
     f.nf_cutoff = p.normal_fluid_cutoff; f.t_zero = p.t_zero_normal_fluid;
+    f.shear_omega = p.norm_shear_omega;
+    f.noise = c->noise.p;
+    f.noise_n = c->noise.p ? (int)c->noise.n : 0;
     // src/boundary.f90:143-175 create_periodic_loop_array; image 0 = no shift
     Shifts &s = c->sh;
     s.s[0][0] = s.s[0][1] = s.s[0][2] = 0.0;
@@ -964,8 +980,9 @@ int nv_create(const nv_params *params, void **out) {
         rc = fail(c, NV_ERR_UNSUPPORTED, "only periodic and open boundaries are supported");
     else if (p.tree_theta > 0 && (p.boundary[0] == 0 || p.boundary[1] == 0 || p.boundary[2] == 0))
         rc = fail(c, NV_ERR_ARG, "init.mod:init_setup: tree approximation cannot run with open boundaries");
-    else if (p.normal_velocity < 0 || p.normal_velocity > 1)
-        rc = fail(c, NV_ERR_UNSUPPORTED, "normal_velocity must be zero or xflow");
+    else if (p.normal_velocity < NV_NF_ZERO || p.normal_velocity > NV_NF_CHANNEL)
+        rc = fail(c, NV_ERR_UNSUPPORTED,
+                  "normal_velocity must be zero, xflow, noisey_xflow, shear_xflow, ychannel or channel");
     else if (p.precision < 0 || p.precision > 1) rc = fail(c, NV_ERR_ARG, "precision must be FP64 or FP32C");
     else if (p.velocity == NV_VEL_TREE && !(p.tree_theta > 0))
         rc = fail(c, NV_ERR_ARG, "velocity Tree needs tree_theta > 0");
@@ -1080,6 +1097,22 @@ int nv_set_time(void *h, double t, int32_t itime) {
     if (!c) return NV_ERR_ARG;
     c->t = t;
     c->itime = itime;
+    return NV_OK;
+}
+
+int nv_set_noise(void *h, int32_t n, const double *noise) {
+    Ctx *c = (Ctx *)h;
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || !noise) return fail(c, NV_ERR_ARG, "nv_set_noise: the table needs at least one value");
+    for (int32_t q = 0; q < n; ++q)
+        if (std::isnan(noise[q])) return fail(c, NV_ERR_ARG, "nv_set_noise: NaN in the noise table");
+    c->noise.release();
+    HIPCHK(c, c->noise.reserve((size_t)n));
+    HIPCHK(c, hipMemcpy(c->noise.p, noise, sizeof(double) * n, hipMemcpyHostToDevice));
+    const int zero = 0;
+    HIPCHK(c, hipMemcpy(&c->ctl->noise_ctr, &zero, sizeof(int), hipMemcpyHostToDevice));
+    set_phys(c);
+    ++c->layout_gen;   // recorded steps hold the old table's address
     return NV_OK;
 }
 
@@ -1219,6 +1252,7 @@ int nv_step(void *h, int32_t nsteps) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     const int total = nsteps;
+    if (noise_missing(c)) return fail(c, NV_ERR_STATE, "normal_velocity noisey_xflow without a noise table (nv_set_noise)");
     hipEvent_t s0, s1;
     HIPCHK(c, hipEventCreate(&s0));
     HIPCHK(c, hipEventCreate(&s1));

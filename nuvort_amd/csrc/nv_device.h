@@ -52,11 +52,15 @@ struct Phys {
     int velocity;           // 0 LIA, 1 BS, 2 Tree
     double tree_theta;
     int extra_corr;
-    int nf;                 // 0 zero, 1 xflow
+    int nf;                 // normal_velocity: NV_NF_* (0 zero, 1 xflow, 2 noisey_xflow, 3 shear_xflow,
+                            // 4 ychannel, 5 channel)
     double xflow;
     double alpha[2];
     double sup[3];
     double nf_cutoff, t_zero;
+    double shear_omega;     // norm_shear_omega
+    const double *noise;    // noisey_xflow table (device), noise_n values
+    int noise_n;
 };
 
 struct Shifts {
@@ -130,6 +134,41 @@ __device__ __forceinline__ bool bs_term64(double d0, double d1, double d2, doubl
     return true;
 }
 
+// src/general.f90:46-51 cosh, which the channel model below calls (the
+// module function hides the intrinsic): 0.5*(exp(x)+exp(-x))
+__device__ __forceinline__ double ref_cosh(double x) { return 0.5 * (exp(x) + exp(-x)); }
+
+// src/normal_fluid.f90:83-118 get_normal_velocity at position x and time t
+// for the closed-form models; `noise` is the noisey_xflow table's current
+// value.  Each expression keeps the Fortran line's evaluation order.
+__device__ __forceinline__ void normal_velocity(const Phys &p, const double x[3], double t, double noise,
+                                                double un[3]) {
+    un[0] = un[1] = un[2] = 0.0;
+    if (p.nf == 1) {
+        un[0] = p.xflow;                                                       // :96-97
+    } else if (p.nf == 2) {
+        un[0] = noise * p.xflow;                                               // :98-99
+    } else if (p.nf == 3) {
+        un[0] = p.xflow * sin(2.0 * NV_PI * x[2] / p.box[2] + t * p.shear_omega);   // :101-102
+    } else if (p.nf == 4) {
+        const double q = 2.0 * x[1] / p.box[1];
+        un[0] = p.xflow * (1.0 - q * q);                                       // :103-104
+    } else if (p.nf == 5) {
+        double cossum = 0.0;                                                   // :105-112
+        for (int ii = 1; ii <= 10; ++ii) {
+            const double ak = (double)(2 * ii - 1) * (NV_PI / 2.0);
+            const double sgn = (ii & 1) ? -1.0 : 1.0;                          // (-1)**ii
+            cossum = cossum + 4.0 * (sgn / (ak * ak * ak)) *
+                                  (ref_cosh(ak * 2.0 * x[1] / p.box[1]) * cos(ak * 2.0 * x[2] / p.box[2])) /
+                                  ref_cosh(ak);
+        }
+        const double q = 2.0 * x[2] / p.box[2];
+        un[0] = p.xflow * (1.0 - q * q + cossum) * 1.2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) un[k] = un[k] + p.sup[k];                      // :117
+}
+
 // AB coefficients, src/cdata.f90:67-71
 #define NV_ONE_HALF (1.0 / 2.0)
 #define NV_THREE_TWOS (3.0 / 2.0)
@@ -140,10 +179,11 @@ __device__ __forceinline__ bool bs_term64(double d0, double d1, double d2, doubl
 // Velocity epilogue of one active point: local term (src/timestep.f90:60-80,
 // 102-105), the non-local sums ub0 (unshifted) and ubs (images), superfluid
 // velocity store (:123), mutual friction (:126-139, src/normal_fluid.f90:83-118),
-// imposed flow (:140); then, if do_ab, the Adams-Bashforth update of
+// imposed flow (:140); `noise` is the noisey_xflow table's value for this
+// call of pmotion; then, if do_ab, the Adams-Bashforth update of
 // src/timestep.f90:28-43 into xn.
 __device__ __forceinline__ void finish_point(const DState &s, const Phys &p, int r, const double ub0[3],
-                                             const double ubs[3], double t, int do_ab) {
+                                             const double ubs[3], double t, double noise, int do_ab) {
     const int cap = s.cap;
     double x[3], gi[3], gb[3], fd[3], fdd[3], c[3], u[3];
 #pragma unroll
@@ -177,11 +217,7 @@ __device__ __forceinline__ void finish_point(const DState &s, const Phys &p, int
     if ((fabs(p.alpha[0]) > NV_EPS) || (fabs(p.alpha[1]) > NV_EPS)) {
         if (t < p.nf_cutoff) {
             double un[3] = {0.0, 0.0, 0.0};
-            if (t < p.t_zero) {
-                if (p.nf == 1) un[0] = p.xflow;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) un[k] = un[k] + p.sup[k];
-            }
+            if (t < p.t_zero) normal_velocity(p, x, t, noise, un);
             double dv[3], c1[3], c2[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) dv[k] = un[k] - u[k];

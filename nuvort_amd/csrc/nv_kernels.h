@@ -24,7 +24,8 @@ struct DCtl {
     int itime;         // the step being taken (src/run.f90 loop index)
     double t;          // simulation time
     unsigned long long nvel_total;  // nvel summed over the steps k_clock_tick closed
-    int pad[2];
+    int noise_ctr;     // noisey_xflow: index of the noise table's current value (0-based)
+    int pad;
 };
 
 // velocity (k_velocity.hip)

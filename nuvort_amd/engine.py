@@ -2,7 +2,7 @@
 
 ``FilamentSolver`` keeps the filament state on the GPU and exposes one method
 per reference routine, with the reference's names and call order
-(src/run.f90:229-270): ``ghostp``, ``construct_tree``, ``pmotion``,
+(src/run.f90:24-65): ``ghostp``, ``construct_tree``, ``pmotion``,
 ``pinsert``, ``pclose`` / ``pclose_tree``, ``precon``, ``premove``,
 ``enforce_boundary``, ``empty_tree``, ``nan_finder``; ``step(k)`` runs k
 iterations of the main-loop body on the device.  Errors raise
@@ -19,6 +19,20 @@ from .runfile import RunParams, validate
 from .state import FilamentState
 
 
+# normal_velocity names (src/normal_fluid.f90:94-115) -> NV_NF_*
+NF_MODELS = {"zero": 0, "xflow": 1, "noisey_xflow": 2, "shear_xflow": 3, "ychannel": 4, "channel": 5}
+
+
+def read_noise(path: str) -> np.ndarray:
+    """The noise table of noisey_xflow: list-directed reals, 100000 of them
+    (src/normal_fluid.f90:156-162 load_noise reads exactly that many)."""
+    with open(path) as fh:
+        vals = np.array(fh.read().replace(",", " ").split(), dtype=np.float64)
+    if vals.size < 100000:
+        raise ValueError("%s holds %d values, load_noise reads 100000" % (path, vals.size))
+    return vals[:100000].copy()
+
+
 def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.NvParams:
     s = _L.NvParams()
     s.delta, s.dt, s.quant_circ, s.corea = p.delta, p.dt, p.quant_circ, p.corea
@@ -30,10 +44,11 @@ def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.
     s.velocity = {"LIA": 0, "BS": 1, "Tree": 2}[p.velocity]
     s.tree_theta = p.tree_theta
     s.tree_extra_correction = 1 if p.tree_extra_correction else 0
-    s.normal_velocity = {"zero": 0, "xflow": 1}.get(p.normal_velocity, 9)
+    s.normal_velocity = NF_MODELS.get(p.normal_velocity, 9)
     s.norm_vel_xflow = p.norm_vel_xflow
     s.alpha[0], s.alpha[1] = p.alpha
     s.normal_fluid_cutoff, s.t_zero_normal_fluid = p.normal_fluid_cutoff, p.t_zero_normal_fluid
+    s.norm_shear_omega = p.norm_shear_omega
     s.recon_shots = p.recon_shots
     s.overide_timestep_check = 1 if p.overide_timestep_check else 0
     s.precision = {"fp64": 0, "fp32c": 1}[p.hip_precision]
@@ -147,6 +162,13 @@ class FilamentSolver:
 
     def set_time(self, t: float, itime: int) -> None:
         self._call("nv_set_time", C.c_double(t), C.c_int32(itime))
+
+    def set_noise(self, noise) -> None:
+        """The noisey_xflow table (load_noise); its counter restarts at the
+        first value."""
+        a = np.ascontiguousarray(noise, dtype=np.float64)
+        self._noise = a
+        self._call("nv_set_noise", C.c_int32(a.size), a.ctypes.data_as(_L._dp))
 
     # ------------------------------------------------ reference procedures
     def ghostp(self):

@@ -1,6 +1,6 @@
 !> iso_c_binding interface to the HIP library (include/nuvort.h).
 !>
-!> The reference's hot-path procedures (src/run.f90:229-270 call sequence:
+!> The reference's hot-path procedures (src/run.f90:24-65 call sequence:
 !> ghostp, construct_tree, pmotion, pinsert, pclose / pclose_tree, precon,
 !> premove, enforce_boundary, empty_tree, NAN_finder) each have one nv_*
 !> counterpart here.  Every nv_* returns an integer status (NV_OK = 0); the
@@ -18,7 +18,8 @@ module nuvort_hip
   integer(c_int), parameter :: NV_OK = 0
   integer(c_int), parameter :: NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2
   integer(c_int), parameter :: NV_BC_OPEN = 0, NV_BC_PERIODIC = 1
-  integer(c_int), parameter :: NV_NF_ZERO = 0, NV_NF_XFLOW = 1
+  integer(c_int), parameter :: NV_NF_ZERO = 0, NV_NF_XFLOW = 1, NV_NF_NOISEY_XFLOW = 2, NV_NF_SHEAR_XFLOW = 3
+  integer(c_int), parameter :: NV_NF_YCHANNEL = 4, NV_NF_CHANNEL = 5
   integer(c_int), parameter :: NV_PREC_FP64 = 0, NV_PREC_FP32C = 1
   integer(c_int), parameter :: NV_CLOSE_AUTO = 0, NV_CLOSE_BRUTE = 1, NV_CLOSE_TREE = 2
 
@@ -34,6 +35,7 @@ module nuvort_hip
     real(c_double) :: alpha(2)
     real(c_double) :: super_velocity(3)
     real(c_double) :: normal_fluid_cutoff, t_zero_normal_fluid
+    real(c_double) :: norm_shear_omega
     integer(c_int32_t) :: recon_shots
     integer(c_int32_t) :: overide_timestep_check
     integer(c_int32_t) :: precision
@@ -85,6 +87,13 @@ module nuvort_hip
       type(c_ptr), value :: ctx
       real(c_double), value :: t
       integer(c_int32_t), value :: itime
+    end function
+    !> the noisey_xflow table that load_noise reads (src/normal_fluid.f90:156-162)
+    integer(c_int) function nv_set_noise(ctx, n, noise) bind(C, name='nv_set_noise')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n
+      real(c_double), dimension(*), intent(in) :: noise
     end function
     integer(c_int) function nv_ghostp(ctx) bind(C, name='nv_ghostp')
       import :: c_ptr, c_int

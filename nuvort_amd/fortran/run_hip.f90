@@ -1,4 +1,4 @@
-!> A Fortran host that runs the reference's main loop (src/run.f90:229-270)
+!> A Fortran host that runs the reference's main loop (src/run.f90:24-65)
 !> with every hot-path procedure replaced by its HIP C-ABI counterpart.
 !>
 !> usage: run_hip <state_in> <state_out> <nsteps>
@@ -27,16 +27,26 @@ program run_hip
   real(c_double), allocatable, target :: closestd(:)
   integer(c_int32_t), allocatable, target :: infront(:), behind(:), closest(:), pinnedi(:), pinnedb(:), wpinned(:,:)
   type(nv_state) :: st
+  character(len=30) :: noise_file
+  real(c_double), allocatable :: noise(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   call get_command_argument(3, arg) ; read(arg,*) nrun
 
-  call read_params(prm, recon_shots, dt, tree_theta)
+  call read_params(prm, recon_shots, dt, tree_theta, noise_file)
   rc = nv_create(prm, ctx)
   if (rc /= NV_OK) then
     write (*,*) 'nv_create rejected the run.in parameters, status ', rc
     stop 1
+  end if
+  if (prm%normal_velocity == NV_NF_NOISEY_XFLOW) then
+    ! load_noise (src/normal_fluid.f90:156-162): 100000 list-directed values
+    allocate(noise(100000))
+    open(unit=37, file=trim(noise_file), status='old')
+    read(37,*) noise
+    close(37)
+    call nv_check(ctx, nv_set_noise(ctx, int(size(noise), c_int32_t), noise), 'normal_fluid.mod:load_noise')
   end if
   call load_state(trim(fin))
   call bind_state(n)
@@ -80,22 +90,24 @@ program run_hip
 contains
 
   !> the reference's run.in semantics for the keys this library reads
-  subroutine read_params(p, rshots, dt_out, theta_out)
+  subroutine read_params(p, rshots, dt_out, theta_out, nfile)
     type(nv_params), intent(out) :: p
     integer, intent(out) :: rshots
     real(c_double), intent(out) :: dt_out, theta_out
+    character(len=30), intent(out) :: nfile
     character(len=100) :: buffer, label
     character(len=40) :: bx, by, bz
     character(len=30) :: vel, nf, prec
     integer :: pos, ios, fh
     logical :: tec, otc
     real(c_double) :: delta, dtv, kappa, corea, box(3), theta, xflow, alpha(2), sup(3), cutoff, tzero, nearr
+    real(c_double) :: omega
     ! defaults of src/cdata.f90:76-127
     delta = 0 ; dtv = 0 ; kappa = 9.97E-4_c_double ; corea = 8.244023E-9_c_double ; box = 0
     bx = 'periodic' ; by = 'periodic' ; bz = 'periodic' ; vel = ' ' ; nf = 'zero'
     theta = 0 ; tec = .true. ; xflow = 0.5_c_double ; alpha = 0 ; sup = 0
     cutoff = 1E8_c_double ; tzero = 1E8_c_double ; rshots = 1 ; otc = .false.
-    prec = 'fp64' ; nearr = 8
+    prec = 'fp64' ; nearr = 8 ; omega = 0 ; nfile = 'set_me!'
     ios = 0
     open(newunit=fh, file='run.in', status='old')
     do while (ios == 0)
@@ -121,6 +133,8 @@ contains
       case ('normal_fluid_cutoff') ; read(buffer, *, iostat=ios) cutoff
       case ('t_zero_normal_fluid') ; read(buffer, *, iostat=ios) tzero
       case ('norm_vel_xflow') ; read(buffer, *, iostat=ios) xflow
+      case ('norm_shear_omega') ; read(buffer, *, iostat=ios) omega
+      case ('noise_file') ; read(buffer, *, iostat=ios) nfile
       case ('alpha') ; read(buffer, *, iostat=ios) alpha
       case ('super_velocity') ; read(buffer, *, iostat=ios) sup
       case ('overide_timestep_check') ; read(buffer, *, iostat=ios) otc
@@ -142,7 +156,16 @@ contains
     end select
     p%tree_theta = theta
     p%tree_extra_correction = merge(1, 0, tec)
-    p%normal_velocity = merge(NV_NF_XFLOW, merge(NV_NF_ZERO, 9, nf == 'zero'), nf == 'xflow')
+    select case (nf)
+    case ('zero') ; p%normal_velocity = NV_NF_ZERO
+    case ('xflow') ; p%normal_velocity = NV_NF_XFLOW
+    case ('noisey_xflow') ; p%normal_velocity = NV_NF_NOISEY_XFLOW
+    case ('shear_xflow') ; p%normal_velocity = NV_NF_SHEAR_XFLOW
+    case ('ychannel') ; p%normal_velocity = NV_NF_YCHANNEL
+    case ('channel') ; p%normal_velocity = NV_NF_CHANNEL
+    case default ; p%normal_velocity = 9
+    end select
+    p%norm_shear_omega = omega
     p%norm_vel_xflow = xflow ; p%alpha = alpha ; p%super_velocity = sup
     p%normal_fluid_cutoff = cutoff ; p%t_zero_normal_fluid = tzero
     p%recon_shots = rshots

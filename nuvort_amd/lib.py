@@ -27,7 +27,8 @@ class NvParams(C.Structure):
                 ("tree_theta", C.c_double), ("tree_extra_correction", C.c_int32), ("normal_velocity", C.c_int32),
                 ("norm_vel_xflow", C.c_double), ("alpha", C.c_double * 2), ("super_velocity", C.c_double * 3),
                 ("normal_fluid_cutoff", C.c_double), ("t_zero_normal_fluid", C.c_double),
-                ("recon_shots", C.c_int32), ("overide_timestep_check", C.c_int32), ("precision", C.c_int32),
+                ("norm_shear_omega", C.c_double), ("recon_shots", C.c_int32),
+                ("overide_timestep_check", C.c_int32), ("precision", C.c_int32),
                 ("near_radius", C.c_double), ("close_search", C.c_int32), ("exact_order", C.c_int32),
                 ("capacity", C.c_int32), ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32),
                 ("step_launch", C.c_int32)]
@@ -64,7 +65,7 @@ class NvStats(C.Structure):
 
 
 EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_state_batch",
-           "nv_get_state_batch", "nv_set_time", "nv_ghostp",
+           "nv_get_state_batch", "nv_set_time", "nv_set_noise", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
            "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
            "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune")
@@ -89,6 +90,7 @@ def load(path: str = LIB_PATH):
     vp = C.c_void_p
     sig = {
         "nv_create": [C.POINTER(NvParams), C.POINTER(vp)], "nv_destroy": [vp], "nv_set_state": [vp, C.POINTER(NvState)],
+        "nv_set_noise": [vp, C.c_int32, C.POINTER(C.c_double)],
         "nv_get_state": [vp, C.POINTER(NvState)], "nv_set_time": [vp, C.c_double, C.c_int32],
         "nv_set_state_batch": [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(NvState)],
         "nv_get_state_batch": [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(NvState)],

@@ -193,8 +193,11 @@ def validate(p: RunParams) -> None:
         raise RunFileError('init.mod:init_setup: correct value for "velocity" in run.in has not been set')
     if p.tree_theta > 0 and "open" in (p.boundary_x, p.boundary_y, p.boundary_z):
         raise RunFileError("init.mod:init_setup: tree approximation cannot run with open boundaries")
-    if p.normal_velocity not in ("zero", "xflow"):
-        raise RunFileError("normal_velocity %r is not supported by this library" % p.normal_velocity)
+    if p.normal_velocity not in ("zero", "xflow", "noisey_xflow", "shear_xflow", "ychannel", "channel"):
+        if p.normal_velocity in ("laizetDNS", "turbDNS"):
+            raise RunFileError("normal_velocity %r needs a DNS data file and is not supported by this library"
+                               % p.normal_velocity)
+        raise RunFileError("normal_fluid.mod:get_normal_fluid: correct parameter for normal_veloctity not set")
     if p.hip_precision not in ("fp64", "fp32c"):
         raise RunFileError("hip_precision must be fp64 or fp32c")
     if p.hip_step_launch not in ("graph", "eager"):

@@ -123,3 +123,10 @@ def ensemble(seed: int, ntangles: int, nmin: int, nmax: int, box: float = 0.1, d
     sizes = ensemble_sizes(seed, ntangles, nmin, nmax)
     return [random_rings(seed * 1000003 + k, 0, rmin, rmax, box, delta, target_points=int(s))
             for k, s in enumerate(sizes)]
+
+
+def noise_table(seed: int, n: int = 100000, spread: float = 0.2) -> np.ndarray:
+    """A noisey_xflow multiplier table (the role of the reference's
+    noise/wn_snr_*.log files, which hold 100000 values): 1 + spread * N(0, 1),
+    seeded."""
+    return 1.0 + spread * _rng(seed).normal(size=n)

@@ -397,10 +397,46 @@ static void biot_savart(const nvo_state *s, const nvo_params *p, int32_t i, cons
     }
 }
 
-/* src/normal_fluid.f90:83-118 get_normal_velocity ('zero', 'xflow') */
-static void normal_velocity(const nvo_params *p, double *un) {
+/* src/general.f90:46-51: the module's cosh, which hides the intrinsic in
+ * normal_fluid (it `use`s general) */
+static double ref_cosh(double x) { return 0.5 * (exp(x) + exp(-x)); }
+
+/* src/normal_fluid.f90:83-118 get_normal_velocity at position x, time t;
+ * the DNS models (laizetDNS, turbDNS) are not restated */
+static void normal_velocity(const nvo_params *p, const double *x, double t, double *un) {
     un[0] = un[1] = un[2] = 0.0;
-    if (p->normal_velocity == 1) un[0] = p->norm_vel_xflow;
+    switch (p->normal_velocity) {
+    case 1: /* :96-97 xflow */
+        un[0] = p->norm_vel_xflow;
+        break;
+    case 2: /* :98-99 noisey_xflow */
+        un[0] = (p->noise && p->noise_counter ? p->noise[*p->noise_counter - 1] : 0.0) * p->norm_vel_xflow;
+        break;
+    case 3: /* :101-102 shear_xflow */
+        un[0] = p->norm_vel_xflow * sin(2 * PI * x[2] / p->box[2] + t * p->norm_shear_omega);
+        break;
+    case 4: { /* :103-104 ychannel */
+        double q = 2. * x[1] / p->box[1];
+        un[0] = p->norm_vel_xflow * (1. - q * q);
+        break;
+    }
+    case 5: { /* :105-112 channel */
+        double cossum = 0.;
+        for (int ii = 1; ii <= 10; ii++) {
+            double ak = (double)(2 * ii - 1) * (PI / 2);
+            int sgn = (ii % 2) ? -1 : 1; /* (-1)**ii */
+            double ak3 = ak * ak * ak;
+            cossum = cossum + 4 * ((double)sgn / ak3) *
+                                  (ref_cosh(ak * 2 * x[1] / p->box[1]) * cos(ak * 2 * x[2] / p->box[2])) /
+                                  ref_cosh(ak);
+        }
+        double q = 2 * x[2] / p->box[2];
+        un[0] = p->norm_vel_xflow * (1. - q * q + cossum) * 1.2;
+        break;
+    }
+    default:
+        break;
+    }
     for (int k = 0; k < 3; k++) un[k] = un[k] + p->super_velocity[k];
 }
 
@@ -450,7 +486,7 @@ static int calc_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, 
     if ((fabs(p->alpha[0]) > EPS) || (fabs(p->alpha[1]) > EPS)) {
         if (s->t < p->normal_fluid_cutoff) {
             double un[3], dv[3], c1[3], c2[3], mf[3];
-            if (s->t < p->t_zero_normal_fluid) normal_velocity(p, un);
+            if (s->t < p->t_zero_normal_fluid) normal_velocity(p, &X(s, i, 0), s->t, un);
             else un[0] = un[1] = un[2] = 0.0;
             for (int k = 0; k < 3; k++) dv[k] = un[k] - u[k];
             cross(fd, dv, c1);
@@ -532,6 +568,11 @@ int nvo_pmotion(nvo_state *s, const nvo_params *p, const nvo_tree *tr) {
     free(u);
     if (rc) return rc;
     nvo_ab_update(s, p);
+    /* src/normal_fluid.f90:340-347 randomise_normal_fluid (:44) */
+    if (p->normal_velocity == 2 && p->noise_counter) {
+        *p->noise_counter = *p->noise_counter + 1;
+        if (*p->noise_counter > p->noise_n) *p->noise_counter = 1;
+    }
     return NVO_OK;
 }
 
@@ -731,7 +772,7 @@ int nvo_nan_check(const nvo_state *s) {
     return NVO_OK;
 }
 
-/* src/run.f90:229-270: one call = nsteps iterations of the main loop body
+/* src/run.f90:24-65: one call = nsteps iterations of the main loop body
  * (diagnostics and file output excluded). */
 int nvo_step(nvo_state *s, const nvo_params *p, int32_t nsteps, nvo_recon_log *log) {
     nvo_tree tr = {0};

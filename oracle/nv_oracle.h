@@ -2,7 +2,7 @@
  * Test infrastructure only: the oracle.
  *
  * A plain-C, FP64, loop-by-loop restatement of NuVort's filament velocity and
- * timestep path (src/run.f90:229-270 and the routines it calls).  It is the
+ * timestep path (src/run.f90:24-65 and the routines it calls).  It is the
  * checker of the HIP library's parity tests and the CPU baseline timed beside
  * it; nothing in the product links or calls it.  Every function cites the
  * reference lines it restates; evaluation order follows the Fortran source
@@ -47,13 +47,17 @@ typedef struct {
     int32_t velocity;             /* 0 LIA, 1 BS, 2 Tree */
     double tree_theta;
     int32_t tree_extra_correction;
-    int32_t normal_velocity;      /* 0 zero, 1 xflow */
+    int32_t normal_velocity;      /* 0 zero, 1 xflow, 2 noisey_xflow, 3 shear_xflow, 4 ychannel, 5 channel */
     double norm_vel_xflow;
     double alpha[2];
     double super_velocity[3];
     double normal_fluid_cutoff, t_zero_normal_fluid;
     int32_t recon_shots;
     int32_t threads;              /* OpenMP threads, 0 = runtime default */
+    double norm_shear_omega;      /* shear_xflow frequency */
+    const double *noise;          /* noisey_xflow table (noise_n values) */
+    int32_t noise_n;
+    int32_t *noise_counter;       /* 1-based current entry; advanced by every pmotion */
 } nvo_params;
 
 /* One octree node as the reference holds it (src/tree.f90:20-32), stored in

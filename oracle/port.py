@@ -36,7 +36,9 @@ class NvoParams(C.Structure):
                 ("tree_theta", C.c_double), ("tree_extra_correction", C.c_int32),
                 ("normal_velocity", C.c_int32), ("norm_vel_xflow", C.c_double), ("alpha", C.c_double * 2),
                 ("super_velocity", C.c_double * 3), ("normal_fluid_cutoff", C.c_double),
-                ("t_zero_normal_fluid", C.c_double), ("recon_shots", C.c_int32), ("threads", C.c_int32)]
+                ("t_zero_normal_fluid", C.c_double), ("recon_shots", C.c_int32), ("threads", C.c_int32),
+                ("norm_shear_omega", C.c_double), ("noise", _dp), ("noise_n", C.c_int32),
+                ("noise_counter", _ip)]
 
 
 class NvoNode(C.Structure):
@@ -83,7 +85,19 @@ def lib():
     return _lib
 
 
-def params_struct(p: RunParams, threads: int = 0) -> NvoParams:
+NF_MODELS = {"zero": 0, "xflow": 1, "noisey_xflow": 2, "shear_xflow": 3, "ychannel": 4, "channel": 5}
+
+
+class Noise:
+    """The noisey_xflow table and its 1-based counter (normal_fluid's module
+    state, src/normal_fluid.f90:29-31), shared by the calls it is passed to."""
+
+    def __init__(self, values):
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        self.counter = np.ones(1, np.int32)
+
+
+def params_struct(p: RunParams, threads: int = 0, noise: Noise | None = None) -> NvoParams:
     s = NvoParams()
     s.delta, s.dt, s.quant_circ, s.corea = p.delta, p.dt, p.quant_circ, p.corea
     for k in range(3):
@@ -94,7 +108,12 @@ def params_struct(p: RunParams, threads: int = 0) -> NvoParams:
     s.velocity = {"LIA": 0, "BS": 1, "Tree": 2}[p.velocity]
     s.tree_theta = p.tree_theta
     s.tree_extra_correction = 1 if p.tree_extra_correction else 0
-    s.normal_velocity = {"zero": 0, "xflow": 1}[p.normal_velocity]
+    s.normal_velocity = NF_MODELS[p.normal_velocity]
+    s.norm_shear_omega = p.norm_shear_omega
+    if noise is not None:
+        s.noise = noise.values.ctypes.data_as(_dp)
+        s.noise_n = noise.values.size
+        s.noise_counter = noise.counter.ctypes.data_as(_ip)
     s.norm_vel_xflow = p.norm_vel_xflow
     s.alpha[0], s.alpha[1] = p.alpha
     s.normal_fluid_cutoff, s.t_zero_normal_fluid = p.normal_fluid_cutoff, p.t_zero_normal_fluid
@@ -186,19 +205,19 @@ class ReconLog:
         return [(e.itime, e.i, e.j, e.t, tuple(e.mid), e.angle, e.dl) for e in self.buf[:min(self.c.n, self.c.cap)]]
 
 
-def step(st: FilamentState, p: RunParams, nsteps: int, threads: int = 0):
+def step(st: FilamentState, p: RunParams, nsteps: int, threads: int = 0, noise: Noise | None = None):
     ps = PortState(st)
     log = ReconLog()
-    rc = lib().nvo_step(ps.ref(), C.byref(params_struct(p, threads)), nsteps, log.ref())
+    rc = lib().nvo_step(ps.ref(), C.byref(params_struct(p, threads, noise)), nsteps, log.ref())
     return rc, ps.to_state(), log.events()
 
 
-def run_phases(st: FilamentState, p: RunParams, threads: int = 0) -> dict:
-    """One step of src/run.f90:229-270, returning the state after each phase
+def run_phases(st: FilamentState, p: RunParams, threads: int = 0, noise: Noise | None = None) -> dict:
+    """One step of src/run.f90:24-65, returning the state after each phase
     (same phase names as the reference build's dumps) and the tree."""
     L = lib()
     ps = PortState(st)
-    pr = C.byref(params_struct(p, threads))
+    pr = C.byref(params_struct(p, threads, noise))
     out = {}
     ps.c.itime += 1
     L.nvo_ghostp(ps.ref(), pr)

@@ -2,7 +2,7 @@
 !> project's own that links the UNMODIFIED NuVort hot-path modules compiled
 !> straight from the reference tree (cdata, derivatives, general, boundary,
 !> normal_fluid, tree, timestep, reconnection, line) and replays the per-step
-!> call sequence of src/run.f90:229-270 on a state read from a file.
+!> call sequence of src/run.f90:24-65 on a state read from a file.
 !>
 !> It exists so that golden fixtures and calibration timings come from the
 !> reference's own FP64 arithmetic.  Nothing in the product links it.
@@ -24,6 +24,7 @@ program ref_harness
   use tree
   use line
   use reconnection
+  use normal_fluid
   implicit none
   character(len=512) :: state_in, out_prefix, arg
   integer :: nrun, dump_phases, istep, itime0
@@ -44,6 +45,9 @@ program ref_harness
   if (boundary_z=='solid') n_mirror=n_mirror+1
   call create_periodic_loop_array
   call create_mirror_loop_array
+  ! the normal-fluid models with set-up (the noise table of noisey_xflow is
+  ! read here from run.in's noise_file), as src/initial.f90 calls it
+  if (normal_velocity/='zero' .and. normal_velocity/='xflow') call setup_normal_fluid
 
   call load_state(trim(state_in))
   itime0 = itime

@@ -60,12 +60,18 @@ def read_tree(path: str) -> np.ndarray:
 
 
 def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, dump_phases: bool = True,
-                  threads: int = 1) -> dict:
+                  threads: int = 1, files: dict | None = None, expect_fatal: bool = False) -> dict:
     """Run the reference build on ``st`` for ``nsteps`` steps in ``workdir``.
-    Returns the dumped states (ghost, tree, motion, insert, close, recon,
-    remove for the first step; final after the last), the reconnection log
-    lines and the timing record."""
+    ``files`` maps file names (relative to ``workdir``) to text written there
+    first, such as a noise table.  Returns the dumped states (ghost, tree,
+    motion, insert, close, recon, remove for the first step; final after the
+    last), the reconnection log lines and the timing record.  With
+    ``expect_fatal`` the run is expected to stop in fatal_error: its exit
+    status and output are returned instead."""
     os.makedirs(os.path.join(workdir, "data"), exist_ok=True)
+    for name, text in (files or {}).items():
+        with open(os.path.join(workdir, name), "w") as fh:
+            fh.write(text)
     with open(os.path.join(workdir, "run.in"), "w") as fh:
         fh.write(run_text)
     log = os.path.join(workdir, "data", "recon_location.log")
@@ -73,8 +79,14 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
         os.remove(log)
     write_state(os.path.join(workdir, "state_in.bin"), st)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    subprocess.run([REF_BIN, "state_in.bin", "out", str(nsteps), "1" if dump_phases else "0"],
-                   cwd=workdir, env=env, check=True, capture_output=True)
+    r = subprocess.run([REF_BIN, "state_in.bin", "out", str(nsteps), "1" if dump_phases else "0"],
+                       cwd=workdir, env=env, capture_output=True, text=True)
+    if expect_fatal:
+        # fatal_error (src/cdata.f90:245-255) prints its location and message
+        # and stops the run
+        return {"returncode": r.returncode, "stdout": r.stdout, "stderr": r.stderr}
+    if r.returncode != 0:
+        raise RuntimeError("reference build failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-2000:]))
     out = {}
     for ph in ("ghost", "motion", "insert", "close", "recon", "remove", "final"):
         p = os.path.join(workdir, "out_%s.bin" % ph)

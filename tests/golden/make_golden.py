@@ -54,13 +54,24 @@ def pack_state(prefix: str, st: FilamentState, d: dict) -> None:
 ALL_PHASES = ("ghost", "motion", "insert", "close", "recon", "remove", "final")
 
 
-def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, phases=ALL_PHASES) -> None:
+def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, phases=ALL_PHASES,
+         noise_seed: int | None = None) -> None:
+    """Run the reference build and store the fixture.  The ghost phase (the
+    input of every per-phase test) is always kept.  With ``noise_seed`` the
+    run reads the noisey_xflow table tangles.noise_table(noise_seed) from
+    ``noise.dat``; the fixture stores the seed, not the table."""
+    files = None
+    if noise_seed is not None:
+        p.noise_file = "noise.dat"
+        files = {"noise.dat": "\n".join("%.17g" % v for v in tangles.noise_table(noise_seed)) + "\n"}
     txt = format_run_text(p)
     with tempfile.TemporaryDirectory(dir="/tmp") as wd:
-        out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=True, threads=1)
+        out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=True, threads=1, files=files)
     d = {"run_text": np.array(txt), "nsteps": np.array(nsteps), "note": np.array(note)}
+    if noise_seed is not None:
+        d["noise_seed"] = np.array(noise_seed)
     pack_state("in", st, d)
-    for ph in tuple(phases) + ("final",):
+    for ph in ("ghost",) + tuple(phases) + ("final",):
         if ph in out and "%s__x" % ph not in d:
             pack_state(ph, out[ph], d)
     if "tree" in out:
@@ -117,6 +128,51 @@ def remesh_case(delta: float) -> FilamentState:
     return out
 
 
+def make_fatal(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str) -> None:
+    """A state on which the reference stops in fatal_error: the fixture keeps
+    the input and the message the reference printed."""
+    txt = format_run_text(p)
+    with tempfile.TemporaryDirectory(dir="/tmp") as wd:
+        out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=False, threads=1, expect_fatal=True)
+    text = out["stdout"] + out["stderr"]
+    lines = [ln.strip() for ln in text.splitlines() if ln.strip()]
+    fatal = [ln for ln in lines if ".mod" in ln or "run.x" in ln or "FATAL" in ln.upper()]
+    assert out["returncode"] != 0 or fatal, "the reference did not stop: %r" % text[-500:]
+    d = {"run_text": np.array(txt), "nsteps": np.array(nsteps), "note": np.array(note),
+         "fatal_text": np.array("\n".join(fatal[-6:])), "returncode": np.array(out["returncode"])}
+    pack_state("in", st, d)
+    np.savez_compressed(os.path.join(OUT, "fatal_" + name + ".npz"), **d)
+    print("%-22s N=%5d fatal: %s" % ("fatal_" + name, st.pcount, " | ".join(fatal[-3:])))
+
+
+def singular_state(delta: float, L: float) -> FilamentState:
+    """Two rings that touch through the periodic x faces: a point of ring A
+    sits exactly on the +x face and a point of ring B exactly on the -x
+    face, with the same y and z, so that in the image shifted by +L the tree
+    leaf of B's point coincides with A's point (src/tree.f90:289-295).  The
+    +x face point lies in no child of the root (half-open boxes), so only
+    that one target meets the singularity."""
+    A = tangles.ring([L / 2 - 0.006, 0.0, 0.0], 0.006, [0, 0, 1], delta)
+    B = tangles.ring([-L / 2 - 0.006, 0.0, 0.0], 0.006, [0, 1, 0], delta, phase=math.pi)
+    A[int(np.argmax(A[:, 0]))] = (L / 2, 0.0, 0.0)
+    B[int(np.argmin(np.abs(B[:, 0] + L / 2)))] = (-L / 2, 0.0, 0.0)
+    return tangles._assemble([A, B], (L, L, L))
+
+
+def fatal_cases(d5: float) -> None:
+    rings = tangles.random_rings(7, 4, 0.004, 0.012, 0.1, d5)
+    nan = rings.copy()
+    nan.x[5, 1] = float("nan")
+    make_fatal("nan", nan, base_params(), 1, "a NaN position: NAN_finder stops the run (src/general.f90:172-191)")
+    empty = rings.copy()
+    empty.infront[:] = 0
+    empty.behind[:] = 0
+    make_fatal("zero_length", empty, base_params(), 1,
+               "every slot empty: pinsert stops the run (src/line.f90:25-27)")
+    make_fatal("singular", singular_state(d5, 0.1), base_params(velocity="Tree", tree_theta=0.5), 1,
+               "a target on a shifted image's leaf: tree_walk stops the run (src/tree.f90:289-295)")
+
+
 def main() -> None:
     d5 = 5e-4
     rings = tangles.random_rings(7, 4, 0.004, 0.012, 0.1, d5)
@@ -152,6 +208,24 @@ def main() -> None:
     make("remesh_bs", rm, base_params(), 2, "stretched/compressed rings with empty slots: pinsert/premove")
     make("remesh_tree", rm, base_params(velocity="Tree", tree_theta=0.5), 2,
          "remesh case under Tree: stale tree with inserted points")
+    # LIA (src/timestep.f90:60-73) with every topology phase
+    make("lia_rings", rings, base_params(velocity="LIA"), 3, "LIA, 4 random rings, 3 steps")
+    # the closed-form normal-fluid models and the noise table
+    # (src/normal_fluid.f90:94-115), with friction, started at t > 0 so the
+    # time-dependent shear has a phase
+    late = rings.copy()
+    late.t = 2.5e-5
+    mfk = dict(norm_vel_xflow=0.5, alpha=(0.206, 0.0083), super_velocity=(0.0, 0.003, 0.0))
+    make("nf_shear_xflow", late, base_params(normal_velocity="shear_xflow", norm_shear_omega=4.0e4, **mfk), 2,
+         "BS, shear_xflow normal flow with friction", ("motion",))
+    make("nf_ychannel", late, base_params(normal_velocity="ychannel", **mfk), 2,
+         "BS, ychannel normal flow with friction", ("motion",))
+    make("nf_channel", late, base_params(normal_velocity="channel", **mfk), 2,
+         "BS, channel normal flow with friction", ("motion",))
+    make("nf_noisey_xflow", late, base_params(normal_velocity="noisey_xflow", **mfk), 3,
+         "BS, noisey_xflow normal flow (seeded noise table) with friction", ("motion",), noise_seed=77)
+    make("nf_channel_tree", late, base_params(velocity="Tree", tree_theta=0.4, normal_velocity="channel", **mfk), 2,
+         "Tree theta 0.4, channel normal flow with friction", ("motion",))
     dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
     few = ("close", "final")
     make("dense_bs_10step", dense, base_params(box_size=(0.025, 0.025, 0.025)), 10,
@@ -162,4 +236,8 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["fatal"]:
+        fatal_cases(5e-4)
+    else:
+        main()
+        fatal_cases(5e-4)

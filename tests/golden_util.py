@@ -17,7 +17,13 @@ STATE_FIELDS = ("x", "u", "u1", "u2", "u_sup", "u_mf", "ghosti", "ghostb", "infr
 
 
 def names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Fixtures of normal runs (the fatal_* ones end in fatal_error)."""
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(p).startswith("fatal_"))
+
+
+def fatal_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "fatal_*.npz")))
 
 
 class Fixture:
@@ -26,7 +32,7 @@ class Fixture:
         self.z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
         self.params = parse_run_text(str(self.z["run_text"]))
         self.nsteps = int(self.z["nsteps"])
-        self.timing = json.loads(str(self.z["timing"]))
+        self.timing = json.loads(str(self.z["timing"])) if "timing" in self.z.files else {}
 
     def has(self, phase: str) -> bool:
         return ("%s__x" % phase) in self.z.files
@@ -48,6 +54,18 @@ class Fixture:
     @property
     def recon_log(self) -> np.ndarray:
         return self.z["recon_log"]
+
+    @property
+    def fatal_text(self) -> str:
+        """What the reference printed when it stopped (fatal_* fixtures)."""
+        return str(self.z["fatal_text"]) if "fatal_text" in self.z.files else ""
+
+    def noise(self):
+        """The noisey_xflow table the reference read, or None."""
+        if "noise_seed" not in self.z.files:
+            return None
+        from nuvort_amd.tangles import noise_table
+        return noise_table(int(self.z["noise_seed"]))
 
 
 def compare_states(ref: FilamentState, got: FilamentState, n_defined: int, fields=None, rtol=0.0,

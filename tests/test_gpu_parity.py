@@ -24,7 +24,11 @@ FIXTURES = names()
 
 def solver(fx, **kw):
     from nuvort_amd.engine import FilamentSolver
-    return FilamentSolver(fx.params, check=False, **kw)
+    s = FilamentSolver(fx.params, check=False, **kw)
+    nz = fx.noise()
+    if nz is not None:
+        s.set_noise(nz)
+    return s
 
 
 def rel(a, b):
@@ -44,9 +48,11 @@ def test_ghostp_bit_exact(name):
     assert compare_states(fx.state("ghost"), got, fx.state("in").pcount, fields=("ghosti", "ghostb")) == []
 
 
-@pytest.mark.parametrize("name", [n for n in FIXTURES if Fixture(n).params.velocity == "BS"])
+@pytest.mark.parametrize("name", [n for n in FIXTURES if Fixture(n).params.velocity in ("BS", "LIA")])
 @pytest.mark.parametrize("exact", [True, False])
 def test_direct_fp64_velocity_and_ab(name, exact):
+    """pmotion from the reference's ghost phase: u, u_sup, u_mf and the AB
+    update (per-point Euler / AB2 / AB3 selection), direct sum or LIA."""
     fx = Fixture(name)
     if not fx.has("motion"):
         pytest.skip("fixture has no motion phase")
@@ -124,7 +130,7 @@ def test_close_search_bit_exact(name):
     s = solver(fx)
     if fx.params.tree_theta > 0:
         # the stale tree: built from the step's start state, searched with
-        # the positions after pmotion and pinsert (src/run.f90:234-243)
+        # the positions after pmotion and pinsert (src/run.f90:29-40)
         s.set_state(fx.state("ghost"))
         s.construct_tree()
         s.set_state(fx.state("insert"))

@@ -14,7 +14,9 @@ FIXTURES = names()
 def test_full_run_bit_exact(name):
     fx = Fixture(name)
     st = fx.state("in")
-    rc, fin, events = port.step(st, fx.params, fx.nsteps, threads=2)
+    nz = fx.noise()
+    rc, fin, events = port.step(st, fx.params, fx.nsteps, threads=2,
+                                noise=port.Noise(nz) if nz is not None else None)
     assert rc == 0
     ref = fx.state("final")
     assert compare_states(ref, fin, st.pcount) == []
@@ -32,7 +34,8 @@ def test_full_run_bit_exact(name):
 def test_first_step_phases_bit_exact(name):
     fx = Fixture(name)
     st = fx.state("in")
-    out = port.run_phases(st, fx.params, threads=2)
+    nz = fx.noise()
+    out = port.run_phases(st, fx.params, threads=2, noise=port.Noise(nz) if nz is not None else None)
     assert out["rc_motion"] == 0
     for ph in ("ghost", "motion", "insert", "close", "recon", "remove"):
         if fx.has(ph):
