@@ -1,31 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the filament velocity + timestep path on one MI355X.
 
-Workload (the headline configuration): 64 vortex rings, radii U[0.01, 0.03]
-cm, centres uniform in a periodic 0.1 cm box, orientations uniform on the
-sphere, delta = 5e-4 cm (~16k points), kappa = 9.97e-4, core 1.3e-8 cm,
-dt = 5e-6 s, direct Biot-Savart over straight segments with the 26
-periodic images, reconnections and re-meshing on, 3rd-order Adams-Bashforth.
-Seed 1236 (1234 + configuration number 2).  Synthetic data, generated here.
+Headline workload (the batched ensemble): 256 independent periodic vortex
+tangles, point counts log-uniform in [1k, 40k] (2.64 M points in all), each
+a seeded random-ring tangle (radii U[0.01, 0.03] cm, centres uniform in a
+periodic 0.1 cm box, orientations uniform on the sphere, delta = 5e-4 cm,
+kappa = 9.97e-4, core 1.3e-8 cm, dt = 5e-6 s), seed 1238.  Synthetic data,
+generated here.  Direct Biot-Savart over straight segments with the 26
+periodic images, FP32 Kahan-compensated far field and FP64 near field,
+reconnections and re-meshing on, 3rd-order Adams-Bashforth.
 
-A "step" is one pass of the whole reference main-loop body on the device:
-ghosts -> velocity (FP32 Kahan-compensated far field, FP64 near field,
-local term, mutual friction) -> AB update -> point insertion -> brute
-close-point search -> reconnection -> point removal -> periodic wrap,
-recorded once as a HIP graph and replayed every step.
-``value`` = point-updates/s = sum over the timed steps of the points moved,
-divided by the wall time of those steps (state resident in HBM).
+A "step" is one pass of the whole reference main-loop body for every tangle
+of the ensemble on the device: ghosts -> velocity (one persistent direct-sum
+launch for all 256 tangles, local term, mutual friction) -> AB update ->
+point insertion -> brute close-point search -> reconnection -> point
+removal -> periodic wrap, recorded once as a HIP graph and replayed every
+step.  ``value`` = point-updates/s = sum over the timed steps of the points
+moved, divided by the wall time of those steps (state resident in HBM).
 
-Also reported: the same step with the plain FP64 kernel, the Barnes-Hut
-(theta = 0.5) step, the single-step velocity error of the FP32 form against
-the FP64 kernel (all points) and against the FP64 oracle (sampled points),
-the roofline of the direct-sum kernel, and the oracle's FP64 CPU velocity
-timed on a bounded sample of targets as the CPU baseline.
+Also reported: the same ensemble step with the plain FP64 kernel (the >= 3x
+target), the FP32C velocity error against the FP64 kernel (every point) and
+against the FP64 oracle (sampled targets of several tangles), the roofline
+of the direct-sum kernel on the ensemble launch, the oracle's FP64 CPU
+velocity on the sampled targets as the CPU baseline, and two more workloads:
+64 rings (16.7k points, one tangle) and the 1M-point counterflow octree step.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import dataclasses
 import json
 import math
 import os
@@ -127,31 +131,6 @@ def kernel_timing(p, st, reps):
     launches = s1["pair_launches"] - s0["pair_launches"]
     return ((s1["pair_ms_total"] - s0["pair_ms_total"]) / launches * 1e-3,
             (s1["pair_evals_total"] - s0["pair_evals_total"]) / launches, launches)
-
-
-def ensemble_run(p, sts, precision, steps):
-    """Batched ensemble: velocities of one pass, then one warm-up step and
-    `steps` timed steps of the whole batch (one persistent launch per phase)."""
-    from nuvort_amd.engine import FilamentSolver
-    q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
-    q.hip_precision = precision
-    s = FilamentSolver(q)
-    s.set_state_batch(sts)
-    if precision == "fp32c":
-        s.autotune(TUNE_CACHE)
-    s.ghostp()
-    s.velocity()
-    u = np.concatenate([g.u_sup[g.infront != 0] for g in s.get_state_batch()])
-    s.set_state_batch(sts)
-    tune = s.autotune(TUNE_CACHE) if precision == "fp32c" else None
-    s.step(1)
-    s0 = s.stats()
-    t0 = time.perf_counter()
-    s.step(steps)
-    sec = time.perf_counter() - t0
-    pu = s.stats()["point_updates"] - s0["point_updates"]
-    s.close()
-    return u, sec, pu, tune
 
 
 def velocity_once(p, st, precision, tune=True):
@@ -374,19 +353,198 @@ def tree1m(args):
     print(json.dumps(line), flush=True)
 
 
+def cpu_info(threads: int) -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "threads_used": threads}
+
+
+def calibration() -> dict | None:
+    """The reference build's time over the restatement's on one input,
+    measured in the development container (oracle/calibrate.py)."""
+    path = os.path.join(ROOT, "oracle", "calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def oracle_sample(sts, p, picks, per_tangle: int, threads: int):
+    """The oracle's FP64 velocity (the reference's arithmetic: local term,
+    direct sum over the 27 images) of `per_tangle` targets of each tangle in
+    `picks`, spread over its rows.  Returns the seconds of the velocity
+    calls, {tangle: (rows, u)} and the evaluated (target, source) count."""
+    from oracle import port
+    L = port.lib()
+    out, secs, pairs = {}, 0.0, 0
+    for k in picks:
+        st = sts[k]
+        ps = port.PortState(st)
+        pr = port.params_struct(p, threads)
+        L.nvo_ghostp(ps.ref(), C.byref(pr))
+        n = st.pcount
+        m = min(per_tangle, n)
+        blocks = 4
+        starts = [int(b * (n - m // blocks) / blocks) for b in range(blocks)]
+        rows, us = [], []
+        for a in starts:
+            cnt = min(m // blocks, n - a)
+            u = np.zeros((cnt, 3))
+            t0 = time.perf_counter()
+            L.nvo_velocity(ps.ref(), C.byref(pr), None, a + 1, cnt, u.ctypes.data_as(port._dp))
+            secs += time.perf_counter() - t0
+            rows.append(np.arange(a, a + cnt))
+            us.append(u)
+            pairs += cnt * st.active
+        out[k] = (np.concatenate(rows), np.concatenate(us))
+    return secs, out, pairs
+
+
+def ensemble_workload():
+    from nuvort_amd import tangles
+    from nuvort_amd.runfile import RunParams
+    sts = tangles.ensemble(1238, 256, 1000, 40000)
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1), velocity="BS",
+                  hip_precision="fp32c", recon_shots=1)
+    return sts, p
+
+
+def ensemble_steps(p, sts, precision, warmup, steps):
+    """Warm-up steps, then `steps` timed steps of the whole batch (wall
+    clock around the nv_step calls, each of which synchronises before it
+    returns), split into up to 5 calls for the spread."""
+    from nuvort_amd.engine import FilamentSolver
+    q = dataclasses.replace(p, hip_precision=precision)
+    s = FilamentSolver(q)
+    s.set_state_batch(sts)
+    tune = s.autotune(TUNE_CACHE) if precision == "fp32c" else None
+    if warmup:
+        s.step(warmup)
+    blocks = min(5, steps)
+    sizes = [steps // blocks + (1 if b < steps % blocks else 0) for b in range(blocks)]
+    s0 = s.stats()
+    per = []
+    t0 = time.perf_counter()
+    for k in sizes:
+        tb = time.perf_counter()
+        s.step(k)
+        per.append((time.perf_counter() - tb) / k * 1e3)
+    sec = time.perf_counter() - t0
+    s1 = s.stats()
+    s.close()
+    return {"seconds": sec, "point_updates": s1["point_updates"] - s0["point_updates"], "block_ms_per_step": per,
+            "tuning": tune}
+
+
+def ensemble_velocity(p, sts, precision, reps):
+    """One velocity pass of the batch (every point), then `reps` more whose
+    direct-sum launches are timed with HIP events on the library's stream."""
+    from nuvort_amd.engine import FilamentSolver
+    q = dataclasses.replace(p, hip_precision=precision)
+    s = FilamentSolver(q)
+    s.set_state_batch(sts)
+    if precision == "fp32c":
+        s.autotune(TUNE_CACHE)
+    s.ghostp()
+    s.velocity()
+    gs = s.get_state_batch()
+    s0 = s.stats()
+    for _ in range(reps):
+        s.velocity()
+    s1 = s.stats()
+    s.close()
+    launches = s1["pair_launches"] - s0["pair_launches"]
+    kern = None
+    if launches:
+        kern = {"ms": (s1["pair_ms_total"] - s0["pair_ms_total"]) / launches,
+                "pair_evals": (s1["pair_evals_total"] - s0["pair_evals_total"]) / launches, "launches": launches}
+    return gs, kern
+
+
+def rings64_extra(steps):
+    """64 rings in one tangle (16.7k points): FP32C and FP64 steps."""
+    st, p = workload()
+    head = timed_steps(p, st, steps, 3, blocks=min(5, steps))
+    q = dataclasses.replace(p, hip_precision="fp64")
+    k64 = max(5, steps // 4)
+    r64 = timed_steps(q, st, k64, 2)
+    t_k, e_k, _ = kernel_timing(p, st, 10)
+    t64, e64, _ = kernel_timing(q, st, 3)
+    v32 = head["point_updates"] / head["seconds"]
+    v64 = r64["point_updates"] / r64["seconds"]
+    return {"points": st.active, "seed": 1236, "fp32c": {"value": v32, "ms_per_step": head["seconds"] / steps * 1e3,
+                                                         "steps": steps, "kernel_ms": t_k * 1e3,
+                                                         "frac_fp32_peak": FLOP_PER_PAIR * e_k / t_k / 1e12 / PEAK_FP32},
+            "fp64_plain": {"value": v64, "ms_per_step": r64["seconds"] / k64 * 1e3, "steps": k64,
+                           "kernel_ms": t64 * 1e3,
+                           "frac_fp64_peak": FLOP_PER_PAIR * e64 / t64 / 1e12 / PEAK_FP64},
+            "fp32c_over_fp64_step": v32 / v64,
+            "note": "64 random rings (radii U[0.01,0.03] cm) in one periodic 0.1 cm tangle, delta 5e-4 cm"}
+
+
+def tree1m_extra(steps, precision="fp32c"):
+    """The 1M-point counterflow octree step (timing only)."""
+    from nuvort_amd import tangles
+    from nuvort_amd.engine import FilamentSolver
+    from nuvort_amd.runfile import RunParams
+    st = tangles.random_rings(1239, 0, 0.01, 0.03, 0.2, 5e-4, target_points=1000000)
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.2, 0.2, 0.2), velocity="Tree", tree_theta=0.4,
+                  normal_velocity="xflow", norm_vel_xflow=0.5, alpha=(0.206, 0.0083), recon_shots=1,
+                  hip_precision=precision)
+    s = FilamentSolver(p)
+    s.set_state(st)
+    s.step(2)
+    s0 = s.stats()
+    t0 = time.perf_counter()
+    walk, build, close = [], [], []
+    for _ in range(steps):
+        a = s.stats()
+        s.step(1)
+        b = s.stats()
+        walk.append(b["walk_ms_total"] - a["walk_ms_total"])
+        build.append(b["ms_tree_build"])
+        close.append(b["ms_close"])
+    sec = time.perf_counter() - t0
+    s1 = s.stats()
+    nodes = (s1["walk_nodes_total"] - s0["walk_nodes_total"]) / max(1, s1["walk_launches"] - s0["walk_launches"])
+    s.close()
+    n = st.active
+    rec = 48 if precision == "fp32c" else 80
+    wms = float(np.mean(walk))
+    walk_bytes = nodes * rec + n * 27 * 48
+    return {"points": n, "seed": 1239, "walk": precision, "value": (s1["point_updates"] - s0["point_updates"]) / sec,
+            "ms_per_step": sec / steps * 1e3, "steps": steps, "warmup": 2, "walk_ms": wms,
+            "tree_build_ms": float(np.mean(build)), "close_search_ms": float(np.mean(close)),
+            "walk_node_reads": nodes, "walk_record_bytes": rec,
+            "walk_algorithmic_GBps": walk_bytes / (wms * 1e-3) / 1e9, "hbm_peak_GBps": 8000.0,
+            "note": "random rings to 1M points in a periodic 0.2 cm box, delta 5e-4 cm, Barnes-Hut theta 0.4 "
+                    "rebuilt every step, v_n = 0.5 cm/s along x, alpha 0.206 / 0.0083, reconnection and "
+                    "re-meshing every step"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=3072, help="targets the CPU baseline evaluates")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=5, help="timed steps of the plain FP64 ensemble step")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ap.add_argument("--cpu-per-tangle", type=int, default=2048, help="oracle targets per sampled tangle")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and tree comparison runs")
-    ap.add_argument("--workload", choices=("rings64", "dense460k", "tree1m"), default="rings64",
-                    help="rings64: the headline step (default); dense460k: the dense 800-ring tangle "
-                         "(467k points), relaxed 50 steps, FP32C error statistics against FP64; tree1m: "
-                         "1M points in a 0.2 cm box, Barnes-Hut theta 0.4, counterflow mutual friction")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 64-ring and 1M-point octree workloads")
+    ap.add_argument("--workload", choices=("ensemble256", "rings64", "dense460k", "tree1m"), default="ensemble256",
+                    help="ensemble256: the headline batched step (default); rings64: 64 rings, 16.7k points; "
+                         "dense460k: the dense 800-ring tangle (467k points), relaxed 50 steps, FP32C error "
+                         "statistics against FP64; tree1m: 1M points in a 0.2 cm box, Barnes-Hut theta 0.4, "
+                         "counterflow mutual friction")
     ap.add_argument("--tree-precision", choices=("fp64", "fp32c"), default="fp32c",
                     help="tree1m: walk arithmetic (FP32 far field + FP64 near field, or FP64)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -417,122 +575,101 @@ def main():
     if args.workload == "tree1m":
         tree1m(args)
         return
-    st, p = workload()
-    n = st.active
-    log("workload: %d points" % n)
+    if args.workload == "rings64":
+        rings64_main(args)
+        return
 
-    head = timed_steps(p, st, args.steps, args.warmup, blocks=min(5, args.steps))
+    sts, p = ensemble_workload()
+    npts = sum(g.active for g in sts)
+    log("ensemble: %d tangles, %d points" % (len(sts), npts))
+    head = ensemble_steps(p, sts, "fp32c", args.warmup, args.steps)
     ms_step = head["seconds"] / args.steps * 1e3
     value = head["point_updates"] / head["seconds"]
     blk = head["block_ms_per_step"]
-    log("fp32c step: %.3f ms, %.4g point-updates/s (blocks %s)" % (ms_step, value,
-                                                                  " ".join("%.3f" % b for b in blk)))
+    log("fp32c ensemble step: %.1f ms, %.4g point-updates/s (blocks %s)" % (
+        ms_step, value, " ".join("%.1f" % b for b in blk)))
+    r64 = ensemble_steps(p, sts, "fp64", 1, args.fp64_steps)
+    v64 = r64["point_updates"] / r64["seconds"]
+    log("fp64 ensemble step: %.1f ms (fp32c x%.2f)" % (r64["seconds"] / args.fp64_steps * 1e3, value / v64))
 
-    # roofline of the dominant kernel (the FP32-compensated direct sum)
-    t_kernel, evals_per_launch, nlaunch = kernel_timing(p, st, max(10, min(args.steps, 50)))
-    achieved = FLOP_PER_PAIR * evals_per_launch / t_kernel / 1e12
+    # velocities of every point in both forms; kernel times for the roofline
+    g32, k32 = ensemble_velocity(p, sts, "fp32c", 3)
+    g64, k64 = ensemble_velocity(p, sts, "fp64", 1)
+    u32 = np.concatenate([g.u_sup[g.infront != 0] for g in g32])
+    u64 = np.concatenate([g.u_sup[g.infront != 0] for g in g64])
+    acc = {"fp32c_vs_fp64_gpu": rel_err_stats(u64, u32)}
+    t_kernel = k32["ms"] * 1e-3
+    achieved = FLOP_PER_PAIR * k32["pair_evals"] / t_kernel / 1e12
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "direct_fp32c_rings64_traffic.json")
+    pmc = os.path.join(ROOT, "profiles", "direct_fp32c_ensemble256_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32, 4), "traffic": traffic,
-                "kernel": "k_bs32c (direct sum, FP32 far field)", "kernel_ms": round(t_kernel * 1e3, 4),
-                "pair_evals_per_launch": int(evals_per_launch), "flop_per_pair": FLOP_PER_PAIR,
-                "launches_timed": nlaunch}
-
-    extra = {"step_ms_blocks": {"median": float(np.median(blk)), "min": float(min(blk)), "max": float(max(blk)),
-                                "blocks": len(blk), "note": "ms/step of each of the consecutive nv_step calls "
-                                                            "the timed steps were split into"}}
-    if not args.no_extra:
-        qe = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
-        qe.hip_step_launch = "eager"
-        ke = max(5, min(args.steps, 20))
-        re_ = timed_steps(qe, st, ke, 2)
-        extra["eager_launch"] = {"ms_per_step": re_["seconds"] / ke * 1e3,
-                                 "value": re_["point_updates"] / re_["seconds"],
-                                 "note": "the same step with every kernel launched eagerly and the host "
-                                         "synchronising after each phase, instead of one recorded HIP graph"}
-        log("eager step: %.3f ms" % (re_["seconds"] / ke * 1e3))
-        q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
-        q.hip_precision = "fp64"
-        k64 = max(5, min(args.steps, 20))
-        r64 = timed_steps(q, st, k64, 2)
-        v64 = r64["point_updates"] / r64["seconds"]
-        t64, e64, _ = kernel_timing(q, st, 5)
-        a64 = FLOP_PER_PAIR * e64 / t64 / 1e12
-        extra["fp64_plain"] = {"value": v64, "ms_per_step": r64["seconds"] / k64 * 1e3,
-                               "kernel_ms": t64 * 1e3, "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
-                               "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": t64 / t_kernel}
-        log("fp64 step: %.3f ms (x%.2f)" % (r64["seconds"] / k64 * 1e3, value / v64))
-        qt = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
-        qt.hip_precision, qt.velocity, qt.tree_theta = "fp64", "Tree", 0.5
-        kt = max(5, min(args.steps, 50))
-        rt = timed_steps(qt, st, kt, 2)
-        tw = rt["walk_ms"] / max(1, rt["walk_launches"]) * 1e-3
-        walk_nodes = rt["walk_nodes"] / max(1, rt["walk_launches"])
-        # algorithmic bytes of one walk: node records read (80 B) + targets
-        # read (27 walks x 12 B... positions as 3 x 8 B FP64) + velocities written
-        walk_bytes = walk_nodes * 80 + n * 27 * 24 + n * 48
-        extra["tree_theta_0.5"] = {"value": rt["point_updates"] / rt["seconds"],
-                                   "ms_per_step": rt["seconds"] / kt * 1e3, "walk_ms": tw * 1e3,
-                                   "walk_GBps": walk_bytes / tw / 1e9, "hbm_peak_GBps": 8000.0,
-                                   "node_reads_per_walk": walk_nodes}
-        log("tree step: %.3f ms" % (rt["seconds"] / kt * 1e3))
-
-        # batched ensemble: 256 independent random-ring tangles in 0.1 cm
-        # boxes, point counts log-uniform in [1k, 40k], seed 1238
-        from nuvort_amd import tangles as T
-        sts = T.ensemble(1238, 256, 1000, 40000)
-        npts = sum(g.active for g in sts)
-        log("ensemble: %d tangles, %d points" % (len(sts), npts))
-        u32, sec32, pu32, tune32 = ensemble_run(p, sts, "fp32c", 2)
-        log("ensemble fp32c step: %.1f ms" % (sec32 / 2 * 1e3))
-        u64, sec64, pu64, _ = ensemble_run(p, sts, "fp64", 1)
-        log("ensemble fp64 step: %.1f ms" % (sec64 * 1e3))
-        v32, v64 = pu32 / sec32, pu64 / sec64
-        extra["ensemble_256"] = {
-            "tangles": len(sts), "points": npts,
-            "pair_evals_per_step": int(sum(27 * g.active * g.active for g in sts)),
-            "fp32c": {"value": v32, "ms_per_step": sec32 / 2 * 1e3, "steps": 2, "tuning": tune32},
-            "fp64_plain": {"value": v64, "ms_per_step": sec64 * 1e3, "steps": 1},
-            "fp32c_over_fp64": v32 / v64,
-            "accuracy_fp32c_vs_fp64_gpu": rel_err_stats(u64, u32),
-            "note": "256 independent periodic random-ring tangles (0.1 cm boxes, delta 5e-4 cm, radii "
-                    "U[0.01,0.03] cm), point counts log-uniform in [1k, 40k], seed 1238; one launch per "
-                    "phase for the whole batch, each step one recorded HIP graph"}
-
-    # accuracy of the FP32 form: against the FP64 GPU kernel (every point)
-    g64 = velocity_once(p, st, "fp64")
-    g32 = velocity_once(p, st, "fp32c")
-    act = g64.infront != 0
-    acc = {"fp32c_vs_fp64_gpu": rel_err_stats(g64.u_sup[act], g32.u_sup[act])}
+                "kernel": "k_bs32c (direct sum of all 256 tangles, FP32 far field)",
+                "kernel_ms": round(k32["ms"], 3), "pair_evals_per_launch": int(k32["pair_evals"]),
+                "flop_per_pair": FLOP_PER_PAIR, "launches_timed": k32["launches"]}
+    a64 = FLOP_PER_PAIR * k64["pair_evals"] / (k64["ms"] * 1e-3) / 1e12
+    fp64 = {"value": v64, "ms_per_step": r64["seconds"] / args.fp64_steps * 1e3, "steps": args.fp64_steps,
+            "warmup": 1, "kernel_ms": k64["ms"], "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
+            "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": k64["ms"] / k32["ms"]}
 
     cpu = None
     if not args.no_cpu:
-        secs, idx, ucpu = cpu_baseline_parallel(st, p, args.cpu_sample, args.cpu_threads)
-        rows = idx
-        # velocity of the sampled points from the oracle: u (with the local
-        # term and imposed flow, no friction in this workload) = u_sup
-        acc["fp32c_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g32.u[rows])
-        acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, g64.u[rows])
-        cpu_rate = args.cpu_sample / secs
-        cpu = {"value": cpu_rate, "unit": "point-updates/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": "FP64 velocity (direct sum, 27 images, local term) of %d of the %d points of the same "
-                         "workload, oracle C restatement, OpenMP; %.2f s; the velocity is >99%% of a CPU step "
-                         "at this size, so point-updates/s = sampled points / time" % (args.cpu_sample, n, secs)}
-        log("cpu baseline: %.4g point-updates/s on %d threads" % (cpu_rate, args.cpu_threads))
+        sizes = np.array([g.active for g in sts])
+        order = np.argsort(sizes)
+        picks = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0.0, 1.0, 8)]
+        log("oracle sample: %d targets of each of %d tangles on %d threads" % (args.cpu_per_tangle, len(picks),
+                                                                             args.cpu_threads))
+        secs, samp, pairs = oracle_sample(sts, p, picks, args.cpu_per_tangle, args.cpu_threads)
+        ucpu = np.concatenate([samp[k][1] for k in picks])
+        ug32 = np.concatenate([g32[k].u[samp[k][0]] for k in picks])
+        ug64 = np.concatenate([g64[k].u[samp[k][0]] for k in picks])
+        acc["fp32c_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, ug32)
+        acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, ug64)
+        # a CPU step of the ensemble is its velocity pass (>99% at these
+        # sizes): sum over tangles of N_k^2 (target, source) evaluations at
+        # the sampled rate per (target, source)
+        per_pair = secs / pairs
+        cpu_step = per_pair * float(np.sum(sizes.astype(np.float64) ** 2))
+        cal = calibration()
+        cpu = {"value": npts / cpu_step, "unit": "point-updates/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": "FP64 velocity (local term + direct sum over the 27 images, the reference's arithmetic) "
+                         "of %d targets from each of %d tangles of the same ensemble (sizes %s), oracle C "
+                         "restatement with OpenMP, %.2f s; extrapolated to the whole ensemble step by its "
+                         "sum of N_k^2 (the velocity is >99%% of a CPU step at these sizes)"
+                         % (args.cpu_per_tangle, len(picks), [int(sizes[k]) for k in picks], secs),
+               "host": cpu_info(args.cpu_threads),
+               "calibration": None if cal is None else {
+                   "reference_over_restatement_time": cal["reference_over_restatement_time"],
+                   "measured_on": cal["cpu_model"], "threads": cal["threads"], "points": cal["points"],
+                   "note": "the reference build itself (not shipped here) took this many times longer than the "
+                           "restatement on the same input: the reference's CPU rate is this baseline divided "
+                           "by it"}}
+        log("cpu baseline: %.4g point-updates/s on %d threads" % (cpu["value"], args.cpu_threads))
+
+    extra = {"step_ms_blocks": {"median": float(np.median(blk)), "min": float(min(blk)), "max": float(max(blk)),
+                                "blocks": len(blk)},
+             "fp64_plain": fp64}
+    if not args.no_extra:
+        log("64-ring tangle")
+        extra["rings64"] = rings64_extra(40)
+        log("1M-point octree step")
+        extra["tree1m"] = tree1m_extra(10)
+        log("tree1m: %.2f ms/step" % extra["tree1m"]["ms_per_step"])
 
     line = {
         "metric": METRIC, "value": value, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
-        "config": {"workload": "64 random vortex rings in a periodic 0.1 cm box, delta 5e-4 cm, direct "
-                               "Biot-Savart with 26 images, FP32 Kahan far field + FP64 near field, AB3, "
-                               "reconnection and re-meshing every step",
-                   "points": n, "rings": 64, "delta_cm": 5e-4, "dt_s": 5e-6, "box_cm": 0.1, "seed": 1236,
-                   "near_radius_delta": p.hip_near_radius, "parallelism": "1 GPU"},
+        "config": {"workload": "batched ensemble: 256 independent periodic random-ring tangles, point counts "
+                               "log-uniform in [1k, 40k], delta 5e-4 cm, direct Biot-Savart with 26 images, "
+                               "FP32 Kahan far field + FP64 near field, AB3, reconnection and re-meshing every "
+                               "step, one recorded HIP graph per step",
+                   "tangles": len(sts), "points": npts, "pair_evals_per_step": int(k32["pair_evals"]),
+                   "seed": 1238, "dt_s": 5e-6, "box_cm": 0.1, "near_radius_delta": p.hip_near_radius,
+                   "parallelism": "1 GPU"},
         "accuracy": acc,
         "max_rel_velocity_err_vs_fp64": acc["fp32c_vs_fp64_gpu"]["max_rel"],
         "roofline": roofline,
@@ -540,6 +677,33 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
+    print(json.dumps(line), flush=True)
+    if os.path.exists(TUNE_CACHE):
+        os.remove(TUNE_CACHE)
+
+
+def rings64_main(args):
+    """The 64-ring tangle as its own bench line (16.7k points)."""
+    st, p = workload()
+    n = st.active
+    head = timed_steps(p, st, args.steps, args.warmup, blocks=min(5, args.steps))
+    value = head["point_updates"] / head["seconds"]
+    t_kernel, evals, nl = kernel_timing(p, st, max(10, min(args.steps, 50)))
+    achieved = FLOP_PER_PAIR * evals / t_kernel / 1e12
+    g64 = velocity_once(p, st, "fp64")
+    g32 = velocity_once(p, st, "fp32c")
+    act = g64.infront != 0
+    line = {"metric": METRIC, "value": value, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["seconds"] / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
+            "config": {"workload": "64 random vortex rings in a periodic 0.1 cm box, delta 5e-4 cm, direct "
+                                   "Biot-Savart with 26 images, FP32C, AB3, reconnection and re-meshing",
+                       "points": n, "seed": 1236, "parallelism": "1 GPU"},
+            "accuracy": {"fp32c_vs_fp64_gpu": rel_err_stats(g64.u_sup[act], g32.u_sup[act])},
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
+                         "frac": round(achieved / PEAK_FP32, 4), "traffic": None, "kernel_ms": t_kernel * 1e3,
+                         "pair_evals_per_launch": int(evals), "launches_timed": nl},
+            "tuning": head["tuning"], "cpu_baseline": None}
     print(json.dumps(line), flush=True)
     if os.path.exists(TUNE_CACHE):
         os.remove(TUNE_CACHE)

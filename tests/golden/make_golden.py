@@ -177,7 +177,7 @@ def main() -> None:
     d5 = 5e-4
     rings = tangles.random_rings(7, 4, 0.004, 0.012, 0.1, d5)
     make("bs_rings_1step", rings, base_params(), 1, "BS, 4 random rings, one step with phase dumps")
-    make("bs_rings_5step", rings, base_params(), 5, "BS, Euler->AB2->AB3 start-up over 5 steps", ("final",))
+    make("bs_rings_5step", rings, base_params(), 5, "BS, Euler->AB2->AB3 start-up over 5 steps", ("motion", "final"))
     make("tree05_rings", rings, base_params(velocity="Tree", tree_theta=0.5), 2,
          "Tree theta=0.5, tree node list + pclose_tree")
     mf = base_params(velocity="Tree", tree_theta=0.4, normal_velocity="xflow", norm_vel_xflow=0.5,
@@ -227,7 +227,7 @@ def main() -> None:
     make("nf_channel_tree", late, base_params(velocity="Tree", tree_theta=0.4, normal_velocity="channel", **mfk), 2,
          "Tree theta 0.4, channel normal flow with friction", ("motion",))
     dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
-    few = ("close", "final")
+    few = ("motion", "close", "final")
     make("dense_bs_10step", dense, base_params(box_size=(0.025, 0.025, 0.025)), 10,
          "dense random rings in a 0.025 cm box: 10 steps with reconnections", few)
     make("dense_tree_10step", dense, base_params(box_size=(0.025, 0.025, 0.025), velocity="Tree",
