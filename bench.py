@@ -545,6 +545,8 @@ def main():
                          "dense460k: the dense 800-ring tangle (467k points), relaxed 50 steps, FP32C error "
                          "statistics against FP64; tree1m: 1M points in a 0.2 cm box, Barnes-Hut theta 0.4, "
                          "counterflow mutual friction")
+    ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp32c",
+                    help="rings64: direct-sum arithmetic of the timed steps")
     ap.add_argument("--tree-precision", choices=("fp64", "fp32c"), default="fp32c",
                     help="tree1m: walk arithmetic (FP32 far field + FP64 near field, or FP64)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -588,16 +590,20 @@ def main():
     blk = head["block_ms_per_step"]
     log("fp32c ensemble step: %.1f ms, %.4g point-updates/s (blocks %s)" % (
         ms_step, value, " ".join("%.1f" % b for b in blk)))
-    r64 = ensemble_steps(p, sts, "fp64", 1, args.fp64_steps)
-    v64 = r64["point_updates"] / r64["seconds"]
-    log("fp64 ensemble step: %.1f ms (fp32c x%.2f)" % (r64["seconds"] / args.fp64_steps * 1e3, value / v64))
+    do64 = args.fp64_steps > 0   # 0: no FP64 work (profiling runs of the FP32C kernel)
+    if do64:
+        r64 = ensemble_steps(p, sts, "fp64", 1, args.fp64_steps)
+        v64 = r64["point_updates"] / r64["seconds"]
+        log("fp64 ensemble step: %.1f ms (fp32c x%.2f)" % (r64["seconds"] / args.fp64_steps * 1e3, value / v64))
 
     # velocities of every point in both forms; kernel times for the roofline
     g32, k32 = ensemble_velocity(p, sts, "fp32c", 3)
-    g64, k64 = ensemble_velocity(p, sts, "fp64", 1)
-    u32 = np.concatenate([g.u_sup[g.infront != 0] for g in g32])
-    u64 = np.concatenate([g.u_sup[g.infront != 0] for g in g64])
-    acc = {"fp32c_vs_fp64_gpu": rel_err_stats(u64, u32)}
+    acc = {}
+    if do64:
+        g64, k64 = ensemble_velocity(p, sts, "fp64", 1)
+        u32 = np.concatenate([g.u_sup[g.infront != 0] for g in g32])
+        u64 = np.concatenate([g.u_sup[g.infront != 0] for g in g64])
+        acc["fp32c_vs_fp64_gpu"] = rel_err_stats(u64, u32)
     t_kernel = k32["ms"] * 1e-3
     achieved = FLOP_PER_PAIR * k32["pair_evals"] / t_kernel / 1e12
     traffic = None
@@ -610,10 +616,12 @@ def main():
                 "kernel": "k_bs32c (direct sum of all 256 tangles, FP32 far field)",
                 "kernel_ms": round(k32["ms"], 3), "pair_evals_per_launch": int(k32["pair_evals"]),
                 "flop_per_pair": FLOP_PER_PAIR, "launches_timed": k32["launches"]}
-    a64 = FLOP_PER_PAIR * k64["pair_evals"] / (k64["ms"] * 1e-3) / 1e12
-    fp64 = {"value": v64, "ms_per_step": r64["seconds"] / args.fp64_steps * 1e3, "steps": args.fp64_steps,
-            "warmup": 1, "kernel_ms": k64["ms"], "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
-            "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": k64["ms"] / k32["ms"]}
+    fp64 = None
+    if do64:
+        a64 = FLOP_PER_PAIR * k64["pair_evals"] / (k64["ms"] * 1e-3) / 1e12
+        fp64 = {"value": v64, "ms_per_step": r64["seconds"] / args.fp64_steps * 1e3, "steps": args.fp64_steps,
+                "warmup": 1, "kernel_ms": k64["ms"], "achieved_tflops": a64, "frac_fp64_peak": a64 / PEAK_FP64,
+                "fp32c_over_fp64_step": value / v64, "fp32c_over_fp64_kernel": k64["ms"] / k32["ms"]}
 
     cpu = None
     if not args.no_cpu:
@@ -625,9 +633,10 @@ def main():
         secs, samp, pairs = oracle_sample(sts, p, picks, args.cpu_per_tangle, args.cpu_threads)
         ucpu = np.concatenate([samp[k][1] for k in picks])
         ug32 = np.concatenate([g32[k].u[samp[k][0]] for k in picks])
-        ug64 = np.concatenate([g64[k].u[samp[k][0]] for k in picks])
+        ug64 = np.concatenate([g64[k].u[samp[k][0]] for k in picks]) if do64 else None
         acc["fp32c_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, ug32)
-        acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, ug64)
+        if do64:
+            acc["fp64_gpu_vs_fp64_oracle_sample"] = rel_err_stats(ucpu, ug64)
         # a CPU step of the ensemble is its velocity pass (>99% at these
         # sizes): sum over tangles of N_k^2 (target, source) evaluations at
         # the sampled rate per (target, source)
@@ -671,7 +680,7 @@ def main():
                    "seed": 1238, "dt_s": 5e-6, "box_cm": 0.1, "near_radius_delta": p.hip_near_radius,
                    "parallelism": "1 GPU"},
         "accuracy": acc,
-        "max_rel_velocity_err_vs_fp64": acc["fp32c_vs_fp64_gpu"]["max_rel"],
+        "max_rel_velocity_err_vs_fp64": acc["fp32c_vs_fp64_gpu"]["max_rel"] if do64 else None,
         "roofline": roofline,
         "tuning": head["tuning"],
         "cpu_baseline": cpu,
@@ -685,6 +694,7 @@ def main():
 def rings64_main(args):
     """The 64-ring tangle as its own bench line (16.7k points)."""
     st, p = workload()
+    p = dataclasses.replace(p, hip_precision=args.precision)
     n = st.active
     head = timed_steps(p, st, args.steps, args.warmup, blocks=min(5, args.steps))
     value = head["point_updates"] / head["seconds"]
@@ -693,15 +703,17 @@ def rings64_main(args):
     g64 = velocity_once(p, st, "fp64")
     g32 = velocity_once(p, st, "fp32c")
     act = g64.infront != 0
+    peak = PEAK_FP32 if args.precision == "fp32c" else PEAK_FP64
     line = {"metric": METRIC, "value": value, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": head["seconds"] / args.steps * 1e3, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32c+f64", "data": "synthetic",
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32c+f64" if args.precision == "fp32c" else "f64",
+            "data": "synthetic",
             "config": {"workload": "64 random vortex rings in a periodic 0.1 cm box, delta 5e-4 cm, direct "
                                    "Biot-Savart with 26 images, FP32C, AB3, reconnection and re-meshing",
-                       "points": n, "seed": 1236, "parallelism": "1 GPU"},
+                       "points": n, "seed": 1236, "precision": args.precision, "parallelism": "1 GPU"},
             "accuracy": {"fp32c_vs_fp64_gpu": rel_err_stats(g64.u_sup[act], g32.u_sup[act])},
-            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
-                         "frac": round(achieved / PEAK_FP32, 4), "traffic": None, "kernel_ms": t_kernel * 1e3,
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                         "frac": round(achieved / peak, 4), "traffic": None, "kernel_ms": t_kernel * 1e3,
                          "pair_evals_per_launch": int(evals), "launches_timed": nl},
             "tuning": head["tuning"], "cpu_baseline": None}
     print(json.dumps(line), flush=True)
