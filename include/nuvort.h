@@ -211,6 +211,26 @@ int nv_get_recon_events(void *ctx, nv_recon_event *buf, int32_t cap, int32_t *n)
 int nv_get_tree(void *ctx, nv_tree_node *buf, int32_t cap, int32_t *n);
 int nv_get_stats(void *ctx, nv_stats *stats);
 
+/* No counterpart in the reference.  The brute close-point search of
+ * nv_pclose (src/line.f90:135-160 pclose, one tangle) for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc.  It takes no context and calls no
+ * allocator.  Inputs and outputs are device memory the caller holds:
+ * positions x component-major with leading dimension ld >= n (component k of
+ * point p at x[k*ld + p-1]), infront/behind 1-based (0 = empty slot), and
+ * closest/closestd, which receive what pclose writes (closestd = 100 and
+ * closest left as it was where no candidate exists).  A point whose links
+ * leave 1..n is treated as empty.  The search's working memory is a static
+ * device array of NV_NOALLOC_SCRATCH_MIB MiB per GPU, held until the process
+ * ends, which bounds n at NV_NOALLOC_MAX_POINTS (NV_ERR_CAPACITY above).
+ * Kernels are launched on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting for them.  Calls that overlap on one
+ * GPU share its array and race. */
+#define NV_NOALLOC_SCRATCH_MIB 64
+#define NV_NOALLOC_MAX_POINTS 5341952
+int nv_pclose_noalloc(int32_t n, int32_t ld, const double *x, const int32_t *infront, const int32_t *behind,
+                      int32_t *closest, double *closestd, void *stream);
+
 /* Autotuner (no reference counterpart).  Times the direct-sum variants on
  * the uploaded state -- points per lane (targets_per_lane), Kahan block and
  * work items per CU, each left at 0 in nv_params -- one coordinate at a

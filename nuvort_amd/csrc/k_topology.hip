@@ -994,15 +994,18 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
     const int base = tg.base[I.k], n = tg.n[I.k], capk = tg.cap[I.k], S = tg.S[I.k];
     if (I.tb * 256 >= n) return;   // item listed for rows the tangle does not fill (yet)
     const int loc = I.tb * 256 + tid, i = base + loc;
-    const bool act = loc < n && s.infront[i] != 0;
+    // a point whose links leave its tangle is skipped like an empty one (the
+    // context's state is checked on upload; caller memory is not)
+    const int fi0 = loc < n ? s.infront[i] - 1 : -1, bi0 = loc < n ? s.behind[i] - 1 : -1;
+    const bool act = loc < n && fi0 >= base && fi0 < base + n && bi0 >= base && bi0 < base + n;
     double x0 = 0, x1 = 0, x2 = 0;
     int fi = -2, bi = -2;
     double best = 100.0;
     if (act) {
         x0 = s.x[i]; x1 = s.x[s.cap + i]; x2 = s.x[2 * s.cap + i];
-        fi = s.infront[i] - 1; bi = s.behind[i] - 1;
+        fi = fi0; bi = bi0;
         const int j2 = s.infront[fi] - 1;
-        if (j2 >= 0 && j2 != i && j2 != fi && j2 != bi && s.infront[j2] != 0) {
+        if (j2 >= base && j2 < base + n && j2 != i && j2 != fi && j2 != bi && s.infront[j2] != 0) {
             const double d0 = x0 - s.x[j2], d1 = x1 - s.x[s.cap + j2], d2 = x2 - s.x[2 * s.cap + j2];
             const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
             if (d < 100.0) best = fmin(100.0, d * (1.0 + 4.0e-16) + 1.0e-300);
@@ -1094,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_pclose(DState s, Tangles tg, const Item
 __global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, const int *pj) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows || s.infront[i] == 0) return;
-    const int k = tg.row2t[i];
+    const int k = tg.row2t ? tg.row2t[i] : 0;   // no row table: one tangle
     const int capk = tg.cap[k], loc = i - tg.base[k], S = tg.S[k];
     double best = 100.0;
     int bj = -1;
@@ -1108,6 +1111,25 @@ __global__ void k_pclose_fin(DState s, Tangles tg, int rows, const double *pd, c
     }
     s.closestd[i] = best;
     if (bj >= 0) s.closest[i] = bj + 1;
+}
+
+// Tables and work list of the one-tangle close search in a caller-given
+// block (nvk_pclose_flat): tangle 0 owns rows [0, n), S chunks per target.
+struct FlatTab {
+    int base, cap, n, S;
+    long long poff;
+};
+
+__global__ void k_pclose_flat_setup(FlatTab *tab, Item *items, int n, int S, int nitems) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {
+        tab->base = 0;
+        tab->cap = n;
+        tab->n = n;
+        tab->S = S;
+        tab->poff = 0;
+    }
+    if (q < nitems) items[q] = Item{0, q / S, q % S, 0};
 }
 
 // ------------------------------------------------------- step begin / end
@@ -1243,6 +1265,54 @@ hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item
     if (rows > 0) hipLaunchKernelGGL(k_seg_spheres, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, sp);
     if (nitems > 0) hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, sp, pd, pj);
     if (rows > 0) hipLaunchKernelGGL(k_pclose_fin, GRID(rows), 0, st, s, tg, rows, pd, pj);
+    return hipGetLastError();
+}
+
+// The brute close search of one tangle whose arrays the caller holds:
+// positions component-major with leading dimension ld (x[k*ld + r]), its
+// working memory (tables, work list, segment spheres, chunk partials) laid
+// out in `work`, which must hold nvk_pclose_flat_bytes(n, S) bytes.
+static size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+size_t nvk_pclose_flat_bytes(int n, int S) {
+    const size_t ntb = ((size_t)n + 255) / 256;
+    return up256(sizeof(FlatTab)) + up256(sizeof(Item) * ntb * S) + up256(sizeof(Sphere64) * ntb * 4) +
+           up256(sizeof(double) * (size_t)S * n) + up256(sizeof(int) * (size_t)S * n);
+}
+
+hipError_t nvk_pclose_flat(hipStream_t st, const double *x, int ld, const int *infront, const int *behind,
+                           int *closest, double *closestd, int n, int S, void *work) {
+    if (n <= 0) return hipSuccess;
+    const int ntb = (n + 255) / 256, nitems = ntb * S;
+    char *w = (char *)work;
+    FlatTab *tab = (FlatTab *)w;
+    w += up256(sizeof(FlatTab));
+    Item *items = (Item *)w;
+    w += up256(sizeof(Item) * (size_t)nitems);
+    Sphere64 *sp = (Sphere64 *)w;
+    w += up256(sizeof(Sphere64) * (size_t)ntb * 4);
+    double *pd = (double *)w;
+    w += up256(sizeof(double) * (size_t)S * n);
+    int *pj = (int *)w;
+    DState s{};
+    s.cap = ld;
+    s.x = const_cast<double *>(x);
+    s.infront = const_cast<int *>(infront);
+    s.behind = const_cast<int *>(behind);
+    s.closest = closest;
+    s.closestd = closestd;
+    Tangles tg{};
+    tg.K = 1;
+    tg.base = &tab->base;
+    tg.cap = &tab->cap;
+    tg.n = &tab->n;
+    tg.S = &tab->S;
+    tg.poff = &tab->poff;
+    tg.row2t = nullptr;
+    hipLaunchKernelGGL(k_pclose_flat_setup, dim3((nitems + 255) / 256), dim3(256), 0, st, tab, items, n, S, nitems);
+    hipLaunchKernelGGL(k_seg_spheres, dim3(ntb), dim3(256), 0, st, s, n, sp);
+    hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, sp, pd, pj);
+    hipLaunchKernelGGL(k_pclose_fin, GRID(n), 0, st, s, tg, n, pd, pj);
     return hipGetLastError();
 }
 

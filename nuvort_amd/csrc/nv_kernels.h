@@ -76,3 +76,8 @@ hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, cons
 hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, void *spheres,
                       double *pd, int *pj);
 size_t nvk_sphere_bytes();
+// the brute close search of one tangle in caller memory: positions [3][ld],
+// working memory `work` of nvk_pclose_flat_bytes(n, S) bytes (S chunks per target)
+size_t nvk_pclose_flat_bytes(int n, int S);
+hipError_t nvk_pclose_flat(hipStream_t st, const double *x, int ld, const int *infront, const int *behind,
+                           int *closest, double *closestd, int n, int S, void *work);

@@ -144,6 +144,14 @@ module nuvort_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
     end function
+    !> pclose on device arrays the caller holds, allocating nothing (x component-major,
+    !> leading dimension ld; stream a hipStream_t or c_null_ptr); returns before the kernels end
+    integer(c_int) function nv_pclose_noalloc(n, ld, x, infront, behind, closest, closestd, stream) &
+        bind(C, name='nv_pclose_noalloc')
+      import :: c_int, c_int32_t, c_ptr
+      integer(c_int32_t), value :: n, ld
+      type(c_ptr), value :: x, infront, behind, closest, closestd, stream
+    end function
     !> cache_path: a c_null_char-terminated file name; c_null_char alone tunes without a cache
     integer(c_int) function nv_autotune(ctx, cache_path) bind(C, name='nv_autotune')
       import :: c_ptr, c_int, c_char

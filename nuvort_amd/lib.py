@@ -68,7 +68,7 @@ EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_s
            "nv_get_state_batch", "nv_set_time", "nv_set_noise", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
            "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
-           "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune")
+           "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune", "nv_pclose_noalloc")
 
 _lib = None
 
@@ -99,6 +99,8 @@ def load(path: str = LIB_PATH):
         "nv_get_tree": [vp, C.POINTER(NvTreeNode), C.c_int32, C.POINTER(C.c_int32)],
         "nv_get_stats": [vp, C.POINTER(NvStats)],
         "nv_autotune": [vp, C.c_char_p],
+        # device pointers (x, infront, behind, closest, closestd) and a hipStream_t
+        "nv_pclose_noalloc": [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp],
     }
     for name in EXPORTS:
         fn = getattr(L, name)
