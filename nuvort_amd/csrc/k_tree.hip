@@ -576,6 +576,9 @@ __device__ __forceinline__ bool next_walk_item(const WalkQueues &wq, int &h, int
     return false;
 }
 
+constexpr double OPEN_BAND = 4.8e-7;
+constexpr int OPEN_BAND_ULPS = 16;
+
 // WNode -> Node48.  The reference accepts a node when pcount == 1 or
 // width / (dist - geo) < theta (width / dist when dist <= geo or without the
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
@@ -597,7 +600,7 @@ __global__ void k_pack48(DState s, const WNode *__restrict__ walk, int nwalk, do
         circ64[3 * q + k] = w.circ[k];
     }
     const double r = (corr ? w.geo : 0.0) + w.width / theta;
-    o.r2 = (float)(r * r) * (1.0f + 2.0e-7f);
+    o.r2 = __double2float_ru(r * r * (1.0 + OPEN_BAND));   // the upper bound of the exact band
     o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1 : w.skip;
 #pragma unroll
     for (int k = 0; k < 3; ++k) o.e[k] = (float)w.circ[k];
@@ -657,13 +660,47 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 // targets per lane of the FP32 walk
 constexpr int WALK_TPT = 2;
 
+// The band around a node's opening radius squared r^2 inside which the walk
+// decides the reference's opening test in FP64 exactly.  The record holds
+// its upper bound, r^2 (1 + 4.8e-7) rounded up to FP32; its lower bound is
+// 16 float ulps below that, at least 9.5e-7 r^2 lower.  The walk compares its
+// squared distance rounded to FP32 (2^-24 = 6e-8 relative; it differs from
+// the reference's FP64 distance squared by under 1e-11 relative) with both.
+// (OPEN_BAND, OPEN_BAND_ULPS: defined before k_pack48)
+
+// The reference's opening test for target row `row`, node centroid `cent`,
+// shift `sm` (src/tree.f90:263-278), with the FP64 walk's arithmetic: true
+// when the node is accepted as a whole.
+__device__ __forceinline__ bool ref_accepts(const DState &s, int row, const double cent[3], const double sm[3],
+                                            double width, double geo, double theta_c, int corr) {
+    const double e0 = s.x[row] - (cent[0] + sm[0]);
+    const double e1 = s.x[s.cap + row] - (cent[1] + sm[1]);
+    const double e2 = s.x[2 * s.cap + row] - (cent[2] + sm[2]);
+    const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    double theta;
+    if (corr) {
+        if (dist - geo > 0.0) theta = width / (dist - geo);
+        else theta = width / dist;
+    } else {
+        theta = width / dist;
+    }
+    return theta < theta_c;
+}
+
+// Five waves per SIMD (96 VGPRs): the exact opening test of the band is
+// rare (about a hundred lanes per walk of 1M points), and the few registers
+// the compiler spills to fit are spilled there.  That test, although rarely
+// taken, costs the loop about 10% (more scalar registers spilled to vector
+// lanes); decided after the walk instead, by walking the items it touched
+// again, it cost more, since one item's walk is a long chain of dependent
+// record fetches.
 template <int BT, int TPT>
-__global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_walk_vel32(DState s, Phys p, Shifts sh, int n, int npad,
                                                    const int *__restrict__ order, int nord, WalkQueues wq,
                                                    const Node48 *__restrict__ walk,
-                                                   const double *__restrict__ circ64, int nwalk,
-                                                   double rn2, double sing2, double *__restrict__ part, DCtl *ctl,
-                                                   unsigned long long *counters) {
+                                                   const double *__restrict__ circ64, const WNode *__restrict__ wn,
+                                                   int nwalk, double rn2, double sing2, double *__restrict__ part,
+                                                   DCtl *ctl, unsigned long long *counters) {
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     const double kappa = p.kappa;
@@ -731,6 +768,12 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         double v[TPT][3], dd[TPT];
         bool down = false;
         unsigned long long anyfar = 0ull, anyneed = 0ull;
+        // accepted surely above r2hi, opened surely below r2lo; between them
+        // the reference's own test decides
+        // the record holds the band's upper bound; the lower one lies
+        // OPEN_BAND_ULPS float ulps below it (an integer step on the scalar unit)
+        const float r2hi = nd.r2;
+        const float r2lo = __int_as_float(__float_as_int(nd.r2) - OPEN_BAND_ULPS);
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
             const unsigned long long mvis = wave_ballot(cur >= lskip[t]);
@@ -738,7 +781,8 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
 #pragma unroll
             for (int k = 0; k < 3; ++k) v[t][k] = nd.cent[k] - xs[t][k];
             dd[t] = __fma_rn(v[t][0], v[t][0], __fma_rn(v[t][1], v[t][1], v[t][2] * v[t][2]));
-            const unsigned long long mgt = wave_ballot(dd[t] > (double)nd.r2);
+            const float ddf = (float)dd[t];
+            unsigned long long mgt = wave_ballot(ddf > r2hi);
             unsigned long long macc, muse, mnear;
             if (leaf) {
                 const int j = -w - 1;
@@ -751,6 +795,13 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                 muse = mvis & ~mexcl & ~msg;
                 mnear = muse & mrn & (j > 0 ? ~0ull : 0ull);
             } else {
+                const unsigned long long mband = mvis & ~mgt & ~wave_ballot(ddf < r2lo);
+                if (mband != 0ull) {
+                    const double wdt = wn[cur].width, geo = wn[cur].geo;
+                    bool ok = false;
+                    if (lane_in(mband)) ok = ref_accepts(s, r[t], nd.cent, sh.s[m], wdt, geo, p.tree_theta, p.extra_corr);
+                    mgt |= wave_ballot(ok) & mband;
+                }
                 macc = mvis & mgt;
                 muse = macc;
                 mnear = 0ull;
@@ -764,6 +815,7 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
         if (anyfar != 0ull) {
             const NodeCold cd = *(const NodeCold *)(wb + (size_t)cur * sizeof(Node48) + sizeof(NodeHot));
             const float e0 = cd.e[0], e1 = cd.e[1], e2 = cd.e[2];
+            const float kee = 64.0f * 5.96e-8f * cd.ee;
 #pragma unroll
             for (int t = 0; t < TPT; ++t) {
                 if (TPT > 1 && mfar[t] == 0ull) continue;
@@ -778,15 +830,18 @@ __global__ __launch_bounds__(BT) void k_walk_vel32(DState s, Phys p, Shifts sh, 
                 const float inv2 = __builtin_amdgcn_rsqf(r2sq);
                 const float r12 = (r1sq * inv1) * (r2sq * inv2);
                 const float den = r12 + (r1sq + de);
-                // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
-                // the pseudo-segment ending close to the target (r2 < r1 / 4, where
-                // the FP32 r2^2 can even round to <= 0); NaN goes there too
-                const unsigned long long mill =
-                    wave_ballot(!(den >= 0.0625f * r12)) | wave_ballot(!(r2sq >= 0.0625f * r1sq));
                 // the reference skips 4ac - b^2 = 4 |d x e|^2 below epsilon
                 // (x 4 is exact: |d x e|^2 < epsilon / 4)
                 const float xx = __fmaf_rn(x0, x0, __fmaf_rn(x1, x1, x2 * x2));
-                const unsigned long long mok = mfar[t] & ~mill & ~wave_ballot(xx < 0.25f * (float)NV_EPS);
+                // FP64 where the FP32 form loses digits: R1.R2 cancelling r1 r2, or
+                // the pseudo-segment ending close to the target (r2 < r1 / 4, where
+                // the FP32 r2^2 can even round to <= 0), or |d x e|^2 within the FP32
+                // form's rounding (64 ulps of |d|^2 |e|^2) of the skip threshold or
+                // under it, where the FP64 term decides the skip; NaN goes there too
+                const unsigned long long mill = wave_ballot(!(den >= 0.0625f * r12)) |
+                                                wave_ballot(!(r2sq >= 0.0625f * r1sq)) |
+                                                wave_ballot(!(xx >= __fmaf_rn(kee, r1sq, 0.25f * (float)NV_EPS)));
+                const unsigned long long mok = mfar[t] & ~mill;
                 // x is finite for every lane, so x * 0 = 0 where the term is not taken
                 const float fac = lane_in(mok) ? (inv1 + inv2) * __builtin_amdgcn_rcpf(den) : 0.0f;
                 fa[t][0] += (double)(x0 * fac);
@@ -1167,10 +1222,10 @@ static void free_nodes(Nodes &nd) {
     int **ip[] = {&nd.beg, &nd.cnt, &nd.pcount, &nd.dropb, &nd.first, &nd.parent, &nd.minrow, &nd.size, &nd.dfs,
                   &nd.leafinf};
     for (int **p : ip)
-        if (*p) { hipFree(*p); *p = nullptr; }
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
     double **dp[] = {&nd.pos, &nd.w, &nd.sum, &nd.circ, &nd.rad, &nd.cent};
     for (double **p : dp)
-        if (*p) { hipFree(*p); *p = nullptr; }
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
     nd.ncap = 0;
 }
 
@@ -1488,8 +1543,8 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         const int grid = std::min(t->grid32, (nit32 + 3) / 4);
         TCHK(nv_fill_int(st, t->wq, 0, XCDS));
         hipLaunchKernelGGL((k_walk_vel32<256, WALK_TPT>), dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           WalkQueues{t->wq, nbt, sh.n}, t->walk48, t->circ64, t->nwalk, rn2, sqrt_threshold(NV_EPS),
-                           part, ctl, t->counters);
+                           WalkQueues{t->wq, nbt, sh.n}, t->walk48, t->circ64, t->walk, t->nwalk, rn2,
+                           sqrt_threshold(NV_EPS), part, ctl, t->counters);
     } else {
         if (t->nwalk > 0)
             hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
