@@ -42,12 +42,15 @@ enum {
     NV_ERR_SINGULAR = 5,     /* tree singularity: src/tree.f90:289-295 */
     NV_ERR_NAN = 6,          /* NaN in a position: src/general.f90:172-191 */
     NV_ERR_UNSUPPORTED = 7,  /* solid walls / pinned points / other modes */
-    NV_ERR_TREE_DEPTH = 8,   /* octree deeper than NV_TREE_MAX_LEVELS */
+    NV_ERR_TREE_DEPTH = 8,   /* octree deeper than NV_TREE_MAX_LEVELS (points closer than box / 2^48) */
     NV_ERR_TIMESTEP = 9,     /* dt above the stability bound: src/initial.f90:169-180 */
     NV_ERR_STATE = 10        /* call out of order (e.g. walk without a tree) */
 };
 
-#define NV_TREE_MAX_LEVELS 16
+/* Octree levels the build supports: box / 2^48 is 7e-16 cm in a 0.2 cm box,
+ * a few units in the last place of a position; points closer than that are
+ * coincident in practice (the reference's recursion does not end on them). */
+#define NV_TREE_MAX_LEVELS 48
 
 enum { NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2 };
 enum { NV_BC_OPEN = 0, NV_BC_PERIODIC = 1 };

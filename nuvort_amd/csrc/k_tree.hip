@@ -33,7 +33,15 @@
 
 namespace {
 
-constexpr int LEVELS = NV_TREE_MAX_LEVELS;   // 16 levels x 4 bits = 64-bit key
+// A point's path is kept as up to KEY_WORDS 64-bit words of KEY_LEVELS
+// 4-bit digits.  A build uses one word (16 levels) until a node at its
+// deepest level still holds more than one point; the tree is then built
+// again with one more word (the reference recurses without a bound,
+// src/tree.f90:64-104).
+constexpr int KEY_LEVELS = 16;
+constexpr int KEY_WORDS = 3;
+constexpr int LEVELS = KEY_LEVELS * KEY_WORDS;   // NV_TREE_MAX_LEVELS
+static_assert(LEVELS == NV_TREE_MAX_LEVELS, "the header states the depth bound");
 
 struct __align__(16) WNode {
     double cent[3];
@@ -88,23 +96,41 @@ __device__ __forceinline__ R pick64(bool c, const R &a, const R &b) {
     return o;
 }
 
-__device__ __forceinline__ int digit_at(unsigned long long key, int level) {
-    return (int)((key >> (4 * (LEVELS - level))) & 0xFull);
+// The sorted paths: word 0 (levels 1-16) and, in deep builds, words 1 and 2
+// (levels 17-32, 33-48), each in the sorted order.
+struct Keys {
+    const unsigned long long *w0, *w1, *w2;
+};
+
+// the digit of level `level` (1-based) of the path at sorted position pos
+__device__ __forceinline__ int key_digit(const Keys &k, int pos, int level) {
+    const int lv = level - 1, word = lv / KEY_LEVELS, dg = lv - word * KEY_LEVELS;
+    const unsigned long long v = word == 0 ? k.w0[pos] : (word == 1 ? k.w1[pos] : k.w2[pos]);
+    return (int)((v >> (4 * (KEY_LEVELS - 1 - dg))) & 0xFull);
 }
 
-__global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals) {
+// The paths of every row through nwords * 16 levels: word 0 into keys,
+// words 1.. into deep[0..] (by row); inactive rows get all ones, which
+// sorts them last.
+__global__ void k_keys(DState s, int n, double bsize, int nwords, unsigned long long *keys,
+                       unsigned long long *deep0, unsigned long long *deep1, int *vals) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     vals[r] = r;
     if (s.infront[r] == 0) {
         keys[r] = ~0ull;
+        if (nwords > 1) deep0[r] = ~0ull;
+        if (nwords > 2) deep1[r] = ~0ull;
         return;
     }
     const double x[3] = {s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r]};
     double pos[3] = {-bsize / 2.0, -bsize / 2.0, -bsize / 2.0};
     double w = bsize;
     unsigned long long key = 0;
-    for (int lev = 1; lev <= LEVELS; ++lev) {
+    int word = 0;
+    bool out = false;
+    for (int lev = 1; lev <= KEY_LEVELS * nwords; ++lev) {
+        const int dg = lev - 1 - word * KEY_LEVELS;   // digit within the word
         int dig = 0;
         bool in = true;
         double lo[3];
@@ -118,14 +144,35 @@ __global__ void k_keys(DState s, int n, double bsize, unsigned long long *keys, 
             dig |= b << (k == 0 ? 0 : (k == 1 ? 2 : 1));
         }
         if (!in) {
-            key |= 8ull << (4 * (LEVELS - lev));
-            break;
+            key |= 8ull << (4 * (KEY_LEVELS - 1 - dg));
+            out = true;
+        } else {
+            key |= (unsigned long long)dig << (4 * (KEY_LEVELS - 1 - dg));
+            pos[0] = lo[0]; pos[1] = lo[1]; pos[2] = lo[2];
+            w = cw;
         }
-        key |= (unsigned long long)dig << (4 * (LEVELS - lev));
-        pos[0] = lo[0]; pos[1] = lo[1]; pos[2] = lo[2];
-        w = cw;
+        if (out || dg == KEY_LEVELS - 1) {
+            // the word is complete (the rest of the path is zeros)
+            if (word == 0) keys[r] = key;
+            else if (word == 1) deep0[r] = key;
+            else deep1[r] = key;
+            key = 0;
+            ++word;
+            if (out) break;
+        }
     }
-    keys[r] = key;
+    for (; word < nwords; ++word) {
+        if (word == 0) keys[r] = 0ull;
+        else if (word == 1) deep0[r] = 0ull;
+        else deep1[r] = 0ull;
+    }
+}
+
+// out[i] = in[perm[i]]
+__global__ void k_gather_keys(const unsigned long long *__restrict__ in, const int *__restrict__ perm, int n,
+                              unsigned long long *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
 }
 
 // The number of active points, from the sorted keys: an active key's top
@@ -189,6 +236,8 @@ __global__ void k_split_flags(Nodes nd, int L0, int L1, int *flags, int at_max, 
 // k_split_flags for a level whose range is on the device: flags[0, bound),
 // zero past the level's end so that a scan over the bound is the level's
 __global__ void k_split_flags_d(Nodes nd, const LevelCtl *lc, int level, int bound, int *flags, DCtl *ctl) {
+    // (only the deepest level any build has, LEVELS, reports the depth as an
+    // error on the device; shallower builds are redone deeper by the host)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bound) return;
     const int L0 = lc->lvl[level], m = lc->lvl[level + 1] - L0;
@@ -202,7 +251,7 @@ __global__ void k_split_flags_d(Nodes nd, const LevelCtl *lc, int level, int bou
 
 // after a level's scan: the next level's start, the children kernel, and
 // whether the level fit its bounds (one thread)
-__global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wgrid, const int *offs,
+__global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int maxlev, int bound, int wgrid, const int *offs,
                             const int *tsum) {
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1], m = L1 - L0;
     int next = L1, wide = 0;
@@ -211,7 +260,7 @@ __global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wg
         lc->ovf_level = level;
     } else if (m > 0 && !lc->overflow) {
         const int nsplit = offs[m - 1] + tsum[(m - 1) / NV_SCAN_TILE] + (nd.pcount[L1 - 1] > 1 ? 1 : 0);
-        if (nsplit && level == LEVELS) {
+        if (nsplit && level == maxlev) {
             lc->depth = 1;
         } else if (nsplit) {
             if ((long long)L1 + 8ll * nsplit > (long long)nd.ncap) {
@@ -227,10 +276,10 @@ __global__ void k_level_end(Nodes nd, LevelCtl *lc, int level, int bound, int wg
     lc->lvl[level + 2] = next;
 }
 
-__device__ __forceinline__ int lower_digit(const unsigned long long *keys, int lo, int hi, int level, int d) {
+__device__ __forceinline__ int lower_digit(const Keys &keys, int lo, int hi, int level, int d) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (digit_at(keys[mid], level) < d) lo = mid + 1;
+        if (key_digit(keys, mid, level) < d) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -295,7 +344,7 @@ __device__ __forceinline__ int scanned(const int *off, const int *tsum, int i) {
 }
 
 __device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int L1, const int *off,
-                                            const int *tsum, const unsigned long long *keys, int level) {
+                                            const int *tsum, const Keys &keys, int level) {
     for_split8(q, q < L1 && nd.pcount[q] > 1, [&](int p, int c) {
         // lane c: the first key of p's range with digit >= c + 1 at the next
         // level (the digits ascend over the range), the end of child c
@@ -311,13 +360,12 @@ __device__ __forceinline__ void children_of(const Nodes &nd, int q, int L0, int 
     });
 }
 
-__global__ __launch_bounds__(256) void k_children(Nodes nd, int L0, int L1, const int *off,
-                                                  const unsigned long long *keys, int level) {
+__global__ __launch_bounds__(256) void k_children(Nodes nd, int L0, int L1, const int *off, Keys keys, int level) {
     children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, nullptr, keys, level);
 }
 
 __global__ __launch_bounds__(256) void k_children_d(Nodes nd, const LevelCtl *lc, const int *off, const int *tsum,
-                             const unsigned long long *keys, int level) {
+                                                    Keys keys, int level) {
     if (lc->overflow || lc->wide[level]) return;
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
     children_of(nd, L0 + blockIdx.x * blockDim.x + threadIdx.x, L0, L1, off, tsum, keys, level);
@@ -327,7 +375,7 @@ __global__ __launch_bounds__(256) void k_children_d(Nodes nd, const LevelCtl *lc
 // 256 per node: 32 lanes find each of the 8 child boundaries by a 32-way
 // search (a node's keys are ascending in the next level's digit).
 __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0, int L1, const int *off,
-                                                 const int *tsum, const unsigned long long *keys, int level) {
+                                                 const int *tsum, const Keys &keys, int level) {
     __shared__ int bnd[9];
     if (q >= L1 || nd.pcount[q] <= 1) return;
     const int b0 = nd.beg[q], b1 = nd.beg[q] + nd.cnt[q];
@@ -337,7 +385,7 @@ __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0,
     while (hi > lo) {
         const int step = (hi - lo + 31) / 32;
         const int pos = lo + li * step;
-        const bool below = pos < hi && digit_at(keys[pos], level + 1) < d;
+        const bool below = pos < hi && key_digit(keys, pos, level + 1) < d;
         const int c = __popcll((wave_ballot(below) >> sh) & 0xffffffffull);
         if (c == 0) { hi = lo; break; }
         const int nlo = lo + (c - 1) * step + 1, nhi = min(hi, lo + c * step);
@@ -355,13 +403,13 @@ __device__ __forceinline__ void children_wide_of(const Nodes &nd, int q, int L0,
     }
 }
 
-__global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off,
-                                                       const unsigned long long *keys, int level) {
+__global__ __launch_bounds__(256) void k_children_wide(Nodes nd, int L0, int L1, const int *off, Keys keys,
+                                                       int level) {
     children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, nullptr, keys, level);
 }
 
 __global__ __launch_bounds__(256) void k_children_wide_d(Nodes nd, const LevelCtl *lc, const int *off,
-                                                         const int *tsum, const unsigned long long *keys, int level) {
+                                                         const int *tsum, Keys keys, int level) {
     if (lc->overflow || !lc->wide[level]) return;
     const int L0 = lc->lvl[level], L1 = lc->lvl[level + 1];
     children_wide_of(nd, L0 + blockIdx.x, L0, L1, off, tsum, keys, level);
@@ -1028,6 +1076,7 @@ struct CNode {
     float rad;   // every centroid and point below lies within rad of cent
 };
 constexpr int CLOSE_SKIP_BITS = 27;
+constexpr int CLOSE_DEEP = 15;   // the largest level code the 4 bits above the skip index hold
 
 
 // With d = RN(sqrt(dd)) and q = RN(w / d): q < 0.5 <=> w / d < 0.5 - 2^-55
@@ -1048,8 +1097,10 @@ __global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const flo
     CNode o;
 #pragma unroll
     for (int k = 0; k < 3; ++k) o.cent[k] = w.cent[k];
-    // width = bsize 2^-level exactly (the build halves it level by level)
-    const int level = ilogb(bsize / w.width);
+    // width = bsize 2^-level exactly (the build halves it level by level);
+    // levels from CLOSE_DEEP on (deep trees only) are marked CLOSE_DEEP and
+    // the walk reads their width from the full record
+    const int level = min(ilogb(bsize / w.width), CLOSE_DEEP);
     o.word = w.pcount == 1 ? -(w.leaf_infront > 0 ? s.behind[w.leaf_infront - 1] : 0) - 1
                            : w.skip | (level << CLOSE_SKIP_BITS);
     o.rad = wrad[q];
@@ -1115,7 +1166,14 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
             // lo, decided by the reference's division in between
             // 4 width^2 = 4 RN(bsize^2) 2^-2 level exactly, as k_pack_close and
             // the reference's width give it
-            const double w4 = ldexp(b4, -2 * (w >> CLOSE_SKIP_BITS));
+            const int lcode = w >> CLOSE_SKIP_BITS;
+            double w4;
+            if (lcode < CLOSE_DEEP) {
+                w4 = ldexp(b4, -2 * lcode);
+            } else {
+                const double fw = full[cur].width;
+                w4 = 4.0 * (fw * fw);   // = 4 RN(bsize^2) 2^-2 level: scaling by 2^-level is exact
+            }
             const double lo = w4 * (1.0 - CLOSE_BAND), hi = w4 * (1.0 + CLOSE_BAND);
             const unsigned long long mfar = wave_ballot(dd > hi), mnear = wave_ballot(dd < lo);
             const unsigned long long mband = mvis & ~mfar & ~mnear;
@@ -1173,6 +1231,11 @@ struct TreeBuf {
     int cap = 0;                         // rows the key buffers hold
     unsigned long long *k0 = nullptr, *k1 = nullptr;
     int *v0 = nullptr, *v1 = nullptr;
+    // deep builds: words 0.. of the paths by row (raw) and words 1.. sorted
+    int kwords = 1;                      // path words of the builds (grows when a tree is deeper)
+    int maxlev = KEY_LEVELS;             // = KEY_LEVELS * kwords
+    unsigned long long *raw[KEY_WORDS] = {}, *srt[KEY_WORDS - 1] = {};
+    int deep_cap = 0;
     int *hist = nullptr, *tmp = nullptr;
     int hist_cap = 0;
     int *nact = nullptr;                 // device
@@ -1241,6 +1304,8 @@ void nv_tree_free(TreeBuf *t) {
     hipFree(t->wq);
     hipHostFree(t->h_int);
     hipFree(t->lc);
+    for (auto *q : t->raw) hipFree(q);
+    for (auto *q : t->srt) hipFree(q);
     hipHostFree(t->lc_h);
     free_nodes(t->nd);
     delete t;
@@ -1303,6 +1368,52 @@ static int ensure_flags(TreeBuf *t, int m) {
     return NV_OK;
 }
 
+Keys sorted_keys(const TreeBuf *t) {
+    return Keys{t->k0, t->kwords > 1 ? t->srt[0] : nullptr, t->kwords > 2 ? t->srt[1] : nullptr};
+}
+
+// buffers of the path words of a deep build
+static int ensure_deep(TreeBuf *t, int n) {
+    if (n <= t->deep_cap) return NV_OK;
+    const int cap = std::max(n, t->cap);
+    for (auto *&q : t->raw) { (void)hipFree(q); q = nullptr; }
+    for (auto *&q : t->srt) { (void)hipFree(q); q = nullptr; }
+    for (auto *&q : t->raw) TCHK(hipMalloc(&q, sizeof(unsigned long long) * (size_t)cap));
+    for (auto *&q : t->srt) TCHK(hipMalloc(&q, sizeof(unsigned long long) * (size_t)cap));
+    t->deep_cap = cap;
+    return NV_OK;
+}
+
+// The paths of rows [0, n) sorted (stable LSD: the last word first), word 0
+// in k0 and the rows in v0 (t->vals); deeper words gathered into srt[].
+static int sort_paths(TreeBuf *t, hipStream_t st, DState d, int n, double bsize) {
+    const int K = t->kwords;
+    const dim3 g((n + 255) / 256), b(256);
+    if (K == 1) {
+        hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, bsize, 1, t->k0, (unsigned long long *)nullptr,
+                           (unsigned long long *)nullptr, t->v0);
+        TCHK(hipGetLastError());
+        // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
+        TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+        return NV_OK;
+    }
+    if (int rc = ensure_deep(t, n)) return rc;
+    hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, bsize, K, t->raw[0], t->raw[1], t->raw[2], t->v0);
+    TCHK(hipGetLastError());
+    // v0 is the identity here: each pass sorts the rows by one more word
+    for (int w = K - 1; w >= 0; --w) {
+        hipLaunchKernelGGL(k_gather_keys, g, b, 0, st, (const unsigned long long *)t->raw[w], (const int *)t->v0, n,
+                           t->k0);
+        TCHK(hipGetLastError());
+        TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+    }
+    for (int w = 1; w < K; ++w)
+        hipLaunchKernelGGL(k_gather_keys, g, b, 0, st, (const unsigned long long *)t->raw[w], (const int *)t->v0, n,
+                           t->srt[w - 1]);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
 // Level loop with a host round trip per level: the level sizes decide the
 // launches. Leaves t->lvl and t->nwalk; nodes are moments-complete.
 // From level `level` on, t->lvl holding the starts of levels 0 .. level + 1
@@ -1316,21 +1427,22 @@ static int level_loop_sync(TreeBuf *t, hipStream_t st, DState d, DCtl *ctl, int 
         // split flags, scanned in place into the children offsets
         hipLaunchKernelGGL(k_split_flags, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->offs,
                            level == LEVELS ? 1 : 0, ctl);
+        const Keys keys = sorted_keys(t);
         TCHK(hipGetLastError());
         TCHK(nv_exclusive_scan(st, t->offs, m, t->scan_tmp));
         TCHK(hipMemcpyAsync(t->h_int, t->offs + (m - 1), sizeof(int), hipMemcpyDeviceToHost, st));
         TCHK(hipMemcpyAsync(t->h_int + 1, t->nd.pcount + (L1 - 1), sizeof(int), hipMemcpyDeviceToHost, st));
         TCHK(hipStreamSynchronize(st));
         const int nsplit = t->h_int[0] + (t->h_int[1] > 1 ? 1 : 0);
-        if (nsplit == 0 || level == LEVELS) {
-            if (nsplit && level == LEVELS) return NV_ERR_TREE_DEPTH;
+        if (nsplit == 0 || level == t->maxlev) {
+            if (nsplit && level == t->maxlev) return NV_ERR_TREE_DEPTH;
             break;
         }
         if (int rc = grow_nodes(t, st, L1 + 8 * nsplit, L1)) return rc;
         if (m <= 4096 && nsplit * 64 <= t->nact_h)   // few nodes holding many keys each
-            hipLaunchKernelGGL(k_children_wide, dim3(m), dim3(256), 0, st, t->nd, L0, L1, t->offs, t->k0, level);
+            hipLaunchKernelGGL(k_children_wide, dim3(m), dim3(256), 0, st, t->nd, L0, L1, t->offs, keys, level);
         else
-            hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->offs, t->k0,
+            hipLaunchKernelGGL(k_children, dim3((m + 255) / 256), dim3(256), 0, st, t->nd, L0, L1, t->offs, keys,
                                level);
         TCHK(hipGetLastError());
         t->lvl.push_back(L1 + 8 * nsplit);
@@ -1373,7 +1485,8 @@ constexpr int BUILD_RETRY = -1000;
 static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
     const int prev = (int)t->lvl.size() - 1;   // levels of the previous build
     if (prev < 1 || !t->lc) return BUILD_RETRY;
-    const int top = std::min(prev, LEVELS);    // deepest level launched
+    const int top = std::min(prev, t->maxlev);   // deepest level launched
+    const Keys keys = sorted_keys(t);
     int bound[LEVELS + 1], wgrid[LEVELS + 1];
     long long total = 1;
     int fmax = 1;
@@ -1396,14 +1509,14 @@ static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, doubl
                            bound[l], t->offs, ctl);
         TCHK(hipGetLastError());
         TCHK(nv_scan_tiles_sums(st, t->offs, bound[l], t->scan_tmp));
-        hipLaunchKernelGGL(k_level_end, dim3(1), dim3(1), 0, st, t->nd, t->lc, l, bound[l], wgrid[l],
+        hipLaunchKernelGGL(k_level_end, dim3(1), dim3(1), 0, st, t->nd, t->lc, l, t->maxlev, bound[l], wgrid[l],
                            (const int *)t->offs, (const int *)t->scan_tmp);
         TCHK(hipGetLastError());
         if (wgrid[l] > 0)
             hipLaunchKernelGGL(k_children_wide_d, dim3(wgrid[l]), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
-                               (const int *)t->offs, (const int *)t->scan_tmp, (const unsigned long long *)t->k0, l);
+                               (const int *)t->offs, (const int *)t->scan_tmp, keys, l);
         hipLaunchKernelGGL(k_children_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc,
-                           (const int *)t->offs, (const int *)t->scan_tmp, (const unsigned long long *)t->k0, l);
+                           (const int *)t->offs, (const int *)t->scan_tmp, keys, l);
         TCHK(hipGetLastError());
     }
     for (int l = top; l >= 0; --l) {
@@ -1438,19 +1551,28 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     if (n <= 0) return NV_ERR_ZERO_LENGTH;
     if (int rc = ensure_rows(t, n)) return rc;
     const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
-    TCHK(hipMemsetAsync(t->nact, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
-    TCHK(hipGetLastError());
-    // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
-    TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
-    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
-    TCHK(hipGetLastError());
-    t->vals = t->v0;
-    t->built_rows = n;
-    int rc = build_levels_async(t, st, d, n, bsize, ctl);
-    if (rc == BUILD_RETRY) {
-        ++t->sync_builds;
-        rc = build_levels_sync(t, st, d, n, bsize, ctl);
+    int rc = NV_OK;
+    for (;;) {
+        t->maxlev = KEY_LEVELS * t->kwords;
+        TCHK(nv_fill_int(st, t->nact, 0, 1));
+        if ((rc = sort_paths(t, st, d, n, bsize))) return rc;
+        hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
+        TCHK(hipGetLastError());
+        t->vals = t->v0;
+        t->built_rows = n;
+        rc = build_levels_async(t, st, d, n, bsize, ctl);
+        if (rc == BUILD_RETRY) {
+            ++t->sync_builds;
+            rc = build_levels_sync(t, st, d, n, bsize, ctl);
+        }
+        if (rc == NV_ERR_TREE_DEPTH && t->kwords < KEY_WORDS) {
+            // a node of the deepest level still holds several points: build
+            // again with 16 more levels (later builds keep the depth)
+            ++t->kwords;
+            t->lvl.clear();
+            continue;
+        }
+        break;
     }
     if (rc) {
         t->lvl.clear();   // no hints from a failed build

@@ -159,6 +159,24 @@ def singular_state(delta: float, L: float) -> FilamentState:
     return tangles._assemble([A, B], (L, L, L))
 
 
+def deep_tree_state(delta: float) -> FilamentState:
+    """Three rings; a point of ring B is moved to 4e-7 cm from a point of ring
+    A and a point of ring C to 5e-12 cm from another point of ring A, so the
+    octree needs more than 16 (box / 2^16 = 1.5e-6 cm) and more than 32
+    levels (box / 2^32 = 2.3e-11 cm) to separate them (src/tree.f90:64-104
+    recurses while a node holds more than one point)."""
+    A = tangles.ring([0.0, 0.0, 0.0], 0.01, [0, 0, 1], delta)
+    B = tangles.ring([0.0, 0.0105, 0.0], 0.006, [1, 0, 0], delta)
+    Cc = tangles.ring([0.0, -0.0105, 0.0], 0.006, [1, 0, 0], delta)
+    ia = int(np.argmax(A[:, 1]))          # A's point nearest +y
+    ib = int(np.argmin(B[:, 1]))          # B's point nearest A
+    B[ib] = A[ia] + np.array([4.0e-7, 1.0e-7, -2.0e-7])
+    ja = int(np.argmin(A[:, 1]))
+    jc = int(np.argmax(Cc[:, 1]))
+    Cc[jc] = A[ja] + np.array([-5.0e-12, 3.0e-12, 0.0])
+    return tangles._assemble([A, B, Cc], (0.1, 0.1, 0.1))
+
+
 def fatal_cases(d5: float) -> None:
     rings = tangles.random_rings(7, 4, 0.004, 0.012, 0.1, d5)
     nan = rings.copy()
@@ -226,6 +244,8 @@ def main() -> None:
          "BS, noisey_xflow normal flow (seeded noise table) with friction", ("motion",), noise_seed=77)
     make("nf_channel_tree", late, base_params(velocity="Tree", tree_theta=0.4, normal_velocity="channel", **mfk), 2,
          "Tree theta 0.4, channel normal flow with friction", ("motion",))
+    make("tree_deep", deep_tree_state(d5), base_params(velocity="Tree", tree_theta=0.5), 1,
+         "Tree theta 0.5 with point pairs 4e-7 and 5e-12 cm apart (octree deeper than 16 and 32 levels)")
     dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
     few = ("motion", "close", "final")
     make("dense_bs_10step", dense, base_params(box_size=(0.025, 0.025, 0.025)), 10,
@@ -238,6 +258,15 @@ def main() -> None:
 if __name__ == "__main__":
     if sys.argv[1:] == ["fatal"]:
         fatal_cases(5e-4)
+    elif sys.argv[1:2] == ["only"]:
+        # one fixture by name (the others are left as they are)
+        want = set(sys.argv[2:])
+        real_make = make
+
+        def make(name, *a, **kw):   # noqa: F811
+            if name in want:
+                real_make(name, *a, **kw)
+        main()
     else:
         main()
         fatal_cases(5e-4)
