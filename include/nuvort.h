@@ -113,7 +113,23 @@ typedef struct {
     double *closestd;                                         /* [capacity] */
     int32_t *pinnedi, *pinnedb;                               /* [capacity] 0/1 */
     int32_t *wpinned;                                         /* [capacity][3] */
+    double *curv, *stretch;  /* [capacity] diagnostics (nv_diagnostics); may be NULL */
 } nv_state;
+
+/* The quantities of calculate_diagnostics (src/diagnostic.f90:9-74) and of
+ * the last pinsert (src/line.f90:38-39,95) that print_info writes to ts.log
+ * (src/output.f90:20-44), for a single tangle. */
+typedef struct {
+    int32_t active;          /* count(f%infront > 0) */
+    int32_t recon_count, remove_count, itime;
+    double t;
+    double maxu, maxdu;      /* max |u|, max |u1 - u2| over f(1:pcount) */
+    double mean_u_sup, mean_u_mf;   /* sum |u_sup|, |u_mf| over f(1:pcount) / active */
+    double kappa_bar, kappa_min, kappa_max;   /* curvature: mean over active, min over > 0, max */
+    double total_length;     /* pinsert: sum of |x - ghosti| over the points it started with */
+    double avg_sep;          /* total_length / (pcount at its start - empty slots at its end) */
+    int64_t tree_eval;       /* interactions of the last tree walk */
+} nv_diag;
 
 /* One reconnection, as src/reconnection.f90:46-49 logs it plus the pair
  * (i, j local to the tangle, 1-based; tangle = 0 for a single system). */
@@ -205,6 +221,36 @@ int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */
  * octree path a fatal condition raised by a phase after pinsert is returned
  * at the end of the step in which it arose. */
 int nv_step(void *ctx, int32_t nsteps);
+
+/* diagnostics, output and restart (src/diagnostic.f90, src/output.f90,
+ * src/initial.f90:151-166) ------------------------------------------------ */
+/* calculate_diagnostics (src/diagnostic.f90:9-18): ghostp, then each point's
+ * curvature and stretching (kept with the state, written by nv_data_dump
+ * and nv_printf) and the ts.log quantities into *d. */
+int nv_diagnostics(void *ctx, nv_diag *d);
+/* data_dump (src/output.f90:85-94): the point array as var.dat, Fortran
+ * unformatted sequential records (pcount, recon_count, itime, t, then f as
+ * 248-byte qvort records) that the reference's data_restore reads. */
+int nv_data_dump(void *ctx, const char *path);
+/* data_restore (src/initial.f90:151-166): a var.dat (the reference's or
+ * nv_data_dump's) uploaded as the state; the next step is itime + 1. */
+int nv_data_restore(void *ctx, const char *path);
+/* printf (src/output.f90:47-70): the var%04d.log stream file the MATLAB
+ * readers load (t, pcount, x, y, z, infront, u, u_mf, curv, stretch). */
+int nv_printf(void *ctx, const char *path);
+/* print_info's ts.log line (src/output.f90:23-37), appended to path, with
+ * the header line first when itime == shots; d from nv_diagnostics. */
+int nv_print_info(void *ctx, const nv_diag *d, int32_t shots, const char *path);
+/* tree_print (src/tree.f90:96-102): one line per leaf of the current tree in
+ * depth-first order, posx, posx+w, posy, posy+w, posz, posz+w, pcount,
+ * appended to path (the reference's tree_mesh%03d.log). */
+int nv_tree_print(void *ctx, const char *path);
+/* The file formats alone, on host arrays (no context, no GPU): write and
+ * read var.dat (on NV_ERR_CAPACITY host->pcount holds the rows needed), and
+ * write a var%04d.log. */
+int nv_var_write(const char *path, const nv_state *host);
+int nv_var_read(const char *path, nv_state *host);
+int nv_var_log_write(const char *path, const nv_state *host);
 
 /* outputs ---------------------------------------------------------------- */
 /* Reconnections since the last call (the pairs of src/reconnection.f90:11-61). */

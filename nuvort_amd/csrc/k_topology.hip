@@ -152,6 +152,119 @@ __global__ void k_nan(DState s, int n, DCtl *ctl) {
     if (isnan(s.x[r]) || isnan(s.x[s.cap + r]) || isnan(s.x[2 * s.cap + r])) atomicOr(&ctl->status, NVK_NAN);
 }
 
+// ------------------------------------------------------------ diagnostics
+// src/diagnostic.f90:21-74 for rows [0, n): each row's curvature |s''|
+// (curv_info, general.f90:54-61) and stretching u . s'' (stretching_rate),
+// zero for empty rows, and per workgroup the maxima of |u| and |u1 - u2|,
+// the sums of |u_sup|, |u_mf| and curv, the smallest positive and the largest
+// curvature and the active count (velocity_info, curv_info).
+constexpr int DIAG_VALS = 8;
+
+__global__ __launch_bounds__(256) void k_diag(DState s, int n, double *part) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cap = s.cap;
+    double v[DIAG_VALS] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0e300, 0.0, 0.0};
+    if (r < n) {
+        double u[3], u1[3], u2[3], us[3], um[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u[k] = ld(s.u, cap, k, r);
+            u1[k] = ld(s.u1, cap, k, r);
+            u2[k] = ld(s.u2, cap, k, r);
+            us[k] = ld(s.usup, cap, k, r);
+            um[k] = ld(s.umf, cap, k, r);
+        }
+        v[0] = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double d0 = u1[0] - u2[0], d1 = u1[1] - u2[1], d2 = u1[2] - u2[2];
+        v[1] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        v[2] = sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+        v[3] = sqrt(um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
+        double curv = 0.0, stretch = 0.0;
+        if (s.infront[r] != 0) {
+            double x[3], gi[3], gb[3], fd[3], fdd[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x[k] = ld(s.x, cap, k, r);
+                gi[k] = ld(s.gi, cap, k, r);
+                gb[k] = ld(s.gb, cap, k, r);
+            }
+            derivs(x, gi, gb, fd, fdd);
+            curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
+            stretch = u[0] * fdd[0] + u[1] * fdd[1] + u[2] * fdd[2];
+            v[7] = 1.0;
+        }
+        s.curv[r] = curv;
+        s.stretch[r] = stretch;
+        v[4] = curv;
+        if (curv > 0.0) v[5] = curv;
+        v[6] = curv;
+    }
+    // workgroup reduction: maxima, sums, minimum, maximum, count
+    __shared__ double red[DIAG_VALS][256];
+#pragma unroll
+    for (int k = 0; k < DIAG_VALS; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const int o = threadIdx.x + w;
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][o]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][o]);
+            red[2][threadIdx.x] += red[2][o];
+            red[3][threadIdx.x] += red[3][o];
+            red[4][threadIdx.x] += red[4][o];
+            red[5][threadIdx.x] = fmin(red[5][threadIdx.x], red[5][o]);
+            red[6][threadIdx.x] = fmax(red[6][threadIdx.x], red[6][o]);
+            red[7][threadIdx.x] += red[7][o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_VALS) part[(size_t)blockIdx.x * DIAG_VALS + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// the workgroups' partials reduced in a fixed order by one workgroup
+__global__ __launch_bounds__(256) void k_diag_fin(const double *part, int nblk, DDiag *out) {
+    double v[DIAG_VALS] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0e300, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+        const double *q = part + (size_t)b * DIAG_VALS;
+        v[0] = fmax(v[0], q[0]);
+        v[1] = fmax(v[1], q[1]);
+        v[2] += q[2];
+        v[3] += q[3];
+        v[4] += q[4];
+        v[5] = fmin(v[5], q[5]);
+        v[6] = fmax(v[6], q[6]);
+        v[7] += q[7];
+    }
+    __shared__ double red[DIAG_VALS][256];
+#pragma unroll
+    for (int k = 0; k < DIAG_VALS; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const int o = threadIdx.x + w;
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][o]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][o]);
+            red[2][threadIdx.x] += red[2][o];
+            red[3][threadIdx.x] += red[3][o];
+            red[4][threadIdx.x] += red[4][o];
+            red[5][threadIdx.x] = fmin(red[5][threadIdx.x], red[5][o]);
+            red[6][threadIdx.x] = fmax(red[6][threadIdx.x], red[6][o]);
+            red[7][threadIdx.x] += red[7][o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out->maxu = red[0][0];
+        out->maxdu = red[1][0];
+        out->sum_usup = red[2][0];
+        out->sum_umf = red[3][0];
+        out->sum_curv = red[4][0];
+        out->curv_min = red[5][0];
+        out->curv_max = red[6][0];
+        out->active = (int)red[7][0];
+    }
+}
+
 // ---------------------------------------------------------------- pinsert
 __global__ void k_pinsert_pre(DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,12 +275,24 @@ __global__ void k_pinsert_pre(DState s, Phys p, Tangles tg, int n, int *cand, in
         empty[r] = (!act && r - tg.base[k] < tg.n[k]) ? 1 : 0;
     }
     int c = 0;
+    double disti = 0.0;
     if (act) {
-        double disti = dist_gen(s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r], s.gi[r], s.gi[s.cap + r],
-                                s.gi[2 * s.cap + r]);
+        disti = dist_gen(s.x[r], s.x[s.cap + r], s.x[2 * s.cap + r], s.gi[r], s.gi[s.cap + r],
+                         s.gi[2 * s.cap + r]);
         c = disti > p.delta ? 1 : 0;
     }
     if (r < n) cand[r] = c;
+    // src/line.f90:38-39 total_length (a diagnostic): the sum of disti over
+    // the points present when pinsert starts, one atomic per workgroup
+    __shared__ double wsum[4];
+    double v = disti;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+        if (b != 0.0) atomicAdd(&ctl->total_length, b);
+    }
 }
 
 // Inputs of one insertion at row r (src/line.f90:30-91): r's position and
@@ -499,7 +624,13 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         }
     }
     __syncthreads();
-    if (tid == 0) tg.n[tk] = nl_sh;
+    if (tid == 0) {
+        // avg_sep's denominator (src/line.f90:95): pcount at the start less
+        // the empty slots left at the end
+        atomicAdd(&ctl->n_old, tg.n[tk]);
+        atomicAdd(&ctl->empty_after, e1 - ei);
+        tg.n[tk] = nl_sh;
+    }
 }
 
 // ----------------------------------------------------------------- precon
@@ -1141,6 +1272,9 @@ __global__ void k_step_begin(DCtl *ctl, int *tnact, int K, int *queue) {
         ctl->itime = ctl->itime + 1;
         ctl->nvel = 0;
         ctl->nactive = 0;
+        ctl->n_old = 0;
+        ctl->total_length = 0.0;
+        ctl->empty_after = 0;
         queue[0] = 0;
     }
 }
@@ -1192,6 +1326,8 @@ __global__ void k_relayout(DState o, DState nu, const int *__restrict__ row2t, c
         nu.gb[c * nu.cap + q] = o.gb[c * o.cap + r];
     }
     nu.closestd[q] = o.closestd[r];
+    nu.curv[q] = o.curv[r];
+    nu.stretch[q] = o.stretch[r];
     const int f = o.infront[r], b = o.behind[r], cl = o.closest[r];
     nu.infront[q] = f > 0 ? f + shift : 0;
     nu.behind[q] = b > 0 ? b + shift : 0;
@@ -1313,6 +1449,15 @@ hipError_t nvk_pclose_flat(hipStream_t st, const double *x, int ld, const int *i
     hipLaunchKernelGGL(k_seg_spheres, dim3(ntb), dim3(256), 0, st, s, n, sp);
     hipLaunchKernelGGL(k_pclose, dim3(nitems), dim3(256), 0, st, s, tg, items, sp, pd, pj);
     hipLaunchKernelGGL(k_pclose_fin, GRID(n), 0, st, s, tg, n, pd, pj);
+    return hipGetLastError();
+}
+
+int nvk_diag_parts(int n) { return ((n + 255) / 256) * DIAG_VALS; }
+
+hipError_t nvk_diagnostics(hipStream_t st, DState s, int n, double *part, DDiag *out) {
+    const int nblk = (n + 255) / 256;
+    if (nblk > 0) hipLaunchKernelGGL(k_diag, dim3(nblk), dim3(256), 0, st, s, n, part);
+    hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(256), 0, st, (const double *)part, nblk, out);
     return hipGetLastError();
 }
 

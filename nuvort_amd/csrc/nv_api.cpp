@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/nuvort.h"
+#include "nv_io.h"
 #include "nv_kernels.h"
 #include "nv_tree.h"
 #include "nv_util.h"
@@ -65,6 +66,7 @@ struct Ctx {
     DevBuf<char> rec32, tinfo, sph;
     DevBuf<int> pj;
     DevBuf<double> noise;            // noisey_xflow table (nv_set_noise)
+    DevBuf<double> diag;             // nv_diagnostics' partial reductions and result
     // work lists of the last direct-sum and close-search launches
     DevBuf<Item> items, pitems;
     DevBuf<int> d_S, d_pS, queue;
@@ -144,7 +146,7 @@ int order_sources(Ctx *c, bool force) {
 }
 
 void free_dstate(DState &d) {
-    double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb, &d.closestd};
+    double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb, &d.closestd, &d.curv, &d.stretch};
     for (double **p : dp)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     int **ip[] = {&d.infront, &d.behind, &d.closest};
@@ -164,8 +166,11 @@ hipError_t alloc_dstate(hipStream_t st, int rows, DState *out) {
         if ((e = hipMalloc(p, sizeof(double) * 3 * (size_t)rows)) != hipSuccess) break;
         if ((e = hipMemsetAsync(*p, 0, sizeof(double) * 3 * (size_t)rows, st)) != hipSuccess) break;
     }
-    if (e == hipSuccess && (e = hipMalloc(&d.closestd, sizeof(double) * rows)) == hipSuccess)
-        e = hipMemsetAsync(d.closestd, 0, sizeof(double) * rows, st);
+    double **sp[] = {&d.closestd, &d.curv, &d.stretch};
+    for (double **p : sp) {
+        if (e != hipSuccess) break;
+        if ((e = hipMalloc(p, sizeof(double) * rows)) == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(double) * rows, st);
+    }
     int **ip[] = {&d.infront, &d.behind, &d.closest};
     for (int **p : ip) {
         if (e != hipSuccess) break;
@@ -348,15 +353,20 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
         }
         HIPCHK(c, hipMemcpy(idst[q], zi.data(), sizeof(int) * R, hipMemcpyHostToDevice));
     }
-    std::vector<double> cd(R, 0.0);
-    if (hs->closestd) {
-        long o = 0;
-        for (int k = 0; k < K; ++k) {
-            for (int r = 0; r < pcounts[k]; ++r) cd[c->tbase[k] + r] = hs->closestd[o + r];
-            o += pcounts[k];
+    const double *ssrc[] = {hs->closestd, hs->curv, hs->stretch};
+    double *sdst[] = {c->d.closestd, c->d.curv, c->d.stretch};
+    std::vector<double> cd(R);
+    for (int q = 0; q < 3; ++q) {
+        std::fill(cd.begin(), cd.end(), 0.0);
+        if (ssrc[q]) {
+            long o = 0;
+            for (int k = 0; k < K; ++k) {
+                for (int r = 0; r < pcounts[k]; ++r) cd[c->tbase[k] + r] = ssrc[q][o + r];
+                o += pcounts[k];
+            }
         }
+        HIPCHK(c, hipMemcpy(sdst[q], cd.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     }
-    HIPCHK(c, hipMemcpy(c->d.closestd, cd.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     c->t = hs->t;
     c->itime = hs->itime;
     // a new point array; the noise table's counter is not part of it
@@ -414,12 +424,15 @@ int get_batch(Ctx *c, int32_t *pcounts, nv_state *hs) {
             o += c->tn[k];
         }
     }
-    if (hs->closestd) {
+    double *sdst[] = {hs->closestd, hs->curv, hs->stretch};
+    const double *ssrc[] = {c->d.closestd, c->d.curv, c->d.stretch};
+    for (int q = 0; q < 3; ++q) {
+        if (!sdst[q]) continue;
         std::vector<double> cd(R);
-        HIPCHK(c, hipMemcpy(cd.data(), c->d.closestd, sizeof(double) * R, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(cd.data(), ssrc[q], sizeof(double) * R, hipMemcpyDeviceToHost));
         long o = 0;
         for (int k = 0; k < c->K; ++k) {
-            for (int r = 0; r < c->tn[k]; ++r) hs->closestd[o + r] = cd[c->tbase[k] + r];
+            for (int r = 0; r < c->tn[k]; ++r) sdst[q][o + r] = cd[c->tbase[k] + r];
             o += c->tn[k];
         }
     }
@@ -540,6 +553,15 @@ int64_t pair_evals(const Ctx *c, int64_t nv, const std::vector<int> &tn) {
 bool noise_missing(const Ctx *c) {
     return c->prm.normal_velocity == NV_NF_NOISEY_XFLOW && !c->noise.p &&
            (std::fabs(c->prm.alpha[0]) > NV_EPS || std::fabs(c->prm.alpha[1]) > NV_EPS);
+}
+
+// pinsert's diagnostic sums start at zero (fills by kernel: recorded steps
+// hold no memset nodes)
+hipError_t clear_pinsert_stats(Ctx *c) {
+    hipError_t e = nv_fill_int(c->st, &c->ctl->n_old, 0, 1);
+    if (e == hipSuccess) e = nv_fill_int(c->st, (int *)&c->ctl->total_length, 0, 2);
+    if (e == hipSuccess) e = nv_fill_int(c->st, &c->ctl->empty_after, 0, 1);
+    return e;
 }
 
 int velocity_impl(Ctx *c, int do_ab) {
@@ -792,6 +814,7 @@ int tree_step(Ctx *c) {
     Tangles tg = tangles(c, nullptr, nullptr);
     HIPCHK(c, nv_fill_int(st, &c->ctl->nactive, 0, 1));
     HIPCHK(c, nv_fill_int(st, c->p_tnact, 0, c->K));
+    HIPCHK(c, clear_pinsert_stats(c));
     HIPCHK(c, nvk_pinsert_pre(st, c->d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
     HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
     HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
@@ -1161,6 +1184,7 @@ int nv_pinsert(void *h) {
     Tangles tg = tangles(c, nullptr, nullptr);
     c->ctl_h->nactive = 0;
     HIPCHK(c, hipMemcpyAsync(&c->ctl->nactive, &c->ctl_h->nactive, sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, clear_pinsert_stats(c));
     HIPCHK(c, nv_fill_int(c->st, c->p_tnact, 0, c->K));
     HIPCHK(c, nvk_pinsert_pre(c->st, c->d, c->phys, tg, c->rows, c->cand.p, c->empty.p, c->ctl));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
@@ -1298,6 +1322,145 @@ int nv_step(void *h, int32_t nsteps) {
     (void)hipEventDestroy(s0);
     (void)hipEventDestroy(s1);
     return rc;
+}
+
+int nv_diagnostics(void *h, nv_diag *out) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !out) return NV_ERR_ARG;
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "nv_diagnostics: one tangle (the reference's single f)");
+    const int n = c->tn[0];
+    HIPCHK(c, c->diag.reserve((size_t)nvk_diag_parts(n) + 16));
+    DDiag *dd = (DDiag *)(c->diag.p + nvk_diag_parts(n));
+    // src/diagnostic.f90:13: the ghosts of the current positions first
+    HIPCHK(c, nvk_ghosts(c->st, c->d, c->phys, c->rows));
+    HIPCHK(c, nvk_diagnostics(c->st, c->d, n, c->diag.p, dd));
+    DDiag hd{};
+    HIPCHK(c, hipMemcpyAsync(&hd, dd, sizeof(DDiag), hipMemcpyDeviceToHost, c->st));
+    if (int rc = read_ctl(c)) return rc;   // synchronises; status and counts
+    std::memset(out, 0, sizeof(*out));
+    const DCtl &k = *c->ctl_h;
+    out->active = hd.active;
+    out->recon_count = c->trecon[0];
+    out->remove_count = c->tremove[0];
+    out->itime = c->itime;
+    out->t = c->t;
+    out->maxu = hd.maxu;
+    out->maxdu = hd.maxdu;
+    const double act = (double)hd.active;
+    out->mean_u_sup = hd.sum_usup / act;
+    out->mean_u_mf = hd.sum_umf / act;
+    out->kappa_bar = hd.sum_curv / act;
+    // minval of an empty selection is huge(0.) in Fortran
+    out->kappa_min = hd.curv_min >= 1.0e300 ? 1.7976931348623157e308 : hd.curv_min;
+    out->kappa_max = hd.curv_max;
+    out->total_length = k.total_length;
+    out->avg_sep = k.total_length / (double)(k.n_old - k.empty_after);
+    out->tree_eval = c->stats.tree_evals;
+    return NV_OK;
+}
+
+int nv_data_dump(void *h, const char *path) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !path) return NV_ERR_ARG;
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "nv_data_dump: one tangle (the reference's single f)");
+    if (int rc = read_ctl(c)) return rc;
+    const int n = c->tn[0];
+    std::vector<double> v3((size_t)8 * 3 * n), v1((size_t)3 * n);
+    std::vector<int32_t> i1((size_t)3 * n);
+    nv_state hs{};
+    hs.capacity = n;
+    double **vp[] = {&hs.x, &hs.u, &hs.u1, &hs.u2, &hs.u_sup, &hs.u_mf, &hs.ghosti, &hs.ghostb};
+    for (int q = 0; q < 8; ++q) *vp[q] = v3.data() + (size_t)q * 3 * n;
+    hs.closestd = v1.data();
+    hs.curv = v1.data() + n;
+    hs.stretch = v1.data() + 2 * (size_t)n;
+    hs.infront = i1.data();
+    hs.behind = i1.data() + n;
+    hs.closest = i1.data() + 2 * (size_t)n;
+    if (int rc = get_batch(c, nullptr, &hs)) return rc;
+    const int rc = nv_var_write(path, &hs);
+    if (rc) return fail(c, rc, std::string("nv_data_dump: cannot write ") + path);
+    return NV_OK;
+}
+
+int nv_data_restore(void *h, const char *path) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !path) return NV_ERR_ARG;
+    nv_state probe{};
+    int rc = nv_var_read(path, &probe);
+    if (rc != NV_ERR_CAPACITY && rc != NV_OK) return fail(c, NV_ERR_ARG, std::string("nv_data_restore: cannot read ") + path);
+    const int n = probe.pcount;
+    std::vector<double> v3((size_t)8 * 3 * n), v1((size_t)3 * n);
+    std::vector<int32_t> i1((size_t)5 * n), w3((size_t)3 * n);
+    nv_state hs{};
+    hs.capacity = n;
+    double **vp[] = {&hs.x, &hs.u, &hs.u1, &hs.u2, &hs.u_sup, &hs.u_mf, &hs.ghosti, &hs.ghostb};
+    for (int q = 0; q < 8; ++q) *vp[q] = v3.data() + (size_t)q * 3 * n;
+    hs.closestd = v1.data();
+    hs.curv = v1.data() + n;
+    hs.stretch = v1.data() + 2 * (size_t)n;
+    hs.infront = i1.data();
+    hs.behind = i1.data() + n;
+    hs.closest = i1.data() + 2 * (size_t)n;
+    hs.pinnedi = i1.data() + 3 * (size_t)n;
+    hs.pinnedb = i1.data() + 4 * (size_t)n;
+    hs.wpinned = w3.data();
+    if ((rc = nv_var_read(path, &hs))) return fail(c, NV_ERR_ARG, std::string("nv_data_restore: cannot read ") + path);
+    // the tree, like the reference's, is not part of the restart
+    c->tree_valid = false;
+    const int32_t pc = hs.pcount;
+    return set_batch(c, 1, &pc, &hs, nullptr);
+}
+
+int nv_printf(void *h, const char *path) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !path) return NV_ERR_ARG;
+    if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "nv_printf: one tangle (the reference's single f)");
+    if (int rc = read_ctl(c)) return rc;
+    const int n = c->tn[0];
+    std::vector<double> v3((size_t)3 * 3 * n), v1((size_t)2 * n);
+    std::vector<int32_t> i1((size_t)n);
+    nv_state hs{};
+    hs.capacity = n;
+    hs.x = v3.data();
+    hs.u = v3.data() + 3 * (size_t)n;
+    hs.u_mf = v3.data() + 6 * (size_t)n;
+    hs.curv = v1.data();
+    hs.stretch = v1.data() + n;
+    hs.infront = i1.data();
+    if (int rc = get_batch(c, nullptr, &hs)) return rc;
+    if (int rc = nv_var_log_write(path, &hs)) return fail(c, rc, std::string("nv_printf: cannot write ") + path);
+    return NV_OK;
+}
+
+int nv_print_info(void *h, const nv_diag *d, int32_t shots, const char *path) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !d || !path || shots < 1) return NV_ERR_ARG;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return fail(c, NV_ERR_ARG, std::string("nv_print_info: cannot open ") + path);
+    std::string s;
+    if (d->itime == shots) s = nvio::ts_header();
+    s += nvio::ts_line(shots, *d, c->prm.delta);
+    const bool ok = std::fputs(s.c_str(), f) >= 0;
+    if (std::fclose(f) != 0 || !ok) return fail(c, NV_ERR_ARG, std::string("nv_print_info: cannot write ") + path);
+    return NV_OK;
+}
+
+int nv_tree_print(void *h, const char *path) {
+    Ctx *c = (Ctx *)h;
+    if (!c || !path) return NV_ERR_ARG;
+    if (!c->tree_valid) return fail(c, NV_ERR_STATE, "nv_tree_print: no tree (construct_tree not called)");
+    int n = 0;
+    if (int rc = nv_tree_dump(c->tree, c->st, nullptr, 0, &n)) return fail(c, rc, "tree dump failed");
+    std::vector<nv_tree_node> nodes((size_t)std::max(n, 1));
+    if (int rc = nv_tree_dump(c->tree, c->st, nodes.data(), n, &n)) return fail(c, rc, "tree dump failed");
+    FILE *f = std::fopen(path, "a");
+    if (!f) return fail(c, NV_ERR_ARG, std::string("nv_tree_print: cannot open ") + path);
+    bool ok = true;
+    for (int q = 0; q < n && ok; ++q)
+        if (nodes[q].pcount <= 1) ok = std::fputs(nvio::tree_leaf_line(nodes[q]).c_str(), f) >= 0;
+    if (std::fclose(f) != 0 || !ok) return fail(c, NV_ERR_ARG, std::string("nv_tree_print: cannot write ") + path);
+    return NV_OK;
 }
 
 int nv_get_recon_events(void *h, nv_recon_event *buf, int32_t cap, int32_t *n) {

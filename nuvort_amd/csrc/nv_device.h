@@ -21,6 +21,7 @@ struct DState {
     double *u, *u1, *u2, *usup, *umf, *gi, *gb;
     int *infront, *behind, *closest;
     double *closestd;
+    double *curv, *stretch;         // diagnostics: |s''| and u . s'' (src/diagnostic.f90:37-74)
 };
 
 // Independent tangles sharing the state arrays.  Tangle k owns the rows

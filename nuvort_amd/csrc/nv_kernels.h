@@ -25,7 +25,16 @@ struct DCtl {
     double t;          // simulation time
     unsigned long long nvel_total;  // nvel summed over the steps k_clock_tick closed
     int noise_ctr;     // noisey_xflow: index of the noise table's current value (0-based)
-    int pad;
+    int n_old;         // pinsert: pcount when it started (summed over tangles)
+    double total_length;   // pinsert: sum of |x - ghosti| over the points it started with
+    int empty_after;   // pinsert: empty slots below pcount when it ended (summed over tangles)
+    int pad2;
+};
+
+// src/diagnostic.f90:9-18 calculate_diagnostics, reduced on the device
+struct DDiag {
+    double maxu, maxdu, sum_usup, sum_umf, sum_curv, curv_min, curv_max;
+    int active, pad;
 };
 
 // velocity (k_velocity.hip)
@@ -63,6 +72,11 @@ hipError_t nvk_step_end(hipStream_t st, DState s, Phys p, int n, DCtl *ctl, doub
 
 // boundary / topology / close search (k_topology.hip)
 hipError_t nvk_ghosts(hipStream_t st, DState s, Phys p, int n);
+// curvature and stretching of every row and the reductions of
+// velocity_info / curv_info (src/diagnostic.f90:21-74) over rows [0, n) of a
+// single tangle; part holds nvk_diag_parts(n) doubles
+int nvk_diag_parts(int n);
+hipError_t nvk_diagnostics(hipStream_t st, DState s, int n, double *part, DDiag *out);
 hipError_t nvk_wrap(hipStream_t st, DState s, Phys p, int n);
 hipError_t nvk_nan(hipStream_t st, DState s, int n, DCtl *ctl);
 hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl);

@@ -112,12 +112,20 @@ class FilamentSolver:
         self._keep = st
         self._call("nv_set_state", C.byref(self._view(st)))
 
-    def get_state(self) -> FilamentState:
+    def get_state(self, diagnostics: bool = False) -> FilamentState:
+        """The point array; with ``diagnostics`` also each point's curvature
+        and stretching (as nv_diagnostics left them) in extra["curv"] and
+        extra["stretch"]."""
         probe = _L.NvState()
         probe.capacity = 0
         self.L.nv_get_state(self.h, C.byref(probe))
         st = FilamentState.empty(probe.pcount)
         v = self._view(st, st.pcount)
+        if diagnostics:
+            st.extra["curv"] = np.zeros(st.pcount)
+            st.extra["stretch"] = np.zeros(st.pcount)
+            v.curv = st.extra["curv"].ctypes.data_as(_L._dp)
+            v.stretch = st.extra["stretch"].ctypes.data_as(_L._dp)
         self._call("nv_get_state", C.byref(v))
         st.t, st.itime, st.recon_count = v.t, v.itime, v.recon_count
         st.extra["remove_count"] = v.remove_count
@@ -209,6 +217,36 @@ class FilamentSolver:
 
     def step(self, nsteps: int = 1):
         self._call("nv_step", C.c_int32(nsteps))
+
+    # ---------------------------------------- diagnostics, output, restart
+    def diagnostics(self) -> dict:
+        """calculate_diagnostics (src/diagnostic.f90:9-18): the ts.log
+        quantities; curvature and stretching are kept with the state."""
+        d = _L.NvDiag()
+        self._call("nv_diagnostics", C.byref(d))
+        self._diag = d
+        return {f: getattr(d, f) for f, _ in _L.NvDiag._fields_}
+
+    def data_dump(self, path: str) -> None:
+        """data_dump (src/output.f90:85-94): var.dat."""
+        self._call("nv_data_dump", path.encode())
+
+    def data_restore(self, path: str) -> None:
+        """data_restore (src/initial.f90:151-166) from var.dat."""
+        self._call("nv_data_restore", path.encode())
+
+    def printf(self, path: str) -> None:
+        """printf (src/output.f90:47-70): a var%04d.log stream file."""
+        self._call("nv_printf", path.encode())
+
+    def print_info(self, shots: int, path: str) -> None:
+        """print_info's ts.log line (src/output.f90:23-37), from the last
+        diagnostics()."""
+        self._call("nv_print_info", C.byref(self._diag), C.c_int32(shots), path.encode())
+
+    def tree_print(self, path: str) -> None:
+        """tree_print (src/tree.f90:96-102): the current tree's leaves."""
+        self._call("nv_tree_print", path.encode())
 
     # ------------------------------------------------------------ outputs
     def recon_events(self):

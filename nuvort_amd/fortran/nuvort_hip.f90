@@ -56,6 +56,15 @@ module nuvort_hip
     type(c_ptr) :: closestd
     type(c_ptr) :: pinnedi, pinnedb
     type(c_ptr) :: wpinned
+    type(c_ptr) :: curv, stretch
+  end type
+
+  !> calculate_diagnostics / the last pinsert (src/diagnostic.f90, src/line.f90:38-39,95)
+  type, bind(C) :: nv_diag
+    integer(c_int32_t) :: active, recon_count, remove_count, itime
+    real(c_double) :: t, maxu, maxdu, mean_u_sup, mean_u_mf, kappa_bar, kappa_min, kappa_max
+    real(c_double) :: total_length, avg_sep
+    integer(c_int64_t) :: tree_eval
   end type
 
   interface
@@ -151,6 +160,39 @@ module nuvort_hip
       import :: c_int, c_int32_t, c_ptr
       integer(c_int32_t), value :: n, ld
       type(c_ptr), value :: x, infront, behind, closest, closestd, stream
+    end function
+    integer(c_int) function nv_diagnostics(ctx, d) bind(C, name='nv_diagnostics')
+      import :: c_ptr, c_int, nv_diag
+      type(c_ptr), value :: ctx
+      type(nv_diag), intent(out) :: d
+    end function
+    !> path arguments: c_null_char-terminated file names
+    integer(c_int) function nv_data_dump(ctx, path) bind(C, name='nv_data_dump')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function
+    integer(c_int) function nv_data_restore(ctx, path) bind(C, name='nv_data_restore')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function
+    integer(c_int) function nv_printf(ctx, path) bind(C, name='nv_printf')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function
+    integer(c_int) function nv_print_info(ctx, d, shots, path) bind(C, name='nv_print_info')
+      import :: c_ptr, c_int, c_char, c_int32_t, nv_diag
+      type(c_ptr), value :: ctx
+      type(nv_diag), intent(in) :: d
+      integer(c_int32_t), value :: shots
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function
+    integer(c_int) function nv_tree_print(ctx, path) bind(C, name='nv_tree_print')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
     end function
     !> cache_path: a c_null_char-terminated file name; c_null_char alone tunes without a cache
     integer(c_int) function nv_autotune(ctx, cache_path) bind(C, name='nv_autotune')

@@ -39,7 +39,16 @@ class NvState(C.Structure):
                 ("remove_count", C.c_int32), ("pad", C.c_int32), ("t", C.c_double),
                 ("x", _dp), ("u", _dp), ("u1", _dp), ("u2", _dp), ("u_sup", _dp), ("u_mf", _dp),
                 ("ghosti", _dp), ("ghostb", _dp), ("infront", _ip), ("behind", _ip), ("closest", _ip),
-                ("closestd", _dp), ("pinnedi", _ip), ("pinnedb", _ip), ("wpinned", _ip)]
+                ("closestd", _dp), ("pinnedi", _ip), ("pinnedb", _ip), ("wpinned", _ip),
+                ("curv", _dp), ("stretch", _dp)]
+
+
+class NvDiag(C.Structure):
+    _fields_ = [("active", C.c_int32), ("recon_count", C.c_int32), ("remove_count", C.c_int32), ("itime", C.c_int32),
+                ("t", C.c_double), ("maxu", C.c_double), ("maxdu", C.c_double), ("mean_u_sup", C.c_double),
+                ("mean_u_mf", C.c_double), ("kappa_bar", C.c_double), ("kappa_min", C.c_double),
+                ("kappa_max", C.c_double), ("total_length", C.c_double), ("avg_sep", C.c_double),
+                ("tree_eval", C.c_int64)]
 
 
 class NvReconEvent(C.Structure):
@@ -68,7 +77,9 @@ EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_s
            "nv_get_state_batch", "nv_set_time", "nv_set_noise", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
            "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
-           "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune", "nv_pclose_noalloc")
+           "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune", "nv_pclose_noalloc",
+           "nv_diagnostics", "nv_data_dump", "nv_data_restore", "nv_printf", "nv_print_info", "nv_tree_print",
+           "nv_var_write", "nv_var_read", "nv_var_log_write")
 
 _lib = None
 
@@ -101,6 +112,11 @@ def load(path: str = LIB_PATH):
         "nv_autotune": [vp, C.c_char_p],
         # device pointers (x, infront, behind, closest, closestd) and a hipStream_t
         "nv_pclose_noalloc": [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp],
+        "nv_diagnostics": [vp, C.POINTER(NvDiag)], "nv_data_dump": [vp, C.c_char_p],
+        "nv_data_restore": [vp, C.c_char_p], "nv_printf": [vp, C.c_char_p],
+        "nv_print_info": [vp, C.POINTER(NvDiag), C.c_int32, C.c_char_p], "nv_tree_print": [vp, C.c_char_p],
+        "nv_var_write": [C.c_char_p, C.POINTER(NvState)], "nv_var_read": [C.c_char_p, C.POINTER(NvState)],
+        "nv_var_log_write": [C.c_char_p, C.POINTER(NvState)],
     }
     for name in EXPORTS:
         fn = getattr(L, name)

@@ -7,10 +7,18 @@
 !> It exists so that golden fixtures and calibration timings come from the
 !> reference's own FP64 arithmetic.  Nothing in the product links it.
 !>
-!> usage: nuvort_ref <state_in> <out_prefix> <nsteps> <dump_phases 0|1>
+!> usage: nuvort_ref <state_in> <out_prefix> <nsteps> <dump_phases 0|1> [output 0|1]
 !>   run.in is read from the working directory by the reference's own parser
 !>   (src/cdata.f90:131-241); ./data must exist (precon appends to
 !>   ./data/recon_location.log, src/reconnection.f90:47-49).
+!>   A state_in named *.dat is read as the reference's restart file, with the
+!>   statements of data_restore (src/initial.f90:151-166).
+!>   With output = 1 every step ends with the reference's output phase
+!>   (src/run.f90:50-56): calculate_diagnostics, then, when mod(itime,shots)
+!>   is 0, the statements of data_dump (./data/var.dat, src/output.f90:85-94),
+!>   of printf (./data/var%04d.log, :47-70) and print_info's ts.log line
+!>   (./data/ts.log, :23-37); output.f90 itself is not compiled (its banner
+!>   calls GNU extensions).
 !>
 !> State file (little-endian stream, also written by oracle/ref_io.py):
 !>   int32 pcount, int32 itime, int32 recon_count, real8 t, then per field
@@ -25,15 +33,20 @@ program ref_harness
   use line
   use reconnection
   use normal_fluid
+  use diagnostic
   implicit none
   character(len=512) :: state_in, out_prefix, arg
-  integer :: nrun, dump_phases, istep, itime0
+  integer :: nrun, dump_phases, istep, itime0, do_output
   real(8) :: t0, t1, t_vel, t_tree, t_close, ta, tb
 
   call get_command_argument(1, state_in)
   call get_command_argument(2, out_prefix)
   call get_command_argument(3, arg) ; read(arg,*) nrun
   call get_command_argument(4, arg) ; read(arg,*) dump_phases
+  do_output = 0
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg) ; read(arg,*) do_output
+  end if
 
   call read_run_file()
   ! boundary bookkeeping exactly as src/initial.f90:37-78 does it
@@ -49,7 +62,11 @@ program ref_harness
   ! read here from run.in's noise_file), as src/initial.f90 calls it
   if (normal_velocity/='zero' .and. normal_velocity/='xflow') call setup_normal_fluid
 
-  call load_state(trim(state_in))
+  if (index(state_in, '.dat') == len_trim(state_in) - 3) then
+    call restore_var(trim(state_in))
+  else
+    call load_state(trim(state_in))
+  end if
   itime0 = itime
   t_vel = 0.d0 ; t_tree = 0.d0 ; t_close = 0.d0
   t0 = omp_get_wtime()
@@ -86,6 +103,14 @@ program ref_harness
     end if
     call enforce_boundary
     t = t + dt
+    if (do_output == 1) then
+      call calculate_diagnostics
+      if (mod(itime, shots) == 0) then
+        call dump_var
+        call write_var_log(itime / shots)
+        call write_ts_line
+      end if
+    end if
     if (tree_theta>0) then
       call empty_tree(vtree)
       deallocate(vtree%parray) ; deallocate(vtree)
@@ -108,6 +133,68 @@ program ref_harness
   close(71)
 
 contains
+
+  !> src/initial.f90:151-166 data_restore's statements
+  subroutine restore_var(fname)
+    character(len=*), intent(in) :: fname
+    integer :: dummy_itime
+    open(unit=63, file=fname, FORM='unformatted')
+      read(63) pcount
+      read(63) recon_count
+      read(63) dummy_itime
+      read(63) t
+      allocate(f(pcount))
+      read(63) f
+    close(63)
+    itime = dummy_itime
+  end subroutine
+
+  !> src/output.f90:85-94 data_dump's statements
+  subroutine dump_var
+    open(unit=53, file="./data/var.dat", FORM='unformatted', status='replace')
+      write(53) pcount
+      write(53) recon_count
+      write(53) itime
+      write(53) t
+      write(53) f
+    close(53)
+  end subroutine
+
+  !> src/output.f90:47-70 printf's statements
+  subroutine write_var_log(filenumber)
+    integer, intent(in) :: filenumber
+    character(len=40) :: print_file
+    write(unit=print_file, fmt="(a,i4.4,a)") "./data/var", filenumber, ".log"
+    open(unit=98, file=print_file, status='replace', form='unformatted', access='stream')
+      write(98) t
+      write(98) pcount
+      write(98) f(:)%x(1)
+      write(98) f(:)%x(2)
+      write(98) f(:)%x(3)
+      write(98) f(:)%infront
+      write(98) f(:)%u(1)
+      write(98) f(:)%u(2)
+      write(98) f(:)%u(3)
+      write(98) f(:)%u_mf(1)
+      write(98) f(:)%u_mf(2)
+      write(98) f(:)%u_mf(3)
+      write(98) f(:)%curv
+      write(98) f(:)%stretch
+    close(98)
+  end subroutine
+
+  !> src/output.f90:35-37 print_info's ts.log line (the header is cosmetic)
+  subroutine write_ts_line
+    open(unit=78, file='data/ts.log', position='append')
+    write(78,'(i6.4,f13.7,i10.1,i13.1,i13.1,f7.4,f13.6,f13.5,f13.5,f10.2,i13.1,i13.1)') &
+      itime/shots, t, count(mask=f(:)%infront>0), recon_count, wall_recon_count, avg_sep/delta, &
+      total_length, maxu, maxdu, kappa_bar, remove_count, tree_eval
+    close(78)
+    open(unit=79, file='data/diag.log', position='append')
+    write(79,'(10es26.17e3)') maxu, maxdu, mean_u_sup, mean_u_mf, kappa_bar, kappa_min, kappa_max, &
+      total_length, avg_sep, real(tree_eval)
+    close(79)
+  end subroutine
 
   subroutine load_state(fname)
     character(len=*), intent(in) :: fname

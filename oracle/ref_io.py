@@ -60,14 +60,18 @@ def read_tree(path: str) -> np.ndarray:
 
 
 def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, dump_phases: bool = True,
-                  threads: int = 1, files: dict | None = None, expect_fatal: bool = False) -> dict:
+                  threads: int = 1, files: dict | None = None, expect_fatal: bool = False, output: bool = False,
+                  state_path: str | None = None) -> dict:
     """Run the reference build on ``st`` for ``nsteps`` steps in ``workdir``.
     ``files`` maps file names (relative to ``workdir``) to text written there
     first, such as a noise table.  Returns the dumped states (ghost, tree,
     motion, insert, close, recon, remove for the first step; final after the
     last), the reconnection log lines and the timing record.  With
     ``expect_fatal`` the run is expected to stop in fatal_error: its exit
-    status and output are returned instead."""
+    status and output are returned instead.  With ``output`` every step ends
+    with the reference's output phase (diagnostics; var.dat, var%04d.log and
+    the ts.log line every shots steps), whose files are returned as bytes
+    under "files".  ``state_path`` (a var.dat) replaces ``st`` as the input."""
     os.makedirs(os.path.join(workdir, "data"), exist_ok=True)
     for name, text in (files or {}).items():
         with open(os.path.join(workdir, name), "w") as fh:
@@ -77,9 +81,15 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
     log = os.path.join(workdir, "data", "recon_location.log")
     if os.path.exists(log):
         os.remove(log)
-    write_state(os.path.join(workdir, "state_in.bin"), st)
+    src = "state_in.bin"
+    if state_path is not None:
+        src = "restart_in.dat"
+        with open(state_path, "rb") as fi, open(os.path.join(workdir, src), "wb") as fo:
+            fo.write(fi.read())
+    else:
+        write_state(os.path.join(workdir, src), st)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    r = subprocess.run([REF_BIN, "state_in.bin", "out", str(nsteps), "1" if dump_phases else "0"],
+    r = subprocess.run([REF_BIN, src, "out", str(nsteps), "1" if dump_phases else "0", "1" if output else "0"],
                        cwd=workdir, env=env, capture_output=True, text=True)
     if expect_fatal:
         # fatal_error (src/cdata.f90:245-255) prints its location and message
@@ -108,4 +118,12 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
             k, v = line.split()
             timing[k] = float(v)
     out["timing"] = timing
+    if output:
+        files = {}
+        dd = os.path.join(workdir, "data")
+        for name in sorted(os.listdir(dd)):
+            if name.startswith("var") or name in ("ts.log", "diag.log"):
+                with open(os.path.join(dd, name), "rb") as fh:
+                    files[name] = fh.read()
+        out["files"] = files
     return out

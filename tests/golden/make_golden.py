@@ -84,6 +84,34 @@ def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, pha
         name, st.pcount, nsteps, len(out["recon_log"]), out["final"].pcount))
 
 
+def make_restart(name: str, st: FilamentState, p: RunParams, nsteps: int, cont: int, note: str) -> None:
+    """The reference's output phase over nsteps steps (shots 1: diagnostics,
+    var.dat, var%04d.log and the ts.log line every step), then a run of
+    `cont` steps restarted from the var.dat it wrote.  Stores the files'
+    bytes (file__<name>), the state after nsteps, and the restarted run's
+    final state (cont__*)."""
+    p.shots = 1
+    txt = format_run_text(p)
+    with tempfile.TemporaryDirectory(dir="/tmp") as wd:
+        out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=False, threads=1, output=True)
+    with tempfile.TemporaryDirectory(dir="/tmp") as wd:
+        var = os.path.join(wd, "var_saved.dat")
+        with open(var, "wb") as fh:
+            fh.write(out["files"]["var.dat"])
+        c = ref_io.run_reference(os.path.join(wd, "run"), None, txt, cont, dump_phases=False, threads=1,
+                                 state_path=var)
+    d = {"run_text": np.array(txt), "nsteps": np.array(nsteps), "cont_steps": np.array(cont), "note": np.array(note)}
+    pack_state("in", st, d)
+    pack_state("final", out["final"], d)
+    pack_state("cont", c["final"], d)
+    for fname, data in out["files"].items():
+        d["file__" + fname] = np.frombuffer(data, np.uint8)
+    d["recon_log"] = out["recon_log"]
+    d["timing"] = np.array(json.dumps(out["timing"]))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+    print("%-22s N=%5d steps=%3d+%d files=%s" % (name, st.pcount, nsteps, cont, sorted(out["files"])))
+
+
 def crossing_rings(y0: float, delta: float) -> FilamentState:
     """Ring A in the xy-plane (centre 0, R=0.01); ring B in the plane y=y0,
     centred at (0.004, y0, 0) with R=0.006, so B passes y0 away from A's
@@ -244,6 +272,11 @@ def main() -> None:
          "BS, noisey_xflow normal flow (seeded noise table) with friction", ("motion",), noise_seed=77)
     make("nf_channel_tree", late, base_params(velocity="Tree", tree_theta=0.4, normal_velocity="channel", **mfk), 2,
          "Tree theta 0.4, channel normal flow with friction", ("motion",))
+    make_restart("restart_bs", rings, base_params(normal_velocity="xflow", norm_vel_xflow=0.5,
+                                                 alpha=(0.206, 0.0083)), 3, 2,
+                 "BS with friction: 3 steps with the output phase (var.dat, var%04d.log, ts.log), 2 restarted")
+    make_restart("restart_tree", crossing_rings(1.0e-4, d5), base_params(velocity="Tree", tree_theta=0.5), 2, 2,
+                 "Tree with a reconnection: 2 steps with the output phase, 2 restarted from var.dat")
     make("tree_deep", deep_tree_state(d5), base_params(velocity="Tree", tree_theta=0.5), 1,
          "Tree theta 0.5 with point pairs 4e-7 and 5e-12 cm apart (octree deeper than 16 and 32 levels)")
     dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
@@ -261,11 +294,15 @@ if __name__ == "__main__":
     elif sys.argv[1:2] == ["only"]:
         # one fixture by name (the others are left as they are)
         want = set(sys.argv[2:])
-        real_make = make
+        real_make, real_restart = make, make_restart
 
         def make(name, *a, **kw):   # noqa: F811
             if name in want:
                 real_make(name, *a, **kw)
+
+        def make_restart(name, *a, **kw):   # noqa: F811
+            if name in want:
+                real_restart(name, *a, **kw)
         main()
     else:
         main()

@@ -17,9 +17,14 @@ STATE_FIELDS = ("x", "u", "u1", "u2", "u_sup", "u_mf", "ghosti", "ghostb", "infr
 
 
 def names():
-    """Fixtures of normal runs (the fatal_* ones end in fatal_error)."""
+    """Fixtures of normal runs (the fatal_* ones end in fatal_error, the
+    restart_* ones hold the output phase's files and a restarted run)."""
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith("fatal_"))
+                  if not os.path.basename(p).startswith(("fatal_", "restart_")))
+
+
+def restart_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "restart_*.npz")))
 
 
 def fatal_names():
@@ -59,6 +64,13 @@ class Fixture:
     def fatal_text(self) -> str:
         """What the reference printed when it stopped (fatal_* fixtures)."""
         return str(self.z["fatal_text"]) if "fatal_text" in self.z.files else ""
+
+    def file(self, name: str) -> bytes:
+        """A file the reference's output phase wrote (restart_* fixtures)."""
+        return self.z["file__" + name].tobytes()
+
+    def files(self):
+        return sorted(k[len("file__"):] for k in self.z.files if k.startswith("file__"))
 
     def noise(self):
         """The noisey_xflow table the reference read, or None."""

@@ -82,7 +82,7 @@ def test_rebuilds_match_fresh_builds():
     deep = Fixture("tree_deep").state("in")   # pairs 4e-7 and 5e-12 cm apart: 34 levels
     seq = [("base", base, True), ("moved", moved, False), ("bigger", bigger, True), ("deeper", deeper, None),
            ("smaller", smaller, False), ("smaller again", smaller, False), ("deep", deep, True),
-           ("base after deep", base, None), ("moved after deep", moved, False)]
+           ("base after deep", base, True), ("moved after deep", moved, False)]
     s = FilamentSolver(params())
     try:
         prev_sync, prev_levels = 0, 0
