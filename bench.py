@@ -253,6 +253,30 @@ def oracle_tree_velocity(st, p, ranges, threads):
     return np.concatenate(out)
 
 
+def tree_phases(p, st, steps):
+    """Octree steps launched eagerly from state st: the wall time per step and
+    the build, velocity walk and close search durations (HIP events), which
+    recorded steps do not time by phase."""
+    from nuvort_amd.engine import FilamentSolver
+    q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+    q.hip_step_launch = "eager"
+    s = FilamentSolver(q)
+    s.set_state(st)
+    s.step(1)
+    out = []
+    for _ in range(steps):
+        a = s.stats()
+        t0 = time.perf_counter()
+        s.step(1)
+        dt_ = time.perf_counter() - t0
+        b = s.stats()
+        out.append({"step_ms": dt_ * 1e3, "tree_build_ms": b["ms_tree_build"], "close_ms": b["ms_close"],
+                    "walk_ms": b["walk_ms_total"] - a["walk_ms_total"],
+                    "walk_nodes": b["walk_nodes_total"] - a["walk_nodes_total"]})
+    s.close()
+    return out
+
+
 def tree1m(args):
     """Random rings (radii U[0.01, 0.03] cm, delta 5e-4 cm, seed 1239) until
     1M points in a periodic 0.2 cm box; Barnes-Hut theta 0.4 (octree rebuilt
@@ -273,27 +297,27 @@ def tree1m(args):
     s.set_state(st)
     s.step(1)
     log("warm-up step done")
-    per, stats = [], []
+    per = []
     s0 = s.stats()
     t_all = 0.0
     for k in range(args.steps if args.steps < 100 else 5):
-        a = s.stats()
         t0 = time.perf_counter()
         s.step(1)
         dt_ = time.perf_counter() - t0
-        b = s.stats()
         t_all += dt_
         per.append(dt_ * 1e3)
-        stats.append({"step_ms": dt_ * 1e3, "tree_build_ms": b["ms_tree_build"], "close_ms": b["ms_close"],
-                      "walk_ms": b["walk_ms_total"] - a["walk_ms_total"],
-                      "walk_nodes": b["walk_nodes_total"] - a["walk_nodes_total"]})
-        log("step: %.1f ms (build %.1f, walk %.1f, close %.1f)" % (dt_ * 1e3, b["ms_tree_build"],
-                                                                   stats[-1]["walk_ms"], b["ms_close"]))
+        log("step: %.1f ms" % (dt_ * 1e3))
     s1 = s.stats()
     nsteps = len(per)
     pu = s1["point_updates"] - s0["point_updates"]
+    launch = {"recorded_steps": s1["tree_replays"] - s0["tree_replays"],
+              "redone_eagerly": s1["tree_redone"] - s0["tree_redone"]}
     cur = s.get_state()
     s.close()
+    stats = tree_phases(p, cur, 3)
+    for x in stats:
+        log("eager step: %.1f ms (build %.1f, walk %.1f, close %.1f)" % (x["step_ms"], x["tree_build_ms"],
+                                                                         x["walk_ms"], x["close_ms"]))
     # velocities on the current state: this walk, the FP64 walk, and the oracle's direct sum on a sample
     def tree_velocity(prec):
         q = p.__class__(**{k: getattr(p, k) for k in p.__dataclass_fields__})
@@ -345,7 +369,7 @@ def tree1m(args):
                                "5e-4 cm, Barnes-Hut theta 0.4 rebuilt every step, v_n = 0.5 cm/s along x, "
                                "alpha 0.206 / 0.0083, reconnection and re-meshing every step",
                    "points": n, "seed": 1239, "dt_s": 5e-6, "walk": args.tree_precision, "parallelism": "1 GPU"},
-        "per_step": stats, "accuracy": acc,
+        "step_ms": per, "steps_launched": launch, "eager_steps_by_phase": stats, "accuracy": acc,
         "walk": {"ms": walk_ms, "node_reads": walk_nodes, "record_bytes": rec,
                  "algorithmic_GBps": walk_bytes / (walk_ms * 1e-3) / 1e9, "hbm_peak_GBps": 8000.0},
         "oracle_sample": {"targets": sample, "seconds": secs, "threads": args.cpu_threads},
@@ -504,18 +528,16 @@ def tree1m_extra(steps, precision="fp32c"):
     s.step(2)
     s0 = s.stats()
     t0 = time.perf_counter()
-    walk, build, close = [], [], []
-    for _ in range(steps):
-        a = s.stats()
-        s.step(1)
-        b = s.stats()
-        walk.append(b["walk_ms_total"] - a["walk_ms_total"])
-        build.append(b["ms_tree_build"])
-        close.append(b["ms_close"])
+    s.step(steps)
     sec = time.perf_counter() - t0
     s1 = s.stats()
-    nodes = (s1["walk_nodes_total"] - s0["walk_nodes_total"]) / max(1, s1["walk_launches"] - s0["walk_launches"])
+    cur = s.get_state()
     s.close()
+    ph = tree_phases(p, cur, 2)
+    walk = [x["walk_ms"] for x in ph]
+    build = [x["tree_build_ms"] for x in ph]
+    close = [x["close_ms"] for x in ph]
+    nodes = float(np.mean([x["walk_nodes"] for x in ph]))
     n = st.active
     rec = 48 if precision == "fp32c" else 80
     wms = float(np.mean(walk))
@@ -523,6 +545,8 @@ def tree1m_extra(steps, precision="fp32c"):
     return {"points": n, "seed": 1239, "walk": precision, "value": (s1["point_updates"] - s0["point_updates"]) / sec,
             "ms_per_step": sec / steps * 1e3, "steps": steps, "warmup": 2, "walk_ms": wms,
             "tree_build_ms": float(np.mean(build)), "close_search_ms": float(np.mean(close)),
+            "recorded_steps": s1["tree_replays"] - s0["tree_replays"],
+            "phases": "walk, build and close-search times from 2 eager steps after the timed ones",
             "walk_node_reads": nodes, "walk_record_bytes": rec,
             "walk_algorithmic_GBps": walk_bytes / (wms * 1e-3) / 1e9, "hbm_peak_GBps": 8000.0,
             "note": "random rings to 1M points in a periodic 0.2 cm box, delta 5e-4 cm, Barnes-Hut theta 0.4 "

@@ -161,7 +161,8 @@ typedef struct {
     int64_t pair_launches;     /* direct-sum kernel launches */
     double pair_ms_total;      /* summed duration of those launches (HIP events) */
     int64_t pair_evals_total;  /* summed pair evaluations of those launches */
-    int64_t walk_launches;     /* tree velocity walk launches */
+    int64_t walk_launches;     /* tree velocity walk launches timed (eager steps and calls; recorded
+                                  octree steps are not timed by phase) */
     double walk_ms_total;      /* summed duration of those launches */
     int64_t walk_nodes_total;  /* summed node records read by those walks */
     int32_t targets_per_lane;  /* direct-sum variant in use: points per lane */
@@ -172,6 +173,9 @@ typedef struct {
     int32_t tree_builds;       /* octree builds */
     int32_t tree_sync_builds;  /* of which ran the level loop with a host round trip per level (the
                                   first build, and a tree that outgrew the previous build's levels) */
+    int32_t tree_replays;      /* octree steps nv_step took as a recorded step (one synchronisation) */
+    int32_t tree_redone;       /* recorded octree steps whose build outgrew the recorded plan and
+                                  that were taken again eagerly */
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */

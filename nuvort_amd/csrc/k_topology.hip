@@ -267,6 +267,7 @@ __global__ __launch_bounds__(256) void k_diag_fin(const double *part, int nblk, 
 
 // ---------------------------------------------------------------- pinsert
 __global__ void k_pinsert_pre(DState s, Phys p, Tangles tg, int n, int *cand, int *empty, DCtl *ctl) {
+    if (ctl->abort) return;   // (a recorded octree step whose build was incomplete: as k_finish)
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < n && s.infront[r] != 0;
     if (r < n) {
@@ -514,6 +515,7 @@ constexpr int INS_CHUNK = 128;
 // after_insert keeps the staged inputs current.
 __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, Tangles tg, const int *cand,
                                                               const int *empty, DCtl *ctl) {
+    if (ctl->abort) return;
     __shared__ Dirty d;
     __shared__ InsIn st[INS_CHUNK];
     __shared__ InsPlan pl[INS_CHUNK];
@@ -847,6 +849,7 @@ __device__ bool precon_visit(const DState &s, const Phys &p, int &nrec, int tk, 
 
 __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
                                                              nv_recon_event *ev, int ev_cap) {
+    if (ctl->abort) return;
     __shared__ ReconIn st[REC_CHUNK];
     __shared__ int strow[REC_CHUNK];
     __shared__ ModSet mod;
@@ -975,6 +978,7 @@ constexpr int RM_CHUNK = 256;
 // unless one of its rows (r, inf, ii) was modified by a removal of this chunk,
 // and then reloads it.
 __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
+    if (ctl->abort) return;
     __shared__ Dirty d;
     __shared__ RmIn st[RM_CHUNK];
     __shared__ int strow[RM_CHUNK];
@@ -1282,6 +1286,7 @@ __global__ void k_step_begin(DCtl *ctl, int *tnact, int K, int *queue) {
 // enforce_boundary (src/boundary.f90:91-138), then t = t + dt, then the NaN
 // test of the wrapped positions (src/general.f90:172-191), in one launch
 __global__ void k_step_end(DState s, Phys p, int n, DCtl *ctl, double dt) {
+    if (ctl->abort) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r == 0) {
         ctl->t = ctl->t + dt;

@@ -96,6 +96,11 @@ __device__ __forceinline__ R pick64(bool c, const R &a, const R &b) {
     return o;
 }
 
+// A row or node count the kernels take either from the host (dn null) or
+// from the device, bounded by the launch (a recorded step's counts change
+// between replays; its launches cover the bounds).
+__device__ __forceinline__ int dev_count(int nmax, const int *dn) { return dn ? min(*dn, nmax) : nmax; }
+
 // The sorted paths: word 0 (levels 1-16) and, in deep builds, words 1 and 2
 // (levels 17-32, 33-48), each in the sorted order.
 struct Keys {
@@ -112,10 +117,10 @@ __device__ __forceinline__ int key_digit(const Keys &k, int pos, int level) {
 // The paths of every row through nwords * 16 levels: word 0 into keys,
 // words 1.. into deep[0..] (by row); inactive rows get all ones, which
 // sorts them last.
-__global__ void k_keys(DState s, int n, double bsize, int nwords, unsigned long long *keys,
+__global__ void k_keys(DState s, int nmax, const int *dn, double bsize, int nwords, unsigned long long *keys,
                        unsigned long long *deep0, unsigned long long *deep1, int *vals) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    if (r >= dev_count(nmax, dn)) return;
     vals[r] = r;
     if (s.infront[r] == 0) {
         keys[r] = ~0ull;
@@ -169,16 +174,17 @@ __global__ void k_keys(DState s, int n, double bsize, int nwords, unsigned long 
 }
 
 // out[i] = in[perm[i]]
-__global__ void k_gather_keys(const unsigned long long *__restrict__ in, const int *__restrict__ perm, int n,
-                              unsigned long long *__restrict__ out) {
+__global__ void k_gather_keys(const unsigned long long *__restrict__ in, const int *__restrict__ perm, int nmax,
+                              const int *dn, unsigned long long *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[perm[i]];
+    if (i < dev_count(nmax, dn)) out[i] = in[perm[i]];
 }
 
 // The number of active points, from the sorted keys: an active key's top
 // nibble is at most 8, so active keys sort before the inactive ~0 ones. One
 // thread finds the boundary and writes it; *nact was cleared beforehand.
-__global__ void k_count_active(const unsigned long long *__restrict__ keys, int n, int *nact) {
+__global__ void k_count_active(const unsigned long long *__restrict__ keys, int nmax, const int *dn, int *nact) {
+    const int n = dev_count(nmax, dn);
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     if (keys[r] != ~0ull && (r + 1 == n || keys[r + 1] == ~0ull)) *nact = r + 1;
@@ -206,7 +212,8 @@ struct LevelCtl {
     int nwalk;              // non-empty nodes, the depth-first walk's length
 };
 
-__global__ void k_root(Nodes nd, DState s, int n, double bsize, const int *nact, LevelCtl *lc) {
+__global__ void k_root(Nodes nd, DState s, int nmax, const int *dn, double bsize, const int *nact, LevelCtl *lc) {
+    const int n = dev_count(nmax, dn);
     if (lc) {
         lc->lvl[0] = 0;
         lc->lvl[1] = 1;
@@ -523,11 +530,21 @@ __global__ void k_up_d(Nodes nd, DState s, const LevelCtl *lc, int level, const 
 
 __global__ void k_build_tail(const Nodes nd, LevelCtl *lc) { lc->nwalk = lc->overflow ? 0 : nd.size[0]; }
 
-// depth-first positions of the children of level-`L` nodes and the walk
-// records of the level-`L` nodes themselves
-__global__ void k_down(Nodes nd, DState s, int L0, int L1, WNode *walk, float *wrad) {
-    int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L1) return;
+// A recorded step's build is complete when no level outgrew its bound or the
+// node arrays, no node of the deepest level holds several points, the tree
+// ends within the levels launched (levels 0 .. top) and its walk fits the
+// records (nwalk_max).  Otherwise the step's later phases are switched off
+// (ctl->abort, no walk records) and the host takes the step again eagerly.
+__global__ void k_build_check(LevelCtl *lc, DCtl *ctl, int top, int nwalk_max, const int *dn) {
+    // (no points at all: the eager build's error)
+    if (lc->overflow || lc->depth || lc->lvl[top + 2] > lc->lvl[top + 1] || lc->nwalk > nwalk_max || *dn <= 0) {
+        ctl->abort = 1;
+        lc->nwalk = 0;
+    }
+}
+
+// depth-first positions of the children of node q and its walk record
+__device__ __forceinline__ void down_of(const Nodes &nd, const DState &s, int q, WNode *walk, float *wrad) {
     const int nc = nd.ncap;
     const int pc = nd.pcount[q];
     if (q == 0) nd.dfs[0] = 0;
@@ -572,10 +589,26 @@ __global__ void k_down(Nodes nd, DState s, int L0, int L1, WNode *walk, float *w
     wrad[my] = rf;
 }
 
+// down_of over the nodes of level-`L`
+__global__ void k_down(Nodes nd, DState s, int L0, int L1, WNode *walk, float *wrad) {
+    const int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < L1) down_of(nd, s, q, walk, wrad);
+}
+
+// k_down on a level whose range is on the device; nothing once a recorded
+// step's build has been found incomplete (k_build_check)
+__global__ void k_down_d(Nodes nd, DState s, const LevelCtl *lc, int level, WNode *walk, float *wrad,
+                         const DCtl *ctl) {
+    if (ctl->abort) return;
+    const int q = lc->lvl[level] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < lc->lvl[level + 1]) down_of(nd, s, q, walk, wrad);
+}
+
 // j = behind(leaf_infront) of every leaf record, read once before a walk
 // (links do not change while it runs), so that the walk loop has no load
 // that depends on the node it evaluates
-__global__ void k_leaf_links(DState s, WNode *__restrict__ walk, int nwalk) {
+__global__ void k_leaf_links(DState s, WNode *__restrict__ walk, int nwmax, const int *dnw) {
+    const int nwalk = dev_count(nwmax, dnw);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nwalk) return;
     const int pc = walk[q].pcount, li = walk[q].leaf_infront;
@@ -600,9 +633,18 @@ constexpr int XCDS = 8;
 
 struct WalkQueues {
     int *cnt;   // XCDS counters, zero at launch
-    int nb;     // blocks of 64 targets
+    int nb;     // blocks of targets (of `per` targets each)
     int nimg;   // images per block
+    const int *dn = nullptr;   // the target count on the device (then nb is derived from it) ...
+    int nmax = 0;              // ... bounded by this
+    int per = 64;              // targets per block
 };
+
+// the queues with their block count taken from the device when it is there
+__device__ __forceinline__ WalkQueues resolve(WalkQueues wq) {
+    if (wq.dn) wq.nb = (dev_count(wq.nmax, wq.dn) + wq.per - 1) / wq.per;
+    return wq;
+}
 
 // The next item of the calling wave (every lane active): false once all
 // queues are empty.  h: the queues this wave has emptied.
@@ -632,8 +674,9 @@ constexpr int OPEN_BAND_ULPS = 16;
 // correction, src/tree.f90:266-278); for theta <= 1.1 (so that w / theta >
 // geo) that is dist > geo + width / theta (width / theta without the
 // correction), compared here on squares.
-__global__ void k_pack48(DState s, const WNode *__restrict__ walk, int nwalk, double theta, int corr,
+__global__ void k_pack48(DState s, const WNode *__restrict__ walk, int nwmax, const int *dnw, double theta, int corr,
                          Node48 *__restrict__ out, double *__restrict__ circ64) {
+    const int nwalk = dev_count(nwmax, dnw);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > nwalk) return;
     if (q == nwalk) {   // the spare record
@@ -748,7 +791,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                                    const Node48 *__restrict__ walk,
                                                    const double *__restrict__ circ64, const WNode *__restrict__ wn,
                                                    int nwalk, double rn2, double sing2, double *__restrict__ part,
-                                                   DCtl *ctl, unsigned long long *counters) {
+                                                   DCtl *ctl, unsigned long long *counters, const int *dn,
+                                                   const int *dnw) {
+    if (ctl->abort) return;
+    if (dn) nord = n = dev_count(n, dn);   // a recorded step: this step's count (its build's rows)
+    if (dnw) nwalk = *dnw;
+    wq = resolve(wq);
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     const double kappa = p.kappa;
@@ -944,7 +992,12 @@ template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, int n, int npad,
                                                  const int *__restrict__ order, int nord, WalkQueues wq,
                                                  const WNode *__restrict__ walk, int nwalk, double *__restrict__ part,
-                                                 DCtl *ctl, unsigned long long *counters) {
+                                                 DCtl *ctl, unsigned long long *counters, const int *dn,
+                                                 const int *dnw) {
+    if (ctl->abort) return;
+    if (dn) nord = n = dev_count(n, dn);
+    if (dnw) nwalk = *dnw;
+    wq = resolve(wq);
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     unsigned long long nev = 0, nvis = 0;
@@ -1044,9 +1097,9 @@ __global__ __launch_bounds__(BT) void k_walk_vel(DState s, Phys p, Shifts sh, in
 // 30-bit Morton keys of the current positions (1024 cells per box edge),
 // inactive rows last: the target order of the close search, which runs
 // after insertions the tree build did not see.
-__global__ void k_target_keys(DState s, int n, double bsize, unsigned long long *keys, int *vals) {
+__global__ void k_target_keys(DState s, int nmax, const int *dn, double bsize, unsigned long long *keys, int *vals) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    if (r >= dev_count(nmax, dn)) return;
     vals[r] = r;
     unsigned code = 0xffffffffu;
     if (s.infront[r] != 0) {
@@ -1085,8 +1138,9 @@ constexpr int CLOSE_DEEP = 15;   // the largest level code the 4 bits above the 
 // factors are far above the rounding of dd, of 4 w^2 and of the bounds.
 constexpr double CLOSE_BAND = 0x1p-46;
 
-__global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const float *__restrict__ wrad, int nwalk,
-                             double bsize, CNode *__restrict__ out) {
+__global__ void k_pack_close(DState s, const WNode *__restrict__ walk, const float *__restrict__ wrad, int nwmax,
+                             const int *dnw, double bsize, CNode *__restrict__ out) {
+    const int nwalk = dev_count(nwmax, dnw);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > nwalk) return;
     if (q == nwalk) {   // the spare record
@@ -1119,7 +1173,11 @@ template <int BT>
 __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *__restrict__ order, int nord,
                                                    WalkQueues wq, const CNode *__restrict__ walk,
                                                    const WNode *__restrict__ full, int nwalk, double pm,
-                                                   double b4) {
+                                                   double b4, const DCtl *ctl, const int *dn, const int *dnw) {
+    if (ctl && ctl->abort) return;
+    if (dn) nord = n = dev_count(n, dn);
+    if (dnw) nwalk = *dnw;
+    wq = resolve(wq);
     const int lane = threadIdx.x & 63;
     int h = 0;   // queues emptied
     for (;;) {
@@ -1260,6 +1318,11 @@ struct TreeBuf {
     const int *vals = nullptr;           // sorted rows of the last build
     LevelCtl *lc = nullptr, *lc_h = nullptr;   // level loop control, device and pinned copy
     int sync_builds = 0;                 // builds that ran the synchronous level loop
+    long gen = 0;                        // bumped whenever a buffer above moves (recorded steps hold them)
+    // the plan of the recorded steps' builds: levels 0 .. g_top launched on
+    // these bounds, over g_rows rows
+    int g_top = -1, g_rows = 0;
+    int g_bound[LEVELS + 1] = {}, g_wgrid[LEVELS + 1] = {}, g_hint[LEVELS + 1] = {};
 };
 
 #define TCHK(x)                                     \
@@ -1294,19 +1357,19 @@ static void free_nodes(Nodes &nd) {
 
 void nv_tree_free(TreeBuf *t) {
     if (!t) return;
-    hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
-    hipFree(t->nact); hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp); hipFree(t->walk);
-    hipFree(t->wrad);
-    hipFree(t->walk48);
-    hipFree(t->circ64);
-    hipFree(t->walkc);
-    hipFree(t->counters);
-    hipFree(t->wq);
-    hipHostFree(t->h_int);
-    hipFree(t->lc);
-    for (auto *q : t->raw) hipFree(q);
-    for (auto *q : t->srt) hipFree(q);
-    hipHostFree(t->lc_h);
+    (void)hipFree(t->k0); (void)hipFree(t->k1); (void)hipFree(t->v0); (void)hipFree(t->v1); (void)hipFree(t->hist); (void)hipFree(t->tmp);
+    (void)hipFree(t->nact); (void)hipFree(t->flags); (void)hipFree(t->offs); (void)hipFree(t->scan_tmp); (void)hipFree(t->walk);
+    (void)hipFree(t->wrad);
+    (void)hipFree(t->walk48);
+    (void)hipFree(t->circ64);
+    (void)hipFree(t->walkc);
+    (void)hipFree(t->counters);
+    (void)hipFree(t->wq);
+    (void)hipHostFree(t->h_int);
+    (void)hipFree(t->lc);
+    for (auto *q : t->raw) (void)hipFree(q);
+    for (auto *q : t->srt) (void)hipFree(q);
+    (void)hipHostFree(t->lc_h);
     free_nodes(t->nd);
     delete t;
 }
@@ -1339,13 +1402,14 @@ static int grow_nodes(TreeBuf *t, hipStream_t st, int want, int keep) {
     TCHK(hipStreamSynchronize(st));
     free_nodes(o);
     t->nd = n;
+    ++t->gen;
     return NV_OK;
 }
 
 static int ensure_rows(TreeBuf *t, int n) {
     if (n <= t->cap) return NV_OK;
     int cap = std::max(n, t->cap + t->cap / 2);
-    hipFree(t->k0); hipFree(t->k1); hipFree(t->v0); hipFree(t->v1); hipFree(t->hist); hipFree(t->tmp);
+    (void)hipFree(t->k0); (void)hipFree(t->k1); (void)hipFree(t->v0); (void)hipFree(t->v1); (void)hipFree(t->hist); (void)hipFree(t->tmp);
     TCHK(hipMalloc(&t->k0, sizeof(unsigned long long) * cap));
     TCHK(hipMalloc(&t->k1, sizeof(unsigned long long) * cap));
     TCHK(hipMalloc(&t->v0, sizeof(int) * cap));
@@ -1354,17 +1418,19 @@ static int ensure_rows(TreeBuf *t, int n) {
     TCHK(hipMalloc(&t->hist, sizeof(int) * 256 * (size_t)nblk));
     TCHK(hipMalloc(&t->tmp, sizeof(int) * (size_t)nv_scan_tmp_ints(256 * nblk)));
     t->cap = cap;
+    ++t->gen;
     return NV_OK;
 }
 
 static int ensure_flags(TreeBuf *t, int m) {
     if (m <= t->flag_cap) return NV_OK;
     int cap = std::max(m, 2 * t->flag_cap);
-    hipFree(t->flags); hipFree(t->offs); hipFree(t->scan_tmp);
+    (void)hipFree(t->flags); (void)hipFree(t->offs); (void)hipFree(t->scan_tmp);
     TCHK(hipMalloc(&t->flags, sizeof(int) * cap));
     TCHK(hipMalloc(&t->offs, sizeof(int) * cap));
     TCHK(hipMalloc(&t->scan_tmp, sizeof(int) * (size_t)nv_scan_tmp_ints(cap)));
     t->flag_cap = cap;
+    ++t->gen;
     return NV_OK;
 }
 
@@ -1381,35 +1447,37 @@ static int ensure_deep(TreeBuf *t, int n) {
     for (auto *&q : t->raw) TCHK(hipMalloc(&q, sizeof(unsigned long long) * (size_t)cap));
     for (auto *&q : t->srt) TCHK(hipMalloc(&q, sizeof(unsigned long long) * (size_t)cap));
     t->deep_cap = cap;
+    ++t->gen;
     return NV_OK;
 }
 
 // The paths of rows [0, n) sorted (stable LSD: the last word first), word 0
 // in k0 and the rows in v0 (t->vals); deeper words gathered into srt[].
-static int sort_paths(TreeBuf *t, hipStream_t st, DState d, int n, double bsize) {
+// With dn the row count is *dn (<= n), read on the device.
+static int sort_paths(TreeBuf *t, hipStream_t st, DState d, int n, const int *dn, double bsize) {
     const int K = t->kwords;
     const dim3 g((n + 255) / 256), b(256);
     if (K == 1) {
-        hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, bsize, 1, t->k0, (unsigned long long *)nullptr,
+        hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, dn, bsize, 1, t->k0, (unsigned long long *)nullptr,
                            (unsigned long long *)nullptr, t->v0);
         TCHK(hipGetLastError());
         // stable LSD radix sort of the 64-bit paths; the result is back in k0/v0
-        TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+        TCHK(nv_radix_sort_pairs_dn(st, t->k0, t->v0, t->k1, t->v1, n, dn, 64, t->hist, t->tmp));
         return NV_OK;
     }
     if (int rc = ensure_deep(t, n)) return rc;
-    hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, bsize, K, t->raw[0], t->raw[1], t->raw[2], t->v0);
+    hipLaunchKernelGGL(k_keys, g, b, 0, st, d, n, dn, bsize, K, t->raw[0], t->raw[1], t->raw[2], t->v0);
     TCHK(hipGetLastError());
     // v0 is the identity here: each pass sorts the rows by one more word
     for (int w = K - 1; w >= 0; --w) {
         hipLaunchKernelGGL(k_gather_keys, g, b, 0, st, (const unsigned long long *)t->raw[w], (const int *)t->v0, n,
-                           t->k0);
+                           dn, t->k0);
         TCHK(hipGetLastError());
-        TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 64, t->hist, t->tmp));
+        TCHK(nv_radix_sort_pairs_dn(st, t->k0, t->v0, t->k1, t->v1, n, dn, 64, t->hist, t->tmp));
     }
     for (int w = 1; w < K; ++w)
         hipLaunchKernelGGL(k_gather_keys, g, b, 0, st, (const unsigned long long *)t->raw[w], (const int *)t->v0, n,
-                           t->srt[w - 1]);
+                           dn, t->srt[w - 1]);
     TCHK(hipGetLastError());
     return NV_OK;
 }
@@ -1466,7 +1534,8 @@ static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double
     TCHK(hipStreamSynchronize(st));
     t->nact_h = t->h_int[0];
     if (int rc = grow_nodes(t, st, std::max(1024, 4 * t->nact_h + 64), 0)) return rc;
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, (LevelCtl *)nullptr);
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, (const int *)nullptr, bsize, t->nact,
+                       (LevelCtl *)nullptr);
     TCHK(hipGetLastError());
     t->lvl.assign(1, 0);
     t->lvl.push_back(1);
@@ -1475,34 +1544,31 @@ static int build_levels_sync(TreeBuf *t, hipStream_t st, DState d, int n, double
 
 constexpr int BUILD_RETRY = -1000;
 
-// The same level loop with no host round trip until its end: every level is
-// launched on a bound taken from the previous build's level sizes (one more
-// level than it had), the kernels take their node range from LevelCtl, and a
-// level that outgrows its bound, the node arrays or the levels launched makes
-// the build return BUILD_RETRY, after which the caller runs the synchronous
-// loop from the root again. Both loops run the same kernels on the same
-// ranges, so the trees are identical.
-static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
-    const int prev = (int)t->lvl.size() - 1;   // levels of the previous build
-    if (prev < 1 || !t->lc) return BUILD_RETRY;
-    const int top = std::min(prev, t->maxlev);   // deepest level launched
-    const Keys keys = sorted_keys(t);
-    int bound[LEVELS + 1], wgrid[LEVELS + 1];
+// Launch bounds of levels 0 .. top from the previous build's level sizes
+// (levels 0 .. prev - 1; 128 for the level past them): the size + 1/8 + 256,
+// rounded up to 256.  Returns the node count holding those levels and the
+// children of level top.
+static long long plan_levels(const TreeBuf *t, int prev, int top, int *bound, int *wgrid, int *fmax) {
     long long total = 1;
-    int fmax = 1;
+    *fmax = 1;
     for (int l = 0; l <= top; ++l) {
         const int hint = l < prev ? t->lvl[l + 1] - t->lvl[l] : 128;
         bound[l] = (hint + hint / 8 + 256 + 255) / 256 * 256;
         wgrid[l] = hint <= 4096 ? std::min(4096, bound[l]) : 0;
         total += bound[l];
-        fmax = std::max(fmax, bound[l]);
+        *fmax = std::max(*fmax, bound[l]);
     }
     // children of the deepest launched level land past its bound
-    total += 8ll * bound[top];
-    if (total > (1ll << 30)) return BUILD_RETRY;
-    if (int rc = grow_nodes(t, st, (int)total, 0)) return rc;
-    if (int rc = ensure_flags(t, fmax)) return rc;
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, bsize, t->nact, t->lc);
+    return total + 8ll * bound[top];
+}
+
+// The level loop on node ranges held in LevelCtl (launches only): levels
+// 0 .. top on their bounds, the moments upwards, the walk length.  Rows
+// [0, n), or [0, *dn) when dn is given.
+static int enqueue_levels(TreeBuf *t, hipStream_t st, DState d, int n, const int *dn, double bsize, DCtl *ctl,
+                          int top, const int *bound, const int *wgrid) {
+    const Keys keys = sorted_keys(t);
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, t->nd, d, n, dn, bsize, t->nact, t->lc);
     TCHK(hipGetLastError());
     for (int l = 0; l <= top; ++l) {
         hipLaunchKernelGGL(k_split_flags_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, (const LevelCtl *)t->lc, l,
@@ -1526,6 +1592,26 @@ static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, doubl
     }
     hipLaunchKernelGGL(k_build_tail, dim3(1), dim3(1), 0, st, t->nd, t->lc);
     TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+// The same level loop with no host round trip until its end: every level is
+// launched on a bound taken from the previous build's level sizes (one more
+// level than it had), the kernels take their node range from LevelCtl, and a
+// level that outgrows its bound, the node arrays or the levels launched makes
+// the build return BUILD_RETRY, after which the caller runs the synchronous
+// loop from the root again. Both loops run the same kernels on the same
+// ranges, so the trees are identical.
+static int build_levels_async(TreeBuf *t, hipStream_t st, DState d, int n, double bsize, DCtl *ctl) {
+    const int prev = (int)t->lvl.size() - 1;   // levels of the previous build
+    if (prev < 1 || !t->lc) return BUILD_RETRY;
+    const int top = std::min(prev, t->maxlev);   // deepest level launched
+    int bound[LEVELS + 1], wgrid[LEVELS + 1], fmax = 1;
+    const long long total = plan_levels(t, prev, top, bound, wgrid, &fmax);
+    if (total > (1ll << 30)) return BUILD_RETRY;
+    if (int rc = grow_nodes(t, st, (int)total, 0)) return rc;
+    if (int rc = ensure_flags(t, fmax)) return rc;
+    if (int rc = enqueue_levels(t, st, d, n, nullptr, bsize, ctl, top, bound, wgrid)) return rc;
     TCHK(hipMemcpyAsync(t->lc_h, t->lc, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     const LevelCtl &h = *t->lc_h;
@@ -1555,8 +1641,9 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     for (;;) {
         t->maxlev = KEY_LEVELS * t->kwords;
         TCHK(nv_fill_int(st, t->nact, 0, 1));
-        if ((rc = sort_paths(t, st, d, n, bsize))) return rc;
-        hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, t->nact);
+        if ((rc = sort_paths(t, st, d, n, nullptr, bsize))) return rc;
+        hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, (const int *)nullptr,
+                           t->nact);
         TCHK(hipGetLastError());
         t->vals = t->v0;
         t->built_rows = n;
@@ -1581,13 +1668,14 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
     const int nlev = (int)t->lvl.size() - 1;
     t->nnodes = t->lvl.back();
     if (t->nwalk > t->walk_cap) {
-        hipFree(t->walk);
-        hipFree(t->wrad);
+        (void)hipFree(t->walk);
+        (void)hipFree(t->wrad);
         t->walk = nullptr;
         t->wrad = nullptr;
         t->walk_cap = std::max(t->nwalk, 2 * t->walk_cap);
         TCHK(hipMalloc(&t->walk, sizeof(WNode) * (size_t)t->walk_cap));
         TCHK(hipMalloc(&t->wrad, sizeof(float) * (size_t)t->walk_cap));
+        ++t->gen;
     }
     for (int lv = 0; lv < nlev; ++lv) {
         const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
@@ -1649,16 +1737,17 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         // one spare record past the last: the walk fetches a successor index
         // up to nwalk without a clamp
         if (t->nwalk + 1 > t->walk48_cap) {
-            hipFree(t->walk48);
+            (void)hipFree(t->walk48);
             t->walk48 = nullptr;
             t->walk48_cap = std::max(t->nwalk + 1, 2 * t->walk48_cap);
             TCHK(hipMalloc(&t->walk48, sizeof(Node48) * (size_t)t->walk48_cap));
-            hipFree(t->circ64);
+            (void)hipFree(t->circ64);
             t->circ64 = nullptr;
             TCHK(hipMalloc(&t->circ64, 3 * sizeof(double) * (size_t)t->walk48_cap));
+            ++t->gen;
         }
         hipLaunchKernelGGL(k_pack48, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
-                           p.tree_theta, p.extra_corr, t->walk48, t->circ64);
+                           (const int *)nullptr, p.tree_theta, p.extra_corr, t->walk48, t->circ64);
         // items: (image, WALK_TPT blocks of 64 targets)
         const int nbt = (n + 64 * WALK_TPT - 1) / (64 * WALK_TPT), nit32 = nbt * sh.n;
         if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256, WALK_TPT>)) == 0) return NV_ERR_HIP;
@@ -1666,15 +1755,17 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
         TCHK(nv_fill_int(st, t->wq, 0, XCDS));
         hipLaunchKernelGGL((k_walk_vel32<256, WALK_TPT>), dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
                            WalkQueues{t->wq, nbt, sh.n}, t->walk48, t->circ64, t->walk, t->nwalk, rn2,
-                           sqrt_threshold(NV_EPS), part, ctl, t->counters);
+                           sqrt_threshold(NV_EPS), part, ctl, t->counters, (const int *)nullptr, (const int *)nullptr);
     } else {
         if (t->nwalk > 0)
-            hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk);
+            hipLaunchKernelGGL(k_leaf_links, dim3((t->nwalk + 255) / 256), dim3(256), 0, st, d, t->walk, t->nwalk,
+                               (const int *)nullptr);
         if (t->grid64 == 0 && (t->grid64 = resident_blocks(k_walk_vel<256>)) == 0) return NV_ERR_HIP;
         const int grid = std::min(t->grid64, (nitems + 3) / 4);
         TCHK(nv_fill_int(st, t->wq, 0, XCDS));
         hipLaunchKernelGGL(k_walk_vel<256>, dim3(grid), dim3(256), 0, st, d, p, sh, n, npad, order, nord,
-                           WalkQueues{t->wq, nb64, sh.n}, t->walk, t->nwalk, part, ctl, t->counters);
+                           WalkQueues{t->wq, nb64, sh.n}, t->walk, t->nwalk, part, ctl, t->counters, (const int *)nullptr,
+                           (const int *)nullptr);
     }
     TCHK(hipGetLastError());
     return NV_OK;
@@ -1719,29 +1810,191 @@ int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n
     // the sort replaces it (the build's keys and rows are not needed after
     // the build; the velocity walk accepts any order of the rows).
     if (int rc = ensure_rows(t, n)) return rc;
-    hipLaunchKernelGGL(k_target_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, bsize, t->k0, t->v0);
+    hipLaunchKernelGGL(k_target_keys, dim3((n + 255) / 256), dim3(256), 0, st, d, n, (const int *)nullptr, bsize,
+                       t->k0, t->v0);
     TCHK(hipGetLastError());
     TCHK(nv_radix_sort_pairs(st, t->k0, t->v0, t->k1, t->v1, n, 32, t->hist, t->tmp));
     t->vals = t->v0;
     t->built_rows = n;
     // close-search records, one spare past the last
     if (t->nwalk + 1 > t->walkc_cap) {
-        hipFree(t->walkc);
+        (void)hipFree(t->walkc);
         t->walkc = nullptr;
         t->walkc_cap = std::max(t->nwalk + 1, 2 * t->walkc_cap);
         TCHK(hipMalloc(&t->walkc, sizeof(CNode) * (size_t)t->walkc_cap));
+        ++t->gen;
     }
     if (t->nwalk >= (1 << CLOSE_SKIP_BITS)) return NV_ERR_UNSUPPORTED;   // the skip index field
     hipLaunchKernelGGL(k_pack_close, dim3((t->nwalk + 1 + 255) / 256), dim3(256), 0, st, d, t->walk,
-                       (const float *)t->wrad, t->nwalk, bsize, t->walkc);
+                       (const float *)t->wrad, t->nwalk, (const int *)nullptr, bsize, t->walkc);
     if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
     const int nitems = (n + 63) / 64;
     TCHK(nv_fill_int(st, t->wq, 0, XCDS));
     hipLaunchKernelGGL(k_walk_close<256>, dim3(std::min(t->gridc, (nitems + 3) / 4)), dim3(256), 0, st, d, n, t->vals,
                        n, WalkQueues{t->wq, nitems, 1}, t->walkc, t->walk, t->nwalk, bsize * 0x1p-40,
-                       4.0 * (bsize * bsize));
+                       4.0 * (bsize * bsize), (const DCtl *)nullptr, (const int *)nullptr, (const int *)nullptr);
     TCHK(hipGetLastError());
     return NV_OK;
+}
+
+// ------------------------------------------------------ recorded steps
+// A recorded octree step reads no size on the host: its build sorts rows
+// [0, *dn) of a fixed row range, launches levels 0 .. top (one more than the
+// last build had) on bounds with slack over that build's level sizes (as the
+// eager build's first attempt does), and holds walk records for every node
+// those bounds allow.  k_build_check switches the step's later phases off
+// when the build outgrows them; the host then takes the step again eagerly
+// and records anew from that build's sizes.
+
+// resident workgroups of each persistent walk
+static int walk_grids(TreeBuf *t) {
+    if (t->grid32 == 0 && (t->grid32 = resident_blocks(k_walk_vel32<256, WALK_TPT>)) == 0) return NV_ERR_HIP;
+    if (t->grid64 == 0 && (t->grid64 = resident_blocks(k_walk_vel<256>)) == 0) return NV_ERR_HIP;
+    if (t->gridc == 0 && (t->gridc = resident_blocks(k_walk_close<256>)) == 0) return NV_ERR_HIP;
+    return NV_OK;
+}
+
+// grow a record array to hold `want` records
+template <typename T>
+static int ensure_records(TreeBuf *t, T **p, int *cap, int want) {
+    if (want <= *cap) return NV_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    TCHK(hipMalloc(p, sizeof(T) * (size_t)want));
+    *cap = want;
+    ++t->gen;
+    return NV_OK;
+}
+
+int nv_tree_graph_prepare(TreeBuf *t, hipStream_t st, int rows) {
+    const int prev = (int)t->lvl.size() - 1;   // levels of the last build
+    if (prev < 1 || rows <= 0) return NV_ERR_STATE;
+    if (int rc = ensure_rows(t, rows)) return rc;
+    if (t->kwords > 1)
+        if (int rc = ensure_deep(t, rows)) return rc;
+    t->maxlev = KEY_LEVELS * t->kwords;
+    const int top = std::min(prev, t->maxlev);
+    int fmax = 1;
+    const long long total = plan_levels(t, prev, top, t->g_bound, t->g_wgrid, &fmax);
+    // the close search's records index their successors in CLOSE_SKIP_BITS bits
+    if (total + 1 >= (1ll << CLOSE_SKIP_BITS)) return NV_ERR_UNSUPPORTED;
+    for (int l = 0; l <= top; ++l) t->g_hint[l] = l < prev ? t->lvl[l + 1] - t->lvl[l] : 0;
+    if (int rc = grow_nodes(t, st, (int)total, 0)) return rc;
+    if (int rc = ensure_flags(t, fmax)) return rc;
+    const int nrec = (int)total + 1;   // every node a build within the bounds makes, and the spare
+    if (nrec > t->walk_cap) {
+        (void)hipFree(t->wrad);
+        t->wrad = nullptr;
+        int cap = t->walk_cap;
+        if (int rc = ensure_records(t, &t->walk, &cap, nrec)) return rc;
+        TCHK(hipMalloc(&t->wrad, sizeof(float) * (size_t)cap));
+        t->walk_cap = cap;
+    }
+    if (nrec > t->walk48_cap) {
+        (void)hipFree(t->circ64);
+        t->circ64 = nullptr;
+        int cap = t->walk48_cap;
+        if (int rc = ensure_records(t, &t->walk48, &cap, nrec)) return rc;
+        TCHK(hipMalloc(&t->circ64, 3 * sizeof(double) * (size_t)cap));
+        t->walk48_cap = cap;
+    }
+    if (int rc = ensure_records(t, &t->walkc, &t->walkc_cap, nrec)) return rc;
+    if (int rc = walk_grids(t)) return rc;
+    t->g_top = top;
+    t->g_rows = rows;
+    return NV_OK;
+}
+
+int nv_tree_graph_build(TreeBuf *t, hipStream_t st, DState d, Phys p, const int *dn, DCtl *ctl) {
+    const int n = t->g_rows, top = t->g_top;
+    if (top < 0) return NV_ERR_STATE;
+    const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
+    TCHK(nv_fill_int(st, t->nact, 0, 1));
+    if (int rc = sort_paths(t, st, d, n, dn, bsize)) return rc;
+    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, dn, t->nact);
+    TCHK(hipGetLastError());
+    t->vals = t->v0;
+    if (int rc = enqueue_levels(t, st, d, n, dn, bsize, ctl, top, t->g_bound, t->g_wgrid)) return rc;
+    const int nwalk_max = std::min(t->walk_cap, std::min(t->walk48_cap, t->walkc_cap)) - 1;
+    hipLaunchKernelGGL(k_build_check, dim3(1), dim3(1), 0, st, t->lc, ctl, top, nwalk_max, dn);
+    TCHK(hipGetLastError());
+    for (int l = 0; l <= top; ++l)
+        hipLaunchKernelGGL(k_down_d, dim3(t->g_bound[l] / 256), dim3(256), 0, st, t->nd, d, (const LevelCtl *)t->lc,
+                           l, t->walk, t->wrad, (const DCtl *)ctl);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_graph_walk(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int precision, double rn2,
+                       double *part, DCtl *ctl, const int *dn) {
+    const int npad = t->g_rows;
+    const int *dnw = &t->lc->nwalk;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(4), 0, st, t->counters);
+    TCHK(nv_fill_int(st, t->wq, 0, XCDS));
+    if (precision == NV_PREC_FP32C && p.tree_theta <= 1.1) {
+        hipLaunchKernelGGL(k_pack48, dim3((t->walk48_cap + 255) / 256), dim3(256), 0, st, d, t->walk,
+                           t->walk48_cap - 1, dnw, p.tree_theta, p.extra_corr, t->walk48, t->circ64);
+        WalkQueues wq{t->wq, 0, sh.n, dn, npad, 64 * WALK_TPT};
+        hipLaunchKernelGGL((k_walk_vel32<256, WALK_TPT>), dim3(t->grid32), dim3(256), 0, st, d, p, sh, npad, npad,
+                           (const int *)t->v0, 0, wq, t->walk48, t->circ64, t->walk, 0, rn2, sqrt_threshold(NV_EPS),
+                           part, ctl, t->counters, dn, dnw);
+    } else {
+        hipLaunchKernelGGL(k_leaf_links, dim3((t->walk_cap + 255) / 256), dim3(256), 0, st, d, t->walk, t->walk_cap,
+                           dnw);
+        WalkQueues wq{t->wq, 0, sh.n, dn, npad, 64};
+        hipLaunchKernelGGL(k_walk_vel<256>, dim3(t->grid64), dim3(256), 0, st, d, p, sh, npad, npad,
+                           (const int *)t->v0, 0, wq, t->walk, 0, part, ctl, t->counters, dn, dnw);
+    }
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_graph_close(TreeBuf *t, hipStream_t st, DState d, double bsize, const int *dn, DCtl *ctl) {
+    const int rows = t->g_rows;
+    const int *dnw = &t->lc->nwalk;
+    hipLaunchKernelGGL(k_target_keys, dim3((rows + 255) / 256), dim3(256), 0, st, d, rows, dn, bsize, t->k0, t->v0);
+    TCHK(hipGetLastError());
+    TCHK(nv_radix_sort_pairs_dn(st, t->k0, t->v0, t->k1, t->v1, rows, dn, 32, t->hist, t->tmp));
+    hipLaunchKernelGGL(k_pack_close, dim3((t->walkc_cap + 255) / 256), dim3(256), 0, st, d, t->walk,
+                       (const float *)t->wrad, t->walkc_cap - 1, dnw, bsize, t->walkc);
+    TCHK(nv_fill_int(st, t->wq, 0, XCDS));
+    hipLaunchKernelGGL(k_walk_close<256>, dim3(t->gridc), dim3(256), 0, st, d, rows, (const int *)t->v0, 0,
+                       WalkQueues{t->wq, 0, 1, dn, rows, 64}, t->walkc, t->walk, 0, bsize * 0x1p-40,
+                       4.0 * (bsize * bsize), (const DCtl *)ctl, dn, dnw);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_graph_gen(const TreeBuf *t) { return (int)(t->gen & 0x7fffffff); }
+
+int nv_tree_graph_take_async(TreeBuf *t, hipStream_t st) {
+    TCHK(hipMemcpyAsync(t->lc_h, t->lc, sizeof(LevelCtl), hipMemcpyDeviceToHost, st));
+    return nv_tree_walk_counters_async(t, st);
+}
+
+bool nv_tree_graph_take(TreeBuf *t, nv_stats *stats) {
+    const LevelCtl &h = *t->lc_h;
+    const int top = t->g_top;
+    int nlev = 0;
+    while (nlev <= top && h.lvl[nlev + 1] > h.lvl[nlev]) ++nlev;
+    t->lvl.assign(h.lvl, h.lvl + nlev + 1);
+    t->nwalk = h.nwalk;
+    t->nact_h = h.nact;
+    t->nnodes = t->lvl.back();
+    t->built_rows = 0;   // the close search reordered the rows
+    nv_tree_walk_counters_take(t, stats);
+    stats->tree_nodes = t->nnodes;
+    stats->tree_levels = nlev;
+    stats->tree_builds += 1;
+    // record anew before the next build may outgrow the plan: a level past
+    // half its slack, or a tree as deep as the levels launched
+    bool stale = nlev > top;
+    for (int l = 0; l < nlev && l <= top && !stale; ++l) {
+        const int m = h.lvl[l + 1] - h.lvl[l];
+        if (2 * (m - t->g_hint[l]) > t->g_bound[l] - t->g_hint[l]) stale = true;
+    }
+    return stale;
 }
 
 int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *nout) {

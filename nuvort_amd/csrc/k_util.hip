@@ -120,7 +120,11 @@ constexpr int RS_BT = 256;
 constexpr int RS_IPT = 16;
 constexpr int RS_TILE = RS_BT * RS_IPT;
 
-__global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int *hist, int nblk) {
+// dn: the item count on the device (at most n), or null for n
+__device__ __forceinline__ int count_of(int n, const int *dn) { return dn ? min(*dn, n) : n; }
+
+__global__ void k_rs_hist(const unsigned long long *keys, int n, const int *dn, int shift, int *hist, int nblk) {
+    n = count_of(n, dn);
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -133,8 +137,9 @@ __global__ void k_rs_hist(const unsigned long long *keys, int n, int shift, int 
     hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, int shift, const int *hist,
-                             const int *tsum, int nblk, unsigned long long *okeys, int *ovals) {
+__global__ void k_rs_scatter(const unsigned long long *keys, const int *vals, int n, const int *dn, int shift,
+                             const int *hist, const int *tsum, int nblk, unsigned long long *okeys, int *ovals) {
+    n = count_of(n, dn);
     __shared__ int running[256];
     __shared__ int wcnt[RS_BT / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -243,6 +248,13 @@ int nv_radix_hist_ints(int n) { return 256 * nv_radix_blocks(n); }
 
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
                                int bits, int *hist, int *scan_tmp) {
+    return nv_radix_sort_pairs_dn(st, k0, v0, k1, v1, n, nullptr, bits, hist, scan_tmp);
+}
+
+// the same for the first *dn (<= n) pairs, a count on the device: the
+// launches cover n, and blocks past the count take no part
+hipError_t nv_radix_sort_pairs_dn(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1,
+                                  int n, const int *dn, int bits, int *hist, int *scan_tmp) {
     if (n <= 0) return hipSuccess;
     const int nblk = nv_radix_blocks(n);
     unsigned long long *ka = k0, *kb = k1;
@@ -251,11 +263,11 @@ hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, 
     // adding the tile prefix: four launches per 8-bit pass
     const int nh = 256 * nblk;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, shift, hist, nblk);
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, st, ka, n, dn, shift, hist, nblk);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if ((e = nv_scan_tiles_sums(st, hist, nh, scan_tmp)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, shift, (const int *)hist,
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, n, dn, shift, (const int *)hist,
                            (const int *)scan_tmp, nblk, kb, vb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         std::swap(ka, kb);

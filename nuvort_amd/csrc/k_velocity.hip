@@ -594,6 +594,7 @@ __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, cons
 
 __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part,
                                                 int mode, int do_ab, DCtl *ctl) {
+    if (ctl->abort) return;
     finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
 }
 
@@ -604,12 +605,14 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
 // direct-sum layouts (hundreds of chunks per point) are slower so.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_finish_tree(
     DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
+    if (ctl->abort) return;   // (a recorded octree step whose build was incomplete)
     finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
 }
 
 // src/normal_fluid.f90:340-347 randomise_normal_fluid: the noise counter
 // moves to the next value, back to the first after the last
 __global__ void k_noise_tick(DCtl *ctl, int n) {
+    if (ctl->abort) return;
     int c = ctl->noise_ctr + 1;
     if (c >= n) c = 0;
     ctl->noise_ctr = c;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -90,6 +91,8 @@ struct Ctx {
         hipGraph_t g = nullptr;
         hipGraphExec_t e = nullptr;
         long gen = -1;
+        int tgen = -1;                           // octree steps: the tree buffers' generation ...
+        std::array<const void *, 12> sig{};     // ... and the other buffers they hold
     } sg[2][2];
     long layout_gen = 0;
     double *xa = nullptr;            // the buffer alloc_state gave to x
@@ -857,6 +860,53 @@ int tree_step(Ctx *c) {
     return rc;
 }
 
+// the buffers a recorded octree step holds besides the state and the tree
+std::array<const void *, 12> tree_step_sig(const Ctx *c) {
+    return {c->xa, c->part.p, c->d_S.p, c->d_poff.p, c->cand.p, c->empty.p, c->lcand.p, c->lempty.p, c->tmp.p,
+            c->queue.p, c->ev, c->noise.p};
+}
+
+// One pass of tree_step as kernel launches only: the step's counts read on
+// the device (pcount from the tangle table, the tree's sizes from its level
+// table), itime and t advanced on the device, the x / xn exchange resolved at
+// record time.  When the build outgrows the recorded plan it sets ctl->abort
+// and every later phase does nothing.
+int enqueue_tree_step(Ctx *c, DState d, bool recon) {
+    hipStream_t st = c->st;
+    const int R = c->rows, nimg = c->sh.n;
+    const int *dn = c->p_tn;   // pcount of the one tangle
+    HIPCHK(c, nvk_step_begin(st, c->ctl, c->p_tnact, c->K, c->queue.p));
+    HIPCHK(c, nvk_ghosts(st, d, c->phys, R));
+    if (nv_tree_graph_build(c->tree, st, d, c->phys, dn, c->ctl)) return fail(c, NV_ERR_HIP, "tree build launch failed");
+    HIPCHK(c, nv_fill_int(st, c->d_S.p, nimg, 1));
+    HIPCHK(c, nv_fill_int(st, (int *)c->d_poff.p, 0, 2));   // one long long
+    const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
+    if (nv_tree_graph_walk(c->tree, st, d, c->phys, c->sh, c->prm.precision, rn * rn, c->part.p, c->ctl, dn))
+        return fail(c, NV_ERR_HIP, "tree walk launch failed");
+    HIPCHK(c, nvk_finish(st, d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p,
+                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl));
+    std::swap(d.x, d.xn);
+    Tangles tg = tangles(c, nullptr, nullptr);
+    // pinsert (its counters were cleared by nvk_step_begin)
+    HIPCHK(c, nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
+    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
+    HIPCHK(c, nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    if (recon) {
+        const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
+        if (nv_tree_graph_close(c->tree, st, d, bsize, dn, c->ctl))
+            return fail(c, NV_ERR_HIP, "tree close search launch failed");
+        HIPCHK(c, nvk_precon_pre(st, d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
+        HIPCHK(c, nvk_premove_pre(st, d, c->phys, R, c->cand.p));
+        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
+        HIPCHK(c, nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl));
+    }
+    HIPCHK(c, nvk_step_end(st, d, c->phys, R, c->ctl, c->prm.dt));
+    return NV_OK;
+}
+
 void drop_graph(Ctx::StepGraph &g) {
     if (g.e) (void)hipGraphExecDestroy(g.e);
     if (g.g) (void)hipGraphDestroy(g.g);
@@ -938,6 +988,98 @@ int step_recorded(Ctx *c, int nsteps) {
         pu = c->ctl_h->nvel_total;
         if (rc) return rc;
 
+    }
+    return NV_OK;
+}
+
+// Record the octree step variants (both x / xn parities; with and without
+// the reconnection phase when recon_shots > 1) on the plan of the last
+// build.  NV_ERR_STATE (no error text) when there has been no build yet.
+int record_tree_steps(Ctx *c) {
+    for (auto &row : c->sg)
+        for (auto &g : row) drop_graph(g);
+    const int R = c->rows;
+    if (int rc = nv_tree_graph_prepare(c->tree, c->st, R)) {
+        if (rc == NV_ERR_STATE || rc == NV_ERR_UNSUPPORTED) return rc;
+        return fail(c, rc, "recorded tree step: buffers");
+    }
+    HIPCHK(c, c->d_S.reserve(1));
+    HIPCHK(c, c->d_poff.reserve(1));
+    HIPCHK(c, c->queue.reserve(1));
+    HIPCHK(c, c->part.reserve((size_t)c->sh.n * 3 * R));
+    DState d0 = c->d;
+    if (d0.x != c->xa) std::swap(d0.x, d0.xn);
+    for (int par = 0; par < 2; ++par) {
+        DState d = d0;
+        if (par) std::swap(d.x, d.xn);
+        for (int recon = 0; recon < 2; ++recon) {
+            if (!recon && c->prm.recon_shots == 1) continue;
+            Ctx::StepGraph &g = c->sg[par][recon];
+            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_tree_step(c, d, recon != 0);
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(c->st, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ee != hipSuccess)
+                return fail(c, NV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+            g.g = graph;
+            HIPCHK(c, hipGraphInstantiate(&g.e, graph, nullptr, nullptr, 0));
+            g.gen = c->layout_gen;
+            g.tgen = nv_tree_graph_gen(c->tree);
+            g.sig = tree_step_sig(c);
+        }
+    }
+    return NV_OK;
+}
+
+// nv_step on the octree: a recorded step replayed per step with one
+// synchronisation (status, pcount, the tree's level table and counters).
+// The first step, a step whose build outgrew the recorded plan (the replay
+// then did nothing past the build) and any step that cannot be recorded run
+// tree_step; the plan is recorded again from the sizes of the latest build.
+int step_tree_recorded(Ctx *c, int nsteps) {
+    hipStream_t st = c->st;
+    for (int k = 0; k < nsteps; ++k) {
+        if (int rc = ensure_room(c)) return rc;
+        if (c->d.x != c->xa && c->d.xn != c->xa) return fail(c, NV_ERR_STATE, "position buffers lost");
+        const int par = c->d.x == c->xa ? 0 : 1;
+        const bool recon = (c->itime + 1) % c->prm.recon_shots == 0;
+        Ctx::StepGraph &g = c->sg[par][recon ? 1 : 0];
+        if (g.gen != c->layout_gen || g.tgen != nv_tree_graph_gen(c->tree) || g.sig != tree_step_sig(c) || !g.e) {
+            const int rc = record_tree_steps(c);
+            if (rc == NV_ERR_STATE || rc == NV_ERR_UNSUPPORTED) {
+                if (int rs = tree_step(c)) return rs;
+                continue;
+            }
+            if (rc) return rc;
+        }
+        HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
+        HIPCHK(c, hipGraphLaunch(g.e, st));
+        if (int rc = nv_tree_graph_take_async(c->tree, st)) return fail(c, rc, "tree counters");
+        const int rc = read_ctl(c);
+        if (c->ctl_h->abort) {
+            // the build outgrew the plan and the replay stopped after it: the
+            // state is the step's start, taken again eagerly
+            HIPCHK(c, nv_fill_int(st, &c->ctl->abort, 0, 1));
+            ++c->stats.tree_redone;
+            for (auto &row : c->sg)
+                for (auto &gg : row) drop_graph(gg);
+            if (int rs = tree_step(c)) return rs;
+            continue;
+        }
+        c->itime += 1;
+        c->t = c->t + c->prm.dt;
+        swap_x(c);
+        c->tree_valid = false;
+        c->stats.point_updates += c->ctl_h->nvel;
+        ++c->stats.tree_replays;
+        if (nv_tree_graph_take(c->tree, &c->stats))
+            for (auto &row : c->sg)
+                for (auto &gg : row) drop_graph(gg);
+        if (rc) return rc;
     }
     return NV_OK;
 }
@@ -1288,7 +1430,11 @@ int nv_step(void *h, int32_t nsteps) {
         nsteps = 0;
     }
     if (use_tree) {
-        for (int k = 0; k < nsteps && rc == NV_OK; ++k) rc = tree_step(c);
+        const bool recorded = c->prm.step_launch == 0 && c->prm.velocity == NV_VEL_TREE &&
+                              c->prm.close_search != NV_CLOSE_BRUTE && c->K == 1;
+        if (recorded) rc = step_tree_recorded(c, nsteps);
+        else
+            for (int k = 0; k < nsteps && rc == NV_OK; ++k) rc = tree_step(c);
         nsteps = 0;
     }
     for (int k = 0; k < nsteps && rc == NV_OK; ++k) {

@@ -28,7 +28,7 @@ struct DCtl {
     int n_old;         // pinsert: pcount when it started (summed over tangles)
     double total_length;   // pinsert: sum of |x - ghosti| over the points it started with
     int empty_after;   // pinsert: empty slots below pcount when it ended (summed over tangles)
-    int pad2;
+    int abort;         // a recorded octree step found its build incomplete: its later phases do nothing
 };
 
 // src/diagnostic.f90:9-18 calculate_diagnostics, reduced on the device

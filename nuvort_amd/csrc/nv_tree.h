@@ -25,3 +25,22 @@ void nv_tree_walk_counters_take(const TreeBuf *t, nv_stats *stats);
 int nv_tree_walk_close(TreeBuf *t, hipStream_t st, DState d, double bsize, int n, nv_stats *stats);
 // every node, depth-first, empty ones included
 int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *n);
+
+// Recorded octree steps (launches only between prepare and take): the build,
+// velocity walk and close search on device-held counts (*dn rows of the row
+// range prepared; the node ranges in the build's level table), within bounds
+// planned from the last build's level sizes.  A build that outgrows them sets
+// ctl->abort, which switches off every later phase of the step.
+// prepare: plan and grow every buffer (outside capture); NV_ERR_STATE before
+// a first build
+int nv_tree_graph_prepare(TreeBuf *t, hipStream_t st, int rows);
+int nv_tree_graph_build(TreeBuf *t, hipStream_t st, DState d, Phys p, const int *dn, DCtl *ctl);
+int nv_tree_graph_walk(TreeBuf *t, hipStream_t st, DState d, Phys p, const Shifts &sh, int precision, double rn2,
+                       double *part, DCtl *ctl, const int *dn);
+int nv_tree_graph_close(TreeBuf *t, hipStream_t st, DState d, double bsize, const int *dn, DCtl *ctl);
+// changes whenever a buffer the recorded launches hold moves
+int nv_tree_graph_gen(const TreeBuf *t);
+// after a replay: queue the level table and walk counters to pinned memory,
+// then (the stream past them) take them; true when the plan should be redone
+int nv_tree_graph_take_async(TreeBuf *t, hipStream_t st);
+bool nv_tree_graph_take(TreeBuf *t, nv_stats *stats);

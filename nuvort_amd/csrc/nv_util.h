@@ -15,6 +15,9 @@ hipError_t nv_scan_tiles_sums(hipStream_t st, int *data, int n, int *sums);
 int nv_radix_hist_ints(int n);
 hipError_t nv_radix_sort_pairs(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n,
                                int bits, int *hist, int *scan_tmp);
+// the same for the first *dn <= n pairs, a count held on the device
+hipError_t nv_radix_sort_pairs_dn(hipStream_t st, unsigned long long *k0, int *v0, unsigned long long *k1, int *v1,
+                                  int n, const int *dn, int bits, int *hist, int *scan_tmp);
 
 // Wave votes on a lane condition. HIP's __ballot and __any take an int, so
 // a condition already held as a lane mask is first turned into 0/1 in a

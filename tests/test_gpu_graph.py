@@ -4,7 +4,8 @@ positions, links, slots and reconnection log, bit for bit, across both x /
 xn parities, steps with and without the reconnection phase
 (recon_shots > 1), a layout change in the middle of a run (insertions
 outgrowing the device rows), eager calls between recorded steps, and
-batches of tangles in FP32C."""
+batches of tangles in FP32C.  The octree step likewise, including a step
+whose build outgrows the recorded plan (taken again eagerly)."""
 import numpy as np
 import pytest
 
@@ -131,3 +132,53 @@ def test_recorded_lia_step():
     p = params(velocity="LIA")
     st = dense()
     assert_same(run(p, st, "eager", [5]), run(p, st, "graph", [5]))
+
+
+def tree_params(**kw):
+    return params(velocity="Tree", tree_theta=0.5, **kw)
+
+
+@pytest.mark.parametrize("shots", [1, 3])
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_recorded_tree_step_equals_eager(shots, prec):
+    p = tree_params(recon_shots=shots, hip_precision=prec)
+    st = dense()
+    ref = run(p, st, "eager", [7])
+    got = run(p, st, "graph", [7])
+    if shots == 1:
+        assert len(ref[1]) > 0
+    assert ref[2]["tree_replays"] == 0
+    # the first step has no build to plan from; the others replay
+    assert got[2]["tree_replays"] == 6 and got[2]["tree_redone"] == 0
+    assert got[2]["tree_builds"] == ref[2]["tree_builds"] == 7
+    assert_same(ref, got)
+
+
+def test_recorded_tree_step_split_calls_and_eager_calls():
+    def tree_velocity(s):
+        s.ghostp()
+        s.construct_tree()
+        s.velocity()
+    p = tree_params(recon_shots=2, hip_precision="fp32c")
+    st = dense()
+    assert_same(run(p, st, "eager", [7]), run(p, st, "graph", [2, 1, 3, 1]))
+    assert_same(run(p, st, "eager", [2, 2, 2], between=tree_velocity),
+                run(p, st, "graph", [2, 2, 2], between=tree_velocity))
+
+
+def test_recorded_tree_step_outgrowing_its_plan():
+    # rings with segments of 1.9 delta: step 1 inserts a point on every
+    # segment, so step 2's build (planned on step 1's tree) outgrows its
+    # level bounds; that replay stops after the build and the step is taken
+    # again eagerly, after which the plan is recorded anew
+    g = np.random.default_rng(11)
+    loops = [tangles.ring(g.uniform(-0.015, 0.015, 3), 0.01, g.normal(size=3), 5e-4, chord=1.9)
+             for _ in range(60)]
+    st = concat(dense(), tangles._assemble(loops, (0.03, 0.03, 0.03)))
+    p = tree_params(recon_shots=1, alpha=(0.206, 0.0083), normal_velocity="xflow", norm_vel_xflow=0.5)
+    cap = 4 * st.pcount + 4096
+    ref = run(p, st, "eager", [5], capacity=cap)
+    got = run(p, st, "graph", [5], capacity=cap)
+    assert ref[0][0].pcount > st.pcount + 3000
+    assert got[2]["tree_redone"] >= 1 and got[2]["tree_replays"] >= 2
+    assert_same(ref, got)
