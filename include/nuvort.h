@@ -221,9 +221,11 @@ int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.
 int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */
 
 /* nsteps iterations of the src/run.f90:24-65 loop body (diagnostics and
- * file output excluded), state kept on the device throughout.  On the
- * octree path a fatal condition raised by a phase after pinsert is returned
- * at the end of the step in which it arose. */
+ * file output excluded), state kept on the device throughout.  With
+ * step_launch 0 each step is one recorded HIP graph replayed with one host
+ * synchronisation (the octree step too, when its velocity is the tree walk
+ * and its close search the tree's); a fatal condition raised on the device
+ * is returned at the end of the step in which it arose. */
 int nv_step(void *ctx, int32_t nsteps);
 
 /* diagnostics, output and restart (src/diagnostic.f90, src/output.f90,

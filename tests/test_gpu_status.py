@@ -78,3 +78,28 @@ def test_zero_length_phase():
         s.pinsert()
     assert e.value.code == 4
     s.close()
+
+
+@pytest.mark.parametrize("launch", ["graph", "eager"])
+def test_noisey_xflow_without_its_table(launch):
+    """noisey_xflow reads its noise table at set-up (src/normal_fluid.f90:
+    25-60); a context given none refuses to step (NV_ERR_STATE), and steps
+    once the table is set."""
+    import numpy as np
+    from nuvort_amd import lib, tangles
+    from nuvort_amd.engine import FilamentSolver
+    from nuvort_amd.runfile import RunParams
+    p = RunParams(dt=5e-6, delta=5e-4, corea=1.3e-8, box_size=(0.1, 0.1, 0.1), velocity="BS",
+                  normal_velocity="noisey_xflow", norm_vel_xflow=0.5, alpha=(0.206, 0.0083),
+                  hip_step_launch=launch)
+    s = FilamentSolver(p, check=False)
+    try:
+        s.set_state(tangles.random_rings(5, 3, 0.01, 0.02, 0.1, 5e-4))
+        with pytest.raises(lib.NuvortError) as e:
+            s.step(1)
+        assert e.value.code == 10 and "noise" in str(e.value)
+        s.set_noise(tangles.noise_table(3))
+        s.step(1)
+        assert s.get_state().itime == 1
+    finally:
+        s.close()
