@@ -53,7 +53,7 @@ enum {
 #define NV_TREE_MAX_LEVELS 48
 
 enum { NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2 };
-enum { NV_BC_OPEN = 0, NV_BC_PERIODIC = 1 };
+enum { NV_BC_OPEN = 0, NV_BC_PERIODIC = 1, NV_BC_SOLID = 2 };
 /* normal_velocity models of src/normal_fluid.f90:83-118 that have a closed
  * form or use the noise table (the DNS fields laizetDNS / turbDNS need data
  * files the reference does not ship) */
@@ -106,7 +106,7 @@ typedef struct {
     int32_t itime;       /* the reference's itime */
     int32_t recon_count; /* the reference's recon_count */
     int32_t remove_count;/* the reference's remove_count */
-    int32_t pad;
+    int32_t wall_recon_count; /* the reference's wall_recon_count (src/cdata.f90:45) */
     double t;            /* the reference's t */
     double *x, *u, *u1, *u2, *u_sup, *u_mf, *ghosti, *ghostb; /* [capacity][3] */
     int32_t *infront, *behind, *closest;                      /* [capacity] */
@@ -129,6 +129,8 @@ typedef struct {
     double total_length;     /* pinsert: sum of |x - ghosti| over the points it started with */
     double avg_sep;          /* total_length / (pcount at its start - empty slots at its end) */
     int64_t tree_eval;       /* interactions of the last tree walk */
+    int32_t wall_recon_count;/* reconnections with solid walls (src/reconnection.f90:66-150) */
+    int32_t pad;
 } nv_diag;
 
 /* One reconnection, as src/reconnection.f90:46-49 logs it plus the pair
@@ -216,6 +218,7 @@ int nv_pclose(void *ctx);           /* src/line.f90:135-160 pclose */
 int nv_pclose_tree(void *ctx);      /* src/tree.f90:190-203 pclose_tree */
 int nv_precon(void *ctx);           /* src/reconnection.f90:11-61 precon */
 int nv_premove(void *ctx);          /* src/line.f90:102-130 premove */
+int nv_wall_recon(void *ctx);       /* src/reconnection.f90:66-150 wall_recon (src/run.f90:43) */
 int nv_enforce_boundary(void *ctx); /* src/boundary.f90:91-138 enforce_boundary */
 int nv_empty_tree(void *ctx);       /* src/tree.f90:160-186 empty_tree, src/run.f90:58-62 */
 int nv_nan_check(void *ctx);        /* src/general.f90:172-191 NAN_finder */

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnuvort_hip.so")
-SOURCES = ["k_util.hip", "k_velocity.hip", "k_topology.hip", "k_tree.hip", "nv_api.cpp", "nv_static.cpp", "nv_io.cpp"]
+SOURCES = ["k_util.hip", "k_velocity.hip", "k_topology.hip", "k_tree.hip", "k_walls.hip", "nv_api.cpp", "nv_static.cpp", "nv_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]

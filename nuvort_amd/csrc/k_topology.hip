@@ -21,8 +21,14 @@ namespace {
 constexpr int DIRTY_MAX = 8192;
 
 __device__ __forceinline__ void clear_row(const DState &s, int r) {
-    // src/general.f90:10-20 clear_particle (u_sup, u_mf untouched)
+    // src/general.f90:10-20 clear_particle (u_sup, u_mf untouched), the
+    // pinned flags included
     const int cap = s.cap;
+    if (s.pin) {
+        s.pin[r] = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s.wp[k * cap + r] = 0;
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         s.x[k * cap + r] = 0.0;
@@ -132,6 +138,20 @@ __global__ void k_ghosts(DState s, Phys p, int n) {
     store_ghosts(s, p, r);
 }
 
+// src/boundary.f90:124-135: a pinned point back onto its wall, the first
+// component of largest |wpinned| (maxloc)
+__device__ __forceinline__ void pin_to_wall(const DState &s, const Phys &p, int r) {
+    if (!p.walls || s.pin[r] == 0) return;
+    int w[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = s.wp[k * s.cap + r];
+    int pc = 0;
+    if (abs(w[1]) > abs(w[pc])) pc = 1;
+    if (abs(w[2]) > abs(w[pc])) pc = 2;
+    if (w[pc] > 0) s.x[pc * s.cap + r] = 0.499999999 * p.box[pc];
+    else if (w[pc] < 0) s.x[pc * s.cap + r] = -0.499999999 * p.box[pc];
+}
+
 __global__ void k_wrap(DState s, Phys p, int n) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n || s.infront[r] == 0) return;
@@ -144,6 +164,7 @@ __global__ void k_wrap(DState s, Phys p, int n) {
         else if (x < (-L / 2.0)) x = x + L;
         s.x[k * s.cap + r] = x;
     }
+    pin_to_wall(s, p, r);
 }
 
 __global__ void k_nan(DState s, int n, DCtl *ctl) {
@@ -1305,6 +1326,11 @@ __global__ void k_step_end(DState s, Phys p, int n, DCtl *ctl, double dt) {
             else if (x[k] < (-L / 2.0)) x[k] = x[k] + L;
             s.x[k * s.cap + r] = x[k];
         }
+        if (p.walls && s.pin[r]) {
+            pin_to_wall(s, p, r);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x[k] = s.x[k * s.cap + r];
+        }
     }
     if (isnan(x[0]) || isnan(x[1]) || isnan(x[2])) atomicOr(&ctl->status, NVK_NAN);
 }
@@ -1333,6 +1359,9 @@ __global__ void k_relayout(DState o, DState nu, const int *__restrict__ row2t, c
     nu.closestd[q] = o.closestd[r];
     nu.curv[q] = o.curv[r];
     nu.stretch[q] = o.stretch[r];
+    nu.pin[q] = o.pin[r];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nu.wp[c * nu.cap + q] = o.wp[c * o.cap + r];
     const int f = o.infront[r], b = o.behind[r], cl = o.closest[r];
     nu.infront[q] = f > 0 ? f + shift : 0;
     nu.behind[q] = b > 0 ? b + shift : 0;
@@ -1353,6 +1382,225 @@ __global__ void k_clock_begin(DCtl *ctl) { ctl->itime = ctl->itime + 1; }
 __global__ void k_clock_end(DCtl *ctl, double dt) {
     ctl->t = ctl->t + dt;
     ctl->nvel_total += (unsigned long long)ctl->nvel;
+}
+
+// ------------------------------------------------------------ solid walls
+// With solid walls the topology phases run as the reference's own loops, one
+// thread per tangle over every row (pinned points change the ghosts an
+// insertion refreshes and the loops' skips, which the staged kernels above
+// do not model).  Links are global 1-based rows.
+
+// src/reconnection.f90:155-182 loop_killer: the walk returns at a pinned point
+__device__ void loop_killer_pinned(const DState &s, int particle, int n) {
+    int counter = 1, next = particle;
+    for (int it = 1; it <= n; ++it) {
+        if (s.pin[next]) return;
+        next = s.infront[next] - 1;
+        if (next == particle) break;
+        if (next < 0) return;   // a broken chain (the reference would read f(0))
+        counter = counter + 1;
+    }
+    if (counter < 4) {
+        next = particle;
+        for (int it = 1; it <= n; ++it) {
+            const int store_next = s.infront[next] - 1;
+            clear_row(s, next);
+            next = store_next;
+            if (next == particle || next < 0) break;
+        }
+    }
+}
+
+// src/line.f90:18-96 pinsert
+__global__ void k_pinsert_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    const int tk = blockIdx.x, base = tg.base[tk], tcap = tg.cap[tk];
+    int n = tg.n[tk];
+    const int cap = s.cap;
+    int active = 0;
+    for (int r = base; r < base + n; ++r) active += s.infront[r] != 0;
+    if (active == 0) {
+        atomicOr(&ctl->status, NVK_ZERO_LENGTH);
+        return;
+    }
+    const int old = n;
+    double total = 0.0;
+    int fe = base;   // the first empty row (slots are only filled here)
+    for (int r = base; r < base + old; ++r) {
+        if (s.infront[r] == 0) continue;
+        if (s.pin[r] & NV_PINNED_I) continue;
+        double x[3], gi[3], gb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            x[k] = s.x[k * cap + r];
+            gi[k] = s.gi[k * cap + r];
+            gb[k] = s.gb[k * cap + r];
+        }
+        const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
+        total = total + disti;
+        if (!(disti > p.delta)) continue;
+        while (fe < base + n && s.infront[fe] != 0) ++fe;
+        int slot;
+        if (fe < base + n) {
+            slot = fe;
+        } else {
+            if (n >= tcap) {
+                atomicOr(&ctl->status, NVK_CAPACITY);
+                break;
+            }
+            slot = base + n;
+            ++n;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                s.usup[k * cap + slot] = 0.0;
+                s.umf[k * cap + slot] = 0.0;
+            }
+            clear_row(s, slot);
+        }
+        double fd[3], fdd[3], xn[3];
+        derivs(x, gi, gb, fd, fdd);
+        double curv = sqrt(fdd[0] * fdd[0] + fdd[1] * fdd[1] + fdd[2] * fdd[2]);
+        if (curv > 1e-5) {
+            curv = 1.0 / curv;
+            if (curv * curv - 0.25 * (disti * disti) > 0.0) {
+                const double rr = (sqrt(curv * curv - 0.25 * (disti * disti)) - curv) * curv;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]) + rr * fdd[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = 0.5 * (x[k] + gi[k]);
+        }
+        const int fi = s.infront[r] - 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s.x[k * cap + slot] = xn[k];
+            s.u[k * cap + slot] = 0.5 * (s.u[k * cap + r] + s.u[k * cap + fi]);
+            s.u1[k * cap + slot] = 0.0;
+            s.u2[k * cap + slot] = 0.0;
+            s.wp[k * cap + slot] = 0;
+        }
+        s.pin[slot] = 0;
+        s.behind[slot] = r + 1;
+        s.infront[slot] = fi + 1;
+        store_ghosts(s, p, slot);
+        s.behind[fi] = slot + 1;
+        store_ghosts(s, p, fi);
+        s.infront[r] = slot + 1;
+        store_ghosts(s, p, r);
+    }
+    int empties = 0;
+    for (int r = base; r < base + n; ++r) empties += s.infront[r] == 0;
+    atomicAdd(&ctl->total_length, total);
+    atomicAdd(&ctl->n_old, old);
+    atomicAdd(&ctl->empty_after, empties);
+    tg.n[tk] = n;
+}
+
+// src/reconnection.f90:11-61 precon (the tests of recon_tests)
+__global__ void k_precon_walls(DState s, Phys p, Tangles tg, DCtl *ctl, nv_recon_event *ev, int ev_cap) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    const int tk = blockIdx.x, base = tg.base[tk], n = tg.n[tk];
+    const double t = ctl->t;
+    const int itime = ctl->itime;
+    int nrec = 0;
+    for (int r = base; r < base + n; ++r) {
+        ReconIn in;
+        load_recon(s, r, in);
+        recon_tests(p, in);
+        if (!in.go) continue;
+        const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
+        ++nrec;
+        const int q = atomicAdd(&ctl->nevents, 1);
+        if (q < ev_cap) {
+            nv_recon_event e;
+            e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
+            e.angle = acos(in.dot);
+            e.dl = in.lb - in.la;
+            ev[q] = e;
+        }
+        s.infront[parjb - 1] = pari;
+        s.behind[pari - 1] = parjb;
+        s.infront[r] = j;
+        s.behind[rj] = r + 1;
+        loop_killer_pinned(s, pari - 1, n);
+        loop_killer_pinned(s, r, n);
+    }
+    if (nrec) tg.recon[tk] += nrec;
+}
+
+// src/reconnection.f90:66-150 wall_recon
+__global__ void k_wall_recon(DState s, Phys p, Tangles tg, DCtl *ctl) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    const int tk = blockIdx.x, base = tg.base[tk], n = tg.n[tk];
+    const int cap = s.cap;
+    int nwall = 0;
+    for (int r = base; r < base + n; ++r) {
+        if (s.infront[r] == 0) continue;
+        // a pinned pair with nothing between (:75-78)
+        if ((s.pin[r] & NV_PINNED_B) && (s.pin[s.infront[r] - 1] & NV_PINNED_I)) {
+            clear_row(s, s.infront[r] - 1);
+            clear_row(s, r);
+        }
+        if (s.pin[r]) continue;
+        for (int k = 0; k < 3; ++k) {
+            if (!p.solid[k]) continue;
+            const double xk = s.x[k * cap + r];
+            int side = 0;
+            if (xk > 0.5 * (p.box[k] - p.delta)) side = 1;
+            else if (xk < -0.5 * (p.box[k] - p.delta)) side = -1;
+            if (!side) continue;
+            const int pari = s.infront[r] - 1, parb = s.behind[r] - 1;
+            s.pin[pari] |= NV_PINNED_B;
+            s.behind[pari] = pari + 1;
+            s.pin[parb] |= NV_PINNED_I;
+            s.infront[parb] = parb + 1;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                s.wp[q * cap + pari] = q == k ? side : 0;
+                s.wp[q * cap + parb] = q == k ? side : 0;
+            }
+            clear_row(s, r);
+            if (s.pin[pari] == (NV_PINNED_I | NV_PINNED_B)) clear_row(s, pari);
+            if (s.pin[parb] == (NV_PINNED_I | NV_PINNED_B)) clear_row(s, parb);
+            ++nwall;
+        }
+    }
+    if (nwall) tg.wall[tk] += nwall;
+}
+
+// src/line.f90:102-130 premove
+__global__ void k_premove_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    const int tk = blockIdx.x, base = tg.base[tk], n = tg.n[tk];
+    const int cap = s.cap;
+    int nrm = 0;
+    for (int r = base; r < base + n; ++r) {
+        if (s.infront[r] == 0) continue;
+        if (s.pin[r] & NV_PINNED_I) continue;
+        const int inf = s.infront[r] - 1;
+        if (s.pin[inf] & NV_PINNED_I) continue;
+        const int ii = s.infront[inf] - 1;
+        const double distii = dist_gen(s.x[r], s.x[cap + r], s.x[2 * cap + r], s.x[ii], s.x[cap + ii], s.x[2 * cap + ii]);
+        if (distii < .99 * p.delta) {
+            s.behind[ii] = r + 1;
+            s.infront[r] = ii + 1;
+            clear_row(s, inf);
+            loop_killer_pinned(s, r, n);
+            ++nrm;
+        }
+        // two points on the boundary (:126-128)
+        if (s.infront[r] != 0 && (s.pin[r] & NV_PINNED_B) && (s.pin[s.infront[r] - 1] & NV_PINNED_I)) {
+            clear_row(s, s.infront[r] - 1);
+            clear_row(s, r);
+        }
+    }
+    if (nrm) tg.remove[tk] += nrm;
 }
 
 }  // namespace
@@ -1492,5 +1740,22 @@ hipError_t nvk_step_begin(hipStream_t st, DCtl *ctl, int *tnact, int K, int *que
 }
 hipError_t nvk_step_end(hipStream_t st, DState s, Phys p, int n, DCtl *ctl, double dt) {
     hipLaunchKernelGGL(k_step_end, dim3((std::max(n, 1) + 255) / 256), dim3(256), 0, st, s, p, n, ctl, dt);
+    return hipGetLastError();
+}
+
+hipError_t nvk_pinsert_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl) {
+    hipLaunchKernelGGL(k_pinsert_walls, dim3(tg.K), dim3(64), 0, st, s, p, tg, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_precon_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl, nv_recon_event *ev, int ev_cap) {
+    hipLaunchKernelGGL(k_precon_walls, dim3(tg.K), dim3(64), 0, st, s, p, tg, ctl, ev, ev_cap);
+    return hipGetLastError();
+}
+hipError_t nvk_wall_recon(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl) {
+    hipLaunchKernelGGL(k_wall_recon, dim3(tg.K), dim3(64), 0, st, s, p, tg, ctl);
+    return hipGetLastError();
+}
+hipError_t nvk_premove_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl) {
+    hipLaunchKernelGGL(k_premove_walls, dim3(tg.K), dim3(64), 0, st, s, p, tg, ctl);
     return hipGetLastError();
 }

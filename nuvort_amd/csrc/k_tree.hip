@@ -122,7 +122,9 @@ __global__ void k_keys(DState s, int nmax, const int *dn, double bsize, int nwor
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= dev_count(nmax, dn)) return;
     vals[r] = r;
-    if (s.infront[r] == 0) {
+    // empty points and points pinned in front are not part of the tree
+    // (src/tree.f90:126-128)
+    if (s.infront[r] == 0 || (s.pin[r] & NV_PINNED_I)) {
         keys[r] = ~0ull;
         if (nwords > 1) deep0[r] = ~0ull;
         if (nwords > 2) deep1[r] = ~0ull;
@@ -428,7 +430,11 @@ __device__ __forceinline__ double dist3(double a0, double a1, double a2, double 
     return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
 }
 
-__device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, const int *vals, int nact) {
+// psum (solid walls, else null): the positions and count of the points
+// pinned in front, which the root's centroid counts (its masked sum over
+// infront > 0, src/tree.f90:50-52) and no other node does
+__device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, const int *vals, int nact,
+                                      const double *psum) {
     const int cap = s.cap, nc = nd.ncap;
     double sm[3] = {0.0, 0.0, 0.0}, ci[3] = {0.0, 0.0, 0.0};
     int mr = 0x7fffffff, size;
@@ -493,8 +499,14 @@ __device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, c
     nd.size[q] = size;
     // empty nodes: no centroid (nothing reads it)
     if (nd.pcount[q] == 0) return;
-    const double cntd = q == 0 ? (double)nact : (double)nd.pcount[q];
-    const double c0 = sm[0] / cntd, c1 = sm[1] / cntd, c2 = sm[2] / cntd;
+    double cntd = q == 0 ? (double)nact : (double)nd.pcount[q];
+    double cs[3] = {sm[0], sm[1], sm[2]};
+    if (q == 0 && psum) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cs[k] = cs[k] + psum[k];
+        cntd = cntd + psum[3];
+    }
+    const double c0 = cs[0] / cntd, c1 = cs[1] / cntd, c2 = cs[2] / cntd;
     double rad = 0.0;
     int p0 = nd.beg[q];
     if (nd.first[q] >= 0) {
@@ -517,15 +529,39 @@ __device__ __forceinline__ void up_of(const Nodes &nd, const DState &s, int q, c
     nd.cent[2 * nc + q] = c2;
 }
 
-__global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals, int nact) {
+__global__ void k_up(Nodes nd, DState s, int L0, int L1, const int *vals, int nact, const double *psum) {
     const int q = L0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < L1) up_of(nd, s, q, vals, nact);
+    if (q < L1) up_of(nd, s, q, vals, nact, psum);
 }
 
-__global__ void k_up_d(Nodes nd, DState s, const LevelCtl *lc, int level, const int *vals) {
+__global__ void k_up_d(Nodes nd, DState s, const LevelCtl *lc, int level, const int *vals, const double *psum) {
     if (lc->overflow) return;
     const int q = lc->lvl[level] + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < lc->lvl[level + 1]) up_of(nd, s, q, vals, lc->nact);
+    if (q < lc->lvl[level + 1]) up_of(nd, s, q, vals, lc->nact, psum);
+}
+
+// the sum of the positions of the points pinned in front and their count
+// (rows [0, n), or [0, *dn)), one workgroup, a fixed reduction order
+__global__ __launch_bounds__(256) void k_pinned_sum(DState s, int nmax, const int *dn, double *psum) {
+    const int n = dev_count(nmax, dn);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = threadIdx.x; r < n; r += 256)
+        if (s.infront[r] != 0 && (s.pin[r] & NV_PINNED_I)) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] += s.x[k * s.cap + r];
+            v[3] += 1.0;
+        }
+    __shared__ double red[4][256];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) psum[threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void k_build_tail(const Nodes nd, LevelCtl *lc) { lc->nwalk = lc->overflow ? 0 : nd.size[0]; }
@@ -1282,6 +1318,121 @@ __global__ __launch_bounds__(BT) void k_walk_close(DState s, int n, const int *_
     }
 }
 
+// ------------------------------------------------------------ solid walls
+// src/tree.f90:321-392 tree_walk_mirror for target row r, summed over the
+// reflections in order (src/timestep.f90:116-120) into mir[3][npad]: each
+// lane walks the depth-first records on its own (wall runs are small and
+// the mirrored walks short).  A node is accepted when it is a leaf or
+// width / dist < theta, dist being to the reflected end of the node's
+// pseudo-segment, centroid + circ (the extra correction's theta is
+// overwritten there, :356); a leaf whose segment goes into a wall, and a
+// pinned target's own image, are skipped (:360-372).
+__global__ __launch_bounds__(64) void k_walk_mirror(DState s, Phys p, int n, const WNode *__restrict__ walk, int nwalk,
+                                                    double *__restrict__ mir, int npad, DCtl *ctl,
+                                                    unsigned long long *counters, const int *dn, const int *dnw) {
+    if (ctl->abort) return;
+    if (dn) n = dev_count(npad, dn);
+    if (dnw) nwalk = *dnw;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= npad) return;
+    double u[3] = {0.0, 0.0, 0.0};
+    unsigned long long nev = 0;
+    bool sing = false;
+    if (r < n && s.infront[r] != 0) {
+        const int cap = s.cap;
+        const double x0 = s.x[r], x1 = s.x[cap + r], x2 = s.x[2 * cap + r];
+        const int pin = s.pin[r], beh = s.behind[r];
+        const int w0 = s.wp[r], w1 = s.wp[cap + r], w2 = s.wp[2 * cap + r];
+        for (int m = 0; m < p.nmir; ++m) {
+            const int rf[3] = {p.mir[m][0], p.mir[m][1], p.mir[m][2]};
+            const bool own = ((w0 - rf[0]) + (w1 - rf[1]) + (w2 - rf[2])) == 0 && (w0 + w1 + w2) < 2;
+            int cur = 0;
+            while (cur < nwalk) {
+                const WNode nd = walk[cur];
+                double sim[3], sgi[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    sgi[k] = reflect1(nd.cent[k], rf[k], p.box[k]);
+                    sim[k] = reflect1(nd.cent[k] + nd.circ[k], rf[k], p.box[k]);
+                }
+                const double v0 = sim[0] - x0, v1 = sim[1] - x1, v2 = sim[2] - x2;
+                const double e0 = x0 - sim[0], e1 = x1 - sim[1], e2 = x2 - sim[2];
+                const double dd = e0 * e0 + e1 * e1 + e2 * e2;
+                const double dist = sqrt(dd);
+                const double theta = nd.width / dist;
+                if (!(nd.pcount == 1 || theta < p.tree_theta)) {
+                    ++cur;   // open: the first child follows
+                    continue;
+                }
+                cur = nd.skip;
+                if (nd.pcount == 1) {
+                    const int j = s.behind[nd.leaf_infront - 1];
+                    if (s.pin[j - 1] & NV_PINNED_I) continue;
+                    if (own) {
+                        if ((pin & NV_PINNED_B) && j == r + 1) continue;
+                        if ((pin & NV_PINNED_I) && j == beh) continue;
+                    }
+                    if (dist < NV_EPS) {
+                        sing = true;
+                        continue;
+                    }
+                }
+                const double c[3] = {sgi[0] - sim[0], sgi[1] - sim[1], sgi[2] - sim[2]};
+                tree_term64(v0, v1, v2, c, dd, p.kappa, u, nev);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mir[(size_t)k * npad + r] = u[k];
+    if (sing) atomicOr(&ctl->status, NVK_SINGULAR);
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(&counters[0], nev);
+}
+
+// src/tree.f90:190-244 pclose_tree / closest_tree with pinned points, each
+// lane walking the stale tree's depth-first records on its own: every slot
+// gets closestd = 100, a pinned target no search, a pinned leaf is no
+// candidate (:226-234).
+__global__ __launch_bounds__(64) void k_close_walls(DState s, int n, const WNode *__restrict__ walk, int nwalk,
+                                                    const DCtl *ctl, const int *dn, const int *dnw, int npad) {
+    if (ctl && ctl->abort) return;
+    if (dn) n = dev_count(npad, dn);
+    if (dnw) nwalk = *dnw;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    s.closestd[r] = 100.0;
+    if (s.infront[r] == 0 || s.pin[r]) return;
+    const int cap = s.cap;
+    const double x0 = s.x[r], x1 = s.x[cap + r], x2 = s.x[2 * cap + r];
+    const int inf = s.infront[r], beh = s.behind[r];
+    double best = 100.0;
+    int cl = s.closest[r];
+    int cur = 0;
+    while (cur < nwalk) {
+        const WNode nd = walk[cur];
+        const double e0 = x0 - nd.cent[0], e1 = x1 - nd.cent[1], e2 = x2 - nd.cent[2];
+        const double dist = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+        const double theta = nd.width / dist;
+        if (!(nd.pcount == 1 || theta < 0.5)) {
+            ++cur;
+            continue;
+        }
+        cur = nd.skip;
+        if (!(dist < best)) continue;
+        if (nd.pcount > 1) {
+            cl = 0;
+            continue;
+        }
+        const int j = s.behind[nd.leaf_infront - 1];
+        if (j == r + 1 || inf == j || beh == j) continue;
+        if (s.pin[j - 1]) continue;
+        cl = j;
+        best = dist;
+    }
+    s.closest[r] = cl;
+    s.closestd[r] = best;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host
@@ -1319,6 +1470,8 @@ struct TreeBuf {
     LevelCtl *lc = nullptr, *lc_h = nullptr;   // level loop control, device and pinned copy
     int sync_builds = 0;                 // builds that ran the synchronous level loop
     long gen = 0;                        // bumped whenever a buffer above moves (recorded steps hold them)
+    double *psum = nullptr;              // solid walls: the pinned-in-front points' position sum and count
+    bool walls = false;                  // the builds count them in the root's centroid
     // the plan of the recorded steps' builds: levels 0 .. g_top launched on
     // these bounds, over g_rows rows
     int g_top = -1, g_rows = 0;
@@ -1336,7 +1489,7 @@ TreeBuf *nv_tree_new() {
     if (hipMalloc(&t->nact, sizeof(int)) != hipSuccess || hipMalloc(&t->counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&t->wq, XCDS * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void **)&t->h_int, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&t->lc, sizeof(LevelCtl)) != hipSuccess ||
+        hipMalloc(&t->lc, sizeof(LevelCtl)) != hipSuccess || hipMalloc(&t->psum, 4 * sizeof(double)) != hipSuccess ||
         hipHostMalloc((void **)&t->lc_h, sizeof(LevelCtl), hipHostMallocDefault) != hipSuccess) {
         delete t;
         return nullptr;
@@ -1367,6 +1520,7 @@ void nv_tree_free(TreeBuf *t) {
     (void)hipFree(t->wq);
     (void)hipHostFree(t->h_int);
     (void)hipFree(t->lc);
+    (void)hipFree(t->psum);
     for (auto *q : t->raw) (void)hipFree(q);
     for (auto *q : t->srt) (void)hipFree(q);
     (void)hipHostFree(t->lc_h);
@@ -1520,7 +1674,8 @@ static int level_loop_sync(TreeBuf *t, hipStream_t st, DState d, DCtl *ctl, int 
     for (int lv = nlev - 1; lv >= 0; --lv) {
         const int L0 = t->lvl[lv], L1 = t->lvl[lv + 1];
         if (L1 > L0)
-            hipLaunchKernelGGL(k_up, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->vals, nact);
+            hipLaunchKernelGGL(k_up, dim3((L1 - L0 + 255) / 256), dim3(256), 0, st, t->nd, d, L0, L1, t->vals, nact,
+                               (const double *)(t->walls ? t->psum : nullptr));
         TCHK(hipGetLastError());
     }
     TCHK(hipMemcpyAsync(t->h_int, t->nd.size, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1587,7 +1742,7 @@ static int enqueue_levels(TreeBuf *t, hipStream_t st, DState d, int n, const int
     }
     for (int l = top; l >= 0; --l) {
         hipLaunchKernelGGL(k_up_d, dim3(bound[l] / 256), dim3(256), 0, st, t->nd, d, (const LevelCtl *)t->lc, l,
-                           t->vals);
+                           t->vals, (const double *)(t->walls ? t->psum : nullptr));
         TCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_build_tail, dim3(1), dim3(1), 0, st, t->nd, t->lc);
@@ -1642,6 +1797,8 @@ int nv_tree_build(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, DCtl *ctl
         t->maxlev = KEY_LEVELS * t->kwords;
         TCHK(nv_fill_int(st, t->nact, 0, 1));
         if ((rc = sort_paths(t, st, d, n, nullptr, bsize))) return rc;
+        t->walls = p.walls != 0;
+        if (t->walls) hipLaunchKernelGGL(k_pinned_sum, dim3(1), dim3(256), 0, st, d, n, (const int *)nullptr, t->psum);
         hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, (const int *)nullptr,
                            t->nact);
         TCHK(hipGetLastError());
@@ -1912,6 +2069,8 @@ int nv_tree_graph_build(TreeBuf *t, hipStream_t st, DState d, Phys p, const int 
     const double bsize = std::max(p.box[0], std::max(p.box[1], p.box[2]));
     TCHK(nv_fill_int(st, t->nact, 0, 1));
     if (int rc = sort_paths(t, st, d, n, dn, bsize)) return rc;
+    t->walls = p.walls != 0;
+    if (t->walls) hipLaunchKernelGGL(k_pinned_sum, dim3(1), dim3(256), 0, st, d, n, dn, t->psum);
     hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, st, t->k0, n, dn, t->nact);
     TCHK(hipGetLastError());
     t->vals = t->v0;
@@ -2010,6 +2169,8 @@ int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *no
     TCHK(hipMemcpyAsync(pos.data(), t->nd.pos, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
     TCHK(hipMemcpyAsync(sum.data(), t->nd.sum, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
     TCHK(hipMemcpyAsync(circ.data(), t->nd.circ, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost, st));
+    double ps[4] = {0.0, 0.0, 0.0, 0.0};   // the root also counts the points pinned in front
+    if (t->walls) TCHK(hipMemcpyAsync(ps, t->psum, sizeof(ps), hipMemcpyDeviceToHost, st));
     TCHK(hipStreamSynchronize(st));
     int o = 0;
     std::vector<int> stack;
@@ -2018,10 +2179,10 @@ int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *no
         int q = stack.back();
         stack.pop_back();
         nv_tree_node &r = buf[o++];
-        const double cntd = q == 0 ? (double)t->nact_h : (double)pc[q];
+        const double cntd = q == 0 ? (double)t->nact_h + ps[3] : (double)pc[q];
         for (int k = 0; k < 3; ++k) {
             r.pos[k] = pos[(size_t)k * nc + q];
-            r.cent[k] = pc[q] > 0 ? sum[(size_t)k * nc + q] / cntd : 0.0;
+            r.cent[k] = pc[q] > 0 ? (sum[(size_t)k * nc + q] + (q == 0 ? ps[k] : 0.0)) / cntd : 0.0;
             r.circ[k] = q == 0 ? 0.0 : circ[(size_t)k * nc + q];
         }
         r.width = w[q];
@@ -2030,5 +2191,38 @@ int nv_tree_dump(TreeBuf *t, hipStream_t st, nv_tree_node *buf, int cap, int *no
         if (first[q] >= 0)
             for (int c = 7; c >= 0; --c) stack.push_back(first[q] + c);
     }
+    return NV_OK;
+}
+
+// ------------------------------------------------------ solid walls (host)
+int nv_tree_walk_mirror(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, int npad, double *mir, DCtl *ctl) {
+    if (npad <= 0) return NV_OK;
+    hipLaunchKernelGGL(k_walk_mirror, dim3((npad + 63) / 64), dim3(64), 0, st, d, p, n, (const WNode *)t->walk,
+                       t->nwalk, mir, npad, ctl, t->counters, (const int *)nullptr, (const int *)nullptr);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_close_walls(TreeBuf *t, hipStream_t st, DState d, int n) {
+    if (n <= 0) return NV_OK;
+    hipLaunchKernelGGL(k_close_walls, dim3((n + 63) / 64), dim3(64), 0, st, d, n, (const WNode *)t->walk, t->nwalk,
+                       (const DCtl *)nullptr, (const int *)nullptr, (const int *)nullptr, n);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_graph_mirror(TreeBuf *t, hipStream_t st, DState d, Phys p, double *mir, DCtl *ctl, const int *dn) {
+    const int npad = t->g_rows;
+    hipLaunchKernelGGL(k_walk_mirror, dim3((npad + 63) / 64), dim3(64), 0, st, d, p, npad, (const WNode *)t->walk, 0,
+                       mir, npad, ctl, t->counters, dn, (const int *)&t->lc->nwalk);
+    TCHK(hipGetLastError());
+    return NV_OK;
+}
+
+int nv_tree_graph_close_walls(TreeBuf *t, hipStream_t st, DState d, const int *dn, DCtl *ctl) {
+    const int npad = t->g_rows;
+    hipLaunchKernelGGL(k_close_walls, dim3((npad + 63) / 64), dim3(64), 0, st, d, npad, (const WNode *)t->walk, 0,
+                       (const DCtl *)ctl, dn, (const int *)&t->lc->nwalk, npad);
+    TCHK(hipGetLastError());
     return NV_OK;
 }

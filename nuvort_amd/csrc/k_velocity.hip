@@ -24,13 +24,19 @@ constexpr int REC64 = 8;   // x0 x1 x2 e0 e1 e2 c valid
 constexpr int TB = 256;    // lanes per workgroup
 constexpr int TS = 256;    // sources per tile
 
+// a row's segment is a source of the direct sums: not empty, and not going
+// into a wall (pinned in front, src/timestep.f90:161,187)
+__device__ __forceinline__ bool is_source(const DState &s, int r) {
+    return s.infront[r] != 0 && !(s.pin && (s.pin[r] & NV_PINNED_I));
+}
+
 // FP64 source records, one per row: x, e = ghosti - x, c = |e|^2
-// (dist_gen_sq(ghosti, x), src/timestep.f90:164), valid = infront != 0.
+// (dist_gen_sq(ghosti, x), src/timestep.f90:164), valid = is_source.
 __global__ void k_src64(DState s, int rows, double *__restrict__ rec) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     double v[REC64] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (s.infront[r] != 0) {
+    if (is_source(s, r)) {
         double e[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
     __shared__ int first;
     const int q0 = blockIdx.x * TS + threadIdx.x;
     const int r = q0 < rows ? perm[q0] : 0;
-    const bool act = q0 < rows && s.infront[r] != 0;
+    const bool act = q0 < rows && is_source(s, r);
     // origin: the row of the tile's first active record
     if (threadIdx.x == 0) first = 0x7fffffff;
     __syncthreads();
@@ -516,7 +522,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
 // mode 4: the FP32 walk's, image 0 as in mode 3 and images 1.. as floats
 // ([image - 1][3][rows] after image 0's block).
 __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, const Tangles &tg, int rows,
-                                            const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
+                                            const double *__restrict__ part, int mode, int do_ab, DCtl *ctl,
+                                            const double *__restrict__ mir) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = r < rows && s.infront[r] != 0;
     // active points counted per workgroup, one atomic each: atomics on one
@@ -588,14 +595,21 @@ __device__ __forceinline__ void finish_body(const DState &s, const Phys &p, cons
             for (int k = 0; k < 3; ++k) ub0[k] = nr[k] + far[k];
         }
     }
+    // the image vortices' sum of solid walls (src/timestep.f90:94-98), zero
+    // without walls: the reference adds u_mir either way
+    double ubm[3] = {0.0, 0.0, 0.0};
+    if (mir) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ubm[k] = mir[(size_t)k * rows + r];
+    }
     const double noise = (p.nf == NV_NF_NOISEY_XFLOW && p.noise_n > 0) ? p.noise[ctl->noise_ctr] : 0.0;
-    finish_point(s, p, r, ub0, ubs, ctl->t, noise, do_ab);
+    finish_point(s, p, r, ub0, ubs, ubm, ctl->t, noise, do_ab);
 }
 
 __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part,
-                                                int mode, int do_ab, DCtl *ctl) {
+                                                int mode, int do_ab, DCtl *ctl, const double *__restrict__ mir) {
     if (ctl->abort) return;
-    finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
+    finish_body(s, p, tg, rows, part, mode, do_ab, ctl, mir);
 }
 
 // The tree walks' layouts (modes 3 and 4, 27 chunks of 3 values) at eight waves per
@@ -604,9 +618,10 @@ __global__ __launch_bounds__(256) void k_finish(DState s, Phys p, Tangles tg, in
 // 1 M points against the compiler's 110 VGPRs and 4 waves). The chunked
 // direct-sum layouts (hundreds of chunks per point) are slower so.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_finish_tree(
-    DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, int do_ab, DCtl *ctl) {
+    DState s, Phys p, Tangles tg, int rows, const double *__restrict__ part, int mode, int do_ab, DCtl *ctl,
+    const double *__restrict__ mir) {
     if (ctl->abort) return;   // (a recorded octree step whose build was incomplete)
-    finish_body(s, p, tg, rows, part, mode, do_ab, ctl);
+    finish_body(s, p, tg, rows, part, mode, do_ab, ctl, mir);
 }
 
 // src/normal_fluid.f90:340-347 randomise_normal_fluid: the noise counter
@@ -745,14 +760,14 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
 }
 
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
-                      int do_ab, DCtl *ctl) {
+                      int do_ab, DCtl *ctl, const double *mir) {
     if (rows > 0) {
         if (mode == 3 || mode == 4)
             hipLaunchKernelGGL(k_finish_tree, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
-                               do_ab, ctl);
+                               do_ab, ctl, mir);
         else
             hipLaunchKernelGGL(k_finish, dim3((rows + 255) / 256), dim3(256), 0, st, s, p, tg, rows, part, mode,
-                               do_ab, ctl);
+                               do_ab, ctl, mir);
     }
     // the end of pmotion advances the noise table (src/timestep.f90:44)
     if (do_ab && p.nf == NV_NF_NOISEY_XFLOW && p.noise_n > 0)

@@ -51,13 +51,13 @@ struct Ctx {
     int ncu = 256;
     // tangle layout: host mirrors and device copies
     int K = 0, rows = 0;             // rows = sum of tangle capacities = the state arrays' stride
-    std::vector<int> tbase, tcap, tn, trecon, tremove;
+    std::vector<int> tbase, tcap, tn, trecon, tremove, twall;
     DevBuf<int> d_tbase, d_tcap, d_row2t;
-    // per-tangle counters, one device block [pcount | recon | remove | nact]
+    // per-tangle counters, one device block [pcount | recon | remove | wall | nact]
     // so that a step's host read is one copy into pinned memory
     DevBuf<int> d_tcnt;
-    int *p_tn = nullptr, *p_trecon = nullptr, *p_tremove = nullptr, *p_tnact = nullptr;
-    int *h_tcnt = nullptr;           // pinned mirror of the first three
+    int *p_tn = nullptr, *p_trecon = nullptr, *p_tremove = nullptr, *p_twall = nullptr, *p_tnact = nullptr;
+    int *h_tcnt = nullptr;           // pinned mirror of the first four
     int h_tcnt_cap = 0;
     DState d{};
     DCtl *ctl = nullptr;             // device
@@ -68,6 +68,7 @@ struct Ctx {
     DevBuf<int> pj;
     DevBuf<double> noise;            // noisey_xflow table (nv_set_noise)
     DevBuf<double> diag;             // nv_diagnostics' partial reductions and result
+    DevBuf<double> mir;              // solid walls: the image vortices' velocity, [3][rows]
     // work lists of the last direct-sum and close-search launches
     DevBuf<Item> items, pitems;
     DevBuf<int> d_S, d_pS, queue;
@@ -129,6 +130,7 @@ Tangles tangles(Ctx *c, const int *S, const long long *poff) {
     g.n = c->p_tn;
     g.recon = c->p_trecon;
     g.remove = c->p_tremove;
+    g.wall = c->p_twall;
     g.nact = c->p_tnact;
     g.row2t = c->d_row2t.p;
     g.S = S;
@@ -152,7 +154,7 @@ void free_dstate(DState &d) {
     double **dp[] = {&d.x, &d.xn, &d.u, &d.u1, &d.u2, &d.usup, &d.umf, &d.gi, &d.gb, &d.closestd, &d.curv, &d.stretch};
     for (double **p : dp)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
-    int **ip[] = {&d.infront, &d.behind, &d.closest};
+    int **ip[] = {&d.infront, &d.behind, &d.closest, &d.pin, &d.wp};
     for (int **p : ip)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
@@ -174,11 +176,12 @@ hipError_t alloc_dstate(hipStream_t st, int rows, DState *out) {
         if (e != hipSuccess) break;
         if ((e = hipMalloc(p, sizeof(double) * rows)) == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(double) * rows, st);
     }
-    int **ip[] = {&d.infront, &d.behind, &d.closest};
+    int **ip[] = {&d.infront, &d.behind, &d.closest, &d.pin, &d.wp};
     for (int **p : ip) {
         if (e != hipSuccess) break;
-        if ((e = hipMalloc(p, sizeof(int) * (size_t)rows)) != hipSuccess) break;
-        e = hipMemsetAsync(*p, 0, sizeof(int) * (size_t)rows, st);
+        const size_t m = (p == &d.wp ? 3 : 1) * (size_t)rows;
+        if ((e = hipMalloc(p, sizeof(int) * m)) != hipSuccess) break;
+        e = hipMemsetAsync(*p, 0, sizeof(int) * m, st);
     }
     if (e != hipSuccess) {
         free_dstate(d);
@@ -207,6 +210,7 @@ int alloc_scratch(Ctx *c) {
     HIPCHK(c, c->iperm.reserve(rows));
     HIPCHK(c, c->ohist.reserve((size_t)nv_radix_hist_ints(rows)));
     HIPCHK(c, c->oscan.reserve((size_t)nv_scan_tmp_ints(nv_radix_hist_ints(rows))));
+    if (c->phys.walls) HIPCHK(c, c->mir.reserve((size_t)3 * rows));
     return NV_OK;
 }
 
@@ -229,17 +233,18 @@ int upload_tables(Ctx *c) {
     const int K = c->K;
     HIPCHK(c, c->d_tbase.reserve(K));
     HIPCHK(c, c->d_tcap.reserve(K));
-    HIPCHK(c, c->d_tcnt.reserve((size_t)4 * K));
+    HIPCHK(c, c->d_tcnt.reserve((size_t)5 * K));
     c->p_tn = c->d_tcnt.p;
     c->p_trecon = c->d_tcnt.p + K;
     c->p_tremove = c->d_tcnt.p + 2 * K;
-    c->p_tnact = c->d_tcnt.p + 3 * K;
-    if (c->h_tcnt_cap < 3 * K) {
+    c->p_twall = c->d_tcnt.p + 3 * K;
+    c->p_tnact = c->d_tcnt.p + 4 * K;
+    if (c->h_tcnt_cap < 4 * K) {
         if (c->h_tcnt) (void)hipHostFree(c->h_tcnt);
         c->h_tcnt = nullptr;
         c->h_tcnt_cap = 0;
-        HIPCHK(c, hipHostMalloc((void **)&c->h_tcnt, sizeof(int) * 3 * (size_t)K, hipHostMallocDefault));
-        c->h_tcnt_cap = 3 * K;
+        HIPCHK(c, hipHostMalloc((void **)&c->h_tcnt, sizeof(int) * 4 * (size_t)K, hipHostMallocDefault));
+        c->h_tcnt_cap = 4 * K;
     }
     HIPCHK(c, c->d_row2t.reserve(c->rows));
     HIPCHK(c, hipMemcpy(c->d_tbase.p, c->tbase.data(), sizeof(int) * K, hipMemcpyHostToDevice));
@@ -247,6 +252,7 @@ int upload_tables(Ctx *c) {
     HIPCHK(c, hipMemcpy(c->p_tn, c->tn.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->p_trecon, c->trecon.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->p_tremove, c->tremove.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->p_twall, c->twall.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     std::vector<int> r2t(c->rows);
     for (int k = 0; k < K; ++k) std::fill(r2t.begin() + c->tbase[k], r2t.begin() + c->tbase[k] + c->tcap[k], k);
     HIPCHK(c, hipMemcpy(c->d_row2t.p, r2t.data(), sizeof(int) * c->rows, hipMemcpyHostToDevice));
@@ -257,12 +263,13 @@ int read_ctl(Ctx *c) {
     HIPCHK(c, hipMemcpyAsync(c->ctl_h, c->ctl, sizeof(DCtl), hipMemcpyDeviceToHost, c->st));
     const int K = c->K;
     if (K > 0)
-        HIPCHK(c, hipMemcpyAsync(c->h_tcnt, c->d_tcnt.p, sizeof(int) * 3 * (size_t)K, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->h_tcnt, c->d_tcnt.p, sizeof(int) * 4 * (size_t)K, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (K > 0) {
         std::copy(c->h_tcnt, c->h_tcnt + K, c->tn.begin());
         std::copy(c->h_tcnt + K, c->h_tcnt + 2 * K, c->trecon.begin());
         std::copy(c->h_tcnt + 2 * K, c->h_tcnt + 3 * K, c->tremove.begin());
+        std::copy(c->h_tcnt + 3 * K, c->h_tcnt + 4 * K, c->twall.begin());
     }
     int s = c->ctl_h->status;
     if (s & NVK_NAN) return fail(c, NV_ERR_NAN, "run.x: there is a NAN in the (filament) f%x array");
@@ -284,15 +291,16 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
     }
     if (hs->capacity < total) return fail(c, NV_ERR_ARG, "state arrays smaller than the sum of pcounts");
     if (K > 1 && c->prm.tree_theta > 0) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
-    // links and the absence of walls, tangle by tangle
+    // links, and pinned points only where there are walls, tangle by tangle
     long off = 0;
     for (int k = 0; k < K; ++k) {
         const int n = pcounts[k];
         for (int q = 0; q < n; ++q) {
             const long r = off + q;
-            if ((hs->pinnedi && hs->pinnedi[r]) || (hs->pinnedb && hs->pinnedb[r]) ||
-                (hs->wpinned && (hs->wpinned[3 * r] | hs->wpinned[3 * r + 1] | hs->wpinned[3 * r + 2])))
-                return fail(c, NV_ERR_UNSUPPORTED, "pinned points (solid walls) are not supported");
+            if (!c->phys.walls &&
+                ((hs->pinnedi && hs->pinnedi[r]) || (hs->pinnedb && hs->pinnedb[r]) ||
+                 (hs->wpinned && (hs->wpinned[3 * r] | hs->wpinned[3 * r + 1] | hs->wpinned[3 * r + 2]))))
+                return fail(c, NV_ERR_ARG, "pinned points without a solid boundary");
             const int f = hs->infront[r], b = hs->behind[r];
             if (f < 0 || f > n || (f != 0 && (b < 1 || b > n))) return fail(c, NV_ERR_ARG, "link out of range");
         }
@@ -305,6 +313,7 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
     c->tn.assign(pcounts, pcounts + K);
     c->trecon.assign(K, 0);
     c->tremove.assign(K, 0);
+    c->twall.assign(K, 0);
     long rows = 0;
     for (int k = 0; k < K; ++k) {
         // the rows a move to a wider layout would give (3 x pcount + margin),
@@ -320,6 +329,7 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
     if (rows > 0x7fffffffL / 8) return fail(c, NV_ERR_CAPACITY, "too many rows");
     c->trecon[0] = hs->recon_count;
     c->tremove[0] = hs->remove_count;
+    c->twall[0] = hs->wall_recon_count;
     c->rows = (int)rows;
     if (int rc = alloc_state(c)) return rc;
     if (int rc = upload_tables(c)) return rc;
@@ -370,6 +380,23 @@ int set_batch(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs, const s
         }
         HIPCHK(c, hipMemcpy(sdst[q], cd.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     }
+    // pinned flags (bit 0 pinnedi, bit 1 pinnedb) and wpinned [3][R]
+    std::vector<int> pin(R, 0), wp((size_t)3 * R, 0);
+    {
+        long o = 0;
+        for (int k = 0; k < K; ++k) {
+            for (int r = 0; r < pcounts[k]; ++r) {
+                const long h = o + r;
+                const int g = c->tbase[k] + r;
+                pin[g] = (hs->pinnedi && hs->pinnedi[h] ? NV_PINNED_I : 0) | (hs->pinnedb && hs->pinnedb[h] ? NV_PINNED_B : 0);
+                if (hs->wpinned)
+                    for (int j = 0; j < 3; ++j) wp[(size_t)j * R + g] = hs->wpinned[3 * h + j];
+            }
+            o += pcounts[k];
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->d.pin, pin.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d.wp, wp.data(), sizeof(int) * 3 * (size_t)R, hipMemcpyHostToDevice));
     c->t = hs->t;
     c->itime = hs->itime;
     // a new point array; the noise table's counter is not part of it
@@ -397,6 +424,7 @@ int get_batch(Ctx *c, int32_t *pcounts, nv_state *hs) {
     hs->itime = c->itime;
     hs->recon_count = std::accumulate(c->trecon.begin(), c->trecon.end(), 0);
     hs->remove_count = std::accumulate(c->tremove.begin(), c->tremove.end(), 0);
+    hs->wall_recon_count = std::accumulate(c->twall.begin(), c->twall.end(), 0);
     if (total == 0) return NV_OK;
     const int R = c->rows;
     std::vector<double> tmp((size_t)3 * R);
@@ -439,9 +467,22 @@ int get_batch(Ctx *c, int32_t *pcounts, nv_state *hs) {
             o += c->tn[k];
         }
     }
-    if (hs->pinnedi) std::memset(hs->pinnedi, 0, sizeof(int32_t) * total);
-    if (hs->pinnedb) std::memset(hs->pinnedb, 0, sizeof(int32_t) * total);
-    if (hs->wpinned) std::memset(hs->wpinned, 0, sizeof(int32_t) * 3 * total);
+    if (hs->pinnedi || hs->pinnedb || hs->wpinned) {
+        std::vector<int> pin(R), wp((size_t)3 * R);
+        HIPCHK(c, hipMemcpy(pin.data(), c->d.pin, sizeof(int) * R, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(wp.data(), c->d.wp, sizeof(int) * 3 * (size_t)R, hipMemcpyDeviceToHost));
+        long o = 0;
+        for (int k = 0; k < c->K; ++k) {
+            for (int r = 0; r < c->tn[k]; ++r) {
+                const int g = c->tbase[k] + r;
+                if (hs->pinnedi) hs->pinnedi[o + r] = (pin[g] & NV_PINNED_I) ? 1 : 0;
+                if (hs->pinnedb) hs->pinnedb[o + r] = (pin[g] & NV_PINNED_B) ? 1 : 0;
+                if (hs->wpinned)
+                    for (int j = 0; j < 3; ++j) hs->wpinned[3 * (o + r) + j] = wp[(size_t)j * R + g];
+            }
+            o += c->tn[k];
+        }
+    }
     return NV_OK;
 }
 
@@ -567,6 +608,25 @@ hipError_t clear_pinsert_stats(Ctx *c) {
     return e;
 }
 
+// Solid walls: the image vortices' velocity into c->mir (launches only),
+// the direct sum's (src/timestep.f90:94-98) or the octree's (:116-120; in a
+// recorded step on the device's counts); LIA has none.
+int enqueue_mirrors(Ctx *c, DState d, bool recorded) {
+    if (!c->phys.walls || c->prm.velocity == NV_VEL_LIA) return NV_OK;
+    if (c->prm.velocity == NV_VEL_BS) {
+        HIPCHK(c, nvk_mirror_bs64(c->st, d, c->phys, tangles(c, nullptr, nullptr), c->rows, c->mir.p));
+        return NV_OK;
+    }
+    const int rc = recorded ? nv_tree_graph_mirror(c->tree, c->st, d, c->phys, c->mir.p, c->ctl, c->p_tn)
+                            : nv_tree_walk_mirror(c->tree, c->st, d, c->phys, c->tn[0], c->rows, c->mir.p, c->ctl);
+    return rc ? fail(c, NV_ERR_HIP, "mirror tree walk launch failed") : NV_OK;
+}
+
+// the image vortices' sum k_finish adds, or none
+const double *mirror_sum(const Ctx *c) {
+    return c->phys.walls && c->prm.velocity != NV_VEL_LIA ? c->mir.p : nullptr;
+}
+
 int velocity_impl(Ctx *c, int do_ab) {
     if (c->K == 0) return NV_OK;
     if (noise_missing(c)) return fail(c, NV_ERR_STATE, "normal_velocity noisey_xflow without a noise table (nv_set_noise)");
@@ -626,10 +686,11 @@ int velocity_impl(Ctx *c, int do_ab) {
         }
         timed_pair = true;
     }
+    if (int rc = enqueue_mirrors(c, c->d, false)) return rc;
     HIPCHK(c, nv_fill_int(c->st, &c->ctl->nvel, 0, 1));
     HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
     Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
-    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, tg, R, c->part.p, mode, do_ab, c->ctl));
+    HIPCHK(c, nvk_finish(c->st, c->d, c->phys, tg, R, c->part.p, mode, do_ab, c->ctl, mirror_sum(c)));
     HIPCHK(c, hipMemcpyAsync(&c->ctl_h->nvel, &c->ctl->nvel, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -696,6 +757,43 @@ int build_close_items(Ctx *c, bool cap_rows, int *nitems_out) {
     return NV_OK;
 }
 
+// ------------------------------------------------------- topology phases
+// pinsert (src/line.f90:18-96) as launches only, its counters cleared by the
+// caller: the staged kernels, or with solid walls the serial form
+hipError_t enqueue_pinsert(Ctx *c, DState d) {
+    hipStream_t st = c->st;
+    const int R = c->rows;
+    Tangles tg = tangles(c, nullptr, nullptr);
+    if (c->phys.walls) return nvk_pinsert_walls(st, d, c->phys, tg, c->ctl);
+    hipError_t e = nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl);
+    if (e == hipSuccess) e = nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p);
+    if (e == hipSuccess) e = nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p);
+    if (e == hipSuccess) e = nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl);
+    return e;
+}
+
+// precon, wall_recon and premove (src/run.f90:42-44) as launches only; the
+// device clock stamps the reconnection log
+hipError_t enqueue_recon_remove(Ctx *c, DState d) {
+    hipStream_t st = c->st;
+    const int R = c->rows;
+    Tangles tg = tangles(c, nullptr, nullptr);
+    hipError_t e;
+    if (c->phys.walls) {
+        e = nvk_precon_walls(st, d, c->phys, tg, c->ctl, c->ev, c->ev_cap);
+        if (e == hipSuccess) e = nvk_wall_recon(st, d, c->phys, tg, c->ctl);
+        if (e == hipSuccess) e = nvk_premove_walls(st, d, c->phys, tg, c->ctl);
+        return e;
+    }
+    e = nvk_precon_pre(st, d, c->phys, R, c->cand.p);
+    if (e == hipSuccess) e = nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p);
+    if (e == hipSuccess) e = nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap);
+    if (e == hipSuccess) e = nvk_premove_pre(st, d, c->phys, R, c->cand.p);
+    if (e == hipSuccess) e = nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p);
+    if (e == hipSuccess) e = nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl);
+    return e;
+}
+
 // ------------------------------------------------------- recorded steps
 // One pass of the src/run.f90 loop body (direct sum or LIA) as kernel
 // launches only: no copies, fills by kernel, no host reads, the step's
@@ -726,23 +824,18 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
             mode = 0;
         }
     }
-    HIPCHK(c, nvk_finish(st, d, c->phys, tv, R, c->part.p, mode, 1, c->ctl));
+    if (int rc = enqueue_mirrors(c, d, true)) return rc;
+    HIPCHK(c, nvk_finish(st, d, c->phys, tv, R, c->part.p, mode, 1, c->ctl, mirror_sum(c)));
     std::swap(d.x, d.xn);
-    Tangles tg = tangles(c, nullptr, nullptr);
-    // pinsert (its counters were cleared by nvk_step_begin)
-    HIPCHK(c, nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
-    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
-    HIPCHK(c, nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    HIPCHK(c, enqueue_pinsert(c, d));
     if (recon) {
-        Tangles tp = tangles(c, c->d_pS.p, c->d_ppoff.p);
-        HIPCHK(c, nvk_pclose(st, d, tp, R, c->pitems.p, npitems, c->sph.p, c->pd.p, c->pj.p));
-        HIPCHK(c, nvk_precon_pre(st, d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-        HIPCHK(c, nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
-        HIPCHK(c, nvk_premove_pre(st, d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-        HIPCHK(c, nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl));
+        if (c->phys.walls) {
+            HIPCHK(c, nvk_pclose_walls(st, d, tangles(c, nullptr, nullptr), R));
+        } else {
+            Tangles tp = tangles(c, c->d_pS.p, c->d_ppoff.p);
+            HIPCHK(c, nvk_pclose(st, d, tp, R, c->pitems.p, npitems, c->sph.p, c->pd.p, c->pj.p));
+        }
+        HIPCHK(c, enqueue_recon_remove(c, d));
     }
     HIPCHK(c, nvk_step_end(st, d, c->phys, R, c->ctl, c->prm.dt));
     return NV_OK;
@@ -766,11 +859,13 @@ int enqueue_tree_pmotion(Ctx *c) {
                               c->ctl, &c->stats))
         return fail(c, NV_ERR_HIP, "tree walk launch failed");
     HIPCHK(c, hipEventRecord(c->e3, st));
+    if (int rc = enqueue_mirrors(c, c->d, false)) return rc;
     if (int rc = nv_tree_walk_counters_async(c->tree, st)) return fail(c, rc, "tree counters");
     HIPCHK(c, nv_fill_int(st, &c->ctl->nvel, 0, 1));
     HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
     HIPCHK(c, nvk_finish(st, c->d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p,
-                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl));
+                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl,
+                         mirror_sum(c)));
     HIPCHK(c, hipEventRecord(c->e1, st));
     swap_x(c);
     return NV_OK;
@@ -814,14 +909,10 @@ int tree_step(Ctx *c) {
     const bool walk = c->prm.velocity == NV_VEL_TREE;
     if ((rc = walk ? enqueue_tree_pmotion(c) : velocity_impl(c, 1))) return rc;
     // pinsert
-    Tangles tg = tangles(c, nullptr, nullptr);
     HIPCHK(c, nv_fill_int(st, &c->ctl->nactive, 0, 1));
     HIPCHK(c, nv_fill_int(st, c->p_tnact, 0, c->K));
     HIPCHK(c, clear_pinsert_stats(c));
-    HIPCHK(c, nvk_pinsert_pre(st, c->d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
-    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
-    HIPCHK(c, nvk_pinsert_serial(st, c->d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    HIPCHK(c, enqueue_pinsert(c, c->d));
     const int rc_ins = read_ctl(c);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->e4, c->e5);
@@ -836,17 +927,13 @@ int tree_step(Ctx *c) {
         } else {
             HIPCHK(c, hipEventRecord(c->e6, st));
             const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
-            if (nv_tree_walk_close(c->tree, st, c->d, bsize, c->tn[0], &c->stats))
-                return fail(c, NV_ERR_HIP, "tree close search launch failed");
+            const int rw = c->phys.walls ? nv_tree_close_walls(c->tree, st, c->d, c->tn[0])
+                                         : nv_tree_walk_close(c->tree, st, c->d, bsize, c->tn[0], &c->stats);
+            if (rw) return fail(c, NV_ERR_HIP, "tree close search launch failed");
             HIPCHK(c, hipEventRecord(c->e7, st));
         }
-        HIPCHK(c, nvk_precon_pre(st, c->d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
         HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
-        HIPCHK(c, nvk_precon_serial(st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
-        HIPCHK(c, nvk_premove_pre(st, c->d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-        HIPCHK(c, nvk_premove_serial(st, c->d, c->phys, tg, c->lcand.p, c->ctl));
+        HIPCHK(c, enqueue_recon_remove(c, c->d));
     }
     HIPCHK(c, nvk_wrap(st, c->d, c->phys, R));
     c->t = c->t + c->prm.dt;
@@ -883,25 +970,19 @@ int enqueue_tree_step(Ctx *c, DState d, bool recon) {
     const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
     if (nv_tree_graph_walk(c->tree, st, d, c->phys, c->sh, c->prm.precision, rn * rn, c->part.p, c->ctl, dn))
         return fail(c, NV_ERR_HIP, "tree walk launch failed");
+    if (int rc = enqueue_mirrors(c, d, true)) return rc;
     HIPCHK(c, nvk_finish(st, d, c->phys, tangles(c, c->d_S.p, c->d_poff.p), R, c->part.p,
-                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl));
+                         nv_tree_walk_partial_mode(c->prm.precision, c->phys.tree_theta), 1, c->ctl,
+                         mirror_sum(c)));
     std::swap(d.x, d.xn);
-    Tangles tg = tangles(c, nullptr, nullptr);
     // pinsert (its counters were cleared by nvk_step_begin)
-    HIPCHK(c, nvk_pinsert_pre(st, d, c->phys, tg, R, c->cand.p, c->empty.p, c->ctl));
-    HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-    HIPCHK(c, nv_compact_flags(st, c->empty.p, R, c->lempty.p, &c->ctl->nempty, c->tmp.p));
-    HIPCHK(c, nvk_pinsert_serial(st, d, c->phys, tg, c->lcand.p, c->lempty.p, c->ctl));
+    HIPCHK(c, enqueue_pinsert(c, d));
     if (recon) {
         const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
-        if (nv_tree_graph_close(c->tree, st, d, bsize, dn, c->ctl))
-            return fail(c, NV_ERR_HIP, "tree close search launch failed");
-        HIPCHK(c, nvk_precon_pre(st, d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-        HIPCHK(c, nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
-        HIPCHK(c, nvk_premove_pre(st, d, c->phys, R, c->cand.p));
-        HIPCHK(c, nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p));
-        HIPCHK(c, nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl));
+        const int rw = c->phys.walls ? nv_tree_graph_close_walls(c->tree, st, d, dn, c->ctl)
+                                     : nv_tree_graph_close(c->tree, st, d, bsize, dn, c->ctl);
+        if (rw) return fail(c, NV_ERR_HIP, "tree close search launch failed");
+        HIPCHK(c, enqueue_recon_remove(c, d));
     }
     HIPCHK(c, nvk_step_end(st, d, c->phys, R, c->ctl, c->prm.dt));
     return NV_OK;
@@ -1100,6 +1181,25 @@ int set_phys(Ctx *c) {
     f.shear_omega = p.norm_shear_omega;
     f.noise = c->noise.p;
     f.noise_n = c->noise.p ? (int)c->noise.n : 0;
+    // src/boundary.f90:180-214 create_mirror_loop_array: the reflections in
+    // the solid walls, i, j, k nested over -1..1 (a direction that is not
+    // solid held at 0)
+    f.walls = 0;
+    f.nmir = 0;
+    for (int k = 0; k < 3; ++k) {
+        f.solid[k] = p.boundary[k] == NV_BC_SOLID;
+        f.walls |= f.solid[k];
+    }
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j)
+            for (int k = -1; k <= 1; ++k) {
+                if ((!f.solid[0] && i) || (!f.solid[1] && j) || (!f.solid[2] && k)) continue;
+                if (!i && !j && !k) continue;
+                f.mir[f.nmir][0] = (signed char)i;
+                f.mir[f.nmir][1] = (signed char)j;
+                f.mir[f.nmir][2] = (signed char)k;
+                ++f.nmir;
+            }
     // src/boundary.f90:143-175 create_periodic_loop_array; image 0 = no shift
     Shifts &s = c->sh;
     s.s[0][0] = s.s[0][1] = s.s[0][2] = 0.0;
@@ -1140,9 +1240,9 @@ int nv_create(const nv_params *params, void **out) {
     else if (!(std::min(p.box_size[0], std::min(p.box_size[1], p.box_size[2])) > 0.0))
         rc = fail(c, NV_ERR_ARG, "init_setup: box size is less than or equal zero");
     else if (p.velocity < 0 || p.velocity > 2) rc = fail(c, NV_ERR_ARG, "velocity must be LIA, BS or Tree");
-    else if (p.boundary[0] < 0 || p.boundary[0] > 1 || p.boundary[1] < 0 || p.boundary[1] > 1 || p.boundary[2] < 0 ||
-             p.boundary[2] > 1)
-        rc = fail(c, NV_ERR_UNSUPPORTED, "only periodic and open boundaries are supported");
+    else if (p.boundary[0] < 0 || p.boundary[0] > 2 || p.boundary[1] < 0 || p.boundary[1] > 2 || p.boundary[2] < 0 ||
+             p.boundary[2] > 2)
+        rc = fail(c, NV_ERR_ARG, "init_setup: incorrect boundary parameter");
     else if (p.tree_theta > 0 && (p.boundary[0] == 0 || p.boundary[1] == 0 || p.boundary[2] == 0))
         rc = fail(c, NV_ERR_ARG, "init.mod:init_setup: tree approximation cannot run with open boundaries");
     else if (p.normal_velocity < NV_NF_ZERO || p.normal_velocity > NV_NF_CHANNEL)
@@ -1328,6 +1428,10 @@ int nv_pinsert(void *h) {
     HIPCHK(c, hipMemcpyAsync(&c->ctl->nactive, &c->ctl_h->nactive, sizeof(int), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, clear_pinsert_stats(c));
     HIPCHK(c, nv_fill_int(c->st, c->p_tnact, 0, c->K));
+    if (c->phys.walls) {
+        HIPCHK(c, nvk_pinsert_walls(c->st, c->d, c->phys, tg, c->ctl));
+        return read_ctl(c);
+    }
     HIPCHK(c, nvk_pinsert_pre(c->st, c->d, c->phys, tg, c->rows, c->cand.p, c->empty.p, c->ctl));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
     if (int rc = compact(c, c->empty, c->lempty, &c->ctl->nempty)) return rc;
@@ -1339,12 +1443,18 @@ int nv_pclose(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     if (c->K == 0) return NV_OK;
-    int nitems = 0;
-    if (int rc = build_close_items(c, false, &nitems)) return rc;
-    Tangles tg = tangles(c, c->d_pS.p, c->d_ppoff.p);
-    HIPCHK(c, hipEventRecord(c->e2, c->st));
-    HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, nitems, c->sph.p, c->pd.p, c->pj.p));
-    HIPCHK(c, hipEventRecord(c->e3, c->st));
+    if (c->phys.walls) {
+        HIPCHK(c, hipEventRecord(c->e2, c->st));
+        HIPCHK(c, nvk_pclose_walls(c->st, c->d, tangles(c, nullptr, nullptr), c->rows));
+        HIPCHK(c, hipEventRecord(c->e3, c->st));
+    } else {
+        int nitems = 0;
+        if (int rc = build_close_items(c, false, &nitems)) return rc;
+        Tangles tg = tangles(c, c->d_pS.p, c->d_ppoff.p);
+        HIPCHK(c, hipEventRecord(c->e2, c->st));
+        HIPCHK(c, nvk_pclose(c->st, c->d, tg, c->rows, c->pitems.p, nitems, c->sph.p, c->pd.p, c->pj.p));
+        HIPCHK(c, hipEventRecord(c->e3, c->st));
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->e2, c->e3);
@@ -1359,7 +1469,8 @@ int nv_pclose_tree(void *h) {
     if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
     const double bsize = std::max(c->phys.box[0], std::max(c->phys.box[1], c->phys.box[2]));
-    int rc = nv_tree_walk_close(c->tree, c->st, c->d, bsize, c->tn[0], &c->stats);
+    int rc = c->phys.walls ? nv_tree_close_walls(c->tree, c->st, c->d, c->tn[0])
+                           : nv_tree_walk_close(c->tree, c->st, c->d, bsize, c->tn[0], &c->stats);
     if (rc) return fail(c, NV_ERR_HIP, "tree close search launch failed");
     HIPCHK(c, hipEventRecord(c->e3, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1374,10 +1485,25 @@ int nv_precon(void *h) {
     if (!c) return NV_ERR_ARG;
     if (c->K == 0) return NV_OK;
     Tangles tg = tangles(c, nullptr, nullptr);
+    if (c->phys.walls) {
+        HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
+        HIPCHK(c, nvk_precon_walls(c->st, c->d, c->phys, tg, c->ctl, c->ev, c->ev_cap));
+        return read_ctl(c);
+    }
     HIPCHK(c, nvk_precon_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
     HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
     HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
+    return read_ctl(c);
+}
+
+int nv_wall_recon(void *h) {
+    Ctx *c = (Ctx *)h;
+    if (!c) return NV_ERR_ARG;
+    // without solid walls there are no pinned points and no wall to reach:
+    // the reference's loop changes nothing
+    if (c->K == 0 || !c->phys.walls) return NV_OK;
+    HIPCHK(c, nvk_wall_recon(c->st, c->d, c->phys, tangles(c, nullptr, nullptr), c->ctl));
     return read_ctl(c);
 }
 
@@ -1386,6 +1512,10 @@ int nv_premove(void *h) {
     if (!c) return NV_ERR_ARG;
     if (c->K == 0) return NV_OK;
     Tangles tg = tangles(c, nullptr, nullptr);
+    if (c->phys.walls) {
+        HIPCHK(c, nvk_premove_walls(c->st, c->d, c->phys, tg, c->ctl));
+        return read_ctl(c);
+    }
     HIPCHK(c, nvk_premove_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
     HIPCHK(c, nvk_premove_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl));
@@ -1453,6 +1583,7 @@ int nv_step(void *h, int32_t nsteps) {
             else rc = nv_pclose(c);
             if (rc) break;
             if ((rc = nv_precon(c))) break;
+            if ((rc = nv_wall_recon(c))) break;
             if ((rc = nv_premove(c))) break;
         }
         if ((rc = nv_enforce_boundary(c))) break;
@@ -1502,6 +1633,7 @@ int nv_diagnostics(void *h, nv_diag *out) {
     out->total_length = k.total_length;
     out->avg_sep = k.total_length / (double)(k.n_old - k.empty_after);
     out->tree_eval = c->stats.tree_evals;
+    out->wall_recon_count = c->twall[0];
     return NV_OK;
 }
 

@@ -22,7 +22,11 @@ struct DState {
     int *infront, *behind, *closest;
     double *closestd;
     double *curv, *stretch;         // diagnostics: |s''| and u . s'' (src/diagnostic.f90:37-74)
+    int *pin;                       // solid walls: bit 0 pinnedi, bit 1 pinnedb (src/cdata.f90:28)
+    int *wp;                        // wpinned, [3][cap] (src/cdata.f90:27)
 };
+
+constexpr int NV_PINNED_I = 1, NV_PINNED_B = 2;
 
 // Independent tangles sharing the state arrays.  Tangle k owns the rows
 // [base[k], base[k] + cap[k]) (both multiples of 256, so source tiles never
@@ -34,6 +38,7 @@ struct Tangles {
     int *n;                 // pcount of each tangle (updated on the device)
     int *recon, *remove;    // recon_count / remove_count of each tangle
     int *nact;              // active points of each tangle (pinsert pre-filter)
+    int *wall;              // wall_recon_count of each tangle
     const int *row2t;       // tangle of every row
     const int *S;           // direct-sum chunks per target of each tangle (this launch)
     const long long *poff;  // offset of each tangle's partial block (this launch)
@@ -62,6 +67,10 @@ struct Phys {
     double shear_omega;     // norm_shear_omega
     const double *noise;    // noisey_xflow table (device), noise_n values
     int noise_n;
+    int solid[3];           // solid walls at +-box/2 in that direction
+    int walls;              // any solid wall: mirror images, pinned points, wall_recon
+    int nmir;               // mirror images (src/boundary.f90:180-214 order)
+    signed char mir[26][3];
 };
 
 struct Shifts {
@@ -77,13 +86,41 @@ __device__ __forceinline__ double dist_gen(double a0, double a1, double a2, doub
     return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
 }
 
-// src/boundary.f90:26-84 get_ghost_p for an unpinned point of row r
+// the reference's reflection of coordinate v in the wall of direction code r
+// (-1, 0, 1): v + 2*abs(r)*(0.5*r*L - v) (src/boundary.f90:33-37,
+// src/timestep.f90:221-222, src/tree.f90:345-348)
+__device__ __forceinline__ double reflect1(double v, int r, double L) {
+    return v + (double)(2 * abs(r)) * ((0.5 * (double)r) * L - v);
+}
+
+// src/boundary.f90:26-84 get_ghost_p of row r: a point pinned in front takes
+// its behind's mirror image as its ghost in front, one pinned behind its
+// infront's as its ghost behind (:31-38)
 __device__ __forceinline__ void ghost_of(const DState &s, const Phys &p, int r, double gi[3], double gb[3]) {
-    int f = s.infront[r] - 1, b = s.behind[r] - 1;
+    const int pin = p.walls ? s.pin[r] : 0;
+    if (pin & NV_PINNED_I) {
+        const int b = s.behind[r] - 1;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        gi[k] = ld(s.x, s.cap, k, f);
-        gb[k] = ld(s.x, s.cap, k, b);
+        for (int k = 0; k < 3; ++k) {
+            const double xb = ld(s.x, s.cap, k, b);
+            gb[k] = xb;
+            gi[k] = reflect1(xb, s.wp[k * s.cap + r], p.box[k]);
+        }
+    } else if (pin & NV_PINNED_B) {
+        const int f = s.infront[r] - 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double xf = ld(s.x, s.cap, k, f);
+            gi[k] = xf;
+            gb[k] = reflect1(xf, s.wp[k * s.cap + r], p.box[k]);
+        }
+    } else {
+        const int f = s.infront[r] - 1, b = s.behind[r] - 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gi[k] = ld(s.x, s.cap, k, f);
+            gb[k] = ld(s.x, s.cap, k, b);
+        }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -178,13 +215,15 @@ __device__ __forceinline__ void normal_velocity(const Phys &p, const double x[3]
 #define NV_5_12 (5.0 / 12.0)
 
 // Velocity epilogue of one active point: local term (src/timestep.f90:60-80,
-// 102-105), the non-local sums ub0 (unshifted) and ubs (images), superfluid
+// 102-105), the non-local sums ub0 (unshifted), ubs (images) and ubm (solid
+// walls' image vortices), superfluid
 // velocity store (:123), mutual friction (:126-139, src/normal_fluid.f90:83-118),
 // imposed flow (:140); `noise` is the noisey_xflow table's value for this
 // call of pmotion; then, if do_ab, the Adams-Bashforth update of
 // src/timestep.f90:28-43 into xn.
 __device__ __forceinline__ void finish_point(const DState &s, const Phys &p, int r, const double ub0[3],
-                                             const double ubs[3], double t, double noise, int do_ab) {
+                                             const double ubs[3], const double ubm[3], double t, double noise,
+                                             int do_ab) {
     const int cap = s.cap;
     double x[3], gi[3], gb[3], fd[3], fdd[3], c[3], u[3];
 #pragma unroll
@@ -211,6 +250,7 @@ __device__ __forceinline__ void finish_point(const DState &s, const Phys &p, int
             u[k] = beta * c[k];
             u[k] = u[k] + ub0[k];
             u[k] = u[k] + ubs[k];
+            u[k] = u[k] + ubm[k];
         }
     }
 #pragma unroll

@@ -50,7 +50,16 @@ hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, in
                      double kappa,
                      float rn2, double *part, int tpt, int kb);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
-                      int do_ab, DCtl *ctl);
+                      int do_ab, DCtl *ctl, const double *mir);
+// solid walls (k_walls.hip, k_topology.hip): the image vortices' sum
+// mir[3][rows] of the direct sum, the brute close search with pinned points,
+// and the topology phases as the reference's serial loops
+hipError_t nvk_mirror_bs64(hipStream_t st, DState s, Phys p, Tangles tg, int rows, double *mir);
+hipError_t nvk_pclose_walls(hipStream_t st, DState s, Tangles tg, int rows);
+hipError_t nvk_pinsert_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl);
+hipError_t nvk_precon_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl, nv_recon_event *ev, int ev_cap);
+hipError_t nvk_wall_recon(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl);
+hipError_t nvk_premove_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl);
 hipError_t nvk_reduce_probe(hipStream_t st, DState s, Tangles tg, int rows, const double *part, double *sink);
 size_t nvk_rec32_bytes();
 size_t nvk_tileinfo_bytes();

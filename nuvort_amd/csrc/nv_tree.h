@@ -44,3 +44,12 @@ int nv_tree_graph_gen(const TreeBuf *t);
 // then (the stream past them) take them; true when the plan should be redone
 int nv_tree_graph_take_async(TreeBuf *t, hipStream_t st);
 bool nv_tree_graph_take(TreeBuf *t, nv_stats *stats);
+
+// Solid walls: tree_walk_mirror (src/tree.f90:321-392) over every reflection
+// into mir[3][npad] (FP64, the reference's arithmetic), and pclose_tree with
+// pinned points (src/tree.f90:190-244); eagerly (host counts) or in a
+// recorded step (device counts, as the graph functions above)
+int nv_tree_walk_mirror(TreeBuf *t, hipStream_t st, DState d, Phys p, int n, int npad, double *mir, DCtl *ctl);
+int nv_tree_close_walls(TreeBuf *t, hipStream_t st, DState d, int n);
+int nv_tree_graph_mirror(TreeBuf *t, hipStream_t st, DState d, Phys p, double *mir, DCtl *ctl, const int *dn);
+int nv_tree_graph_close_walls(TreeBuf *t, hipStream_t st, DState d, const int *dn, DCtl *ctl);

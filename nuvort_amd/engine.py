@@ -40,7 +40,7 @@ def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.
         s.box_size[k] = p.box_size[k]
         s.super_velocity[k] = p.super_velocity[k]
     for k, b in enumerate((p.boundary_x, p.boundary_y, p.boundary_z)):
-        s.boundary[k] = {"open": 0, "periodic": 1}.get(b, 2)
+        s.boundary[k] = {"open": 0, "periodic": 1, "solid": 2}.get(b, 9)
     s.velocity = {"LIA": 0, "BS": 1, "Tree": 2}[p.velocity]
     s.tree_theta = p.tree_theta
     s.tree_extra_correction = 1 if p.tree_extra_correction else 0
@@ -97,6 +97,7 @@ class FilamentSolver:
         v.capacity = cap if cap is not None else st.pcount
         v.itime, v.recon_count, v.t = st.itime, st.recon_count, st.t
         v.remove_count = int(st.extra.get("remove_count", 0))
+        v.wall_recon_count = int(st.extra.get("wall_recon_count", 0))
         m = {"x": st.x, "u": st.u, "u1": st.u1, "u2": st.u2, "u_sup": st.u_sup, "u_mf": st.u_mf,
              "ghosti": st.ghosti, "ghostb": st.ghostb, "closestd": st.closestd}
         for k, a in m.items():
@@ -129,6 +130,7 @@ class FilamentSolver:
         self._call("nv_get_state", C.byref(v))
         st.t, st.itime, st.recon_count = v.t, v.itime, v.recon_count
         st.extra["remove_count"] = v.remove_count
+        st.extra["wall_recon_count"] = v.wall_recon_count
         return st
 
     def set_state_batch(self, states) -> None:
@@ -202,6 +204,9 @@ class FilamentSolver:
 
     def precon(self):
         self._call("nv_precon")
+
+    def wall_recon(self):
+        self._call("nv_wall_recon")
 
     def premove(self):
         self._call("nv_premove")

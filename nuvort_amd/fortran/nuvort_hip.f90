@@ -17,7 +17,7 @@ module nuvort_hip
 
   integer(c_int), parameter :: NV_OK = 0
   integer(c_int), parameter :: NV_VEL_LIA = 0, NV_VEL_BS = 1, NV_VEL_TREE = 2
-  integer(c_int), parameter :: NV_BC_OPEN = 0, NV_BC_PERIODIC = 1
+  integer(c_int), parameter :: NV_BC_OPEN = 0, NV_BC_PERIODIC = 1, NV_BC_SOLID = 2
   integer(c_int), parameter :: NV_NF_ZERO = 0, NV_NF_XFLOW = 1, NV_NF_NOISEY_XFLOW = 2, NV_NF_SHEAR_XFLOW = 3
   integer(c_int), parameter :: NV_NF_YCHANNEL = 4, NV_NF_CHANNEL = 5
   integer(c_int), parameter :: NV_PREC_FP64 = 0, NV_PREC_FP32C = 1
@@ -49,7 +49,7 @@ module nuvort_hip
   end type
 
   type, bind(C) :: nv_state
-    integer(c_int32_t) :: pcount, capacity, itime, recon_count, remove_count, pad
+    integer(c_int32_t) :: pcount, capacity, itime, recon_count, remove_count, wall_recon_count
     real(c_double) :: t
     type(c_ptr) :: x, u, u1, u2, u_sup, u_mf, ghosti, ghostb
     type(c_ptr) :: infront, behind, closest
@@ -65,6 +65,7 @@ module nuvort_hip
     real(c_double) :: t, maxu, maxdu, mean_u_sup, mean_u_mf, kappa_bar, kappa_min, kappa_max
     real(c_double) :: total_length, avg_sep
     integer(c_int64_t) :: tree_eval
+    integer(c_int32_t) :: wall_recon_count, pad
   end type
 
   interface
@@ -129,6 +130,10 @@ module nuvort_hip
       type(c_ptr), value :: ctx
     end function
     integer(c_int) function nv_precon(ctx) bind(C, name='nv_precon')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nv_wall_recon(ctx) bind(C, name='nv_wall_recon')
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
     end function

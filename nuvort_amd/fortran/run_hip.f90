@@ -66,6 +66,7 @@ program run_hip
         call nv_check(ctx, nv_pclose(ctx), 'line.mod:pclose')
       end if
       call nv_check(ctx, nv_precon(ctx), 'reconnection.mod:precon')
+      call nv_check(ctx, nv_wall_recon(ctx), 'reconnection.mod:wall_recon')
       call nv_check(ctx, nv_premove(ctx), 'line.mod:premove')
     end if
     call nv_check(ctx, nv_enforce_boundary(ctx), 'boundary.mod:enforce_boundary')
@@ -145,9 +146,7 @@ contains
     end do
     close(fh)
     p%delta = delta ; p%dt = dtv ; p%quant_circ = kappa ; p%corea = corea ; p%box_size = box
-    p%boundary(1) = merge(NV_BC_PERIODIC, merge(NV_BC_OPEN, 2, bx == 'open'), bx == 'periodic')
-    p%boundary(2) = merge(NV_BC_PERIODIC, merge(NV_BC_OPEN, 2, by == 'open'), by == 'periodic')
-    p%boundary(3) = merge(NV_BC_PERIODIC, merge(NV_BC_OPEN, 2, bz == 'open'), bz == 'periodic')
+    p%boundary(1) = bc_code(bx) ; p%boundary(2) = bc_code(by) ; p%boundary(3) = bc_code(bz)
     select case (vel)
     case ('LIA') ; p%velocity = NV_VEL_LIA
     case ('BS') ; p%velocity = NV_VEL_BS
@@ -178,6 +177,17 @@ contains
     dt_out = dtv ; theta_out = theta
   end subroutine
 
+  !> run.in's boundary word as NV_BC_* (9 for a word init_setup rejects)
+  integer(c_int) function bc_code(word)
+    character(len=*), intent(in) :: word
+    select case (trim(word))
+    case ('periodic') ; bc_code = NV_BC_PERIODIC
+    case ('open') ; bc_code = NV_BC_OPEN
+    case ('solid') ; bc_code = NV_BC_SOLID
+    case default ; bc_code = 9
+    end select
+  end function
+
   subroutine resize_state(m)
     integer, intent(in) :: m
     if (allocated(x)) deallocate(x, u, u1, u2, usup, umf, gi, gb, closestd, infront, behind, closest, &
@@ -189,7 +199,7 @@ contains
   subroutine bind_state(cap)
     integer, intent(in) :: cap
     st%pcount = n ; st%capacity = cap ; st%itime = itime ; st%recon_count = recon_count
-    st%remove_count = 0 ; st%pad = 0 ; st%t = t
+    st%remove_count = 0 ; st%wall_recon_count = 0 ; st%t = t
     st%x = c_loc(x) ; st%u = c_loc(u) ; st%u1 = c_loc(u1) ; st%u2 = c_loc(u2)
     st%u_sup = c_loc(usup) ; st%u_mf = c_loc(umf) ; st%ghosti = c_loc(gi) ; st%ghostb = c_loc(gb)
     st%infront = c_loc(infront) ; st%behind = c_loc(behind) ; st%closest = c_loc(closest)

@@ -36,7 +36,7 @@ class NvParams(C.Structure):
 
 class NvState(C.Structure):
     _fields_ = [("pcount", C.c_int32), ("capacity", C.c_int32), ("itime", C.c_int32), ("recon_count", C.c_int32),
-                ("remove_count", C.c_int32), ("pad", C.c_int32), ("t", C.c_double),
+                ("remove_count", C.c_int32), ("wall_recon_count", C.c_int32), ("t", C.c_double),
                 ("x", _dp), ("u", _dp), ("u1", _dp), ("u2", _dp), ("u_sup", _dp), ("u_mf", _dp),
                 ("ghosti", _dp), ("ghostb", _dp), ("infront", _ip), ("behind", _ip), ("closest", _ip),
                 ("closestd", _dp), ("pinnedi", _ip), ("pinnedb", _ip), ("wpinned", _ip),
@@ -48,7 +48,7 @@ class NvDiag(C.Structure):
                 ("t", C.c_double), ("maxu", C.c_double), ("maxdu", C.c_double), ("mean_u_sup", C.c_double),
                 ("mean_u_mf", C.c_double), ("kappa_bar", C.c_double), ("kappa_min", C.c_double),
                 ("kappa_max", C.c_double), ("total_length", C.c_double), ("avg_sep", C.c_double),
-                ("tree_eval", C.c_int64)]
+                ("tree_eval", C.c_int64), ("wall_recon_count", C.c_int32), ("pad", C.c_int32)]
 
 
 class NvReconEvent(C.Structure):
@@ -76,7 +76,7 @@ class NvStats(C.Structure):
 EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_state_batch",
            "nv_get_state_batch", "nv_set_time", "nv_set_noise", "nv_ghostp",
            "nv_construct_tree", "nv_pmotion", "nv_velocity", "nv_pinsert", "nv_pclose", "nv_pclose_tree",
-           "nv_precon", "nv_premove", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
+           "nv_precon", "nv_premove", "nv_wall_recon", "nv_enforce_boundary", "nv_empty_tree", "nv_nan_check", "nv_step",
            "nv_get_recon_events", "nv_get_tree", "nv_get_stats", "nv_autotune", "nv_pclose_noalloc",
            "nv_diagnostics", "nv_data_dump", "nv_data_restore", "nv_printf", "nv_print_info", "nv_tree_print",
            "nv_var_write", "nv_var_read", "nv_var_log_write")

@@ -187,8 +187,6 @@ def validate(p: RunParams) -> None:
     for b in (p.boundary_x, p.boundary_y, p.boundary_z):
         if b not in ("periodic", "open", "solid"):
             raise RunFileError("init_setup: incorrect boundary parameter")
-        if b == "solid":
-            raise RunFileError("solid boundaries are not supported by this library")
     if p.velocity not in ("LIA", "BS", "Tree"):
         raise RunFileError('init.mod:init_setup: correct value for "velocity" in run.in has not been set')
     if p.tree_theta > 0 and "open" in (p.boundary_x, p.boundary_y, p.boundary_z):

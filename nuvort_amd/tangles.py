@@ -130,3 +130,87 @@ def noise_table(seed: int, n: int = 100000, spread: float = 0.2) -> np.ndarray:
     noise/wn_snr_*.log files, which hold 100000 values): 1 + spread * N(0, 1),
     seeded."""
     return 1.0 + spread * _rng(seed).normal(size=n)
+
+
+WALL = 0.499999999   # where src/boundary.f90:124-135 puts a pinned point: this fraction of the box
+
+
+def wall_tangle(seed: int, box: float = 0.1, delta: float = 5e-4, nlines: int = 3, nhalf: int = 2,
+                nrings: int = 2) -> FilamentState:
+    """A tangle for solid walls at x = +-box/2 and z = +-box/2 (y periodic).
+
+    * ``nlines`` wavy lines from the +z wall to the -z wall (the shape of the
+      reference's lattice lines, src/initial_cond.f90:296-318): the first
+      point pinned behind on the +z wall (behind = itself, wpinned (0,0,1)),
+      the last pinned in front on the -z wall (infront = itself, wpinned
+      (0,0,-1));
+    * ``nhalf`` half loops standing on the x walls, alternately +x and -x
+      (the shape of src/initial_cond.f90:41-75, in a random plane through the
+      wall normal), both ends pinned to that wall;
+    * ``nrings`` rings, the first one reaching to within delta/5 of the +x
+      wall, which wall_recon (src/reconnection.f90:84-93) cuts there;
+    * one two-point segment lying on the -z wall (pinned behind, then pinned
+      in front), which wall_recon and premove clear.
+
+    Pinned points sit at +-0.499999999 box, where enforce_boundary puts them.
+    Chords are at most 0.98 delta."""
+    g = _rng(seed)
+    L = box
+    segs = []   # (points, pinned ends: None for a closed loop, else (wpinned of the first, of the last))
+    for _ in range(nlines):
+        x0, y0 = g.uniform(-0.3 * L, 0.3 * L, size=2)
+        amp, waves, phi = g.uniform(0.001, 0.003), int(g.integers(1, 4)), g.uniform(0, 2 * math.pi)
+        k = 2.0 * math.pi * waves / L
+        n = math.ceil(2 * WALL * L * math.sqrt(1.0 + (amp * k) ** 2) / (0.98 * delta)) + 1
+        z = WALL * L - 2 * WALL * L * np.arange(n) / (n - 1)
+        pts = np.stack([x0 + amp * np.cos(k * z + phi), y0 + amp * np.sin(k * z + phi), z], 1)
+        segs.append((pts, ((0, 0, 1), (0, 0, -1))))
+    for h in range(nhalf):
+        side = 1 if h % 2 == 0 else -1
+        r = g.uniform(0.008, 0.02)
+        ang = g.uniform(0, 2 * math.pi)
+        d = np.array([0.0, math.cos(ang), math.sin(ang) * 0.3])   # a direction in the wall, mostly along y
+        d /= np.linalg.norm(d)
+        c = np.array([side * WALL * L, g.uniform(-0.3 * L, 0.3 * L), g.uniform(-0.2 * L, 0.2 * L)])
+        n = max(5, math.ceil(math.pi * r / (0.98 * delta)) + 1)
+        th = math.pi * np.arange(n) / (n - 1)
+        pts = c[None, :] + r * np.cos(th)[:, None] * d[None, :]
+        pts[:, 0] = side * WALL * L - side * r * np.sin(th)
+        segs.append((pts, ((side, 0, 0), (side, 0, 0))))
+    for q in range(nrings):
+        r = g.uniform(0.006, 0.012)
+        if q == 0:
+            # in a plane holding the x axis, its +x point delta/5 inside the wall
+            c = np.array([0.5 * L - 0.2 * delta - r, g.uniform(-0.3 * L, 0.3 * L), g.uniform(-0.2 * L, 0.2 * L)])
+            nv = np.array([0.0, math.cos(0.7), math.sin(0.7)])
+        else:
+            c = g.uniform(-0.25 * L, 0.25 * L, size=3)
+            nv = g.normal(size=3)
+        segs.append((ring(c, r, nv, delta, g.uniform(0, 2 * math.pi)), None))
+    zf = -WALL * L
+    segs.append((np.array([[0.01, 0.02, zf], [0.01 + 0.5 * delta, 0.02, zf]]), ((0, 0, -1), (0, 0, -1))))
+    n = sum(len(p) for p, _ in segs)
+    st = FilamentState.empty(n)
+    off = 0
+    for pts, pins in segs:
+        m = len(pts)
+        idx = np.arange(m)
+        st.x[off:off + m] = pts
+        if pins is None:
+            st.infront[off:off + m] = off + (idx + 1) % m + 1
+            st.behind[off:off + m] = off + (idx - 1) % m + 1
+        else:
+            st.infront[off:off + m] = off + idx + 2
+            st.behind[off:off + m] = off + idx
+            st.behind[off] = off + 1                  # pinned behind: behind = itself
+            st.infront[off + m - 1] = off + m         # pinned in front: infront = itself
+            st.pinnedb[off] = 1
+            st.pinnedi[off + m - 1] = 1
+            st.wpinned[off] = pins[0]
+            st.wpinned[off + m - 1] = pins[1]
+        off += m
+    # y is periodic: wrap it alone
+    y = st.x[:, 1]
+    y[y > L / 2] -= L
+    y[y < -L / 2] += L
+    return st

@@ -111,6 +111,38 @@ int nvo_shift_table(const nvo_params *p, double shifts[26][3]) {
     return n;
 }
 
+/* src/boundary.f90:180-214 create_mirror_loop_array: the reflections, i, j, k
+ * nested over -1..1, a direction that is not solid held at 0 */
+int nvo_mirror_table(const nvo_params *p, int32_t reflect[26][3]) {
+    int n = 0;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                int ii, jj, kk;
+                if (p->solid[0]) ii = i; else { if (i != 0) continue; ii = 0; }
+                if (p->solid[1]) jj = j; else { if (j != 0) continue; jj = 0; }
+                if (p->solid[2]) kk = k; else { if (k != 0) continue; kk = 0; }
+                if (ii == 0 && jj == 0 && kk == 0) continue;
+                reflect[n][0] = ii; reflect[n][1] = jj; reflect[n][2] = kk;
+                n++;
+            }
+    return n;
+}
+
+/* the reference's reflection of v in the walls of r, v + 2*abs(r)*(0.5*r*box_size - v)
+ * (src/timestep.f90:221-222, src/tree.f90:345-348) */
+static inline void reflect_point(const double *v, const int32_t *r, const double *box, double *out) {
+    for (int k = 0; k < 3; k++) out[k] = v[k] + (double)(2 * abs(r[k])) * ((0.5 * r[k]) * box[k] - v[k]);
+}
+
+/* the own-image test of biot_savart_mirror and tree_walk_mirror
+ * (src/timestep.f90:215, src/tree.f90:369): sum(wpinned - reflect) == 0 and
+ * sum(wpinned) < 2 (sums of components, as the reference writes them) */
+static inline int own_image(const nvo_state *s, int32_t i, const int32_t *r) {
+    const int32_t *w = &s->wpinned[(i - 1) * 3];
+    return ((w[0] - r[0]) + (w[1] - r[1]) + (w[2] - r[2])) == 0 && (w[0] + w[1] + w[2]) < 2;
+}
+
 /* src/boundary.f90:26-84 get_ghost_p (pinned reflections at :31-38) */
 static void get_ghost_p(const nvo_state *s, const nvo_params *p, int32_t i, double *gin, double *gbe) {
     const int32_t *w = &s->wpinned[(i - 1) * 3];
@@ -323,6 +355,53 @@ static int tree_walk(const nvo_state *s, const nvo_params *p, const nvo_tree *tr
     return NVO_OK;
 }
 
+/* src/tree.f90:321-392 tree_walk_mirror.  theta is width / dist with dist to
+ * the reflected end centroid + circ: the extra correction's value is
+ * overwritten at :356. */
+static int tree_walk_mirror(const nvo_state *s, const nvo_params *p, const nvo_tree *tr, int32_t nd, int32_t i,
+                            const int32_t *refl, double *u, int64_t *evals) {
+    const nvo_node *n = &tr->nodes[nd];
+    if (n->pcount == 0) return NVO_OK;
+    const double *xi = &X(s, i, 0);
+    double ce[3], simg[3], sgi[3];
+    for (int k = 0; k < 3; k++) ce[k] = n->cent[k] + n->circ[k];
+    reflect_point(n->cent, refl, p->box, sgi);
+    reflect_point(ce, refl, p->box, simg);
+    double dist = dist_gen(xi, simg);
+    double theta = n->width / dist;
+    if (n->pcount == 1 || theta < p->tree_theta) {
+        if (n->pcount == 1) {
+            int32_t j = BEH(s, n->leaf_infront);
+            if (s->pinnedi[j - 1]) return NVO_OK;
+            if (own_image(s, i, refl)) {
+                if (s->pinnedb[i - 1] && i == j) return NVO_OK;
+                if (s->pinnedi[i - 1] && j == BEH(s, i)) return NVO_OK;
+            }
+            if (dist < EPS) return NVO_ERR_SINGULAR;
+        }
+        double d[3], e[3];
+        for (int k = 0; k < 3; k++) { d[k] = simg[k] - xi[k]; e[k] = sgi[k] - simg[k]; }
+        double a = dist * dist;
+        double b = 2.0 * dot3(d, e);
+        double c = dist_gen_sq(simg, sgi);
+        if (4.0 * a * c - b * b < EPS) return NVO_OK;
+        double D = 4.0 * a * c - b * b;
+        double ub[3];
+        cross(d, e, ub);
+        double den = (2.0 * PI) * D;
+        for (int k = 0; k < 3; k++) ub[k] = ub[k] * p->quant_circ / den;
+        double fac = (2.0 * c + b) / sqrt(a + b + c) - (b / sqrt(a));
+        for (int k = 0; k < 3; k++) u[k] = u[k] + ub[k] * fac;
+        (*evals)++;
+        return NVO_OK;
+    }
+    for (int c = 0; c < 8; c++) {
+        int rc = tree_walk_mirror(s, p, tr, n->child[c], i, refl, u, evals);
+        if (rc) return rc;
+    }
+    return NVO_OK;
+}
+
 /* src/tree.f90:206-244 closest_tree */
 static void closest_tree(nvo_state *s, const nvo_tree *tr, int32_t nd, int32_t i) {
     const nvo_node *n = &tr->nodes[nd];
@@ -397,6 +476,36 @@ static void biot_savart(const nvo_state *s, const nvo_params *p, int32_t i, cons
     }
 }
 
+/* src/timestep.f90:201-234 biot_savart_mirror: the image segment runs from
+ * the reflected ghost to the reflected point */
+static void biot_savart_mirror(const nvo_state *s, const nvo_params *p, int32_t i, const int32_t *refl, double *u) {
+    const double *xi = &X(s, i, 0);
+    const int own = own_image(s, i, refl);
+    for (int32_t j = 1; j <= s->n; j++) {
+        if ((INF(s, j) == 0) || s->pinnedi[j - 1]) continue;
+        if (own) {
+            if (s->pinnedb[i - 1] && (i == j)) continue;
+            if (s->pinnedi[i - 1] && (j == BEH(s, i))) continue;
+        }
+        double sgi[3], simg[3];
+        reflect_point(&X(s, j, 0), refl, p->box, sgi);
+        reflect_point(&GI(s, j, 0), refl, p->box, simg);
+        double a = dist_gen_sq(xi, simg);
+        double d[3], e[3];
+        for (int k = 0; k < 3; k++) { d[k] = simg[k] - xi[k]; e[k] = sgi[k] - simg[k]; }
+        double b = 2.0 * dot3(d, e);
+        double c = dist_gen_sq(simg, sgi);
+        double D = 4.0 * a * c - b * b;
+        if (D == 0.0) continue;
+        double ub[3];
+        cross(d, e, ub);
+        double den = (2.0 * PI) * D;
+        for (int k = 0; k < 3; k++) ub[k] = ub[k] * p->quant_circ / den;
+        double fac = (2.0 * c + b) / sqrt(a + b + c) - (b / sqrt(a));
+        for (int k = 0; k < 3; k++) u[k] = u[k] + ub[k] * fac;
+    }
+}
+
 /* src/general.f90:46-51: the module's cosh, which hides the intrinsic in
  * normal_fluid (it `use`s general) */
 static double ref_cosh(double x) { return 0.5 * (exp(x) + exp(-x)); }
@@ -444,7 +553,7 @@ static int shift_table_cache(const nvo_params *p, double sh[26][3]) { return nvo
 
 /* src/timestep.f90:49-141 calc_velocity (LIA :60-73, BS :74-98, Tree :99-121) */
 static int calc_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, int32_t i, int nsh,
-                         double sh[26][3], double *u, int64_t *evals) {
+                         double sh[26][3], int nmir, int32_t mir[26][3], double *u, int64_t *evals) {
     double fd[3], fdd[3], beta;
     get_deriv_1(s, i, fd);
     get_deriv_2(s, i, fdd);
@@ -481,6 +590,17 @@ static int calc_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, 
             }
         }
         for (int k = 0; k < 3; k++) u[k] = u[k] + ub[k];
+        /* solid walls: image vortices (:94-98, :116-120) */
+        double um[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < nmir; m++) {
+            if (p->velocity == 1) {
+                biot_savart_mirror(s, p, i, mir[m], um);
+            } else {
+                int rc = tree_walk_mirror(s, p, tr, 0, i, mir[m], um, evals);
+                if (rc) return rc;
+            }
+        }
+        for (int k = 0; k < 3; k++) u[k] = u[k] + um[k];
     }
     for (int k = 0; k < 3; k++) s->u_sup[(i - 1) * 3 + k] = u[k];
     if ((fabs(p->alpha[0]) > EPS) || (fabs(p->alpha[1]) > EPS)) {
@@ -506,6 +626,8 @@ int nvo_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, int32_t 
                  double *u_out) {
     double sh[26][3];
     int nsh = shift_table_cache(p, sh);
+    int32_t mir[26][3];
+    int nmir = nvo_mirror_table(p, mir);
     int err = NVO_OK;
 #ifdef _OPENMP
     if (p->threads > 0) omp_set_num_threads(p->threads);
@@ -516,7 +638,7 @@ int nvo_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr, int32_t 
         int32_t i = first + q;
         if (INF(s, i) == 0) continue;
         double u[3];
-        int rc = calc_velocity(s, p, tr, i, nsh, sh, u, &evals);
+        int rc = calc_velocity(s, p, tr, i, nsh, sh, nmir, mir, u, &evals);
         if (rc) {
 #pragma omp critical
             err = rc;
@@ -752,7 +874,9 @@ int nvo_precon(nvo_state *s, const nvo_params *p, nvo_recon_log *log) {
     return NVO_OK;
 }
 
-/* src/boundary.f90:91-138 enforce_boundary (periodic part) */
+/* src/boundary.f90:91-138 enforce_boundary: the periodic wrap, then a
+ * pinned point back onto its wall (the first component of largest |wpinned|,
+ * maxloc) */
 void nvo_enforce_boundary(nvo_state *s, const nvo_params *p) {
     for (int32_t i = 1; i <= s->n; i++) {
         if (INF(s, i) == 0) continue;
@@ -761,6 +885,49 @@ void nvo_enforce_boundary(nvo_state *s, const nvo_params *p) {
             double L = p->box[k];
             if (X(s, i, k) > (L / 2.0)) X(s, i, k) = X(s, i, k) - L;
             else if (X(s, i, k) < (-L / 2.0)) X(s, i, k) = X(s, i, k) + L;
+        }
+        if (s->pinnedi[i - 1] || s->pinnedb[i - 1]) {
+            const int32_t *w = &s->wpinned[(i - 1) * 3];
+            int pc = 0;
+            for (int k = 1; k < 3; k++)
+                if (abs(w[k]) > abs(w[pc])) pc = k;
+            if (w[pc] > 0) X(s, i, pc) = 0.499999999 * p->box[pc];
+            else if (w[pc] < 0) X(s, i, pc) = -0.499999999 * p->box[pc];
+        }
+    }
+}
+
+/* src/reconnection.f90:66-150 wall_recon: a point within delta/2 of a solid
+ * wall is removed and its neighbours pinned to that wall (behind pinned
+ * infront, infront pinned behind); a point pinned at both ends is cleared,
+ * as is a pinned pair with nothing between. */
+void nvo_wall_recon(nvo_state *s, const nvo_params *p) {
+    for (int32_t i = 1; i <= s->n; i++) {
+        if (INF(s, i) == 0) continue;
+        if (s->pinnedb[i - 1] && s->pinnedi[INF(s, i) - 1]) {
+            clear_particle(s, INF(s, i));
+            clear_particle(s, i);
+        }
+        if (s->pinnedi[i - 1] || s->pinnedb[i - 1]) continue;
+        for (int k = 0; k < 3; k++) {
+            if (!p->solid[k]) continue;
+            int side = 0;
+            if (X(s, i, k) > 0.5 * (p->box[k] - p->delta)) side = 1;
+            else if (X(s, i, k) < -0.5 * (p->box[k] - p->delta)) side = -1;
+            if (!side) continue;
+            int32_t pari = INF(s, i), parb = BEH(s, i);
+            s->pinnedb[pari - 1] = 1;
+            BEH(s, pari) = pari;
+            s->pinnedi[parb - 1] = 1;
+            INF(s, parb) = parb;
+            for (int q = 0; q < 3; q++) {
+                s->wpinned[(pari - 1) * 3 + q] = q == k ? side : 0;
+                s->wpinned[(parb - 1) * 3 + q] = q == k ? side : 0;
+            }
+            clear_particle(s, i);
+            if (s->pinnedi[pari - 1] && s->pinnedb[pari - 1]) clear_particle(s, pari);
+            if (s->pinnedi[parb - 1] && s->pinnedb[parb - 1]) clear_particle(s, parb);
+            s->wall_recon_count = s->wall_recon_count + 1;
         }
     }
 }
@@ -792,6 +959,7 @@ int nvo_step(nvo_state *s, const nvo_params *p, int32_t nsteps, nvo_recon_log *l
             if (p->tree_theta > 0) nvo_pclose_tree(s, p, &tr);
             else nvo_pclose(s, p);
             nvo_precon(s, p, log);
+            nvo_wall_recon(s, p);
             nvo_premove(s, p);
         }
         nvo_enforce_boundary(s, p);

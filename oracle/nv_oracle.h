@@ -31,7 +31,7 @@ typedef struct {
     int32_t itime;
     int32_t recon_count;
     int32_t remove_count;
-    int32_t pad;
+    int32_t wall_recon_count; /* wall_recon's counter (src/reconnection.f90:66-150) */
     double t;
     double *x, *u, *u1, *u2, *u_sup, *u_mf, *gi, *gb; /* [cap][3] */
     int32_t *infront, *behind, *closest;              /* [cap] */
@@ -43,7 +43,7 @@ typedef struct {
 typedef struct {
     double delta, dt, quant_circ, corea;
     double box[3];
-    int32_t periodic[3];          /* 1 = periodic, 0 = open (solid unsupported) */
+    int32_t periodic[3];          /* 1 = periodic, 0 = open or solid */
     int32_t velocity;             /* 0 LIA, 1 BS, 2 Tree */
     double tree_theta;
     int32_t tree_extra_correction;
@@ -58,6 +58,8 @@ typedef struct {
     const double *noise;          /* noisey_xflow table (noise_n values) */
     int32_t noise_n;
     int32_t *noise_counter;       /* 1-based current entry; advanced by every pmotion */
+    int32_t solid[3];             /* 1 = solid walls at +-box/2 in that direction */
+    int32_t pad_solid;
 } nvo_params;
 
 /* One octree node as the reference holds it (src/tree.f90:20-32), stored in
@@ -85,6 +87,7 @@ typedef struct {
 } nvo_recon_log;
 
 int nvo_shift_table(const nvo_params *p, double shifts[26][3]);
+int nvo_mirror_table(const nvo_params *p, int32_t reflect[26][3]);
 void nvo_ghostp(nvo_state *s, const nvo_params *p);
 int nvo_tree_build(const nvo_state *s, const nvo_params *p, nvo_tree *tr);
 void nvo_tree_free(nvo_tree *tr);
@@ -99,6 +102,7 @@ int nvo_precon(nvo_state *s, const nvo_params *p, nvo_recon_log *log);
 void nvo_premove(nvo_state *s, const nvo_params *p);
 void nvo_loop_killer(nvo_state *s, int32_t particle);
 void nvo_enforce_boundary(nvo_state *s, const nvo_params *p);
+void nvo_wall_recon(nvo_state *s, const nvo_params *p);
 int nvo_nan_check(const nvo_state *s);
 int nvo_step(nvo_state *s, const nvo_params *p, int32_t nsteps, nvo_recon_log *log);
 int64_t nvo_tree_eval_count(void);

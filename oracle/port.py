@@ -24,7 +24,7 @@ _ip = C.POINTER(C.c_int32)
 
 class NvoState(C.Structure):
     _fields_ = [("n", C.c_int32), ("cap", C.c_int32), ("itime", C.c_int32), ("recon_count", C.c_int32),
-                ("remove_count", C.c_int32), ("pad", C.c_int32), ("t", C.c_double),
+                ("remove_count", C.c_int32), ("wall_recon_count", C.c_int32), ("t", C.c_double),
                 ("x", _dp), ("u", _dp), ("u1", _dp), ("u2", _dp), ("u_sup", _dp), ("u_mf", _dp),
                 ("gi", _dp), ("gb", _dp), ("infront", _ip), ("behind", _ip), ("closest", _ip),
                 ("closestd", _dp), ("pinnedi", _ip), ("pinnedb", _ip), ("wpinned", _ip)]
@@ -38,7 +38,7 @@ class NvoParams(C.Structure):
                 ("super_velocity", C.c_double * 3), ("normal_fluid_cutoff", C.c_double),
                 ("t_zero_normal_fluid", C.c_double), ("recon_shots", C.c_int32), ("threads", C.c_int32),
                 ("norm_shear_omega", C.c_double), ("noise", _dp), ("noise_n", C.c_int32),
-                ("noise_counter", _ip)]
+                ("noise_counter", _ip), ("solid", C.c_int32 * 3), ("pad_solid", C.c_int32)]
 
 
 class NvoNode(C.Structure):
@@ -76,7 +76,8 @@ def lib():
                            "nvo_ab_update": [S, P], "nvo_pinsert": [S, P], "nvo_pclose": [S, P],
                            "nvo_pclose_tree": [S, P, T], "nvo_precon": [S, P, G], "nvo_premove": [S, P],
                            "nvo_loop_killer": [S, C.c_int32], "nvo_enforce_boundary": [S, P],
-                           "nvo_nan_check": [S], "nvo_step": [S, P, C.c_int32, G]}.items():
+                           "nvo_nan_check": [S], "nvo_step": [S, P, C.c_int32, G],
+                           "nvo_wall_recon": [S, P]}.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
@@ -105,6 +106,7 @@ def params_struct(p: RunParams, threads: int = 0, noise: Noise | None = None) ->
         s.super_velocity[k] = p.super_velocity[k]
     for k, b in enumerate((p.boundary_x, p.boundary_y, p.boundary_z)):
         s.periodic[k] = 1 if b == "periodic" else 0
+        s.solid[k] = 1 if b == "solid" else 0
     s.velocity = {"LIA": 0, "BS": 1, "Tree": 2}[p.velocity]
     s.tree_theta = p.tree_theta
     s.tree_extra_correction = 1 if p.tree_extra_correction else 0
@@ -156,6 +158,7 @@ class PortState:
             a[:] = self.arr[name][:n]
         st.t, st.itime, st.recon_count = self.c.t, self.c.itime, self.c.recon_count
         st.extra["remove_count"] = self.c.remove_count
+        st.extra["wall_recon_count"] = self.c.wall_recon_count
         return st
 
 
@@ -240,6 +243,8 @@ def run_phases(st: FilamentState, p: RunParams, threads: int = 0, noise: Noise |
         out["close"] = ps.to_state()
         L.nvo_precon(ps.ref(), pr, log.ref())
         out["recon"] = ps.to_state()
+        L.nvo_wall_recon(ps.ref(), pr)
+        out["wall"] = ps.to_state()
         L.nvo_premove(ps.ref(), pr)
         out["remove"] = ps.to_state()
     out["events"] = log.events()

@@ -98,6 +98,7 @@ program ref_harness
       call precon
       if (dump_phases==1 .and. istep==1) call dump_state(trim(out_prefix)//'_recon.bin')
       call wall_recon
+      if (dump_phases==1 .and. istep==1) call dump_state(trim(out_prefix)//'_wall.bin')
       call premove
       if (dump_phases==1 .and. istep==1) call dump_state(trim(out_prefix)//'_remove.bin')
     end if
@@ -129,6 +130,7 @@ program ref_harness
   write(71,'(a,es24.16)') 'close_search_s ', t_close
   write(71,'(a,i0)') 'recon_count ', recon_count
   write(71,'(a,i0)') 'remove_count ', remove_count
+  write(71,'(a,i0)') 'wall_recon_count ', wall_recon_count
   write(71,'(a,i0)') 'pcount ', pcount
   close(71)
 

@@ -66,7 +66,8 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
     ``files`` maps file names (relative to ``workdir``) to text written there
     first, such as a noise table.  Returns the dumped states (ghost, tree,
     motion, insert, close, recon, remove for the first step; final after the
-    last), the reconnection log lines and the timing record.  With
+    last; wall after wall_recon), the reconnection log lines and the timing
+    record.  With
     ``expect_fatal`` the run is expected to stop in fatal_error: its exit
     status and output are returned instead.  With ``output`` every step ends
     with the reference's output phase (diagnostics; var.dat, var%04d.log and
@@ -98,7 +99,7 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
     if r.returncode != 0:
         raise RuntimeError("reference build failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-2000:]))
     out = {}
-    for ph in ("ghost", "motion", "insert", "close", "recon", "remove", "final"):
+    for ph in ("ghost", "motion", "insert", "close", "recon", "wall", "remove", "final"):
         p = os.path.join(workdir, "out_%s.bin" % ph)
         if os.path.exists(p):
             out[ph] = read_state(p)

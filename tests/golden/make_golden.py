@@ -34,7 +34,7 @@ from oracle import ref_io  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ("x", "u", "u1", "u2", "u_sup", "u_mf", "ghosti", "ghostb", "infront", "behind", "closest",
-          "closestd", "pinnedi", "pinnedb")
+          "closestd", "pinnedi", "pinnedb", "wpinned")
 
 
 def base_params(**kw) -> RunParams:
@@ -51,7 +51,7 @@ def pack_state(prefix: str, st: FilamentState, d: dict) -> None:
     d["%s__meta" % prefix] = np.array([st.t, st.itime, st.recon_count], np.float64)
 
 
-ALL_PHASES = ("ghost", "motion", "insert", "close", "recon", "remove", "final")
+ALL_PHASES = ("ghost", "motion", "insert", "close", "recon", "wall", "remove", "final")
 
 
 def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, phases=ALL_PHASES,
@@ -279,6 +279,19 @@ def main() -> None:
                  "Tree with a reconnection: 2 steps with the output phase, 2 restarted from var.dat")
     make("tree_deep", deep_tree_state(d5), base_params(velocity="Tree", tree_theta=0.5), 1,
          "Tree theta 0.5 with point pairs 4e-7 and 5e-12 cm apart (octree deeper than 16 and 32 levels)")
+    # solid walls at x = +-L/2 and z = +-L/2, y periodic: pinned lines and
+    # half loops, a ring cut by wall_recon, a pinned pair cleared
+    # (src/timestep.f90:201-234, src/tree.f90:321-392, src/boundary.f90:31-38,
+    # 124-135, src/reconnection.f90:66-150)
+    walls = dict(boundary_x="solid", boundary_z="solid")
+    wst = tangles.wall_tangle(41)
+    make("walls_bs", wst, base_params(**walls), 3, "BS with solid x and z walls: mirror images, pinned points, "
+         "wall reconnection, 3 steps")
+    make("walls_tree", wst, base_params(velocity="Tree", tree_theta=0.5, **walls), 3,
+         "Tree theta 0.5 with solid x and z walls: mirrored tree walks, pinned points, wall reconnection")
+    make("walls_xflow", wst, base_params(normal_velocity="xflow", norm_vel_xflow=0.5, alpha=(0.206, 0.0083),
+                                         boundary_x="solid", boundary_y="solid", boundary_z="solid"), 2,
+         "BS with friction and walls in all three directions (26 mirror images)", ("motion", "wall", "final"))
     dense = tangles.random_rings(21, 24, 0.003, 0.008, 0.025, d5)
     few = ("motion", "close", "final")
     make("dense_bs_10step", dense, base_params(box_size=(0.025, 0.025, 0.025)), 10,

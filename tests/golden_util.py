@@ -13,7 +13,7 @@ from nuvort_amd.state import FilamentState
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STATE_FIELDS = ("x", "u", "u1", "u2", "u_sup", "u_mf", "ghosti", "ghostb", "infront", "behind", "closest",
-                "closestd", "pinnedi", "pinnedb")
+                "closestd", "pinnedi", "pinnedb", "wpinned")
 
 
 def names():
@@ -46,7 +46,9 @@ class Fixture:
         x = self.z["%s__x" % phase]
         st = FilamentState.empty(x.shape[0])
         for f in STATE_FIELDS:
-            getattr(st, f)[:] = self.z["%s__%s" % (phase, f)]
+            key = "%s__%s" % (phase, f)
+            if key in self.z.files:   # (fixtures without walls hold no wpinned)
+                getattr(st, f)[:] = self.z[key]
         meta = self.z["%s__meta" % phase]
         st.t, st.itime, st.recon_count = float(meta[0]), int(meta[1]), int(meta[2])
         return st

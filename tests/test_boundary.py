@@ -62,7 +62,7 @@ def _create(**kw):
     (dict(box_size=(0.1, 0.0, 0.1)), 1),              # src/initial.f90:80 box size <= 0
     (dict(delta=0.0), 1),                             # src/initial.f90:21 delta must be set
     (dict(tree_theta=0.5, boundary_x="open"), 1),     # src/initial.f90:133-136 tree with open boundaries
-    (dict(boundary_y="solid"), 7),                    # walls are not on this path
+    (dict(boundary_y="sideways"), 1),                 # src/initial.f90:37-78 incorrect boundary parameter
     (dict(velocity="Tree"), 1),                       # Tree without tree_theta walks a null tree
 ])
 def test_parameter_checks(kw, code):

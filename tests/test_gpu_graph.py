@@ -182,3 +182,22 @@ def test_recorded_tree_step_outgrowing_its_plan():
     assert ref[0][0].pcount > st.pcount + 3000
     assert got[2]["tree_redone"] >= 1 and got[2]["tree_replays"] >= 2
     assert_same(ref, got)
+
+
+@pytest.mark.parametrize("velocity", ["BS", "Tree"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_recorded_step_with_walls(velocity, prec):
+    """Solid x and z walls (mirror images, pinned points, wall_recon) in the
+    recorded step, against eager launches."""
+    from nuvort_amd import tangles
+    p = params(box_size=(0.1, 0.1, 0.1), boundary_x="solid", boundary_z="solid", velocity=velocity,
+               hip_precision=prec, recon_shots=1)
+    if velocity == "Tree":
+        p.tree_theta = 0.5
+    st = tangles.wall_tangle(41)
+    ref = run(p, st, "eager", [4])
+    got = run(p, st, "graph", [4])
+    assert ref[0][0].extra["wall_recon_count"] > 0
+    assert_same(ref, got)
+    assert np.array_equal(ref[0][0].pinnedi, got[0][0].pinnedi)
+    assert np.array_equal(ref[0][0].wpinned, got[0][0].wpinned)

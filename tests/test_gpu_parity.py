@@ -165,11 +165,24 @@ def test_precon_bit_exact(name):
         assert abs(e[5] - row[4]) <= 4e-16 * max(1.0, abs(row[4]))
 
 
+@pytest.mark.parametrize("name", [n for n in FIXTURES if Fixture(n).has("wall") and Fixture(n).has("recon")])
+def test_wall_recon_bit_exact(name):
+    """wall_recon (src/reconnection.f90:66-150) from the reference's state
+    after precon: points cut at a wall, neighbours pinned, pinned pairs
+    cleared, and the counter."""
+    fx = Fixture(name)
+    s = solver(fx)
+    s.set_state(fx.state("recon"))
+    s.wall_recon()
+    got = s.get_state()
+    assert compare_states(fx.state("wall"), got, fx.state("in").pcount) == []
+
+
 @pytest.mark.parametrize("name", [n for n in FIXTURES if Fixture(n).has("remove")])
 def test_premove_bit_exact(name):
     fx = Fixture(name)
     s = solver(fx)
-    s.set_state(fx.state("recon"))
+    s.set_state(fx.state("wall" if fx.has("wall") else "recon"))
     s.premove()
     got = s.get_state()
     assert compare_states(fx.state("remove"), got, fx.state("in").pcount) == []
@@ -192,5 +205,9 @@ def test_full_run_matches_reference(name):
     assert np.array_equal(got.infront, ref.infront)
     assert np.array_equal(got.behind, ref.behind)
     assert got.recon_count == ref.recon_count
+    assert np.array_equal(got.pinnedi, ref.pinnedi) and np.array_equal(got.pinnedb, ref.pinnedb)
+    assert np.array_equal(got.wpinned, ref.wpinned)
+    if "wall_recon_count" in fx.timing:
+        assert got.extra["wall_recon_count"] == fx.timing["wall_recon_count"]
     act = ref.infront != 0
     assert np.abs(got.x[act] - ref.x[act]).max() <= 1e-10 * fx.params.box_size[0]

@@ -37,9 +37,9 @@ def test_first_step_phases_bit_exact(name):
     nz = fx.noise()
     out = port.run_phases(st, fx.params, threads=2, noise=port.Noise(nz) if nz is not None else None)
     assert out["rc_motion"] == 0
-    for ph in ("ghost", "motion", "insert", "close", "recon", "remove"):
+    for ph in ("ghost", "motion", "insert", "close", "recon", "wall", "remove"):
         if fx.has(ph):
-            searched = ph in ("close", "recon", "remove")
+            searched = ph in ("close", "recon", "wall", "remove")
             assert compare_states(fx.state(ph), out[ph], st.pcount, searched=searched) == [], ph
     tree = fx.tree()
     if tree is not None:
@@ -59,3 +59,26 @@ def test_fixtures_exercise_every_topology_path():
     assert Fixture("recon_bs").recon_log.shape[0] >= 1
     assert Fixture("recon_tree").recon_log.shape[0] >= 1
     assert Fixture("dense_tree_10step").recon_log.shape[0] >= 1
+
+
+def test_wall_fixtures_exercise_the_wall_paths():
+    """The wall fixtures pin points, cut a filament at a wall and clear a
+    pinned pair (wall_recon, src/reconnection.f90:66-150), and their image
+    vortices carry weight: with the walls made open, the oracle's velocities
+    of the same state differ from the reference's by far more than rounding."""
+    import dataclasses
+    for name in ("walls_bs", "walls_tree"):
+        fx = Fixture(name)
+        assert fx.timing["wall_recon_count"] > 0
+        ghost, motion = fx.state("ghost"), fx.state("motion")
+        assert ghost.pinnedi.sum() > 0 and ghost.pinnedb.sum() > 0
+        rec, wall = fx.state("recon"), fx.state("wall")
+        assert np.count_nonzero(rec.infront != wall.infront) > 0
+        cleared = (rec.infront != 0) & (wall.infront == 0) & (rec.pinnedb != 0)
+        assert cleared.any()   # the pinned pair on the -z wall
+        p = dataclasses.replace(fx.params, boundary_x="open", boundary_z="open") if name == "walls_bs" else \
+            dataclasses.replace(fx.params, boundary_x="periodic", boundary_z="periodic")
+        out = port.run_phases(fx.state("in"), p, threads=2)
+        act = ghost.infront != 0
+        du = np.abs(out["motion"].u_sup[act] - motion.u_sup[act]).max() / np.abs(motion.u_sup[act]).max()
+        assert du > 1e-3, (name, du)
