@@ -206,7 +206,7 @@ std::string ts_line(int shots, const nv_diag &d, double delta) {
     put_f(s, d.t, 13, 7);
     put_i(s, d.active, 10, 1);
     put_i(s, d.recon_count, 13, 1);
-    put_i(s, 0, 13, 1);   // wall_recon_count: no solid walls
+    put_i(s, d.wall_recon_count, 13, 1);
     put_f(s, d.avg_sep / delta, 7, 4);
     put_f(s, d.total_length, 13, 6);
     put_f(s, d.maxu, 13, 5);
