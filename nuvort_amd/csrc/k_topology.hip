@@ -492,9 +492,15 @@ __device__ __forceinline__ int staged_index(const int *strow, int k, int m, int 
 // that visit inserts -- a revisit tests |x - ghost_infront| > delta, and the
 // ghost it will see is the one just computed (a row's infront ghost changes
 // only when the row itself inserts).
+// The new point in a slot above r and below the loop's end is visited by
+// the reference loop too: its |x - ghost_infront| (fixed from now on) joins
+// total_length (src/line.f90:38-39), which the pre-filter summed only over
+// the points present at the start; `len` collects it.
 __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn &in, int slot, const InsOut &o,
-                             InsIn *st, unsigned char *stale, const int *strow, int k, int m, Dirty &d, int n_old) {
+                             InsIn *st, unsigned char *stale, const int *strow, int k, int m, Dirty &d, int n_old,
+                             double &len) {
     const int fi = in.fi, bb = in.bb;
+    if (slot > r && slot < n_old) len += sqrt(dist2_gen(o.xn, o.gs_i));
     // the staged rows are ascending and those not yet visited are above r:
     // find fi and bb among them (usually fi is the next one; either can be
     // far from r, e.g. a new slot); their plans are then stale
@@ -565,6 +571,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         ei_sh = e0;
     }
     int ei = e0;   // used by thread 0 only
+    double len = 0.0;   // thread 0: total_length of the points inserted above the cursor
     for (int cs = 0; cs <= ncand; cs += INS_CHUNK) {
         const int m = min(INS_CHUNK, ncand - cs);
         __syncthreads();
@@ -612,7 +619,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 const int slot = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                 if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
                 if (slot < 0) continue;
-                if (!after_insert(s, p, r, in, slot, o, st, stale, strow, k, m, d, n_old)) {
+                if (!after_insert(s, p, r, in, slot, o, st, stale, strow, k, m, d, n_old, len)) {
                     atomicOr(&ctl->status, NVK_CAPACITY);
                     stop_sh = 1;
                     break;
@@ -636,7 +643,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                     const int slot = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
                     if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
                     if (slot < 0) continue;
-                    if (!after_insert(s, p, r, in, slot, o, st, stale, strow, 0, 0, d, n_old)) {
+                    if (!after_insert(s, p, r, in, slot, o, st, stale, strow, 0, 0, d, n_old, len)) {
                         atomicOr(&ctl->status, NVK_CAPACITY);
                         break;
                     }
@@ -652,6 +659,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         // the empty slots left at the end
         atomicAdd(&ctl->n_old, tg.n[tk]);
         atomicAdd(&ctl->empty_after, e1 - ei);
+        if (len != 0.0) atomicAdd(&ctl->total_length, len);
         tg.n[tk] = nl_sh;
     }
 }

@@ -1644,7 +1644,7 @@ int nv_data_dump(void *h, const char *path) {
     if (int rc = read_ctl(c)) return rc;
     const int n = c->tn[0];
     std::vector<double> v3((size_t)8 * 3 * n), v1((size_t)3 * n);
-    std::vector<int32_t> i1((size_t)3 * n);
+    std::vector<int32_t> i1((size_t)3 * n), ip((size_t)5 * n);
     nv_state hs{};
     hs.capacity = n;
     double **vp[] = {&hs.x, &hs.u, &hs.u1, &hs.u2, &hs.u_sup, &hs.u_mf, &hs.ghosti, &hs.ghostb};
@@ -1655,6 +1655,9 @@ int nv_data_dump(void *h, const char *path) {
     hs.infront = i1.data();
     hs.behind = i1.data() + n;
     hs.closest = i1.data() + 2 * (size_t)n;
+    hs.pinnedi = ip.data();
+    hs.pinnedb = ip.data() + n;
+    hs.wpinned = ip.data() + 2 * (size_t)n;
     if (int rc = get_batch(c, nullptr, &hs)) return rc;
     const int rc = nv_var_write(path, &hs);
     if (rc) return fail(c, rc, std::string("nv_data_dump: cannot write ") + path);

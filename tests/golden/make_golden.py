@@ -277,6 +277,11 @@ def main() -> None:
                  "BS with friction: 3 steps with the output phase (var.dat, var%04d.log, ts.log), 2 restarted")
     make_restart("restart_tree", crossing_rings(1.0e-4, d5), base_params(velocity="Tree", tree_theta=0.5), 2, 2,
                  "Tree with a reconnection: 2 steps with the output phase, 2 restarted from var.dat")
+    make_restart("restart_remesh", remesh_case(d5), base_params(), 2, 1,
+                 "stretched/compressed rings with empty slots: pinsert's total length over reused slots, "
+                 "the output phase over 2 steps, 1 restarted")
+    make_restart("restart_walls", tangles.wall_tangle(41), base_params(boundary_x="solid", boundary_z="solid"), 2, 1,
+                 "solid x and z walls: pinned points in var.dat, wall_recon_count in ts.log, 1 step restarted")
     make("tree_deep", deep_tree_state(d5), base_params(velocity="Tree", tree_theta=0.5), 1,
          "Tree theta 0.5 with point pairs 4e-7 and 5e-12 cm apart (octree deeper than 16 and 32 levels)")
     # solid walls at x = +-L/2 and z = +-L/2, y periodic: pinned lines and

@@ -25,9 +25,9 @@ pytestmark = pytest.mark.gpu
 COMPARE = ("x", "u", "u1", "u2", "u_sup", "u_mf", "infront", "behind", "closest", "closestd")
 
 
-def solver(p):
+def solver(p, exact_order=False):
     from nuvort_amd.engine import FilamentSolver
-    return FilamentSolver(p, check=False)
+    return FilamentSolver(p, check=False, exact_order=exact_order)
 
 
 @pytest.mark.parametrize("name", restart_names())
@@ -102,21 +102,26 @@ def test_diagnostics_and_output_files(name, tmp_path):
 
 @pytest.mark.parametrize("name", restart_names())
 def test_dump_restore_continue_bit_exact(name, tmp_path):
+    """FP64 in the reference's summation order: the direct sum's chunking
+    otherwise follows the point count at the time a step is recorded, which
+    a restart from a later state sees differently (the sums then agree to
+    rounding, not bit for bit)."""
     fx = Fixture(name)
     p = dataclasses.replace(fx.params, hip_precision="fp64")
-    whole = solver(p)
+    exact = p.velocity == "BS"
+    whole = solver(p, exact)
     whole.set_state(fx.state("in"))
     whole.step(fx.nsteps + 2)
     want = whole.get_state()
     whole.close()
-    first = solver(p)
+    first = solver(p, exact)
     first.set_state(fx.state("in"))
     first.step(fx.nsteps)
     first.diagnostics()
     path = tmp_path / "var.dat"
     first.data_dump(str(path))
     first.close()
-    second = solver(p)
+    second = solver(p, exact)
     second.data_restore(str(path))
     second.step(2)
     got = second.get_state()
