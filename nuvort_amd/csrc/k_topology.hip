@@ -534,6 +534,12 @@ __device__ bool after_insert(const DState &s, const Phys &p, int r, const InsIn 
     return true;
 }
 
+// the reference's loops, which the staged kernels hand over to when their
+// queue of rows to revisit is full (defined with the solid-wall kernels)
+__device__ int pinsert_loop(const DState &s, const Phys &p, const Tangles &tg, DCtl *ctl, int tk, int from, int end,
+                            bool len_at_visit);
+__device__ void premove_loop(const DState &s, const Phys &p, const Tangles &tg, int tk, int from);
+
 constexpr int INS_CHUNK = 128;
 
 // One workgroup per tangle.  Candidates are taken INS_CHUNK at a time: every
@@ -548,7 +554,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
     __shared__ InsPlan pl[INS_CHUNK];
     __shared__ unsigned char stale[INS_CHUNK];   // the staged inputs changed after the plan
     __shared__ int strow[INS_CHUNK];
-    __shared__ int nl_sh, stop_sh, ei_sh;
+    __shared__ int nl_sh, stop_sh, ei_sh, res_sh;
     __shared__ int est[INS_CHUNK];   // the next entries of the empty list
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
@@ -569,6 +575,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         nl_sh = tg.n[tk];
         stop_sh = 0;
         ei_sh = e0;
+        res_sh = -1;
     }
     int ei = e0;   // used by thread 0 only
     double len = 0.0;   // thread 0: total_length of the points inserted above the cursor
@@ -620,7 +627,9 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
                 if (slot < 0) continue;
                 if (!after_insert(s, p, r, in, slot, o, st, stale, strow, k, m, d, n_old, len)) {
-                    atomicOr(&ctl->status, NVK_CAPACITY);
+                    // no room for a row to revisit: the plain loop takes
+                    // over after r
+                    res_sh = r + 1;
                     stop_sh = 1;
                     break;
                 }
@@ -644,7 +653,7 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                     if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); break; }
                     if (slot < 0) continue;
                     if (!after_insert(s, p, r, in, slot, o, st, stale, strow, 0, 0, d, n_old, len)) {
-                        atomicOr(&ctl->status, NVK_CAPACITY);
+                        res_sh = r + 1;
                         break;
                     }
                 }
@@ -658,9 +667,12 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         // avg_sep's denominator (src/line.f90:95): pcount at the start less
         // the empty slots left at the end
         atomicAdd(&ctl->n_old, tg.n[tk]);
-        atomicAdd(&ctl->empty_after, e1 - ei);
         if (len != 0.0) atomicAdd(&ctl->total_length, len);
         tg.n[tk] = nl_sh;
+        if (res_sh < 0)
+            atomicAdd(&ctl->empty_after, e1 - ei);
+        else   // the rest of the loop as the reference writes it (it counts the empty slots at its end)
+            pinsert_loop(s, p, tg, ctl, tk, res_sh, n_old, false);
     }
 }
 
@@ -1012,7 +1024,7 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
     __shared__ RmIn st[RM_CHUNK];
     __shared__ int strow[RM_CHUNK];
     __shared__ ModSet mod;
-    __shared__ int stop_sh;
+    __shared__ int stop_sh, res_sh;
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk];
@@ -1023,6 +1035,7 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
     if (tid == 0) {
         d.hd = d.nd = 0;
         stop_sh = 0;
+        res_sh = -1;
     }
     int nrm = 0;   // removals (the serial thread's)
     for (int cs = 0; cs < ncand; cs += RM_CHUNK) {
@@ -1077,14 +1090,21 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
             if (!small) {
                 const int b = in.beh;
                 if (b > r && !dirty_push(d, b)) {
-                    atomicOr(&ctl->status, NVK_CAPACITY);
+                    // no room for a row to revisit: the plain loop takes
+                    // over after r
+                    res_sh = r + 1;
                     stop_sh = 1;
                     break;
                 }
             }
         }
     }
-    if (tid == 0 && nrm) tg.remove[tk] += nrm;
+    __syncthreads();
+    if (tid == 0) {
+        if (nrm) tg.remove[tk] += nrm;
+        // the rest of the loop as the reference writes it
+        if (res_sh >= 0) premove_loop(s, p, tg, tk, res_sh);
+    }
 }
 
 // ------------------------------------------------------ brute close search
@@ -1419,22 +1439,20 @@ __device__ void loop_killer_pinned(const DState &s, int particle, int n) {
     }
 }
 
-// src/line.f90:18-96 pinsert
-__global__ void k_pinsert_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
-    if (ctl->abort || threadIdx.x != 0) return;
-    const int tk = blockIdx.x, base = tg.base[tk], tcap = tg.cap[tk];
+// src/line.f90:18-96 pinsert's loop over rows [from, end) of tangle tk
+// (end: the tangle's pcount at entry), from the tangle's current count; the
+// lengths of the rows it visits join total_length when len_at_visit, else
+// only those of points it inserts into empty slots it has still to visit (a
+// loop resumed after the staged kernel, whose pre-filter counted the rest).
+// Returns the tangle's count after it.
+__device__ int pinsert_loop(const DState &s, const Phys &p, const Tangles &tg, DCtl *ctl, int tk, int from, int end,
+                            bool len_at_visit) {
+    const int base = tg.base[tk], tcap = tg.cap[tk];
     int n = tg.n[tk];
     const int cap = s.cap;
-    int active = 0;
-    for (int r = base; r < base + n; ++r) active += s.infront[r] != 0;
-    if (active == 0) {
-        atomicOr(&ctl->status, NVK_ZERO_LENGTH);
-        return;
-    }
-    const int old = n;
     double total = 0.0;
     int fe = base;   // the first empty row (slots are only filled here)
-    for (int r = base; r < base + old; ++r) {
+    for (int r = from; r < end; ++r) {
         if (s.infront[r] == 0) continue;
         if (s.pin[r] & NV_PINNED_I) continue;
         double x[3], gi[3], gb[3];
@@ -1445,7 +1463,7 @@ __global__ void k_pinsert_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
             gb[k] = s.gb[k * cap + r];
         }
         const double disti = dist_gen(x[0], x[1], x[2], gi[0], gi[1], gi[2]);
-        total = total + disti;
+        if (len_at_visit) total = total + disti;
         if (!(disti > p.delta)) continue;
         while (fe < base + n && s.infront[fe] != 0) ++fe;
         int slot;
@@ -1499,13 +1517,33 @@ __global__ void k_pinsert_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
         store_ghosts(s, p, fi);
         s.infront[r] = slot + 1;
         store_ghosts(s, p, r);
+        if (!len_at_visit && slot > r && slot < end) {
+            double g[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[k] = s.gi[k * cap + slot];
+            total = total + dist_gen(xn[0], xn[1], xn[2], g[0], g[1], g[2]);
+        }
     }
     int empties = 0;
     for (int r = base; r < base + n; ++r) empties += s.infront[r] == 0;
-    atomicAdd(&ctl->total_length, total);
-    atomicAdd(&ctl->n_old, old);
+    if (total != 0.0) atomicAdd(&ctl->total_length, total);
     atomicAdd(&ctl->empty_after, empties);
     tg.n[tk] = n;
+    return n;
+}
+
+// pinsert as the reference's loop, one thread per tangle (solid walls)
+__global__ void k_pinsert_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    const int tk = blockIdx.x, base = tg.base[tk], n = tg.n[tk];
+    int active = 0;
+    for (int r = base; r < base + n; ++r) active += s.infront[r] != 0;
+    if (active == 0) {
+        atomicOr(&ctl->status, NVK_ZERO_LENGTH);
+        return;
+    }
+    atomicAdd(&ctl->n_old, n);
+    pinsert_loop(s, p, tg, ctl, tk, base, base + n, true);
 }
 
 // src/reconnection.f90:11-61 precon (the tests of recon_tests)
@@ -1582,13 +1620,12 @@ __global__ void k_wall_recon(DState s, Phys p, Tangles tg, DCtl *ctl) {
     if (nwall) tg.wall[tk] += nwall;
 }
 
-// src/line.f90:102-130 premove
-__global__ void k_premove_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
-    if (ctl->abort || threadIdx.x != 0) return;
-    const int tk = blockIdx.x, base = tg.base[tk], n = tg.n[tk];
+// src/line.f90:102-130 premove's loop over rows [from, base + pcount) of tangle tk
+__device__ void premove_loop(const DState &s, const Phys &p, const Tangles &tg, int tk, int from) {
+    const int base = tg.base[tk], n = tg.n[tk];
     const int cap = s.cap;
     int nrm = 0;
-    for (int r = base; r < base + n; ++r) {
+    for (int r = from; r < base + n; ++r) {
         if (s.infront[r] == 0) continue;
         if (s.pin[r] & NV_PINNED_I) continue;
         const int inf = s.infront[r] - 1;
@@ -1609,6 +1646,12 @@ __global__ void k_premove_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
         }
     }
     if (nrm) tg.remove[tk] += nrm;
+}
+
+// premove as the reference's loop, one thread per tangle (solid walls)
+__global__ void k_premove_walls(DState s, Phys p, Tangles tg, DCtl *ctl) {
+    if (ctl->abort || threadIdx.x != 0) return;
+    premove_loop(s, p, tg, blockIdx.x, tg.base[blockIdx.x]);
 }
 
 }  // namespace

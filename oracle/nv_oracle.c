@@ -700,15 +700,23 @@ int nvo_pmotion(nvo_state *s, const nvo_params *p, const nvo_tree *tr) {
 
 /* ------------------------------------------------------------ topology --- */
 /* src/line.f90:18-96 pinsert */
+/* total_length and avg_sep of the last pinsert (module globals of the
+ * reference, written by pinsert and printed by the diagnostics) */
+static double g_total_length, g_avg_sep;
+double nvo_total_length(void) { return g_total_length; }
+double nvo_avg_sep(void) { return g_avg_sep; }
+
 int nvo_pinsert(nvo_state *s, const nvo_params *p) {
     int32_t active = 0;
     for (int32_t i = 1; i <= s->n; i++) active += INF(s, i) > 0;
     if (active == 0) return NVO_ERR_ZERO_LENGTH;
     int32_t old = s->n;
+    double total = 0.0;
     for (int32_t i = 1; i <= old; i++) {
         if (INF(s, i) == 0) continue;
         if (s->pinnedi[i - 1]) continue;
         double disti = dist_gen(&X(s, i, 0), &GI(s, i, 0));
+        total = total + disti;
         if (disti > p->delta) {
             int32_t slot = 0;
             for (int32_t q = 1; q <= s->n; q++)
@@ -759,6 +767,11 @@ int nvo_pinsert(nvo_state *s, const nvo_params *p) {
             get_ghost_p(s, p, i, &GI(s, i, 0), &GB(s, i, 0));
         }
     }
+    /* :95: the denominator is old_pcount less the empty slots at the end */
+    int32_t empty = 0;
+    for (int32_t q = 1; q <= s->n; q++) empty += INF(s, q) == 0;
+    g_total_length = total;
+    g_avg_sep = total / (double)(old - empty);
     return NVO_OK;
 }
 

@@ -96,6 +96,8 @@ int nvo_velocity(nvo_state *s, const nvo_params *p, const nvo_tree *tr,
 int nvo_pmotion(nvo_state *s, const nvo_params *p, const nvo_tree *tr);
 void nvo_ab_update(nvo_state *s, const nvo_params *p);
 int nvo_pinsert(nvo_state *s, const nvo_params *p);
+double nvo_total_length(void); /* total_length of the last nvo_pinsert (src/line.f90:29,38) */
+double nvo_avg_sep(void);      /* avg_sep of the last nvo_pinsert (src/line.f90:95) */
 void nvo_pclose(nvo_state *s, const nvo_params *p);
 void nvo_pclose_tree(nvo_state *s, const nvo_params *p, const nvo_tree *tr);
 int nvo_precon(nvo_state *s, const nvo_params *p, nvo_recon_log *log);

@@ -82,6 +82,9 @@ def lib():
             fn.argtypes = args
             fn.restype = C.c_int
         L.nvo_tree_eval_count.restype = C.c_int64
+        for name in ("nvo_total_length", "nvo_avg_sep"):
+            getattr(L, name).argtypes = []
+            getattr(L, name).restype = C.c_double
         _lib = L
     return _lib
 
@@ -213,6 +216,44 @@ def step(st: FilamentState, p: RunParams, nsteps: int, threads: int = 0, noise: 
     log = ReconLog()
     rc = lib().nvo_step(ps.ref(), C.byref(params_struct(p, threads, noise)), nsteps, log.ref())
     return rc, ps.to_state(), log.events()
+
+
+def step_lengths(st: FilamentState, p: RunParams, nsteps: int, threads: int = 0):
+    """nsteps steps one at a time; pinsert's (total_length, avg_sep) after each
+    (src/line.f90:29-95, the values the diagnostics print) and the final state."""
+    L = lib()
+    ps = PortState(st)
+    log = ReconLog()
+    pr = C.byref(params_struct(p, threads))
+    out = []
+    for _ in range(nsteps):
+        rc = L.nvo_step(ps.ref(), pr, 1, log.ref())
+        if rc:
+            raise RuntimeError("oracle step failed with %d" % rc)
+        out.append((L.nvo_total_length(), L.nvo_avg_sep()))
+    return out, ps.to_state()
+
+
+def ghostp(st: FilamentState, p: RunParams) -> FilamentState:
+    """src/boundary.f90:10-19 ghostp (the ghosts of every point) on a copy of st."""
+    ps = PortState(st)
+    lib().nvo_ghostp(ps.ref(), C.byref(params_struct(p)))
+    return ps.to_state()
+
+
+def pinsert(st: FilamentState, p: RunParams):
+    """src/line.f90:18-96 on a copy of st: (status, state, total_length, avg_sep)."""
+    L = lib()
+    ps = PortState(st)
+    rc = L.nvo_pinsert(ps.ref(), C.byref(params_struct(p)))
+    return rc, ps.to_state(), L.nvo_total_length(), L.nvo_avg_sep()
+
+
+def premove(st: FilamentState, p: RunParams):
+    """src/line.f90:102-130 on a copy of st."""
+    ps = PortState(st)
+    lib().nvo_premove(ps.ref(), C.byref(params_struct(p)))
+    return ps.to_state()
 
 
 def run_phases(st: FilamentState, p: RunParams, threads: int = 0, noise: Noise | None = None) -> dict:

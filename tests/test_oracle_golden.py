@@ -4,7 +4,7 @@ fixture's full run must be bit-identical."""
 import numpy as np
 import pytest
 
-from golden_util import Fixture, compare_states, names
+from golden_util import Fixture, compare_states, names, restart_names
 from oracle import port
 
 FIXTURES = names()
@@ -82,3 +82,20 @@ def test_wall_fixtures_exercise_the_wall_paths():
         act = ghost.infront != 0
         du = np.abs(out["motion"].u_sup[act] - motion.u_sup[act]).max() / np.abs(motion.u_sup[act]).max()
         assert du > 1e-3, (name, du)
+
+
+@pytest.mark.parametrize("name", restart_names())
+def test_pinsert_total_length_matches_reference_diagnostics(name):
+    """pinsert's total_length and avg_sep (src/line.f90:29-95) after every
+    step, against the reference's own diag.log lines (columns 8 and 9) for the
+    same steps: restart_remesh inserts into empty slots the loop has still to
+    visit and leaves empty slots at its end, restart_walls skips pinned
+    points.  Serial sums in the reference's row order: equal to the printed
+    17 digits."""
+    fx = Fixture(name)
+    rows = [[float(v) for v in ln.split()] for ln in fx.file("diag.log").decode().splitlines() if ln.strip()]
+    got, _ = port.step_lengths(fx.state("in"), fx.params, int(fx.z["nsteps"]), threads=1)
+    assert len(got) == len(rows)
+    for (tl, sep), row in zip(got, rows):
+        assert tl == pytest.approx(row[7], rel=1e-15, abs=0)
+        assert sep == pytest.approx(row[8], rel=1e-15, abs=0)
