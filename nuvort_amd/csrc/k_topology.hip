@@ -408,6 +408,47 @@ __device__ void plan_insertion(const Phys &p, const InsIn &in, InsPlan &pl) {
 // One visit of the reference loop at row r from its plan: returns the slot
 // used (or -1 when r does not insert); -2 when the tangle's rows are
 // exhausted.
+// The stores of a planned insertion at r (old infront fi) into slot; r's
+// and fi's ghosts unless a later visit stores them again.
+__device__ __forceinline__ void write_insertion(const DState &s, int r, int fi, const InsPlan &pl, int slot,
+                                                bool ghosts_r = true, bool ghosts_fi = true) {
+    const int cap = s.cap;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.x[k * cap + slot] = pl.xn[k];
+        s.u[k * cap + slot] = pl.un[k];
+        s.u1[k * cap + slot] = 0.0;
+        s.u2[k * cap + slot] = 0.0;
+        s.gi[k * cap + slot] = pl.gs_i[k];
+        s.gb[k * cap + slot] = pl.gs_b[k];
+        // fi before r, the reference's refresh order (they coincide only for
+        // a one-point loop)
+        if (ghosts_fi) {
+            s.gi[k * cap + fi] = pl.gf_i[k];
+            s.gb[k * cap + fi] = pl.gf_b[k];
+        }
+        if (ghosts_r) {
+            s.gi[k * cap + r] = pl.gr_i[k];
+            s.gb[k * cap + r] = pl.gr_b[k];
+        }
+    }
+    s.behind[slot] = r + 1;
+    s.infront[slot] = fi + 1;
+    s.behind[fi] = slot + 1;
+    s.infront[r] = slot + 1;
+}
+
+// the slot of a grown row, as the reference's new array element: zeroed
+__device__ __forceinline__ void clear_grown(const DState &s, int slot) {
+    const int cap = s.cap;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.usup[k * cap + slot] = 0.0;
+        s.umf[k * cap + slot] = 0.0;
+    }
+    clear_row(s, slot);
+}
+
 __device__ int insert_at(const DState &s, int r, const InsIn &in, const InsPlan &pl, const int *empty, int &ei,
                          int e1, int &nl, int base, int tcap, InsOut &out, const int *est, int es0, int es1) {
     const int cap = s.cap;
@@ -421,33 +462,9 @@ __device__ int insert_at(const DState &s, int r, const InsIn &in, const InsPlan 
         if (nl >= tcap) return -2;
         slot = base + nl;
         ++nl;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            s.usup[k * cap + slot] = 0.0;
-            s.umf[k * cap + slot] = 0.0;
-        }
-        clear_row(s, slot);
+        clear_grown(s, slot);
     }
-    const int fi = in.fi;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        s.x[k * cap + slot] = pl.xn[k];
-        s.u[k * cap + slot] = pl.un[k];
-        s.u1[k * cap + slot] = 0.0;
-        s.u2[k * cap + slot] = 0.0;
-        s.gi[k * cap + slot] = pl.gs_i[k];
-        s.gb[k * cap + slot] = pl.gs_b[k];
-        // fi before r, the reference's refresh order (they coincide only for
-        // a one-point loop)
-        s.gi[k * cap + fi] = pl.gf_i[k];
-        s.gb[k * cap + fi] = pl.gf_b[k];
-        s.gi[k * cap + r] = pl.gr_i[k];
-        s.gb[k * cap + r] = pl.gr_b[k];
-    }
-    s.behind[slot] = r + 1;
-    s.infront[slot] = fi + 1;
-    s.behind[fi] = slot + 1;
-    s.infront[r] = slot + 1;
+    write_insertion(s, r, in.fi, pl, slot);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         out.xn[k] = pl.xn[k];
@@ -556,6 +573,13 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
     __shared__ int strow[INS_CHUNK];
     __shared__ int nl_sh, stop_sh, ei_sh, res_sh;
     __shared__ int est[INS_CHUNK];   // the next entries of the empty list
+    __shared__ unsigned long long gomask[INS_CHUNK / 64];
+    __shared__ double lenv[INS_CHUNK];
+    __shared__ unsigned char lenq[INS_CHUNK];
+    __shared__ unsigned char fin[INS_CHUNK];     // the plan is final
+    __shared__ int skipg[INS_CHUNK];             // ghosts a later visit stores again (1: r's, 2: fi's)
+    __shared__ int slotv[INS_CHUNK];
+    __shared__ int pushv[2 * INS_CHUNK];          // revisits queued after the chunk's stores
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk], tcap = tg.cap[tk];
@@ -592,9 +616,150 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
         }
         if (es0 + tid < es1) est[tid] = empty[es0 + tid];
         __syncthreads();
+        // The chunk's insertions at once.  When every row queued for a
+        // revisit among the chunk's rows is one of its candidates, the visits
+        // insert where the staged plans say (whether a row inserts depends
+        // on its own position and its infront's ghost, which no other
+        // insertion moves), so the k-th insertion takes the k-th empty slot,
+        // then new rows.  An insertion at r changes the staged inputs of its
+        // old infront fi and behind bb (fi: ghosts and behind; bb: its
+        // infront's infront): a later candidate that is fi or bb of an
+        // earlier insertion is planned again, in rounds, once that insertion
+        // is final.  Where two insertions store the ghosts of the same row
+        // (r's own, and those of an earlier or later insertion's fi), the
+        // later visit's stores are kept.  Revisits of rows beyond the chunk
+        // are queued after its stores.  A chunk whose insertions would queue
+        // a revisit among its own rows, or that runs out of room, is staged
+        // again and made one visit at a time.
+        const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
+        // rows queued for a revisit below limit that are not candidates of
+        // the chunk (a queued candidate is visited once, as a candidate)
+        bool qin = false;
+        for (int q = d.hd + tid; q < d.nd && d.v[q] < limit; q += INS_CHUNK)
+            qin = qin || staged_index(strow, 0, m, d.v[q]) == m;
+        if (!__syncthreads_or(qin) && !stop_sh) {
+            const bool go = tid < m && pl[tid].go;
+            const unsigned long long bm = __ballot(go);
+            const int lane = tid & 63;
+            if (lane == 0) gomask[tid >> 6] = bm;
+            if (tid < INS_CHUNK) {
+                lenq[tid] = 0;
+                skipg[tid] = 0;
+            }
+            __syncthreads();
+            int kq = __popcll(bm & ((1ull << lane) - 1ull)), total = 0;
+#pragma unroll
+            for (int w = 0; w < INS_CHUNK / 64; ++w) {
+                const int c = __popcll(gomask[w]);
+                total += c;
+                if (w < (tid >> 6)) kq += c;
+            }
+            const int navail = es1 - es0;   // staged empty slots (all that are left when < INS_CHUNK)
+            bool bad = false;
+            int slot = -1, r = -1, predb = -1, predf = -1;
+            if (go) {
+                r = strow[tid];
+                if (kq < navail) {
+                    slot = est[kq];
+                } else {
+                    const int a = kq - navail;
+                    if (nl_sh + a >= tcap) bad = true;
+                    else slot = base + nl_sh + a;
+                }
+                // earlier insertions of the chunk whose fi (predb) or bb (predf) this row is
+                const int bb = st[tid].bb, fi = st[tid].fi;
+                const int qb = bb < r ? staged_index(strow, 0, tid, bb) : tid;
+                if (qb < tid && pl[qb].go) predb = qb;
+                const int qf = fi < r ? staged_index(strow, 0, tid, fi) : tid;
+                if (qf < tid && pl[qf].go) predf = qf;
+                // the ghosts stored last: a later insertion whose fi is r
+                // stores r's ghosts after r's own visit; a later insertion
+                // at fi stores fi's after this visit
+                if (predf >= 0) atomicOr(&skipg[predf], 1);   // predf's own ghosts
+                if (predb >= 0) atomicOr(&skipg[predb], 2);   // predb's fi ghosts
+            }
+            slotv[tid] = slot;
+            fin[tid] = !(predb >= 0 || predf >= 0);
+            // plan again, in rounds, the insertions whose inputs earlier ones change
+            bool more = __syncthreads_or(!fin[tid]);
+            for (int it = 0; more && it < 16; ++it) {
+                const bool ready = !fin[tid] && (predb < 0 || fin[predb]) && (predf < 0 || fin[predf]);
+                __syncthreads();
+                if (ready) {
+                    if (predb >= 0) {   // this row is predb's old infront
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            st[tid].gi[c] = pl[predb].gf_i[c];
+                            st[tid].gb[c] = pl[predb].gf_b[c];
+                            st[tid].xb[c] = pl[predb].xn[c];
+                        }
+                        st[tid].bb = slotv[predb];
+                    }
+                    if (predf >= 0) {   // this row is predf's old behind
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) st[tid].xff[c] = pl[predf].xn[c];
+                    }
+                    plan_insertion(p, st[tid], pl[tid]);
+                }
+                __syncthreads();
+                if (ready) fin[tid] = 1;
+                more = __syncthreads_or(!fin[tid]);
+            }
+            bad = bad || more;
+            // revisits (after_insert's): of a row beyond the chunk, queued
+            // after its stores; of a row among the chunk's, one visit at a time
+            int pushf = -1, pushs = -1;
+            if (go && !more) {
+                const int fi = st[tid].fi;
+                const double lim2 = p.delta * p.delta * (1.0 - 1.0e-9);
+                if (fi > r && fi < n_old && staged_index(strow, tid + 1, m, fi) == m &&
+                    dist2_gen(st[tid].xf, pl[tid].gf_i) > lim2)
+                    pushf = fi;
+                if (slot > r && slot < n_old && dist2_gen(pl[tid].xn, pl[tid].gs_i) > lim2) pushs = slot;
+                if ((pushf >= 0 && pushf < limit) || (pushs >= 0 && pushs < limit)) bad = true;
+            }
+            if (tid < INS_CHUNK) {
+                pushv[2 * tid] = pushf;
+                pushv[2 * tid + 1] = pushs;
+            }
+            if (!__syncthreads_or(bad)) {
+                if (go) {
+                    if (kq >= navail) clear_grown(s, slot);
+                    write_insertion(s, r, st[tid].fi, pl[tid], slot, !(skipg[tid] & 1), !(skipg[tid] & 2));
+                    if (slot > r && slot < n_old) {
+                        lenq[tid] = 1;
+                        lenv[tid] = sqrt(dist2_gen(pl[tid].xn, pl[tid].gs_i));
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    while (d.hd < d.nd && d.v[d.hd] < limit) ++d.hd;   // visited as candidates
+                    // total_length in visit order
+                    for (int q = 0; q < m; ++q)
+                        if (lenq[q]) len += lenv[q];
+                    ei = es0 + min(total, navail);
+                    ei_sh = ei;
+                    nl_sh += max(0, total - navail);
+                    for (int q = 0; q < 2 * m; ++q) {
+                        if (pushv[q] >= 0 && !dirty_push(d, pushv[q])) {
+                            // no room: the plain loop takes over at the next chunk's rows
+                            res_sh = limit;
+                            stop_sh = 1;
+                            break;
+                        }
+                    }
+                }
+                continue;
+            }
+            // staged again for the visits one at a time
+            if (tid < m) {
+                load_insertion(s, strow[tid], st[tid]);
+                plan_insertion(p, st[tid], pl[tid]);
+            }
+            __syncthreads();
+        }
         if (tid == 0 && !stop_sh) {
             // rows below the next chunk's first candidate are this chunk's
-            const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
             int nl = nl_sh;
             int k = 0;
             while (true) {
@@ -1013,6 +1178,37 @@ __device__ __forceinline__ bool rm_loop_small(int r, const RmIn &in) {
 
 constexpr int RM_CHUNK = 256;
 
+// The rows whose infront a chunk's removals change (r) or clear (r's
+// infront), each with the index q in the chunk of the first removal that
+// does: an LDS hash of RM_HASH keys and values (RM_CHUNK - q, so that the
+// first removal is the largest value), laid over a ModSet's slots, all -1
+// when empty.
+constexpr int RM_HASH = MOD_SLOTS / 2;
+
+__device__ __forceinline__ void rm_hash_put(int *keys, int *vals, int row, int q) {
+    unsigned h = ((unsigned)row * 2654435761u) & (RM_HASH - 1);
+    for (int t = 0; t < RM_HASH; ++t) {
+        const int old = atomicCAS(&keys[h], -1, row);
+        if (old == -1 || old == row) {
+            atomicMax(&vals[h], RM_CHUNK - q);
+            return;
+        }
+        h = (h + 1) & (RM_HASH - 1);
+    }
+}
+
+// the first removal of the chunk that changes row's infront, or RM_CHUNK
+__device__ __forceinline__ int rm_hash_get(const int *keys, const int *vals, int row) {
+    unsigned h = ((unsigned)row * 2654435761u) & (RM_HASH - 1);
+    for (int t = 0; t < RM_HASH; ++t) {
+        const int k = keys[h];
+        if (k == row) return RM_CHUNK - vals[h];
+        if (k == -1) break;
+        h = (h + 1) & (RM_HASH - 1);
+    }
+    return RM_CHUNK;
+}
+
 // The reference's loop over the candidates in row order (merged with the
 // dirty rows), RM_CHUNK candidates at a time: the workgroup loads every
 // candidate's inputs and test at once; the serial thread uses a staged visit
@@ -1038,6 +1234,7 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
         res_sh = -1;
     }
     int nrm = 0;   // removals (the serial thread's)
+    int *hkeys = mod.v, *hvals = mod.v + RM_HASH;
     for (int cs = 0; cs < ncand; cs += RM_CHUNK) {
         const int m = min(RM_CHUNK, ncand - cs);
         __syncthreads();
@@ -1048,10 +1245,51 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
             load_rm(s, p, strow[tid], st[tid]);
         }
         __syncthreads();
-        if (tid != 0 || stop_sh) continue;
+        if (stop_sh) continue;
         // rows below the next chunk's first candidate are this chunk's; the
         // last chunk also takes every dirty row left
         const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
+        // Independent removals at once.  When no row queued for a revisit
+        // lies among the chunk's rows, its removals can be made together if
+        // none changes what a later visit of the chunk reads: the infront of
+        // its r, inf, ii (and iii, whose infront is iv, when it removes).  A
+        // removal changes infront(r) and clears inf.  It also sets behind(ii)
+        // to r, which a later visit of ii reads only to decide whether to
+        // queue a revisit of its behind, and r, being visited first, lies
+        // below ii then: no revisit.  A removal that would queue a revisit
+        // (behind(r) > r) or clear a loop keeps the chunk serial.
+        if (d.hd == d.nd || d.v[d.hd] >= limit) {
+            const int r = tid < m ? strow[tid] : -1;
+            RmIn in;
+            in.go = 0;
+            if (tid < m) in = st[tid];
+            bool bad = false;
+            if (in.go) {
+                bad = rm_loop_small(r, in) || in.beh > r;
+                rm_hash_put(hkeys, hvals, r, tid);
+                rm_hash_put(hkeys, hvals, in.inf, tid);
+            }
+            __syncthreads();
+            if (tid < m && !bad) {
+                const int rows[4] = {r, in.inf, in.ii, in.go ? in.iii : -1};
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (rows[q] >= 0 && rm_hash_get(hkeys, hvals, rows[q]) < tid) bad = true;
+            }
+            if (!__syncthreads_or(bad)) {
+                if (in.go) {
+                    s.behind[in.ii] = r + 1;
+                    s.infront[r] = in.ii + 1;
+                    clear_row(s, in.inf);
+                }
+                const int cnt = __syncthreads_count(in.go);
+                if (tid == 0) nrm += cnt;
+                continue;
+            }
+            for (int q = tid; q < MOD_SLOTS; q += RM_CHUNK) mod.v[q] = -1;
+            __syncthreads();
+        }
+        if (tid != 0) continue;
         bool tracked = true;   // the modified set is complete
         int k = 0;
         while (true) {

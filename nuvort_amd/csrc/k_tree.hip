@@ -909,6 +909,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
 #pragma unroll
         for (int t = 0; t < TPT; ++t) {
             const unsigned long long mvis = wave_ballot(cur >= lskip[t]);
+            mfar[t] = 0ull;
+            mneed[t] = 0ull;
+            // a block none of whose lanes visits this node (all accepted an
+            // ancestor) changes nothing here: skip its work (a wave-uniform
+            // branch; about a sixth of the iterations for each block)
+            if (TPT > 1 && mvis == 0ull) continue;
             nvis += (unsigned long long)__popcll(mvis);
 #pragma unroll
             for (int k = 0; k < 3; ++k) v[t][k] = nd.cent[k] - xs[t][k];

@@ -3,7 +3,7 @@ staged kernels' queue holds (8,192 per tangle). Then the kernel's serial
 thread continues with the reference's plain loop from the next row
 (src/line.f90:18-96, :102-130). The results must match the oracle bit for
 bit: links, slots, positions, pcount and counts. For pinsert the check also
-covers total_length and avg_sep.
+covers total_length and avg_sep, to 1e-12 since they are summed in parallel.
 
 * pinsert: 9,000 points on rings whose chords are 2.5 delta, followed by
   12,000 empty rows. Each point inserts into an empty slot the loop has still
@@ -81,8 +81,8 @@ def test_pinsert_past_the_queue():
     assert ref.pcount - st.pcount + 12000 > 2 * QUEUE
     got, d = gpu(st, "pinsert")
     assert compare_states(ref, got, st.pcount, searched=False) == []
-    assert d["total_length"] == pytest.approx(tl, rel=1e-13, abs=0)
-    assert d["avg_sep"] == pytest.approx(sep, rel=1e-13, abs=0)
+    assert d["total_length"] == pytest.approx(tl, rel=1e-12, abs=0)   # a parallel sum
+    assert d["avg_sep"] == pytest.approx(sep, rel=1e-12, abs=0)
 
 
 def test_premove_past_the_queue():
