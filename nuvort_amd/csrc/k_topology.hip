@@ -579,7 +579,10 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
     __shared__ unsigned char fin[INS_CHUNK];     // the plan is final
     __shared__ int skipg[INS_CHUNK];             // ghosts a later visit stores again (1: r's, 2: fi's)
     __shared__ int slotv[INS_CHUNK];
-    __shared__ int pushv[2 * INS_CHUNK];          // revisits queued after the chunk's stores
+    __shared__ int pushv[2 * INS_CHUNK];          // revisits queued after a segment's stores
+    __shared__ int a_sh, vmin_sh, cut_sh;         // the segment's first candidate, first queued row, end
+    __shared__ unsigned char rel[INS_CHUNK];      // inputs to load again after a segment
+    __shared__ unsigned long long lenmask[INS_CHUNK / 64], pushmask[INS_CHUNK / 64];
     const int tid = threadIdx.x;
     const int tk = blockIdx.x;
     const int base = tg.base[tk], tcap = tg.cap[tk];
@@ -614,69 +617,85 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
             plan_insertion(p, st[tid], pl[tid]);
             stale[tid] = 0;
         }
+        if (tid < INS_CHUNK) rel[tid] = 0;
         if (es0 + tid < es1) est[tid] = empty[es0 + tid];
         __syncthreads();
-        // The chunk's insertions at once.  When every row queued for a
-        // revisit among the chunk's rows is one of its candidates, the visits
-        // insert where the staged plans say (whether a row inserts depends
-        // on its own position and its infront's ghost, which no other
-        // insertion moves), so the k-th insertion takes the k-th empty slot,
-        // then new rows.  An insertion at r changes the staged inputs of its
-        // old infront fi and behind bb (fi: ghosts and behind; bb: its
-        // infront's infront): a later candidate that is fi or bb of an
-        // earlier insertion is planned again, in rounds, once that insertion
-        // is final.  Where two insertions store the ghosts of the same row
-        // (r's own, and those of an earlier or later insertion's fi), the
-        // later visit's stores are kept.  Revisits of rows beyond the chunk
-        // are queued after its stores.  A chunk whose insertions would queue
-        // a revisit among its own rows, or that runs out of room, is staged
-        // again and made one visit at a time.
         const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
-        // rows queued for a revisit below limit that are not candidates of
-        // the chunk (a queued candidate is visited once, as a candidate)
-        bool qin = false;
-        for (int q = d.hd + tid; q < d.nd && d.v[q] < limit; q += INS_CHUNK)
-            qin = qin || staged_index(strow, 0, m, d.v[q]) == m;
-        if (!__syncthreads_or(qin) && !stop_sh) {
-            const bool go = tid < m && pl[tid].go;
-            const unsigned long long bm = __ballot(go);
-            const int lane = tid & 63;
-            if (lane == 0) gomask[tid >> 6] = bm;
+        if (tid == 0) a_sh = 0;
+        // The chunk in segments of candidates whose visits are made at once,
+        // between visits made one at a time.
+        //
+        // Whether a candidate inserts depends on its own position and its
+        // infront's ghost, which no other insertion moves; so, while no row
+        // queued for a revisit comes first, the visits of candidates a, a+1,
+        // ... insert where their plans say, and the k-th insertion takes the
+        // next empty slot but k, then new rows.  An insertion at r changes
+        // the inputs of its old infront fi and behind bb (fi: ghosts and
+        // behind; bb: its infront's infront): a later candidate that is fi or
+        // bb of an earlier insertion of the segment is planned again, in
+        // rounds, once that insertion is final.  Where two insertions store
+        // the ghosts of the same row (r's own, and those of another
+        // insertion's fi), the later visit's stores are kept.  The segment
+        // ends before the first queued row that is not a candidate, before a
+        // candidate that finds no room or whose plan is not final after the
+        // rounds, and after a candidate that queues a revisit of a row of the
+        // chunk; revisits of later rows are queued after its stores.  The
+        // serial thread then makes the visits one at a time until the next
+        // visit is a candidate again.
+        while (true) {
+            __syncthreads();
+            const int a = a_sh;
+            if (stop_sh) break;
+            if (tid == 0) {
+                vmin_sh = 0x7fffffff;
+                cut_sh = m;
+            }
             if (tid < INS_CHUNK) {
                 lenq[tid] = 0;
                 skipg[tid] = 0;
+                pushv[2 * tid] = pushv[2 * tid + 1] = -1;
             }
             __syncthreads();
-            int kq = __popcll(bm & ((1ull << lane) - 1ull)), total = 0;
-#pragma unroll
-            for (int w = 0; w < INS_CHUNK / 64; ++w) {
-                const int c = __popcll(gomask[w]);
-                total += c;
-                if (w < (tid >> 6)) kq += c;
+            // the first queued row of the chunk's rows that is not one of its candidates
+            for (int q = d.hd + tid; q < d.nd && d.v[q] < limit; q += INS_CHUNK)
+                if (staged_index(strow, a, m, d.v[q]) == m) atomicMin(&vmin_sh, d.v[q]);
+            const bool in_seg0 = tid >= a && tid < m;
+            if (in_seg0 && stale[tid]) {
+                plan_insertion(p, st[tid], pl[tid]);
+                stale[tid] = 0;
             }
-            const int navail = es1 - es0;   // staged empty slots (all that are left when < INS_CHUNK)
-            bool bad = false;
+            __syncthreads();
+            const int vmin = vmin_sh;
+            const int hi = vmin == 0x7fffffff ? m : lower_index(strow, a, m, vmin);
+            const bool in_seg = tid >= a && tid < hi;
+            const bool go = in_seg && pl[tid].go;
+            const unsigned long long bm = __ballot(go);
+            const int lane = tid & 63;
+            if (lane == 0) gomask[tid >> 6] = bm;
+            __syncthreads();
+            int kq = __popcll(bm & ((1ull << lane) - 1ull));
+#pragma unroll
+            for (int w = 0; w < INS_CHUNK / 64; ++w)
+                if (w < (tid >> 6)) kq += __popcll(gomask[w]);
+            const int e_next = ei_sh, n_next = nl_sh;
+            bool stop_here = false;   // this candidate cannot be visited in the segment
             int slot = -1, r = -1, predb = -1, predf = -1;
             if (go) {
                 r = strow[tid];
-                if (kq < navail) {
-                    slot = est[kq];
+                const int e = e_next + kq;
+                if (e < e1) {
+                    slot = (e >= es0 && e < es1) ? est[e - es0] : empty[e];
                 } else {
-                    const int a = kq - navail;
-                    if (nl_sh + a >= tcap) bad = true;
-                    else slot = base + nl_sh + a;
+                    const int a2 = e - e1;
+                    if (n_next + a2 >= tcap) stop_here = true;
+                    else slot = base + n_next + a2;
                 }
-                // earlier insertions of the chunk whose fi (predb) or bb (predf) this row is
+                // earlier insertions of the segment whose fi (predb) or bb (predf) this row is
                 const int bb = st[tid].bb, fi = st[tid].fi;
-                const int qb = bb < r ? staged_index(strow, 0, tid, bb) : tid;
+                const int qb = bb < r ? staged_index(strow, a, tid, bb) : tid;
                 if (qb < tid && pl[qb].go) predb = qb;
-                const int qf = fi < r ? staged_index(strow, 0, tid, fi) : tid;
+                const int qf = fi < r ? staged_index(strow, a, tid, fi) : tid;
                 if (qf < tid && pl[qf].go) predf = qf;
-                // the ghosts stored last: a later insertion whose fi is r
-                // stores r's ghosts after r's own visit; a later insertion
-                // at fi stores fi's after this visit
-                if (predf >= 0) atomicOr(&skipg[predf], 1);   // predf's own ghosts
-                if (predb >= 0) atomicOr(&skipg[predb], 2);   // predb's fi ghosts
             }
             slotv[tid] = slot;
             fin[tid] = !(predb >= 0 || predf >= 0);
@@ -705,102 +724,146 @@ __global__ __launch_bounds__(INS_CHUNK) void k_pinsert_serial(DState s, Phys p, 
                 if (ready) fin[tid] = 1;
                 more = __syncthreads_or(!fin[tid]);
             }
-            bad = bad || more;
-            // revisits (after_insert's): of a row beyond the chunk, queued
-            // after its stores; of a row among the chunk's, one visit at a time
+            stop_here = stop_here || !fin[tid];
+            // revisits (after_insert's)
             int pushf = -1, pushs = -1;
-            if (go && !more) {
+            if (go && !stop_here) {
                 const int fi = st[tid].fi;
                 const double lim2 = p.delta * p.delta * (1.0 - 1.0e-9);
                 if (fi > r && fi < n_old && staged_index(strow, tid + 1, m, fi) == m &&
                     dist2_gen(st[tid].xf, pl[tid].gf_i) > lim2)
                     pushf = fi;
                 if (slot > r && slot < n_old && dist2_gen(pl[tid].xn, pl[tid].gs_i) > lim2) pushs = slot;
-                if ((pushf >= 0 && pushf < limit) || (pushs >= 0 && pushs < limit)) bad = true;
             }
-            if (tid < INS_CHUNK) {
+            if (in_seg && stop_here) atomicMin(&cut_sh, tid);
+            if ((pushf >= 0 && pushf < limit) || (pushs >= 0 && pushs < limit)) atomicMin(&cut_sh, tid + 1);
+            __syncthreads();
+            const int seg_end = min(cut_sh, hi);
+            const bool mine = go && tid < seg_end;
+            if (mine) {
+                if (predf >= 0) atomicOr(&skipg[predf], 1);   // predf's own ghosts, stored again by this visit
+                if (predb >= 0) atomicOr(&skipg[predb], 2);   // predb's fi ghosts: this row's own
                 pushv[2 * tid] = pushf;
                 pushv[2 * tid + 1] = pushs;
             }
-            if (!__syncthreads_or(bad)) {
-                if (go) {
-                    if (kq >= navail) clear_grown(s, slot);
-                    write_insertion(s, r, st[tid].fi, pl[tid], slot, !(skipg[tid] & 1), !(skipg[tid] & 2));
-                    if (slot > r && slot < n_old) {
-                        lenq[tid] = 1;
-                        lenv[tid] = sqrt(dist2_gen(pl[tid].xn, pl[tid].gs_i));
-                    }
+            __syncthreads();
+            if (mine) {
+                if (e_next + kq >= e1) clear_grown(s, slot);
+                const int fi = st[tid].fi, bb = st[tid].bb;
+                write_insertion(s, r, fi, pl[tid], slot, !(skipg[tid] & 1), !(skipg[tid] & 2));
+                if (slot > r && slot < n_old) {
+                    lenq[tid] = 1;
+                    lenv[tid] = sqrt(dist2_gen(pl[tid].xn, pl[tid].gs_i));
                 }
-                __syncthreads();
-                if (tid == 0) {
-                    while (d.hd < d.nd && d.v[d.hd] < limit) ++d.hd;   // visited as candidates
-                    // total_length in visit order
-                    for (int q = 0; q < m; ++q)
-                        if (lenq[q]) len += lenv[q];
-                    ei = es0 + min(total, navail);
-                    ei_sh = ei;
-                    nl_sh += max(0, total - navail);
-                    for (int q = 0; q < 2 * m; ++q) {
-                        if (pushv[q] >= 0 && !dirty_push(d, pushv[q])) {
-                            // no room: the plain loop takes over at the next chunk's rows
-                            res_sh = limit;
-                            stop_sh = 1;
-                            break;
-                        }
-                    }
-                }
-                continue;
+                // candidates after the segment whose inputs this insertion changes
+                const int qf = fi > r ? staged_index(strow, seg_end, m, fi) : m;
+                if (qf < m) rel[qf] = 1;
+                const int qb = bb > r ? staged_index(strow, seg_end, m, bb) : m;
+                if (qb < m) rel[qb] = 1;
             }
-            // staged again for the visits one at a time
-            if (tid < m) {
-                load_insertion(s, strow[tid], st[tid]);
-                plan_insertion(p, st[tid], pl[tid]);
+            // and those the rounds planned again on insertions not made
+            if (tid >= seg_end && tid < hi && (predb >= 0 || predf >= 0)) rel[tid] = 1;
+            // per wave: the segment's insertions, those whose length counts,
+            // those that queue revisits
+            // (ballots of the whole wave, outside any branch)
+            const unsigned long long mins = __ballot(mine);
+            const unsigned long long mlen = __ballot(lenq[tid] != 0);
+            const unsigned long long mpush = __ballot(mine && (pushf >= 0 || pushs >= 0));
+            if (lane == 0) {
+                gomask[tid >> 6] = (unsigned long long)__popcll(mins);
+                lenmask[tid >> 6] = mlen;
+                pushmask[tid >> 6] = mpush;
             }
             __syncthreads();
-        }
-        if (tid == 0 && !stop_sh) {
-            // rows below the next chunk's first candidate are this chunk's
-            int nl = nl_sh;
-            int k = 0;
-            while (true) {
-                const int rc = k < m ? strow[k] : 0x7fffffff;
-                const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
-                const int r = rc < rd ? rc : rd;
-                if (r == 0x7fffffff || r >= limit) break;
-                InsIn in;
-                InsPlan plan;
-                const bool staged = rc == r;
-                if (rc == r) ++k;
-                if (rd == r) ++d.hd;
-                if (staged && stale[k - 1]) {
-                    in = st[k - 1];
-                    plan_insertion(p, in, plan);
-                } else if (staged) {
-                    // a planned visit: insert_at and after_insert read only
-                    // these of its inputs
-                    in.fi = st[k - 1].fi;
-                    in.bb = st[k - 1].bb;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) in.xf[c] = st[k - 1].xf[c];
-                    plan = pl[k - 1];
-                } else {
-                    load_insertion(s, r, in);
-                    plan_insertion(p, in, plan);
-                }
-                InsOut o;
-                const int slot = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
-                if (slot == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
-                if (slot < 0) continue;
-                if (!after_insert(s, p, r, in, slot, o, st, stale, strow, k, m, d, n_old, len)) {
-                    // no room for a row to revisit: the plain loop takes
-                    // over after r
-                    res_sh = r + 1;
-                    stop_sh = 1;
-                    break;
-                }
+            // their inputs from memory, which now holds the segment's insertions
+            if (tid < m && rel[tid]) {
+                load_insertion(s, strow[tid], st[tid]);
+                stale[tid] = 1;
+                rel[tid] = 0;
             }
-            nl_sh = nl;
-            ei_sh = ei;
+            if (tid == 0) {
+                int total = 0;
+#pragma unroll
+                for (int w = 0; w < INS_CHUNK / 64; ++w) total += (int)gomask[w];
+                // queued rows visited as candidates of the segment
+                if (seg_end > a)
+                    while (d.hd < d.nd && d.v[d.hd] <= strow[seg_end - 1]) ++d.hd;
+                // total_length in visit order
+                for (int w = 0; w < INS_CHUNK / 64; ++w)
+                    for (unsigned long long b = lenmask[w]; b; b &= b - 1) len += lenv[64 * w + __ffsll(b) - 1];
+                const int from_list = min(total, max(0, e1 - e_next));
+                ei = e_next + from_list;
+                ei_sh = ei;
+                nl_sh = n_next + (total - from_list);
+                bool room = true;
+                for (int w = 0; w < INS_CHUNK / 64 && room; ++w)
+                    for (unsigned long long b = pushmask[w]; b && room; b &= b - 1) {
+                        const int q = 64 * w + __ffsll(b) - 1;
+                        room = (pushv[2 * q] < 0 || dirty_push(d, pushv[2 * q])) &&
+                               (pushv[2 * q + 1] < 0 || dirty_push(d, pushv[2 * q + 1]));
+                    }
+                if (!room) {
+                    // no room: the plain loop takes over after the segment
+                    res_sh = strow[seg_end - 1] + 1;
+                    stop_sh = 1;
+                }
+                a_sh = seg_end;
+            }
+            __syncthreads();
+            if (stop_sh) break;
+            const int a2 = a_sh;
+            const int rd0 = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+            if (a2 >= m && rd0 >= limit) break;   // the chunk is done
+            const bool serial = a2 == a || (rd0 < limit && (a2 >= m || rd0 < strow[a2]));
+            if (!serial) continue;
+            if (tid == 0) {
+                // one visit at a time, until the next visit is a candidate
+                int nl = nl_sh;
+                int k = a2;
+                bool visited = false;
+                while (true) {
+                    const int rc = k < m ? strow[k] : 0x7fffffff;
+                    const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+                    const int r = rc < rd ? rc : rd;
+                    if (r == 0x7fffffff || r >= limit) break;
+                    if (visited && rc <= rd) break;
+                    visited = true;
+                    InsIn in;
+                    InsPlan plan;
+                    const bool staged = rc == r;
+                    if (rc == r) ++k;
+                    if (rd == r) ++d.hd;
+                    if (staged && stale[k - 1]) {
+                        in = st[k - 1];
+                        plan_insertion(p, in, plan);
+                    } else if (staged) {
+                        // a planned visit: insert_at and after_insert read only
+                        // these of its inputs
+                        in.fi = st[k - 1].fi;
+                        in.bb = st[k - 1].bb;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) in.xf[c] = st[k - 1].xf[c];
+                        plan = pl[k - 1];
+                    } else {
+                        load_insertion(s, r, in);
+                        plan_insertion(p, in, plan);
+                    }
+                    InsOut o;
+                    const int slot2 = insert_at(s, r, in, plan, empty, ei, e1, nl, base, tcap, o, est, es0, es1);
+                    if (slot2 == -2) { atomicOr(&ctl->status, NVK_CAPACITY); stop_sh = 1; break; }
+                    if (slot2 < 0) continue;
+                    if (!after_insert(s, p, r, in, slot2, o, st, stale, strow, k, m, d, n_old, len)) {
+                        // no room for a row to revisit: the plain loop takes
+                        // over after r
+                        res_sh = r + 1;
+                        stop_sh = 1;
+                        break;
+                    }
+                }
+                nl_sh = nl;
+                ei_sh = ei;
+                a_sh = k;
+            }
         }
         if (cs + m >= ncand) {
             // the last chunk: dirty rows beyond it are visited from memory
