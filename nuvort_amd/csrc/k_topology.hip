@@ -1272,19 +1272,53 @@ __device__ __forceinline__ int rm_hash_get(const int *keys, const int *vals, int
     return RM_CHUNK;
 }
 
+// loop_killer (src/reconnection.f90:155-182): clears particle's loop when it
+// has fewer than 4 points (the count stops at 4, which decides the same)
+__device__ void loop_killer_plain(const DState &s, int particle, int n) {
+    int counter = 1, next = particle;
+    for (int it = 1; it <= n; ++it) {
+        next = s.infront[next] - 1;
+        if (next == particle) break;
+        if (next < 0) return;
+        counter = counter + 1;
+        if (counter >= 4) return;
+    }
+    next = particle;
+    for (int it = 1; it <= n; ++it) {
+        const int store_next = s.infront[next] - 1;
+        clear_row(s, next);
+        next = store_next;
+        if (next == particle || next < 0) break;
+    }
+}
+
 // The reference's loop over the candidates in row order (merged with the
-// dirty rows), RM_CHUNK candidates at a time: the workgroup loads every
-// candidate's inputs and test at once; the serial thread uses a staged visit
-// unless one of its rows (r, inf, ii) was modified by a removal of this chunk,
-// and then reloads it.
+// rows queued for a revisit), RM_CHUNK candidates at a time, in segments of
+// candidates whose visits are made at once, between visits made one at a
+// time.
+//
+// A removal at r changes infront(r), clears inf = infront(r), and sets
+// behind(ii) to r.  The visits of a segment are made at once when none reads
+// what an earlier removal of the segment changes: the infront of its r,
+// inf, ii (and iii, whose infront is iv, when it removes); behind(ii) is read
+// only by a later visit of ii, to decide whether to queue its behind, and r,
+// visited first, lies below ii: no revisit.  The segment ends before the
+// first queued row that is not a candidate, before a visit that reads what
+// an earlier removal of the segment changed or that clears a loop, and after
+// a removal that queues a revisit of a row of the chunk; revisits of later
+// rows are queued after its stores.  The serial thread then makes the visits
+// one at a time, from memory, until the next visit is a candidate again; the
+// next segment loads its inputs again.
 __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl) {
     if (ctl->abort) return;
     __shared__ Dirty d;
     __shared__ RmIn st[RM_CHUNK];
     __shared__ int strow[RM_CHUNK];
-    __shared__ ModSet mod;
-    __shared__ int stop_sh, res_sh;
-    const int tid = threadIdx.x;
+    __shared__ int hkeys[RM_HASH], hvals[RM_HASH];
+    __shared__ int pushv[RM_CHUNK];
+    __shared__ unsigned long long nmask[RM_CHUNK / 64], pmask[RM_CHUNK / 64];
+    __shared__ int stop_sh, res_sh, a_sh, vmin_sh, cut_sh;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int tk = blockIdx.x;
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
@@ -1296,108 +1330,135 @@ __global__ __launch_bounds__(RM_CHUNK) void k_premove_serial(DState s, Phys p, T
         stop_sh = 0;
         res_sh = -1;
     }
-    int nrm = 0;   // removals (the serial thread's)
-    int *hkeys = mod.v, *hvals = mod.v + RM_HASH;
+    int nrm = 0;   // removals (thread 0's count)
     for (int cs = 0; cs < ncand; cs += RM_CHUNK) {
         const int m = min(RM_CHUNK, ncand - cs);
         __syncthreads();
-        for (int q = tid; q < MOD_SLOTS; q += RM_CHUNK) mod.v[q] = -1;
-        if (tid == 0) mod.n = 0;
         if (tid < m) {
             strow[tid] = cl[cs + tid];
             load_rm(s, p, strow[tid], st[tid]);
         }
-        __syncthreads();
-        if (stop_sh) continue;
         // rows below the next chunk's first candidate are this chunk's; the
-        // last chunk also takes every dirty row left
+        // last chunk also takes every queued row left
         const int limit = cs + m < ncand ? cl[cs + m] : 0x7fffffff;
-        // Independent removals at once.  When no row queued for a revisit
-        // lies among the chunk's rows, its removals can be made together if
-        // none changes what a later visit of the chunk reads: the infront of
-        // its r, inf, ii (and iii, whose infront is iv, when it removes).  A
-        // removal changes infront(r) and clears inf.  It also sets behind(ii)
-        // to r, which a later visit of ii reads only to decide whether to
-        // queue a revisit of its behind, and r, being visited first, lies
-        // below ii then: no revisit.  A removal that would queue a revisit
-        // (behind(r) > r) or clear a loop keeps the chunk serial.
-        if (d.hd == d.nd || d.v[d.hd] >= limit) {
-            const int r = tid < m ? strow[tid] : -1;
+        if (tid == 0) a_sh = 0;
+        bool fresh = true;   // the staged inputs are current
+        while (true) {
+            __syncthreads();
+            const int a = a_sh;
+            if (stop_sh) break;
+            if (tid == 0) {
+                vmin_sh = 0x7fffffff;
+                cut_sh = m;
+            }
+            for (int q = tid; q < RM_HASH; q += RM_CHUNK) hkeys[q] = hvals[q] = -1;
+            pushv[tid] = -1;
+            if (!fresh && tid >= a && tid < m) load_rm(s, p, strow[tid], st[tid]);
+            __syncthreads();
+            // the first queued row of the chunk's rows that is not one of its candidates
+            for (int q = d.hd + tid; q < d.nd && d.v[q] < limit; q += RM_CHUNK)
+                if (staged_index(strow, a, m, d.v[q]) == m) atomicMin(&vmin_sh, d.v[q]);
+            const bool in0 = tid >= a && tid < m;
             RmIn in;
             in.go = 0;
-            if (tid < m) in = st[tid];
-            bool bad = false;
-            if (in.go) {
-                bad = rm_loop_small(r, in) || in.beh > r;
+            in.inf = in.ii = in.beh = in.iii = in.iv = -1;
+            if (in0) in = st[tid];
+            const int r = in0 ? strow[tid] : -1;
+            if (in0 && in.go) {
                 rm_hash_put(hkeys, hvals, r, tid);
                 rm_hash_put(hkeys, hvals, in.inf, tid);
             }
             __syncthreads();
-            if (tid < m && !bad) {
+            const int vmin = vmin_sh;
+            const int hi = vmin == 0x7fffffff ? m : lower_index(strow, a, m, vmin);
+            const bool in_seg = tid >= a && tid < hi;
+            int cut = 0x7fffffff, push = -1;
+            if (in_seg) {
                 const int rows[4] = {r, in.inf, in.ii, in.go ? in.iii : -1};
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (rows[q] >= 0 && rm_hash_get(hkeys, hvals, rows[q]) < tid) bad = true;
-            }
-            if (!__syncthreads_or(bad)) {
+                    if (rows[q] >= 0 && rm_hash_get(hkeys, hvals, rows[q]) < tid) cut = tid;
                 if (in.go) {
-                    s.behind[in.ii] = r + 1;
-                    s.infront[r] = in.ii + 1;
-                    clear_row(s, in.inf);
+                    if (rm_loop_small(r, in)) {
+                        cut = tid;
+                    } else if (in.beh > r) {
+                        push = in.beh;
+                        if (in.beh < limit) cut = min(cut, tid + 1);
+                    }
                 }
-                const int cnt = __syncthreads_count(in.go);
-                if (tid == 0) nrm += cnt;
-                continue;
             }
-            for (int q = tid; q < MOD_SLOTS; q += RM_CHUNK) mod.v[q] = -1;
+            if (cut != 0x7fffffff) atomicMin(&cut_sh, cut);
             __syncthreads();
-        }
-        if (tid != 0) continue;
-        bool tracked = true;   // the modified set is complete
-        int k = 0;
-        while (true) {
-            const int rc = k < m ? strow[k] : 0x7fffffff;
-            const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
-            const int r = rc < rd ? rc : rd;
-            if (r == 0x7fffffff || r >= limit) break;
-            const bool staged = rc == r;
-            if (staged) ++k;
-            if (rd == r) ++d.hd;
-            RmIn in;
-            bool fresh = false;
-            if (staged) {
-                // staged rows are current unless a removal of this chunk
-                // changed r, inf, ii (the test, r's behind, ii's infront) or
-                // iii (its infront)
-                in = st[k - 1];
-                fresh = tracked && (mod.n == 0 || (!mod_has(mod, r) && (in.inf < 0 || !mod_has(mod, in.inf)) &&
-                                                   (in.ii < 0 || !mod_has(mod, in.ii)) &&
-                                                   (in.iii < 0 || !mod_has(mod, in.iii))));
+            const int seg_end = min(cut_sh, hi);
+            const bool mine = in_seg && tid < seg_end && in.go;
+            if (mine) {
+                s.behind[in.ii] = r + 1;
+                s.infront[r] = in.ii + 1;
+                clear_row(s, in.inf);
+                pushv[tid] = push;
             }
-            if (!fresh) load_rm(s, p, r, in);
-            if (!in.go) continue;
-            const int inf = in.inf, tinfront = in.ii;
-            s.behind[tinfront] = r + 1;
-            s.infront[r] = tinfront + 1;
-            clear_row(s, inf);
-            tracked = mod_add(mod, tinfront) && tracked;
-            tracked = mod_add(mod, r) && tracked;
-            tracked = mod_add(mod, inf) && tracked;
-            ++nrm;
-            // loop_killer(r): it clears r's loop when that has fewer than 4
-            // points, and otherwise changes nothing
-            const bool small = rm_loop_small(r, in);
-            if (small) tracked = loop_killer_rec(s, r, n, mod) && tracked;
-            if (!small) {
-                const int b = in.beh;
-                if (b > r && !dirty_push(d, b)) {
-                    // no room for a row to revisit: the plain loop takes
-                    // over after r
-                    res_sh = r + 1;
+            // (ballots of the whole wave, outside any branch)
+            const unsigned long long mins = __ballot(mine);
+            const unsigned long long mpush = __ballot(mine && push >= 0);
+            if (lane == 0) {
+                nmask[tid >> 6] = mins;
+                pmask[tid >> 6] = mpush;
+            }
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int w = 0; w < RM_CHUNK / 64; ++w) nrm += __popcll(nmask[w]);
+                // queued rows visited as candidates of the segment
+                if (seg_end > a)
+                    while (d.hd < d.nd && d.v[d.hd] <= strow[seg_end - 1]) ++d.hd;
+                bool room = true;
+                for (int w = 0; w < RM_CHUNK / 64 && room; ++w)
+                    for (unsigned long long b = pmask[w]; b && room; b &= b - 1)
+                        room = dirty_push(d, pushv[64 * w + __ffsll(b) - 1]);
+                if (!room) {
+                    // no room: the plain loop takes over after the segment
+                    res_sh = strow[seg_end - 1] + 1;
+                    stop_sh = 1;
+                }
+                a_sh = seg_end;
+            }
+            __syncthreads();
+            fresh = false;
+            if (stop_sh) break;
+            const int a2 = a_sh;
+            const int rd0 = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+            if (a2 >= m && rd0 >= limit) break;   // the chunk is done
+            const bool serial = a2 == a || (rd0 < limit && (a2 >= m || rd0 < strow[a2]));
+            if (!serial || tid != 0) continue;
+            // one visit at a time, from memory, until the next visit is a candidate
+            int k = a2;
+            bool visited = false;
+            while (true) {
+                const int rc = k < m ? strow[k] : 0x7fffffff;
+                const int rd = d.hd < d.nd ? d.v[d.hd] : 0x7fffffff;
+                const int rv = rc < rd ? rc : rd;
+                if (rv == 0x7fffffff || rv >= limit) break;
+                if (visited && rc <= rd) break;
+                visited = true;
+                if (rc == rv) ++k;
+                if (rd == rv) ++d.hd;
+                RmIn iv;
+                load_rm(s, p, rv, iv);
+                if (!iv.go) continue;
+                s.behind[iv.ii] = rv + 1;
+                s.infront[rv] = iv.ii + 1;
+                clear_row(s, iv.inf);
+                ++nrm;
+                if (rm_loop_small(rv, iv)) {
+                    loop_killer_plain(s, rv, n);
+                } else if (iv.beh > rv && !dirty_push(d, iv.beh)) {
+                    // no room for a row to revisit: the plain loop takes over after rv
+                    res_sh = rv + 1;
                     stop_sh = 1;
                     break;
                 }
             }
+            a_sh = k;
         }
     }
     __syncthreads();
