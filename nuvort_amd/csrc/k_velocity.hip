@@ -117,14 +117,18 @@ __global__ __launch_bounds__(TB) void k_bs64(DState s, Tangles tg, const Item *_
 }
 
 // FP32 source records, one per row: position relative to its tile's origin
-// (the first active row of the tile, FP64), e in FP32, c = |e|^2, row, and
-// xe = x x e (from the FP64 position and segment), with which the far-tile
-// loop forms sum (x - t) x e fac as sum x x e fac - t x sum e fac.
+// (the first active row of the tile, FP64), c = |e|^2, xe = x x e (from the
+// FP64 position and segment), e in FP32, and the row.  The far loop forms
+// sum (x - t) x e fac as sum x x e fac - t x sum e fac.  Position and c come
+// first: the near loop reads them (one 16-byte read) for every pair and the
+// rest only for pairs beyond the near radius.
 struct __align__(16) Rec32 {
-    float x0, x1, x2, e0, e1, e2, c;
+    float x0, x1, x2, c;
+    float xe0, xe1, xe2, e0;
+    float e1, e2, pad;
     int j;
-    float xe0, xe1, xe2, pad;
 };
+static_assert(sizeof(Rec32) == 48, "Rec32 is twelve words");
 
 // Spatial order of the FP32 source records.  Within each tangle, active
 // rows by the Morton code of their cell (10 bits per axis over the periodic
@@ -356,11 +360,14 @@ __device__ __forceinline__ void far_block(const Rec32 *tile, int jb, const float
 // through the reference's FP64 arithmetic instead.  Each lane carries TPT
 // target points, so one LDS record read serves TPT pairs.
 // Partial block of tangle k: [S][6][cap_k] = far xyz (times k/4pi), near xyz.
+// Waves per SIMD the compiler is held to: 4 with two targets per lane (128
+// VGPRs; a few spills outside the far loop), 5 with one.  On the 256-tangle
+// ensemble 4 waves ran the kernel in 915 ms against 943 ms with 3.
 #ifndef NV_BS32_WAVES_PER_EU
-#define NV_BS32_WAVES_PER_EU 3
+#define NV_BS32_WAVES_PER_EU(TPT) ((TPT) == 1 ? 5 : 4)
 #endif
 template <int TPT, int KB>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVES_PER_EU))) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVES_PER_EU(TPT)))) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
                                               int *queue, const Rec32 *__restrict__ rec,
                                               const TileInfo *__restrict__ tiles, const int *__restrict__ iperm,
                                               const double *__restrict__ rec64,
