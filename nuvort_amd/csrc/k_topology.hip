@@ -982,62 +982,8 @@ __device__ __forceinline__ void norm_tanf_of(const double x[3], const double gi[
     for (int k = 0; k < 3; ++k) t[k] = t[k] / length;
 }
 
-// rows whose links, validity or contents a reconnection changed since the
-// chunk was staged: open-addressing set in LDS
+// LDS hash size of premove's changed-row table (below)
 constexpr int MOD_SLOTS = 4096;
-
-struct ModSet {
-    int v[MOD_SLOTS];
-    int n;
-};
-
-__device__ __forceinline__ bool mod_has(const ModSet &m, int r) {
-    unsigned h = ((unsigned)r * 2654435761u) & (MOD_SLOTS - 1);
-    for (int q = 0; q < MOD_SLOTS; ++q) {
-        const int v = m.v[h];
-        if (v == r) return true;
-        if (v == -1) return false;
-        h = (h + 1) & (MOD_SLOTS - 1);
-    }
-    return false;
-}
-
-// false once the set is half full (the caller then reloads every visit)
-__device__ __forceinline__ bool mod_add(ModSet &m, int r) {
-    if (m.n >= MOD_SLOTS / 2) return false;
-    unsigned h = ((unsigned)r * 2654435761u) & (MOD_SLOTS - 1);
-    while (m.v[h] != -1) {
-        if (m.v[h] == r) return true;
-        h = (h + 1) & (MOD_SLOTS - 1);
-    }
-    m.v[h] = r;
-    ++m.n;
-    return true;
-}
-
-// loop_killer (above) that also records the rows it clears
-__device__ bool loop_killer_rec(const DState &s, int particle, int n, ModSet &m) {
-    int counter = 1, next = particle;
-    for (int it = 1; it <= n; ++it) {
-        next = s.infront[next] - 1;
-        if (next == particle) break;
-        if (next < 0) return true;
-        counter = counter + 1;
-        if (counter >= 4) return true;
-    }
-    bool ok = true;
-    if (counter < 4) {
-        next = particle;
-        for (int it = 1; it <= n; ++it) {
-            int store_next = s.infront[next] - 1;
-            clear_row(s, next);
-            ok = mod_add(m, next) && ok;
-            next = store_next;
-            if (next == particle || next < 0) break;
-        }
-    }
-    return ok;
-}
 
 // src/reconnection.f90:20-43: the tests of one visit; go = reconnect
 __device__ void recon_tests(const Phys &p, ReconIn &in) {
@@ -1069,126 +1015,440 @@ __device__ void recon_tests(const Phys &p, ReconIn &in) {
 constexpr int REC_CHUNK = 256;
 constexpr int REC_PARALLEL_MAX = 16;
 
-// One workgroup per tangle: candidates staged REC_CHUNK at a time (every
-// thread loads one), thread 0 visits them in row order.  A staged visit is
-// reloaded from memory when a reconnection earlier in the chunk changed any
-// row it read (r, its closest, infront(r), behind(closest)).
-// One visit of the reference's precon loop (src/reconnection.f90:11-61) at
-// staged index kk, by the serial thread: from the staged inputs unless one
-// of the visit's rows was modified (then from memory), and the reconnection
-// when the tests pass.  Returns whether the modified set is still complete.
-__device__ bool precon_visit(const DState &s, const Phys &p, int &nrec, int tk, int base, int n, int kk,
-                             const int *strow, const ReconIn *st, ModSet &mod, bool tracked, DCtl *ctl,
-                             nv_recon_event *ev, int ev_cap, int itime, double t) {
-    const int r = strow[kk];
-    ReconIn in = st[kk];
-    bool fresh = tracked && (mod.n == 0 || !mod_has(mod, r));
-    if (fresh && mod.n > 0 && in.act && in.j > 0)
-        fresh = !mod_has(mod, in.j - 1) && (in.pari <= 0 || !mod_has(mod, in.pari - 1)) &&
-                (in.parjb <= 0 || !mod_has(mod, in.parjb - 1));
-    if (!fresh) {
-        load_recon(s, r, in);
-        recon_tests(p, in);
-    }
-    if (!in.go) return tracked;
-    const int pari = in.pari, j = in.j, rj = j - 1, parjb = in.parjb;
-    ++nrec;
-    const int q = atomicAdd(&ctl->nevents, 1);
-    if (q < ev_cap) {
-        nv_recon_event e;
-        e.itime = itime; e.i = r + 1 - base; e.j = j - base; e.tangle = tk; e.t = t;
+// What one visit of the staged precon reads, loaded for a whole chunk of
+// candidates at once.  The parts of the reference's tests that no
+// reconnection can change are evaluated at staging: the tangents' dot
+// product, l_before, and the first three terms of l_after (positions and
+// stale ghosts of r and its closest point j are not written by precon,
+// except by clearing, which empties r or j and ends the visit early).
+// What a reconnection earlier in the chunk can change -- the links of r and
+// j, and so infront(r), behind(j) and their positions -- is read through
+// the chunk's overlay.
+struct RecStage {
+    double x[3], xj[3], xpi[3], xpjb[3];   // r, j = closest(r), infront(r), behind(j)
+    double cd, dot, lb, s3, la;             // closestd, t_r . t_j, l_before, l_after's first three terms,
+                                            // l_after as staged
+    int act, pari, behr, j, actj, parjb;    // 1-based links as staged
+    int ii, iii, iv, ji, jii;               // infront chains from pari and from j, for loop_killer's walks
+    int go;
+};
+
+// stage one candidate from the current state in memory
+__device__ void stage_recon(const DState &s, const Phys &p, int r, RecStage &in) {
+    const int cap = s.cap;
+    in.act = s.infront[r] != 0;
+    in.pari = s.infront[r];
+    in.behr = s.behind[r];
+    in.cd = s.closestd[r];
+    in.j = s.closest[r];
+    in.actj = 0;
+    in.parjb = 0;
+    in.ii = in.iii = in.iv = in.ji = in.jii = 0;
+    in.dot = 0.0;
+    in.lb = in.s3 = in.la = 0.0;
+    in.go = 0;
+    double gi[3], gb[3];
 #pragma unroll
-        for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
-        e.angle = acos(in.dot);
-        e.dl = in.lb - in.la;
-        ev[q] = e;
+    for (int k = 0; k < 3; ++k) {
+        in.x[k] = s.x[k * cap + r];
+        gi[k] = s.gi[k * cap + r];
+        gb[k] = s.gb[k * cap + r];
+        in.xj[k] = in.xpi[k] = in.xpjb[k] = 0.0;
     }
+    if (in.j <= 0 || !in.act) return;
+    const int rj = in.j - 1;
+    in.actj = s.infront[rj] != 0;
+    in.parjb = s.behind[rj];
+    const int pi = in.pari - 1, pjb = in.parjb > 0 ? in.parjb - 1 : rj;
+    double gij[3], gbj[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        in.xj[k] = s.x[k * cap + rj];
+        gij[k] = s.gi[k * cap + rj];
+        gbj[k] = s.gb[k * cap + rj];
+        in.xpi[k] = s.x[k * cap + pi];
+        in.xpjb[k] = s.x[k * cap + pjb];
+    }
+    in.ii = s.infront[pi];
+    if (in.ii > 0) in.iii = s.infront[in.ii - 1];
+    if (in.iii > 0) in.iv = s.infront[in.iii - 1];
+    in.ji = s.infront[rj];
+    if (in.ji > 0) in.jii = s.infront[in.ji - 1];
+    if (!in.actj) return;
+    double t1[3], t2[3];
+    norm_tanf_of(in.x, gi, gb, t1);
+    norm_tanf_of(in.xj, gij, gbj, t2);
+    in.dot = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
+    in.lb = dist_gen(in.x[0], in.x[1], in.x[2], gi[0], gi[1], gi[2]) +
+            dist_gen(in.x[0], in.x[1], in.x[2], gb[0], gb[1], gb[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], gij[0], gij[1], gij[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], gbj[0], gbj[1], gbj[2]);
+    in.s3 = dist_gen(in.x[0], in.x[1], in.x[2], gb[0], gb[1], gb[2]) +
+            dist_gen(in.x[0], in.x[1], in.x[2], in.xj[0], in.xj[1], in.xj[2]) +
+            dist_gen(in.xj[0], in.xj[1], in.xj[2], gij[0], gij[1], gij[2]);
+}
+
+// src/reconnection.f90:20-43 on the staged inputs with the links as given:
+// returns l_after in la when the visit reconnects
+__device__ __forceinline__ bool recon_decide(const Phys &p, const RecStage &in, int pari, int behr, bool actj,
+                                             const double xpi[3], const double xpjb[3], double &la) {
+    if (!in.act) return false;
+    if (!((in.cd < p.delta / 2.0) && (in.cd > NV_EPS))) return false;
+    const int j = in.j;
+    if (j == pari) return false;
+    if (j == behr) return false;
+    if (j == 0) return false;
+    if (!actj) return false;
+    if (in.dot > 0.9) return false;
+    la = in.s3 + dist_gen(xpi[0], xpi[1], xpi[2], xpjb[0], xpjb[1], xpjb[2]);
+    return la <= in.lb;
+}
+
+// The rows the pass's reconnections have changed so far, with their current
+// links: one 16-byte slot per row (key, inf, beh, cleared), read in one LDS
+// access.  inf / beh hold the 1-based link (0 once cleared), or -1 where
+// that link is still the one the visits were staged with; xs is the row's
+// own position (a changed row is r, j, infront(r) or behind(j) of some
+// reconnection, whose positions the visit had).  Open addressing over
+// OV_SLOTS slots, at most OV_MAX of them used.
+constexpr int OV_SLOTS = 2048;
+constexpr int OV_MAX = 1024;
+
+struct __align__(16) OvSlot {
+    int key, inf, beh, cl;
+};
+
+struct Overlay {
+    OvSlot hd[OV_SLOTS];
+    double xs[OV_SLOTS][3];
+    int n;
+};
+
+__device__ __forceinline__ unsigned ov_hash(int r) { return ((unsigned)r * 2654435761u) >> (32 - 11); }
+static_assert(OV_SLOTS == 1 << 11, "ov_hash keeps 11 bits");
+
+// the slot of row r (its contents in h), or -1
+__device__ __forceinline__ int ov_find(const Overlay &o, int r, OvSlot &h) {
+    unsigned i = ov_hash(r);
+    for (;;) {
+        h = o.hd[i];
+        if (h.key == r) return (int)i;
+        if (h.key == -1) return -1;
+        i = (i + 1) & (OV_SLOTS - 1);
+    }
+}
+
+__device__ __forceinline__ bool ov_has(const Overlay &o, int r) {
+    OvSlot h;
+    return ov_find(o, r, h) >= 0;
+}
+
+// the slot of row r, created (links unchanged, position xs) when absent;
+// the caller has checked that a slot is free
+__device__ int ov_get(Overlay &o, int r, const double xs[3]) {
+    unsigned i = ov_hash(r);
+    for (;;) {
+        const int k = o.hd[i].key;
+        if (k == r) return (int)i;
+        if (k == -1) break;
+        i = (i + 1) & (OV_SLOTS - 1);
+    }
+    ++o.n;
+    o.hd[i] = OvSlot{r, -1, -1, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.xs[i][k] = xs[k];
+    return (int)i;
+}
+
+// The serial thread's view of one visit: infront of any row (1-based), from
+// the overlay, then from the chains staged for this visit, then from memory.
+struct VisitView {
+    const DState &s;
+    const Overlay &o;
+    const RecStage &in;
+    bool tracked;
+    __device__ int inf(int row) const {
+        if (!tracked) return s.infront[row];
+        OvSlot h;
+        if (ov_find(o, row, h) >= 0 && h.inf >= 0) return h.inf;
+        if (in.pari > 0 && row == in.pari - 1) return in.ii;
+        if (in.ii > 0 && row == in.ii - 1) return in.iii;
+        if (in.iii > 0 && row == in.iii - 1) return in.iv;
+        if (in.j > 0 && row == in.j - 1) return in.ji;
+        if (in.ji > 0 && row == in.ji - 1) return in.jii;
+        return s.infront[row];
+    }
+};
+
+__device__ void clear_tracked(const DState &s, Overlay &o, bool tracked, int row) {
+    clear_row(s, row);
+    if (tracked) {
+        const double z[3] = {0.0, 0.0, 0.0};
+        const int i = ov_get(o, row, z);
+        o.hd[i] = OvSlot{row, 0, 0, 1};
+    }
+}
+
+// src/reconnection.f90:155-182 loop_killer(particle): as loop_killer above,
+// reading links through the visit's view and recording what it clears
+__device__ void loop_killer_view(const DState &s, Overlay &o, const VisitView &v, int particle) {
+    int counter = 1, next = particle;
+    for (int it = 1; it <= 3; ++it) {
+        next = v.inf(next) - 1;
+        if (next == particle) break;
+        if (next < 0) return;
+        counter = counter + 1;
+        if (counter >= 4) return;
+    }
+    next = particle;
+    for (int it = 0; it < 4; ++it) {
+        const int store_next = v.inf(next) - 1;
+        clear_tracked(s, o, v.tracked, next);
+        next = store_next;
+        if (next == particle || next < 0) break;
+    }
+}
+
+// position of the row behind the 1-based link `link`, where the staged visit
+// had link `staged` with position xst
+__device__ __forceinline__ void pos_of(const DState &s, const Overlay &o, int link, int staged, const double xst[3],
+                                       double out[3]) {
+    if (link == staged) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = xst[k];
+        return;
+    }
+    OvSlot h;
+    const int i = ov_find(o, link - 1, h);
+    if (i >= 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = o.xs[i][k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = s.x[k * s.cap + link - 1];
+}
+
+// One visit of the reference's precon loop (src/reconnection.f90:11-61) at
+// staged index kk, by the serial thread.  With the overlay complete
+// (tracked), the visit decides from its staged inputs and the overlay's
+// links; otherwise it stages itself again from memory.  A reconnection is
+// written to memory (links, cleared loops) and to the overlay, and queued
+// for the chunk's event log.
+__device__ void precon_visit(const DState &s, const Phys &p, int r, RecStage &sin, Overlay &o, bool &tracked,
+                             int &nrec, int kk, int *evk, double *evla, int &nev) {
+    int pari, behr, parjb;
+    bool actj;
+    double xpi[3], xpjb[3], la = 0.0;
+    RecStage in;
+    if (tracked) {
+        // The decision depends on the links of r and j alone (the positions
+        // it reads are those of rows linked to r and j, which only a clear of
+        // their loop changes): where neither row has changed since staging,
+        // the staged decision stands.
+        OvSlot hr, hj;
+        const int sj = sin.j;
+        const int er = ov_find(o, r, hr);
+        const int ej = sj > 0 ? ov_find(o, sj - 1, hj) : -1;
+        if (er < 0 && ej < 0 && !sin.go) return;
+        if (er >= 0 && hr.cl) return;    // emptied by a loop_killer of this pass
+        // the staged record in registers: one LDS read of all of it, not
+        // one dependent read per field
+        in = sin;
+        pari = (er >= 0 && hr.inf >= 0) ? hr.inf : in.pari;
+        behr = (er >= 0 && hr.beh >= 0) ? hr.beh : in.behr;
+        actj = in.actj;
+        parjb = in.parjb;
+        if (ej >= 0) {
+            actj = actj && !hj.cl;
+            if (hj.beh >= 0) parjb = hj.beh;
+        }
+        if (er < 0 && ej < 0) {
+            la = in.la;   // unchanged, and its staged tests say reconnect
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { xpi[k] = in.xpi[k]; xpjb[k] = in.xpjb[k]; }
+        } else {
+            if (!in.act || !actj || in.j == pari || in.j == behr) return;
+            pos_of(s, o, pari, in.pari, in.xpi, xpi);
+            pos_of(s, o, parjb, in.parjb, in.xpjb, xpjb);
+            if (!recon_decide(p, in, pari, behr, actj, xpi, xpjb, la)) return;
+        }
+    } else {
+        stage_recon(s, p, r, in);
+        pari = in.pari; behr = in.behr; parjb = in.parjb; actj = in.actj;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { xpi[k] = in.xpi[k]; xpjb[k] = in.xpjb[k]; }
+        if (!recon_decide(p, in, pari, behr, actj, xpi, xpjb, la)) return;
+        sin = in;   // staged again from memory: the event log reads it
+    }
+    const int j = in.j, rj = j - 1;
+    ++nrec;
+    evk[nev] = kk;
+    evla[nev] = la;
+    ++nev;
+    // four link slots and up to 3 + 3 cleared rows
+    if (tracked && o.n + 10 > OV_MAX) tracked = false;
     s.infront[parjb - 1] = pari;
     s.behind[pari - 1] = parjb;
     s.infront[r] = j;
     s.behind[rj] = r + 1;
-    tracked = mod_add(mod, parjb - 1) && tracked;
-    tracked = mod_add(mod, pari - 1) && tracked;
-    tracked = mod_add(mod, r) && tracked;
-    tracked = mod_add(mod, rj) && tracked;
-    tracked = loop_killer_rec(s, pari - 1, n, mod) && tracked;
-    tracked = loop_killer_rec(s, r, n, mod) && tracked;
-    return tracked;
+    if (tracked) {
+        o.hd[ov_get(o, parjb - 1, xpjb)].inf = pari;
+        o.hd[ov_get(o, pari - 1, xpi)].beh = parjb;
+        o.hd[ov_get(o, r, in.x)].inf = j;
+        o.hd[ov_get(o, rj, in.xj)].beh = r + 1;
+    }
+    const VisitView v{s, o, in, tracked};
+    loop_killer_view(s, o, v, pari - 1);
+    loop_killer_view(s, o, v, r);
 }
 
+// Every candidate of the pass staged at once, before any reconnection:
+// candidate q into rs[q] for q < rs_cap.
+__global__ void k_precon_stage(DState s, Phys p, const int *cand, const DCtl *ctl, RecStage *rs, int rs_cap) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctl->abort || q >= ctl->ncand || q >= rs_cap) return;
+    RecStage in;
+    stage_recon(s, p, cand[q], in);
+    double la = 0.0;
+    in.go = recon_decide(p, in, in.pari, in.behr, in.actj != 0, in.xpi, in.xpjb, la) ? 1 : 0;
+    in.la = la;
+    rs[q] = in;
+}
+
+// One workgroup per tangle.  Candidates are taken REC_CHUNK at a time into
+// LDS, as staged before the pass (k_precon_stage) while the overlay holds
+// every change of the pass, else staged again from memory at the chunk's
+// start with the overlay emptied.  The serial thread makes the visits in
+// row order from the staged inputs and the overlay.  With few staged
+// reconnections in a chunk, the workgroup finds the next visit that can
+// change anything -- its staged tests say reconnect, or r or j (or the rows
+// it read through them) is in the overlay -- and the serial thread makes
+// only those.  The chunk's reconnections are logged at its end, in visit
+// order, with one atomic for the whole chunk.
 __global__ __launch_bounds__(REC_CHUNK) void k_precon_serial(DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
-                                                             nv_recon_event *ev, int ev_cap) {
+                                                             nv_recon_event *ev, int ev_cap, const RecStage *rs,
+                                                             int rs_cap) {
     if (ctl->abort) return;
-    __shared__ ReconIn st[REC_CHUNK];
+    __shared__ RecStage st[REC_CHUNK];
     __shared__ int strow[REC_CHUNK];
-    __shared__ ModSet mod;
+    __shared__ Overlay ov;
+    __shared__ int evk[REC_CHUNK];
+    __shared__ double evla[REC_CHUNK];
     __shared__ int nk[2];          // the next visit that needs the serial thread
-    __shared__ int tracked_sh;     // the modified set is complete
+    __shared__ int tracked_sh;     // the overlay holds every change since staging
+    __shared__ int prestaged;      // ... since the pass's staging
     __shared__ int ngo;            // staged visits whose tests say "reconnect"
+    __shared__ int nev_sh, ev0_sh;
     const int tid = threadIdx.x;
     const double t = ctl->t;
     const int itime = ctl->itime;
     const int tk = blockIdx.x;
     const int base = tg.base[tk];
     const int c0 = lower_row(cand, ctl->ncand, base), c1 = lower_row(cand, ctl->ncand, base + tg.cap[tk]);
-    const int n = tg.n[tk];
     int nrec = 0;   // reconnections (the serial thread's)
+    for (int q = tid; q < OV_SLOTS; q += REC_CHUNK) ov.hd[q].key = -1;
+    if (tid == 0) {
+        ov.n = 0;
+        tracked_sh = 1;
+        prestaged = 1;
+    }
     for (int cs = c0; cs < c1; cs += REC_CHUNK) {
         const int m = min(REC_CHUNK, c1 - cs);
         __syncthreads();
-        for (int q = tid; q < MOD_SLOTS; q += REC_CHUNK) mod.v[q] = -1;
+        // the staging of the pass is usable while the overlay holds every
+        // change since, and it has room for this chunk's
+        const bool pre = prestaged && tracked_sh && ov.n <= OV_MAX / 2 && cs + m <= rs_cap;
+        __syncthreads();
+        if (!pre) {
+            for (int q = tid; q < OV_SLOTS; q += REC_CHUNK) ov.hd[q].key = -1;
+        }
         if (tid == 0) {
-            mod.n = 0;
-            tracked_sh = 1;
+            if (!pre) {
+                ov.n = 0;
+                prestaged = 0;
+                tracked_sh = 1;
+            }
             nk[0] = nk[1] = m;
             ngo = 0;
+            nev_sh = 0;
         }
         __syncthreads();
         if (tid < m) {
+            RecStage &in = st[tid];
             strow[tid] = cand[cs + tid];
-            load_recon(s, strow[tid], st[tid]);
-            recon_tests(p, st[tid]);
-            if (st[tid].go) atomicAdd(&ngo, 1);
+            if (pre) {
+                in = rs[cs + tid];
+            } else {
+                stage_recon(s, p, strow[tid], in);
+                double la = 0.0;
+                in.go = recon_decide(p, in, in.pari, in.behr, in.actj != 0, in.xpi, in.xpjb, la) ? 1 : 0;
+                in.la = la;
+            }
+            if (in.go) atomicAdd(&ngo, 1);
         }
         __syncthreads();
         if (ngo > REC_PARALLEL_MAX) {
             // many reconnections: the serial thread visits every candidate
             if (tid == 0) {
-                bool tracked = true;
+                bool tracked = tracked_sh != 0;
+                int nev = 0;
                 for (int kk = 0; kk < m; ++kk)
-                    tracked = precon_visit(s, p, nrec, tk, base, n, kk, strow, st, mod, tracked, ctl, ev, ev_cap,
-                                           itime, t);
+                    precon_visit(s, p, strow[kk], st[kk], ov, tracked, nrec, kk, evk, evla, nev);
+                nev_sh = nev;
+                tracked_sh = tracked ? 1 : 0;
             }
-            continue;
-        }
-        // The reference visits the candidates in order.  A visit whose staged
-        // tests say "no reconnection" and none of whose rows a reconnection of
-        // this chunk has modified changes nothing: the block finds the next
-        // visit that is not such a one, and the serial thread runs it.
-        int k = 0, par = 0;
-        while (true) {
-            if (tid >= k && tid < m) {
-                const ReconIn &in = st[tid];
-                bool need = in.go;
-                if (!need && mod.n > 0) {
-                    need = !tracked_sh || mod_has(mod, strow[tid]) ||
-                           (in.act && in.j > 0 &&
-                            (mod_has(mod, in.j - 1) || (in.pari > 0 && mod_has(mod, in.pari - 1)) ||
-                             (in.parjb > 0 && mod_has(mod, in.parjb - 1))));
+        } else {
+            // A visit whose staged tests say "no reconnection" and none of
+            // whose rows a reconnection of this pass has changed changes
+            // nothing: the block finds the next visit that is not such a
+            // one, and the serial thread runs it.
+            int k = 0, par = 0;
+            while (true) {
+                if (tid >= k && tid < m) {
+                    const RecStage &in = st[tid];
+                    bool need = in.go;
+                    if (!need && ov.n > 0) {
+                        need = !tracked_sh || ov_has(ov, strow[tid]) ||
+                               (in.act && in.j > 0 &&
+                                (ov_has(ov, in.j - 1) || (in.pari > 0 && ov_has(ov, in.pari - 1)) ||
+                                 (in.parjb > 0 && ov_has(ov, in.parjb - 1))));
+                    }
+                    if (need) atomicMin(&nk[par], tid);
                 }
-                if (need) atomicMin(&nk[par], tid);
+                __syncthreads();
+                const int kk = nk[par];
+                if (kk >= m) break;
+                if (tid == 0) {
+                    nk[par ^ 1] = m;
+                    bool tracked = tracked_sh != 0;
+                    int nev = nev_sh;
+                    precon_visit(s, p, strow[kk], st[kk], ov, tracked, nrec, kk, evk, evla, nev);
+                    tracked_sh = tracked ? 1 : 0;
+                    nev_sh = nev;
+                }
+                k = kk + 1;
+                par ^= 1;
+                __syncthreads();
             }
-            __syncthreads();
-            const int kk = nk[par];
-            if (kk >= m) break;
-            if (tid == 0) {
-                nk[par ^ 1] = m;
-                tracked_sh = precon_visit(s, p, nrec, tk, base, n, kk, strow, st, mod, tracked_sh != 0, ctl, ev,
-                                          ev_cap, itime, t) ? 1 : 0;
-            }
-            k = kk + 1;
-            par ^= 1;
-            __syncthreads();
+        }
+        // the chunk's reconnection log, in visit order
+        __syncthreads();
+        const int nev = nev_sh;
+        if (nev == 0) continue;
+        if (tid == 0) ev0_sh = atomicAdd(&ctl->nevents, nev);
+        __syncthreads();
+        for (int q = tid; q < nev; q += REC_CHUNK) {
+            const int slot = ev0_sh + q;
+            if (slot >= ev_cap) break;
+            const RecStage &in = st[evk[q]];
+            nv_recon_event e;
+            e.itime = itime; e.i = strow[evk[q]] + 1 - base; e.j = in.j - base; e.tangle = tk; e.t = t;
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) e.mid[k2] = 0.5 * (in.x[k2] + in.xj[k2]);
+            e.angle = acos(in.dot);
+            e.dl = in.lb - evla[q];
+            ev[slot] = e;
         }
     }
     if (tid == 0 && nrec) tg.recon[tk] += nrec;
@@ -2046,9 +2306,13 @@ hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {
     if (n > 0) hipLaunchKernelGGL(k_precon_pre, GRID(n), 0, st, s, p, n, cand);
     return hipGetLastError();
 }
+size_t nvk_recstage_bytes() { return sizeof(RecStage); }
 hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
-                             nv_recon_event *ev, int ev_cap) {
-    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(REC_CHUNK), 0, st, s, p, tg, cand, ctl, ev, ev_cap);
+                             nv_recon_event *ev, int ev_cap, void *rs, int rs_cap) {
+    RecStage *r = (RecStage *)rs;
+    if (rs_cap > 0) hipLaunchKernelGGL(k_precon_stage, GRID(rs_cap), 0, st, s, p, cand, ctl, r, rs_cap);
+    hipLaunchKernelGGL(k_precon_serial, dim3(tg.K), dim3(REC_CHUNK), 0, st, s, p, tg, cand, ctl, ev, ev_cap, r,
+                       rs_cap);
     return hipGetLastError();
 }
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand) {

@@ -65,6 +65,8 @@ struct Ctx {
     DevBuf<int> cand, empty, lcand, lempty, tmp;
     DevBuf<double> rec64, part, pd;
     DevBuf<char> rec32, tinfo, sph;
+    DevBuf<char> rstage;             // precon's candidates staged before its pass
+    int rs_cap = 0;                  // ... their number
     DevBuf<int> pj;
     DevBuf<double> noise;            // noisey_xflow table (nv_set_noise)
     DevBuf<double> diag;             // nv_diagnostics' partial reductions and result
@@ -200,6 +202,10 @@ int alloc_scratch(Ctx *c) {
     HIPCHK(c, c->lempty.reserve(rows));
     HIPCHK(c, c->tmp.reserve((size_t)nv_compact_tmp_ints(rows)));
     HIPCHK(c, c->rec64.reserve((size_t)8 * rows));
+    // precon stages at most this many candidates before its pass; the rest
+    // are staged chunk by chunk from memory
+    c->rs_cap = std::min(rows, 32768);
+    HIPCHK(c, c->rstage.reserve(nvk_recstage_bytes() * (size_t)std::max(1, c->rs_cap)));
     HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
     HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes() * (size_t)(rows / 256 + 1)));
     HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 64 + 4)));
@@ -787,7 +793,7 @@ hipError_t enqueue_recon_remove(Ctx *c, DState d) {
     }
     e = nvk_precon_pre(st, d, c->phys, R, c->cand.p);
     if (e == hipSuccess) e = nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p);
-    if (e == hipSuccess) e = nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap);
+    if (e == hipSuccess) e = nvk_precon_serial(st, d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap, c->rstage.p, c->rs_cap);
     if (e == hipSuccess) e = nvk_premove_pre(st, d, c->phys, R, c->cand.p);
     if (e == hipSuccess) e = nv_compact_flags(st, c->cand.p, R, c->lcand.p, &c->ctl->ncand, c->tmp.p);
     if (e == hipSuccess) e = nvk_premove_serial(st, d, c->phys, tg, c->lcand.p, c->ctl);
@@ -1303,7 +1309,7 @@ int nv_destroy(void *h) {
     DevBuf<int> *ib[] = {&c->d_tbase, &c->d_tcap, &c->d_tcnt, &c->d_row2t,
                          &c->cand, &c->empty, &c->lcand, &c->lempty, &c->tmp, &c->pj, &c->d_S, &c->d_pS, &c->queue};
     for (auto *b : ib) b->release();
-    c->rec64.release(); c->tinfo.release(); c->sph.release(); c->part.release(); c->pd.release(); c->rec32.release();
+    c->rec64.release(); c->tinfo.release(); c->sph.release(); c->part.release(); c->pd.release(); c->rec32.release(); c->rstage.release();
     c->items.release(); c->pitems.release(); c->d_poff.release(); c->d_ppoff.release();
     if (c->ev) (void)hipFree(c->ev);
     if (c->ctl) (void)hipFree(c->ctl);
@@ -1493,7 +1499,8 @@ int nv_precon(void *h) {
     HIPCHK(c, nvk_precon_pre(c->st, c->d, c->phys, c->rows, c->cand.p));
     if (int rc = compact(c, c->cand, c->lcand, &c->ctl->ncand)) return rc;
     HIPCHK(c, nvk_clock_set(c->st, c->ctl, c->t, c->itime));
-    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap));
+    HIPCHK(c, nvk_precon_serial(c->st, c->d, c->phys, tg, c->lcand.p, c->ctl, c->ev, c->ev_cap, c->rstage.p,
+                                 c->rs_cap));
     return read_ctl(c);
 }
 

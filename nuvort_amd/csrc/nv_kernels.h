@@ -92,8 +92,11 @@ hipError_t nvk_pinsert_pre(hipStream_t st, DState s, Phys p, Tangles tg, int n, 
 hipError_t nvk_pinsert_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, const int *empty,
                               DCtl *ctl);
 hipError_t nvk_precon_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
+// precon: every candidate staged at once into rs (rs_cap records of
+// nvk_recstage_bytes()), then the serial pass
+size_t nvk_recstage_bytes();
 hipError_t nvk_precon_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl,
-                             nv_recon_event *ev, int ev_cap);
+                             nv_recon_event *ev, int ev_cap, void *rs, int rs_cap);
 hipError_t nvk_premove_pre(hipStream_t st, DState s, Phys p, int n, int *cand);
 hipError_t nvk_premove_serial(hipStream_t st, DState s, Phys p, Tangles tg, const int *cand, DCtl *ctl);
 hipError_t nvk_pclose(hipStream_t st, DState s, Tangles tg, int rows, const Item *items, int nitems, void *spheres,
