@@ -94,8 +94,9 @@ typedef struct {
     int32_t kahan_block;         /* FP32C: terms summed in plain FP32 between two compensated
                                     adds, 1 (every term), 2, 4, 8, 16 or 32 (0 = default 8) */
     int32_t step_launch;         /* nv_step: 0 = record one step's launches in a HIP graph and
-                                    replay it every step (direct sum and LIA; the octree step
-                                    launches eagerly), 1 = eager launches */
+                                    replay it every step, 1 = eager launches */
+    int32_t source_tile;         /* FP32C direct sum: sources staged per LDS tile, 128 or 256
+                                    (0 = tuned; default 256) */
 } nv_params;
 
 /* Host view of the filament point array, structure of arrays (the fields
@@ -178,6 +179,8 @@ typedef struct {
     int32_t tree_replays;      /* octree steps nv_step took as a recorded step (one synchronisation) */
     int32_t tree_redone;       /* recorded octree steps whose build outgrew the recorded plan and
                                   that were taken again eagerly */
+    int32_t source_tile;       /* direct-sum variant in use: sources per LDS tile */
+    int32_t pad;
 } nv_stats;
 
 /* lifecycle -------------------------------------------------------------- */
@@ -290,9 +293,9 @@ int nv_pclose_noalloc(int32_t n, int32_t ld, const double *x, const int32_t *inf
                       int32_t *closest, double *closestd, void *stream);
 
 /* Autotuner (no reference counterpart).  Times the direct-sum variants on
- * the uploaded state -- points per lane (targets_per_lane), Kahan block and
- * work items per CU, each left at 0 in nv_params -- one coordinate at a
- * time, and keeps the fastest.  The state is not modified (only scratch
+ * the uploaded state -- points per lane (targets_per_lane), Kahan block,
+ * sources per LDS tile (source_tile) and work items per CU, each left at 0
+ * in nv_params -- one coordinate at a time, and keeps the fastest.  The state is not modified (only scratch
  * buffers are written).  With a cache_path, a choice is looked up first
  * under (GPU architecture, precision, tangle count, log2 of the total and
  * of the largest tangle's points) and a new one is appended to the file;

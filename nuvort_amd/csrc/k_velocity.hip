@@ -167,12 +167,13 @@ __global__ void k_order_inverse(const int *__restrict__ perm, int rows, int *__r
 // a sphere (FP32, relative to the origin) holding its active sources;
 // rad < 0 marks a tile without active sources.
 constexpr int SUB = 16;     // sources per sub-block of a tile
+template <int TSZ>
 struct TileInfo {
     double o[3];
     float c[3], rad;
     // bounding sphere of each sub-block of SUB sources (centre xyz, radius;
     // radius -1 where the sub-block holds no active source)
-    float sub[TS / SUB][4];
+    float sub[TSZ / SUB][4];
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -184,13 +185,14 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// One workgroup per tile of TS records (rows is a multiple of TS); record
+// One workgroup per tile of TSZ records (rows is a multiple of TSZ); record
 // q of the tile sequence holds row perm[q].
-__global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__restrict__ perm,
-                                              Rec32 *__restrict__ rec, TileInfo *__restrict__ tiles) {
-    __shared__ float red[TS / 64][6];
+template <int TSZ>
+__global__ __launch_bounds__(TSZ) void k_src32(DState s, int rows, const int *__restrict__ perm,
+                                               Rec32 *__restrict__ rec, TileInfo<TSZ> *__restrict__ tiles) {
+    __shared__ float red[TSZ / 64][6];
     __shared__ int first;
-    const int q0 = blockIdx.x * TS + threadIdx.x;
+    const int q0 = blockIdx.x * TSZ + threadIdx.x;
     const int r = q0 < rows ? perm[q0] : 0;
     const bool act = q0 < rows && is_source(s, r);
     // origin: the row of the tile's first active record
@@ -274,13 +276,13 @@ __global__ __launch_bounds__(TS) void k_src32(DState s, int rows, const int *__r
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 1; q < TS / 64; ++q)
+        for (int q = 1; q < TSZ / 64; ++q)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 lo[k] = fminf(lo[k], red[q][k]);
                 hi[k] = fmaxf(hi[k], red[q][3 + k]);
             }
-        TileInfo &ti = tiles[blockIdx.x];
+        TileInfo<TSZ> &ti = tiles[blockIdx.x];
         ti.o[0] = o[0]; ti.o[1] = o[1]; ti.o[2] = o[2];
         if (lo[0] > hi[0]) {
             ti.c[0] = ti.c[1] = ti.c[2] = 0.f;
@@ -361,18 +363,19 @@ __device__ __forceinline__ void far_block(const Rec32 *tile, int jb, const float
 // target points, so one LDS record read serves TPT pairs.
 // Partial block of tangle k: [S][6][cap_k] = far xyz (times k/4pi), near xyz.
 // Waves per SIMD the compiler is held to: 4 with two targets per lane (128
-// VGPRs; a few spills outside the far loop), 5 with one.  On the 256-tangle
-// ensemble 4 waves ran the kernel in 915 ms against 943 ms with 3.
+// VGPRs; a few spills outside the far loop), 5 with one (96 VGPRs).  On the
+// 256-tangle ensemble: two per lane 943 ms at 3 waves, 915 ms at 4; one per
+// lane 834 ms at 5 waves (and the same at 6, where it spills).
 #ifndef NV_BS32_WAVES_PER_EU
 #define NV_BS32_WAVES_PER_EU(TPT) ((TPT) == 1 ? 5 : 4)
 #endif
-template <int TPT, int KB>
+template <int TPT, int KB, int TSZ>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVES_PER_EU(TPT)))) void k_bs32c(DState s, Tangles tg, const Item *__restrict__ items, int nitems,
                                               int *queue, const Rec32 *__restrict__ rec,
-                                              const TileInfo *__restrict__ tiles, const int *__restrict__ iperm,
+                                              const TileInfo<TSZ> *__restrict__ tiles, const int *__restrict__ iperm,
                                               const double *__restrict__ rec64,
                                               Shifts sh, double kappa, float rn2, double *__restrict__ part) {
-    __shared__ Rec32 tile[TS];
+    __shared__ Rec32 tile[TSZ];
     __shared__ int slot;
     const int tid = threadIdx.x;
     const float rnf = sqrtf(rn2);
@@ -405,17 +408,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
             bpos[q] = bi[q] >= 0 ? iperm[bi[q]] : -1;   // record position of the behind row
             any = any || act[q];
         }
-        const int ntiles = (n + TS - 1) / TS;
+        const int ntiles = (n + TSZ - 1) / TSZ;
         const long U = (long)sh.n * ntiles;
         const long u0 = U * I.c / S, u1 = U * (I.c + 1) / S;
         for (long un = u0; un < u1; ++un) {
             const int m = (int)(un / ntiles), t = (int)(un % ntiles);
-            const int gt = base / TS + t;
+            const int gt = base / TSZ + t;
             __syncthreads();
-            for (int q = tid; q < TS; q += TB) tile[q] = rec[(size_t)gt * TS + q];
+            for (int q = tid; q < TSZ; q += TB) tile[q] = rec[(size_t)gt * TSZ + q];
             __syncthreads();
             if (!any) continue;
-            const TileInfo &ti = tiles[gt];
+            const TileInfo<TSZ> &ti = tiles[gt];
             if (ti.rad < 0.f) continue;     // no active source in this tile
             const double s0 = sh.s[m][0], s1 = sh.s[m][1], s2 = sh.s[m][2];
             float tt[TPT][3];
@@ -432,16 +435,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
                 const float dd = c0 * c0 + c1 * c1 + c2 * c2;
                 const float mag = fabsf(tt[q][0]) + fabsf(tt[q][1]) + fabsf(tt[q][2]) + ti.rad;
                 const float lim = (ti.rad + rnf) * 1.0001f + 1.0e-6f * mag;
-                const bool fq = dd > lim * lim && (unsigned)(bpos[q] - gt * TS) >= (unsigned)TS;
+                const bool fq = dd > lim * lim && (unsigned)(bpos[q] - gt * TSZ) >= (unsigned)TSZ;
                 far = far && (fq || !act[q]);
             }
             if (__all(far)) {
                 // far tile for the whole wave: no pair needs the near-field
                 // test or an exclusion, every term is FP32
-                for (int jb = 0; jb < TS; jb += KB) far_block<TPT, KB>(tile, jb, tt, f, k);
+                for (int jb = 0; jb < TSZ; jb += KB) far_block<TPT, KB>(tile, jb, tt, f, k);
                 continue;
             }
-            for (int jb = 0; jb < TS; jb += KB) {
+            for (int jb = 0; jb < TSZ; jb += KB) {
                 // A block of KB sources that is far for the whole wave (its
                 // sub-block spheres tested as the tile's was) takes the far
                 // path: most blocks of a tile that is near some lane are far
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NV_BS32_WAVE
                         const float dd = c0 * c0 + c1 * c1 + c2 * c2;
                         const float mag = fabsf(tt[q][0]) + fabsf(tt[q][1]) + fabsf(tt[q][2]) + srad;
                         const float lim = (srad + rnf) * 1.0001f + 1.0e-6f * mag;
-                        const bool fq = dd > lim * lim && (unsigned)(bpos[q] - (gt * TS + sb * SUB)) >= (unsigned)SUB;
+                        const bool fq = dd > lim * lim && (unsigned)(bpos[q] - (gt * TSZ + sb * SUB)) >= (unsigned)SUB;
                         bfar = bfar && (fq || !act[q]);
                     }
                 }
@@ -677,32 +680,44 @@ int nvk_occupancy_bs64() {
     return b;
 }
 
-template <int TPT, int KB>
+template <int TPT, int KB, int TSZ>
 int occ_bs32c() {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<TPT, KB>, TB, 0) != hipSuccess || b < 1) b = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bs32c<TPT, KB, TSZ>, TB, 0) != hipSuccess || b < 1) b = 2;
     return b;
 }
 
-int nvk_occupancy_bs32c(int tpt, int kb) {
-    if (tpt == 2) {
-        switch (kb) {
-            case 1: return occ_bs32c<2, 1>();
-            case 2: return occ_bs32c<2, 2>();
-            case 4: return occ_bs32c<2, 4>();
-            case 16: return occ_bs32c<2, 16>();
-            case 32: return occ_bs32c<2, 32>();
-            default: return occ_bs32c<2, 8>();
-        }
-    }
-    switch (kb) {
-        case 1: return occ_bs32c<1, 1>();
-        case 2: return occ_bs32c<1, 2>();
-        case 4: return occ_bs32c<1, 4>();
-        case 16: return occ_bs32c<1, 16>();
-        case 32: return occ_bs32c<1, 32>();
-        default: return occ_bs32c<1, 8>();
-    }
+// The FP32C variants: points per lane (1, 2), Kahan block (1, 2, 4, 8, 16,
+// 32; 4 and up with tiles of 128 sources) and sources per LDS tile (128,
+// 256).  F(TPT, KB, TSZ) is expanded for the variant asked for.
+#define NV_BS32_DISPATCH(F, tpt, kb, ts)                                         \
+    do {                                                                         \
+        const int kb_ = (ts) == 128 && (kb) < 4 ? 4 : (kb);                      \
+        if ((ts) == 128) {                                                       \
+            if ((tpt) == 2) {                                                    \
+                if (kb_ == 4) F(2, 4, 128); else if (kb_ == 16) F(2, 16, 128);   \
+                else if (kb_ == 32) F(2, 32, 128); else F(2, 8, 128);            \
+            } else {                                                             \
+                if (kb_ == 4) F(1, 4, 128); else if (kb_ == 16) F(1, 16, 128);   \
+                else if (kb_ == 32) F(1, 32, 128); else F(1, 8, 128);            \
+            }                                                                    \
+        } else if ((tpt) == 2) {                                                 \
+            if (kb_ == 8) F(2, 8, 256); else if (kb_ == 16) F(2, 16, 256);       \
+            else if (kb_ == 32) F(2, 32, 256); else if (kb_ == 4) F(2, 4, 256);  \
+            else if (kb_ == 2) F(2, 2, 256); else F(2, 1, 256);                  \
+        } else {                                                                 \
+            if (kb_ == 8) F(1, 8, 256); else if (kb_ == 16) F(1, 16, 256);       \
+            else if (kb_ == 32) F(1, 32, 256); else if (kb_ == 4) F(1, 4, 256);  \
+            else if (kb_ == 2) F(1, 2, 256); else F(1, 1, 256);                  \
+        }                                                                        \
+    } while (0)
+
+int nvk_occupancy_bs32c(int tpt, int kb, int ts) {
+    int b = 2;
+#define NV_OCC(T_, K_, S_) b = occ_bs32c<T_, K_, S_>()
+    NV_BS32_DISPATCH(NV_OCC, tpt, kb, ts);
+#undef NV_OCC
+    return b;
 }
 
 hipError_t nvk_src64(hipStream_t st, DState s, int rows, double *rec) {
@@ -716,7 +731,12 @@ hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int
     return hipGetLastError();
 }
 
-size_t nvk_tileinfo_bytes() { return sizeof(TileInfo); }
+// the tile records for `rows` rows with either tile size
+size_t nvk_tileinfo_bytes(int rows) {
+    const size_t a = sizeof(TileInfo<256>) * (size_t)(rows / 256 + 1);
+    const size_t b = sizeof(TileInfo<128>) * (size_t)(rows / 128 + 1);
+    return a > b ? a : b;
+}
 
 hipError_t nvk_order(hipStream_t st, DState s, Tangles tg, int rows, const double *box, unsigned long long *k0,
                      int *v0, unsigned long long *k1, int *v1, int *iperm, int *hist, int *scan_tmp) {
@@ -731,37 +751,25 @@ hipError_t nvk_order(hipStream_t st, DState s, Tangles tg, int rows, const doubl
     return hipGetLastError();
 }
 
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles) {
-    if (rows > 0)
-        hipLaunchKernelGGL(k_src32, dim3((rows + TS - 1) / TS), dim3(TS), 0, st, s, rows, perm, (Rec32 *)rec,
-                           (TileInfo *)tiles);
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles, int ts) {
+    if (rows <= 0) return hipGetLastError();
+    if (ts == 128)
+        hipLaunchKernelGGL(k_src32<128>, dim3((rows + 127) / 128), dim3(128), 0, st, s, rows, perm, (Rec32 *)rec,
+                           (TileInfo<128> *)tiles);
+    else
+        hipLaunchKernelGGL(k_src32<256>, dim3((rows + 255) / 256), dim3(256), 0, st, s, rows, perm, (Rec32 *)rec,
+                           (TileInfo<256> *)tiles);
     return hipGetLastError();
 }
 
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
                      const void *rec, const void *tiles, const int *iperm, const double *rec64, const Shifts *sh,
-                     double kappa,
-                     float rn2, double *part, int tpt, int kb) {
+                     double kappa, float rn2, double *part, int tpt, int kb, int ts) {
     const Rec32 *r = (const Rec32 *)rec;
-    const TileInfo *ti = (const TileInfo *)tiles;
-#define NV_BS32(TPT_, KB_)                                                                                   \
-    hipLaunchKernelGGL((k_bs32c<TPT_, KB_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r, ti, iperm, \
-                       rec64, *sh, kappa, rn2, part)
-    if (tpt == 2) {
-        if (kb == 8) NV_BS32(2, 8);
-        else if (kb == 16) NV_BS32(2, 16);
-        else if (kb == 32) NV_BS32(2, 32);
-        else if (kb == 4) NV_BS32(2, 4);
-        else if (kb == 2) NV_BS32(2, 2);
-        else NV_BS32(2, 1);
-    } else {
-        if (kb == 8) NV_BS32(1, 8);
-        else if (kb == 16) NV_BS32(1, 16);
-        else if (kb == 32) NV_BS32(1, 32);
-        else if (kb == 4) NV_BS32(1, 4);
-        else if (kb == 2) NV_BS32(1, 2);
-        else NV_BS32(1, 1);
-    }
+#define NV_BS32(T_, K_, S_)                                                                                     \
+    hipLaunchKernelGGL((k_bs32c<T_, K_, S_>), dim3(grid), dim3(TB), 0, st, s, tg, items, nitems, queue, r,        \
+                       (const TileInfo<S_> *)tiles, iperm, rec64, *sh, kappa, rn2, part)
+    NV_BS32_DISPATCH(NV_BS32, tpt, kb, ts);
 #undef NV_BS32
     return hipGetLastError();
 }

@@ -85,7 +85,8 @@ struct Ctx {
     std::string err;
     TreeBuf *tree = nullptr;
     bool tree_valid = false;
-    int tpt = 2, kb = 8;             // FP32C kernel variant: targets per lane, compensation block
+    int tpt = 2, kb = 8, ts = 256;   // FP32C kernel variant: targets per lane, compensation block,
+                                     // sources per LDS tile
     int ipc = 16;                    // direct-sum work items per CU
     int occ64 = 4, occ32 = 4;        // resident workgroups per CU of the persistent kernels
     // recorded steps, [x/xn parity][reconnection phase on], valid while
@@ -207,7 +208,7 @@ int alloc_scratch(Ctx *c) {
     c->rs_cap = std::min(rows, 32768);
     HIPCHK(c, c->rstage.reserve(nvk_recstage_bytes() * (size_t)std::max(1, c->rs_cap)));
     HIPCHK(c, c->rec32.reserve(nvk_rec32_bytes() * (size_t)rows));
-    HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes() * (size_t)(rows / 256 + 1)));
+    HIPCHK(c, c->tinfo.reserve(nvk_tileinfo_bytes(rows)));
     HIPCHK(c, c->sph.reserve(nvk_sphere_bytes() * (size_t)(rows / 64 + 4)));
     HIPCHK(c, c->okey0.reserve(rows));
     HIPCHK(c, c->okey1.reserve(rows));
@@ -543,7 +544,7 @@ int ensure_room(Ctx *c) {
 // With cap_rows the list covers every row of each tangle's range, so a
 // recorded launch stays valid while pcount grows; the kernels skip blocks
 // past the tangle's current pcount.
-int build_items(Ctx *c, int tpt, bool exact, bool cap_rows, int *nitems_out) {
+int build_items(Ctx *c, int tpt, int ts, bool exact, bool cap_rows, int *nitems_out) {
     const int K = c->K;
     std::vector<int> order(K);
     std::iota(order.begin(), order.end(), 0);
@@ -552,7 +553,7 @@ int build_items(Ctx *c, int tpt, bool exact, bool cap_rows, int *nitems_out) {
     std::vector<int> ntb(K), U(K);
     for (int k = 0; k < K; ++k) {
         ntb[k] = (c->tn[k] + 256 * tpt - 1) / (256 * tpt);
-        U[k] = c->sh.n * ((c->tn[k] + 255) / 256);
+        U[k] = c->sh.n * ((c->tn[k] + ts - 1) / ts);
         work += (long)ntb[k] * U[k];
     }
     const long want = (long)c->ipc * c->ncu;
@@ -666,7 +667,7 @@ int velocity_impl(Ctx *c, int do_ab) {
     } else {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
         int nitems = 0;
-        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, false, &nitems)) return rc;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, fp32 ? c->ts : 256, c->prm.exact_order != 0, false, &nitems)) return rc;
         HIPCHK(c, c->queue.reserve(1));
         HIPCHK(c, nv_fill_int(c->st, c->queue.p, 0, 1));
         HIPCHK(c, nvk_src64(c->st, c->d, R, c->rec64.p));
@@ -674,12 +675,12 @@ int velocity_impl(Ctx *c, int do_ab) {
         if (fp32) {
             if (int rc = order_sources(c, false)) return rc;
             ++c->order_age;
-            HIPCHK(c, nvk_src32(c->st, c->d, R, c->operm.p, c->rec32.p, c->tinfo.p));
+            HIPCHK(c, nvk_src32(c->st, c->d, R, c->operm.p, c->rec32.p, c->tinfo.p, c->ts));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
             HIPCHK(c, hipEventRecord(c->e2, c->st));
             HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
-                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
+                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb, c->ts));
             HIPCHK(c, hipEventRecord(c->e3, c->st));
             mode = 1;
         } else {
@@ -817,11 +818,11 @@ int enqueue_step(Ctx *c, DState d, bool recon, int nitems, int npitems) {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
         HIPCHK(c, nvk_src64(st, d, R, c->rec64.p));
         if (fp32) {
-            HIPCHK(c, nvk_src32(st, d, R, c->operm.p, c->rec32.p, c->tinfo.p));
+            HIPCHK(c, nvk_src32(st, d, R, c->operm.p, c->rec32.p, c->tinfo.p, c->ts));
             const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
             const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
             HIPCHK(c, nvk_bs32c(st, d, tv, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
-                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
+                                c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb, c->ts));
             mode = 1;
         } else {
             const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));
@@ -1011,7 +1012,7 @@ int record_steps(Ctx *c) {
     int nitems = 0, npitems = 0;
     if (c->prm.velocity == NV_VEL_BS) {
         const bool fp32 = c->prm.precision == NV_PREC_FP32C;
-        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, true, &nitems)) return rc;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, fp32 ? c->ts : 256, c->prm.exact_order != 0, true, &nitems)) return rc;
         HIPCHK(c, c->queue.reserve(1));
     } else {
         HIPCHK(c, c->d_S.reserve(c->K));
@@ -1274,6 +1275,7 @@ int nv_create(const nv_params *params, void **out) {
     // term on the 16k-point ring tangle, 1.27x faster)
     c->kb = (p.kahan_block == 1 || p.kahan_block == 2 || p.kahan_block == 4 || p.kahan_block == 8 ||
              p.kahan_block == 16 || p.kahan_block == 32) ? p.kahan_block : 8;
+    c->ts = p.source_tile == 128 ? 128 : 256;
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
@@ -1290,7 +1292,7 @@ int nv_create(const nv_params *params, void **out) {
     }
     c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     c->occ64 = nvk_occupancy_bs64();
-    c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb);
+    c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb, c->ts);
     std::memset(c->ctl_h, 0, sizeof(DCtl));
     (void)hipMemcpy(c->ctl, c->ctl_h, sizeof(DCtl), hipMemcpyHostToDevice);
     c->tree = nv_tree_new();
@@ -1788,6 +1790,7 @@ int nv_get_stats(void *h, nv_stats *s) {
     c->stats.targets_per_lane = c->prm.precision == NV_PREC_FP32C ? c->tpt : 1;
     c->stats.kahan_block = c->prm.precision == NV_PREC_FP32C ? c->kb : 0;
     c->stats.items_per_cu = c->ipc;
+    c->stats.source_tile = c->prm.precision == NV_PREC_FP32C ? c->ts : 256;
     *s = c->stats;
     return NV_OK;
 }
@@ -1814,33 +1817,39 @@ int nv_autotune(void *h, const char *cache_path) {
     char key[160];
     std::snprintf(key, sizeof key, "%s|%s|K=%d|n=%d|m=%d", arch.c_str(), fp32 ? "fp32c" : "fp64",
                   c->K == 1 ? 1 : 1 << lg(c->K), lg(total), lg(largest));
-    auto apply = [&](int tpt, int kb, int ipc) {
+    auto apply = [&](int tpt, int kb, int ipc, int ts) {
         c->tpt = tpt;
         c->kb = kb;
         c->ipc = ipc;
-        c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb);
+        c->ts = ts;
+        c->occ32 = nvk_occupancy_bs32c(c->tpt, c->kb, c->ts);
         ++c->layout_gen;   // recorded steps hold the old variant's work lists
+    };
+    auto valid = [](int t, int k, int i, int ts) {
+        return (t == 1 || t == 2) && (k == 1 || k == 2 || k == 4 || k == 8 || k == 16 || k == 32) && i >= 1 &&
+               i <= 256 && (ts == 256 || (ts == 128 && k >= 4));
     };
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "r")) {
             char line[512];
-            int ht = 0, hk = 0, hi = 0;
+            int ht = 0, hk = 0, hi = 0, hs = 256;
             double hm = 0;
             bool hit = false;
             while (std::fgets(line, sizeof line, f)) {
                 char k2[160];
-                int t2, b2, i2;
+                int t2, b2, i2, s2 = 256;
                 double m2;
-                if (std::sscanf(line, "%159s tpt=%d kb=%d ipc=%d ms=%lf", k2, &t2, &b2, &i2, &m2) == 5 &&
+                // lines written before the tile size was tuned have no ts=
+                if ((std::sscanf(line, "%159s tpt=%d kb=%d ipc=%d ts=%d ms=%lf", k2, &t2, &b2, &i2, &s2, &m2) == 6 ||
+                     (s2 = 256, std::sscanf(line, "%159s tpt=%d kb=%d ipc=%d ms=%lf", k2, &t2, &b2, &i2, &m2) == 5)) &&
                     std::strcmp(k2, key) == 0) {
-                    ht = t2; hk = b2; hi = i2; hm = m2;
+                    ht = t2; hk = b2; hi = i2; hs = s2; hm = m2;
                     hit = true;   // the last matching line wins
                 }
             }
             std::fclose(f);
-            if (hit && (ht == 1 || ht == 2) && (hk == 1 || hk == 2 || hk == 4 || hk == 8 || hk == 16 || hk == 32) && hi >= 1 &&
-                hi <= 256) {
-                apply(ht, hk, hi);
+            if (hit && valid(ht, hk, hi, hs)) {
+                apply(ht, hk, hi, hs);
                 c->stats.tuned = 2;
                 c->stats.tune_ms = hm;
                 return NV_OK;
@@ -1849,15 +1858,16 @@ int nv_autotune(void *h, const char *cache_path) {
     }
     // time one variant: warm-up launches, then reps timed launches
     int warm = 1, reps = 3;
-    auto time_variant = [&](int tpt, int kb, int ipc, float *ms_out) -> int {
-        apply(tpt, kb, ipc);
+    auto time_variant = [&](int tpt, int kb, int ipc, int ts, float *ms_out) -> int {
+        apply(tpt, kb, ipc, ts);
         int nitems = 0;
-        if (int rc = build_items(c, fp32 ? c->tpt : 1, c->prm.exact_order != 0, false, &nitems)) return rc;
+        if (int rc = build_items(c, fp32 ? c->tpt : 1, fp32 ? c->ts : 256, c->prm.exact_order != 0, false, &nitems))
+            return rc;
         HIPCHK(c, c->queue.reserve(1));
         HIPCHK(c, nvk_src64(c->st, c->d, c->rows, c->rec64.p));
         if (fp32) {
             if (int rc = order_sources(c, false)) return rc;
-            HIPCHK(c, nvk_src32(c->st, c->d, c->rows, c->operm.p, c->rec32.p, c->tinfo.p));
+            HIPCHK(c, nvk_src32(c->st, c->d, c->rows, c->operm.p, c->rec32.p, c->tinfo.p, c->ts));
         }
         Tangles tg = tangles(c, c->d_S.p, c->d_poff.p);
         const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
@@ -1869,7 +1879,7 @@ int nv_autotune(void *h, const char *cache_path) {
             if (fp32) {
                 const int grid = std::max(1, std::min(nitems, c->occ32 * c->ncu));
                 HIPCHK(c, nvk_bs32c(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec32.p, c->tinfo.p, c->iperm.p,
-                                    c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb));
+                                    c->rec64.p, &c->sh, c->phys.kappa, (float)(rn * rn), c->part.p, c->tpt, c->kb, c->ts));
             } else {
                 const int grid = std::max(1, std::min(nitems, c->occ64 * c->ncu));
                 HIPCHK(c, nvk_bs64(c->st, c->d, tg, c->items.p, nitems, c->queue.p, grid, c->rec64.p, &c->sh,
@@ -1888,36 +1898,40 @@ int nv_autotune(void *h, const char *cache_path) {
     };
     const bool free_tpt = fp32 && c->prm.targets_per_lane == 0;
     const bool free_kb = fp32 && c->prm.kahan_block == 0;
-    int bt = fp32 ? c->tpt : 1, bk = c->kb, bi = c->ipc;
+    const bool free_ts = fp32 && c->prm.source_tile == 0;
+    int bt = fp32 ? c->tpt : 1, bk = c->kb, bi = c->ipc, bs = fp32 ? c->ts : 256;
     float best = 0.f;
-    if (int rc = time_variant(bt, bk, bi, &best)) return rc;
+    if (int rc = time_variant(bt, bk, bi, bs, &best)) return rc;
     // keep the whole search to a few seconds on large batches
     if (best > 200.f) { warm = 0; reps = 1; }
     else if (best > 20.f) reps = 2;
-    auto consider = [&](int t, int k, int i) -> int {
-        if (t == bt && k == bk && i == bi) return NV_OK;
+    auto consider = [&](int t, int k, int i, int ts) -> int {
+        if ((t == bt && k == bk && i == bi && ts == bs) || !valid(t, k, i, ts)) return NV_OK;
         float ms = 0.f;
-        if (int rc = time_variant(t, k, i, &ms)) return rc;
+        if (int rc = time_variant(t, k, i, ts, &ms)) return rc;
         if (ms < best) {
             best = ms;
-            bt = t; bk = k; bi = i;
+            bt = t; bk = k; bi = i; bs = ts;
         }
         return NV_OK;
     };
     for (int i : {8, 16, 32})
-        if (int rc = consider(bt, bk, i)) return rc;
+        if (int rc = consider(bt, bk, i, bs)) return rc;
     if (free_tpt)
         for (int t : {1, 2})
-            if (int rc = consider(t, bk, bi)) return rc;
+            if (int rc = consider(t, bk, bi, bs)) return rc;
     if (free_kb)
         for (int k : {4, 8, 16, 32})
-            if (int rc = consider(bt, k, bi)) return rc;
-    apply(bt, bk, bi);
+            if (int rc = consider(bt, k, bi, bs)) return rc;
+    if (free_ts)
+        for (int ts : {128, 256})
+            if (int rc = consider(bt, bk, bi, ts)) return rc;
+    apply(bt, bk, bi, bs);
     c->stats.tuned = 1;
     c->stats.tune_ms = best;
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s tpt=%d kb=%d ipc=%d ms=%.4f\n", key, bt, bk, bi, best);
+            std::fprintf(f, "%s tpt=%d kb=%d ipc=%d ts=%d ms=%.4f\n", key, bt, bk, bi, bs, best);
             std::fclose(f);
         }
     }

@@ -44,11 +44,11 @@ hipError_t nvk_bs64(hipStream_t st, DState s, Tangles tg, const Item *items, int
 // spatial order of the FP32 source records: perm (in v0) and its inverse
 hipError_t nvk_order(hipStream_t st, DState s, Tangles tg, int rows, const double *box, unsigned long long *k0,
                      int *v0, unsigned long long *k1, int *v1, int *iperm, int *hist, int *scan_tmp);
-hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles);
+// ts: sources per tile of the FP32C direct sum (128 or 256)
+hipError_t nvk_src32(hipStream_t st, DState s, int rows, const int *perm, void *rec, void *tiles, int ts);
 hipError_t nvk_bs32c(hipStream_t st, DState s, Tangles tg, const Item *items, int nitems, int *queue, int grid,
                      const void *rec, const void *tiles, const int *iperm, const double *rec64, const Shifts *sh,
-                     double kappa,
-                     float rn2, double *part, int tpt, int kb);
+                     double kappa, float rn2, double *part, int tpt, int kb, int ts);
 hipError_t nvk_finish(hipStream_t st, DState s, Phys p, Tangles tg, int rows, const double *part, int mode,
                       int do_ab, DCtl *ctl, const double *mir);
 // solid walls (k_walls.hip, k_topology.hip): the image vortices' sum
@@ -62,9 +62,9 @@ hipError_t nvk_wall_recon(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ct
 hipError_t nvk_premove_walls(hipStream_t st, DState s, Phys p, Tangles tg, DCtl *ctl);
 hipError_t nvk_reduce_probe(hipStream_t st, DState s, Tangles tg, int rows, const double *part, double *sink);
 size_t nvk_rec32_bytes();
-size_t nvk_tileinfo_bytes();
+size_t nvk_tileinfo_bytes(int rows);
 int nvk_occupancy_bs64();
-int nvk_occupancy_bs32c(int tpt, int kb);
+int nvk_occupancy_bs32c(int tpt, int kb, int ts);
 
 // move the state to a wider layout (k_topology.hip)
 hipError_t nvk_relayout(hipStream_t st, DState o, DState nu, const int *row2t, const int *base_old,

@@ -33,7 +33,9 @@ def read_noise(path: str) -> np.ndarray:
     return vals[:100000].copy()
 
 
-def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.NvParams:
+def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0, variant: dict | None = None) -> _L.NvParams:
+    """variant: FP32C direct-sum kernel fields of nv_params (targets_per_lane,
+    kahan_block, source_tile) to fix instead of the defaults / the autotuner."""
     s = _L.NvParams()
     s.delta, s.dt, s.quant_circ, s.corea = p.delta, p.dt, p.quant_circ, p.corea
     for k in range(3):
@@ -57,17 +59,22 @@ def nv_params(p: RunParams, exact_order: bool = False, capacity: int = 0) -> _L.
     s.step_launch = {"graph": 0, "eager": 1}[p.hip_step_launch]
     s.exact_order = 1 if exact_order else 0
     s.capacity = capacity
+    for k, v in (variant or {}).items():
+        if k not in ("targets_per_lane", "kahan_block", "source_tile"):
+            raise ValueError("not a direct-sum variant field: %s" % k)
+        setattr(s, k, int(v))
     return s
 
 
 class FilamentSolver:
-    def __init__(self, params: RunParams, exact_order: bool = False, capacity: int = 0, check: bool = True):
+    def __init__(self, params: RunParams, exact_order: bool = False, capacity: int = 0, check: bool = True,
+                 variant: dict | None = None):
         if check:
             validate(params)
         self.params = params
         self.L = _L.load()
         self.h = C.c_void_p()
-        rc = self.L.nv_create(C.byref(nv_params(params, exact_order, capacity)), C.byref(self.h))
+        rc = self.L.nv_create(C.byref(nv_params(params, exact_order, capacity, variant)), C.byref(self.h))
         if rc != 0:
             raise _L.NuvortError(rc, "nv_create rejected the parameters")
         self._events = []
@@ -274,7 +281,8 @@ class FilamentSolver:
         (cached per GPU architecture and size class when a path is given)."""
         self._call("nv_autotune", cache_path.encode() if cache_path else None)
         st = self.stats()
-        return {k: st[k] for k in ("targets_per_lane", "kahan_block", "items_per_cu", "tuned", "tune_ms")}
+        return {k: st[k] for k in ("targets_per_lane", "kahan_block", "source_tile", "items_per_cu", "tuned",
+                                   "tune_ms")}
 
     def stats(self) -> dict:
         s = _L.NvStats()

@@ -11,6 +11,7 @@ program layout_check
   write(*,'(a,i0)') 'nv_params.precision ', loc(p%precision) - loc(p)
   write(*,'(a,i0)') 'nv_params.kahan_block ', loc(p%kahan_block) - loc(p)
   write(*,'(a,i0)') 'nv_params.step_launch ', loc(p%step_launch) - loc(p)
+  write(*,'(a,i0)') 'nv_params.source_tile ', loc(p%source_tile) - loc(p)
   write(*,'(a,i0)') 'nv_state.t ', loc(s%t) - loc(s)
   write(*,'(a,i0)') 'nv_state.wpinned ', loc(s%wpinned) - loc(s)
 end program

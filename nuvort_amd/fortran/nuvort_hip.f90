@@ -46,6 +46,7 @@ module nuvort_hip
     integer(c_int32_t) :: targets_per_lane
     integer(c_int32_t) :: kahan_block
     integer(c_int32_t) :: step_launch
+    integer(c_int32_t) :: source_tile
   end type
 
   type, bind(C) :: nv_state

@@ -173,7 +173,7 @@ contains
     p%near_radius = nearr
     p%close_search = NV_CLOSE_AUTO
     p%exact_order = 0 ; p%capacity = 0 ; p%targets_per_lane = 0 ; p%kahan_block = 0
-    p%step_launch = 0
+    p%step_launch = 0 ; p%source_tile = 0
     dt_out = dtv ; theta_out = theta
   end subroutine
 

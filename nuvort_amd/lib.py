@@ -31,7 +31,7 @@ class NvParams(C.Structure):
                 ("overide_timestep_check", C.c_int32), ("precision", C.c_int32),
                 ("near_radius", C.c_double), ("close_search", C.c_int32), ("exact_order", C.c_int32),
                 ("capacity", C.c_int32), ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32),
-                ("step_launch", C.c_int32)]
+                ("step_launch", C.c_int32), ("source_tile", C.c_int32)]
 
 
 class NvState(C.Structure):
@@ -70,7 +70,8 @@ class NvStats(C.Structure):
                 ("walk_launches", C.c_int64), ("walk_ms_total", C.c_double), ("walk_nodes_total", C.c_int64),
                 ("targets_per_lane", C.c_int32), ("kahan_block", C.c_int32), ("items_per_cu", C.c_int32),
                 ("tuned", C.c_int32), ("tune_ms", C.c_double), ("tree_builds", C.c_int32),
-                ("tree_sync_builds", C.c_int32), ("tree_replays", C.c_int32), ("tree_redone", C.c_int32)]
+                ("tree_sync_builds", C.c_int32), ("tree_replays", C.c_int32), ("tree_redone", C.c_int32),
+                ("source_tile", C.c_int32), ("pad", C.c_int32)]
 
 
 EXPORTS = ("nv_create", "nv_destroy", "nv_last_error", "nv_set_state", "nv_get_state", "nv_set_state_batch",

@@ -43,6 +43,7 @@ def test_fortran_types_have_the_c_layout():
     assert int(got["nv_params.precision"]) == nvlib.NvParams.precision.offset
     assert int(got["nv_params.kahan_block"]) == nvlib.NvParams.kahan_block.offset
     assert int(got["nv_params.step_launch"]) == nvlib.NvParams.step_launch.offset
+    assert int(got["nv_params.source_tile"]) == nvlib.NvParams.source_tile.offset
     assert int(got["nv_state.t"]) == nvlib.NvState.t.offset
     assert int(got["nv_state.wpinned"]) == nvlib.NvState.wpinned.offset
 

@@ -31,6 +31,7 @@ def test_autotune_keeps_state_and_accuracy(tmp_path):
     t = s.autotune(cache)
     assert t["tuned"] == 1 and t["tune_ms"] > 0
     assert t["targets_per_lane"] in (1, 2) and t["kahan_block"] in (4, 8, 16, 32) and t["items_per_cu"] in (8, 16, 32)
+    assert t["source_tile"] in (128, 256)
     after = s.get_state()
     for f in FIELDS:
         assert np.array_equal(getattr(before, f), getattr(after, f)), f
@@ -51,8 +52,8 @@ def test_autotune_keeps_state_and_accuracy(tmp_path):
     s2.set_state(tangles.random_rings(22, 24, 0.01, 0.03, 0.1, 5e-4))
     t2 = s2.autotune(cache)
     if t2["tuned"] == 2:
-        assert (t2["targets_per_lane"], t2["kahan_block"], t2["items_per_cu"]) == \
-               (t["targets_per_lane"], t["kahan_block"], t["items_per_cu"])
+        assert (t2["targets_per_lane"], t2["kahan_block"], t2["items_per_cu"], t2["source_tile"]) == \
+               (t["targets_per_lane"], t["kahan_block"], t["items_per_cu"], t["source_tile"])
     else:   # a different size class (log2 of the point count changed)
         assert t2["tuned"] == 1
     s2.close()
