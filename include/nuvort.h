@@ -207,7 +207,11 @@ int nv_set_noise(void *ctx, int32_t n, const double *noise);
  * form of the same path).  The arrays hold the tangles one after another,
  * tangle k with pcounts[k] rows and links local to it (1-based); every
  * phase and nv_step then advance all tangles together, the direct sums of
- * all of them in one launch.  The octree path takes one tangle. */
+ * all of them in one launch.  With tree_theta > 0 each tangle gets its own
+ * octree and device buffers on a stream of its own; nv_step launches every
+ * tangle's recorded step before waiting for any.  The calls that return
+ * one tangle's data (nv_get_state, nv_get_tree, the on-disk formats and the
+ * diagnostics) take a single-tangle state. */
 int nv_set_state_batch(void *ctx, int32_t ntangles, const int32_t *pcounts, const nv_state *host);
 int nv_get_state_batch(void *ctx, int32_t ntangles, int32_t *pcounts, nv_state *host);
 

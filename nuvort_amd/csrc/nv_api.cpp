@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -107,6 +108,12 @@ struct Ctx {
     DevBuf<int> operm, oval1, iperm, ohist, oscan;
     long state_gen = 0, order_gen = -1;
     int order_age = 0;
+    // A batch of tangles with an octree (tree_theta > 0): one context per
+    // tangle, each with its own octree, buffers and stream, to which this
+    // context forwards every call (K and tn mirror theirs).
+    std::vector<Ctx *> subs;
+    std::vector<double> noise_h;     // the noise table, for contexts made after nv_set_noise
+    bool replay_pending = false;     // a recorded octree step launched, not yet finished
 };
 
 constexpr int ORDER_EVERY = 8;       // steps between two source orderings
@@ -1128,46 +1135,63 @@ int record_tree_steps(Ctx *c) {
 // The first step, a step whose build outgrew the recorded plan (the replay
 // then did nothing past the build) and any step that cannot be recorded run
 // tree_step; the plan is recorded again from the sizes of the latest build.
-int step_tree_recorded(Ctx *c, int nsteps) {
+// Launch one recorded octree step, or take it eagerly where it cannot be
+// recorded; tree_replay_finish waits for a launched one.  Several contexts
+// (the tangles of a batch) launch their steps before any waits, so the
+// steps run concurrently.
+int tree_replay_launch(Ctx *c) {
     hipStream_t st = c->st;
-    for (int k = 0; k < nsteps; ++k) {
-        if (int rc = ensure_room(c)) return rc;
-        if (c->d.x != c->xa && c->d.xn != c->xa) return fail(c, NV_ERR_STATE, "position buffers lost");
-        const int par = c->d.x == c->xa ? 0 : 1;
-        const bool recon = (c->itime + 1) % c->prm.recon_shots == 0;
-        Ctx::StepGraph &g = c->sg[par][recon ? 1 : 0];
-        if (g.gen != c->layout_gen || g.tgen != nv_tree_graph_gen(c->tree) || g.sig != tree_step_sig(c) || !g.e) {
-            const int rc = record_tree_steps(c);
-            if (rc == NV_ERR_STATE || rc == NV_ERR_UNSUPPORTED) {
-                if (int rs = tree_step(c)) return rs;
-                continue;
-            }
-            if (rc) return rc;
-        }
-        HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
-        HIPCHK(c, hipGraphLaunch(g.e, st));
-        if (int rc = nv_tree_graph_take_async(c->tree, st)) return fail(c, rc, "tree counters");
-        const int rc = read_ctl(c);
-        if (c->ctl_h->abort) {
-            // the build outgrew the plan and the replay stopped after it: the
-            // state is the step's start, taken again eagerly
-            HIPCHK(c, nv_fill_int(st, &c->ctl->abort, 0, 1));
-            ++c->stats.tree_redone;
-            for (auto &row : c->sg)
-                for (auto &gg : row) drop_graph(gg);
-            if (int rs = tree_step(c)) return rs;
-            continue;
-        }
-        c->itime += 1;
-        c->t = c->t + c->prm.dt;
-        swap_x(c);
-        c->tree_valid = false;
-        c->stats.point_updates += c->ctl_h->nvel;
-        ++c->stats.tree_replays;
-        if (nv_tree_graph_take(c->tree, &c->stats))
-            for (auto &row : c->sg)
-                for (auto &gg : row) drop_graph(gg);
+    c->replay_pending = false;
+    const bool recorded = c->prm.step_launch == 0 && c->prm.velocity == NV_VEL_TREE &&
+                          c->prm.close_search != NV_CLOSE_BRUTE && c->K == 1;
+    if (!recorded) return tree_step(c);
+    if (int rc = ensure_room(c)) return rc;
+    if (c->d.x != c->xa && c->d.xn != c->xa) return fail(c, NV_ERR_STATE, "position buffers lost");
+    const int par = c->d.x == c->xa ? 0 : 1;
+    const bool recon = (c->itime + 1) % c->prm.recon_shots == 0;
+    Ctx::StepGraph &g = c->sg[par][recon ? 1 : 0];
+    if (g.gen != c->layout_gen || g.tgen != nv_tree_graph_gen(c->tree) || g.sig != tree_step_sig(c) || !g.e) {
+        const int rc = record_tree_steps(c);
+        if (rc == NV_ERR_STATE || rc == NV_ERR_UNSUPPORTED) return tree_step(c);
         if (rc) return rc;
+    }
+    HIPCHK(c, nvk_clock_set(st, c->ctl, c->t, c->itime));
+    HIPCHK(c, hipGraphLaunch(g.e, st));
+    if (int rc = nv_tree_graph_take_async(c->tree, st)) return fail(c, rc, "tree counters");
+    c->replay_pending = true;
+    return NV_OK;
+}
+
+int tree_replay_finish(Ctx *c) {
+    if (!c->replay_pending) return NV_OK;
+    c->replay_pending = false;
+    hipStream_t st = c->st;
+    const int rc = read_ctl(c);
+    if (c->ctl_h->abort) {
+        // the build outgrew the plan and the replay stopped after it: the
+        // state is the step's start, taken again eagerly
+        HIPCHK(c, nv_fill_int(st, &c->ctl->abort, 0, 1));
+        ++c->stats.tree_redone;
+        for (auto &row : c->sg)
+            for (auto &gg : row) drop_graph(gg);
+        return tree_step(c);
+    }
+    c->itime += 1;
+    c->t = c->t + c->prm.dt;
+    swap_x(c);
+    c->tree_valid = false;
+    c->stats.point_updates += c->ctl_h->nvel;
+    ++c->stats.tree_replays;
+    if (nv_tree_graph_take(c->tree, &c->stats))
+        for (auto &row : c->sg)
+            for (auto &gg : row) drop_graph(gg);
+    return rc;
+}
+
+int step_tree_recorded(Ctx *c, int nsteps) {
+    for (int k = 0; k < nsteps; ++k) {
+        if (int rc = tree_replay_launch(c)) return rc;
+        if (int rc = tree_replay_finish(c)) return rc;
     }
     return NV_OK;
 }
@@ -1229,6 +1253,181 @@ int set_phys(Ctx *c) {
 int compact(Ctx *c, DevBuf<int> &flags, DevBuf<int> &list, int *count) {
     HIPCHK(c, nv_compact_flags(c->st, flags.p, c->rows, list.p, count, c->tmp.p));
     return NV_OK;
+}
+
+// ------------------------------------------------------- octree batches
+// A batch of tangles with an octree (tree_theta > 0) runs one context per
+// tangle (Ctx::subs): each has its own octree, its own buffers and stream,
+// and the batch context forwards every call to them in tangle order.  A
+// step launches every tangle's recorded octree step before waiting for any,
+// so the tangles' steps run concurrently on the device.
+bool forest(const Ctx *c) { return !c->subs.empty(); }
+
+void drop_subs(Ctx *c) {
+    for (Ctx *s : c->subs) nv_destroy(s);
+    c->subs.clear();
+}
+
+// a call on every tangle's context; an error names the tangle
+template <class F>
+int each_sub(Ctx *c, F f) {
+    for (size_t k = 0; k < c->subs.size(); ++k) {
+        const int rc = f(c->subs[k]);
+        if (rc) return fail(c, rc, "tangle " + std::to_string(k) + ": " + c->subs[k]->err);
+    }
+    return NV_OK;
+}
+
+// tangle k's part of the host arrays: the tangles are concatenated, n rows
+// from row o on
+nv_state tangle_view(const nv_state *hs, long o, int n) {
+    nv_state v = *hs;
+    v.pcount = n;
+    v.capacity = n;
+    double **d3[] = {&v.x, &v.u, &v.u1, &v.u2, &v.u_sup, &v.u_mf, &v.ghosti, &v.ghostb};
+    for (double **p : d3)
+        if (*p) *p += 3 * o;
+    if (v.closestd) v.closestd += o;
+    if (v.curv) v.curv += o;
+    if (v.stretch) v.stretch += o;
+    int32_t **i1[] = {&v.infront, &v.behind, &v.closest, &v.pinnedi, &v.pinnedb};
+    for (int32_t **p : i1)
+        if (*p) *p += o;
+    if (v.wpinned) v.wpinned += 3 * o;
+    return v;
+}
+
+int set_forest(Ctx *c, int K, const int32_t *pcounts, const nv_state *hs) {
+    long total = 0;
+    for (int k = 0; k < K; ++k) {
+        if (pcounts[k] < 0) return fail(c, NV_ERR_ARG, "negative pcount");
+        total += pcounts[k];
+    }
+    if (hs->capacity < total) return fail(c, NV_ERR_ARG, "state arrays smaller than the sum of pcounts");
+    drop_subs(c);
+    for (auto &row : c->sg)
+        for (auto &g : row) drop_graph(g);
+    free_state(c);
+    ++c->layout_gen;
+    c->K = 0;
+    c->tree_valid = false;
+    for (int k = 0; k < K; ++k) {
+        void *h = nullptr;
+        int rc = nv_create(&c->prm, &h);
+        if (rc) {
+            drop_subs(c);
+            return fail(c, rc, "tangle " + std::to_string(k) + ": context");
+        }
+        Ctx *s = (Ctx *)h;
+        c->subs.push_back(s);
+        if (!c->noise_h.empty() && (rc = nv_set_noise(s, (int32_t)c->noise_h.size(), c->noise_h.data()))) {
+            const std::string m = s->err;
+            drop_subs(c);
+            return fail(c, rc, "tangle " + std::to_string(k) + ": " + m);
+        }
+    }
+    long o = 0;
+    for (int k = 0; k < K; ++k) {
+        nv_state v = tangle_view(hs, o, pcounts[k]);
+        // the batch's counters belong to its first tangle, as in a batch
+        // without an octree
+        if (k > 0) v.recon_count = v.remove_count = v.wall_recon_count = 0;
+        if (int rc = nv_set_state(c->subs[k], &v)) {
+            const std::string m = c->subs[k]->err;
+            drop_subs(c);
+            return fail(c, rc, "tangle " + std::to_string(k) + ": " + m);
+        }
+        o += pcounts[k];
+    }
+    c->K = K;
+    c->tn.assign(pcounts, pcounts + K);
+    c->t = hs->t;
+    c->itime = hs->itime;
+    return NV_OK;
+}
+
+int get_forest(Ctx *c, int32_t *pcounts, nv_state *hs) {
+    const int K = (int)c->subs.size();
+    long total = 0;
+    for (int k = 0; k < K; ++k) {
+        Ctx *s = c->subs[k];
+        if (int rc = read_ctl(s)) (void)rc;
+        c->tn[k] = s->tn[0];
+        total += s->tn[0];
+    }
+    if (pcounts)
+        for (int k = 0; k < K; ++k) pcounts[k] = c->tn[k];
+    if (hs->capacity < total) {
+        hs->pcount = (int32_t)total;
+        return fail(c, NV_ERR_CAPACITY, "nv_get_state: caller arrays hold fewer rows than pcount");
+    }
+    long o = 0;
+    int32_t rec = 0, rem = 0, wall = 0;
+    for (int k = 0; k < K; ++k) {
+        nv_state v = tangle_view(hs, o, c->tn[k]);
+        if (int rc = nv_get_state(c->subs[k], &v)) return fail(c, rc, "tangle " + std::to_string(k) + ": " + c->subs[k]->err);
+        rec += v.recon_count;
+        rem += v.remove_count;
+        wall += v.wall_recon_count;
+        o += c->tn[k];
+    }
+    hs->pcount = (int32_t)total;
+    hs->t = c->subs[0]->t;
+    hs->itime = c->subs[0]->itime;
+    hs->recon_count = rec;
+    hs->remove_count = rem;
+    hs->wall_recon_count = wall;
+    return NV_OK;
+}
+
+// nsteps steps of every tangle: each step launched on every tangle's
+// stream, then finished tangle by tangle
+int step_forest(Ctx *c, int nsteps) {
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = NV_OK;
+        size_t q = 0;
+        for (; q < c->subs.size(); ++q)
+            if ((rc = tree_replay_launch(c->subs[q]))) break;
+        // every launched step is finished, also after an error
+        for (size_t u = 0; u < c->subs.size(); ++u) {
+            const int rf = tree_replay_finish(c->subs[u]);
+            if (!rc && rf) {
+                rc = rf;
+                q = u;
+            }
+        }
+        if (rc) return fail(c, rc, "tangle " + std::to_string(q) + ": " + c->subs[q]->err);
+    }
+    for (size_t k = 0; k < c->subs.size(); ++k) c->tn[k] = c->subs[k]->tn[0];
+    c->t = c->subs[0]->t;
+    c->itime = c->subs[0]->itime;
+    return NV_OK;
+}
+
+// the batch's statistics: counters summed over its tangles, the kernel
+// variant of the first
+void forest_stats(Ctx *c, nv_stats *out) {
+    nv_stats a{};
+    for (size_t k = 0; k < c->subs.size(); ++k) {
+        nv_stats s{};
+        nv_get_stats(c->subs[k], &s);
+        if (k == 0) {
+            a = s;
+            continue;
+        }
+        a.pair_evals += s.pair_evals; a.near_evals += s.near_evals; a.tree_evals += s.tree_evals;
+        a.tree_nodes_visited += s.tree_nodes_visited; a.tree_nodes += s.tree_nodes;
+        a.tree_levels = std::max(a.tree_levels, s.tree_levels);
+        a.ms_velocity += s.ms_velocity; a.ms_tree_build += s.ms_tree_build; a.ms_close += s.ms_close;
+        a.ms_topology += s.ms_topology; a.point_updates += s.point_updates;
+        a.pair_launches += s.pair_launches; a.pair_ms_total += s.pair_ms_total;
+        a.pair_evals_total += s.pair_evals_total; a.walk_launches += s.walk_launches;
+        a.walk_ms_total += s.walk_ms_total; a.walk_nodes_total += s.walk_nodes_total;
+        a.tree_builds += s.tree_builds; a.tree_sync_builds += s.tree_sync_builds;
+        a.tree_replays += s.tree_replays; a.tree_redone += s.tree_redone;
+    }
+    a.ms_step = c->stats.ms_step;
+    *out = a;
 }
 
 }  // namespace
@@ -1303,6 +1502,7 @@ int nv_create(const nv_params *params, void **out) {
 int nv_destroy(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    drop_subs(c);
     (void)hipStreamSynchronize(c->st);
     for (auto &row : c->sg)
         for (auto &g : row) drop_graph(g);
@@ -1338,6 +1538,7 @@ int nv_set_state(void *h, const nv_state *hs) {
     // own copies of the build-time data, so a close search after a state
     // upload sees the stale tree exactly as pclose_tree does
     const int32_t n = hs->pcount;
+    drop_subs(c);
     return set_batch(c, 1, &n, hs, nullptr);
 }
 
@@ -1355,6 +1556,8 @@ int nv_get_state(void *h, nv_state *hs) {
 int nv_set_state_batch(void *h, int32_t ntangles, const int32_t *pcounts, const nv_state *hs) {
     Ctx *c = (Ctx *)h;
     if (!c || !hs || !pcounts || ntangles < 1) return NV_ERR_ARG;
+    if (ntangles > 1 && c->prm.tree_theta > 0) return set_forest(c, ntangles, pcounts, hs);
+    drop_subs(c);
     return set_batch(c, ntangles, pcounts, hs, nullptr);
 }
 
@@ -1362,6 +1565,7 @@ int nv_get_state_batch(void *h, int32_t ntangles, int32_t *pcounts, nv_state *hs
     Ctx *c = (Ctx *)h;
     if (!c || !hs || !pcounts) return NV_ERR_ARG;
     if (ntangles != c->K) return fail(c, NV_ERR_ARG, "ntangles differs from the uploaded batch");
+    if (forest(c)) return get_forest(c, pcounts, hs);
     return get_batch(c, pcounts, hs);
 }
 
@@ -1370,6 +1574,7 @@ int nv_set_time(void *h, double t, int32_t itime) {
     if (!c) return NV_ERR_ARG;
     c->t = t;
     c->itime = itime;
+    if (forest(c)) return each_sub(c, [&](Ctx *s) { return nv_set_time(s, t, itime); });
     return NV_OK;
 }
 
@@ -1379,6 +1584,9 @@ int nv_set_noise(void *h, int32_t n, const double *noise) {
     if (n < 1 || !noise) return fail(c, NV_ERR_ARG, "nv_set_noise: the table needs at least one value");
     for (int32_t q = 0; q < n; ++q)
         if (std::isnan(noise[q])) return fail(c, NV_ERR_ARG, "nv_set_noise: NaN in the noise table");
+    c->noise_h.assign(noise, noise + n);
+    if (forest(c))
+        if (int rc = each_sub(c, [&](Ctx *s) { return nv_set_noise(s, n, noise); })) return rc;
     c->noise.release();
     HIPCHK(c, c->noise.reserve((size_t)n));
     HIPCHK(c, hipMemcpy(c->noise.p, noise, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1392,6 +1600,7 @@ int nv_set_noise(void *h, int32_t n, const double *noise) {
 int nv_ghostp(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_ghostp(s); });
     HIPCHK(c, nvk_ghosts(c->st, c->d, c->phys, c->rows));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
@@ -1400,6 +1609,7 @@ int nv_ghostp(void *h) {
 int nv_construct_tree(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_construct_tree(s); });
     if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
     int rc = nv_tree_build(c->tree, c->st, c->d, c->phys, c->tn[0], c->ctl, &c->stats);
@@ -1417,18 +1627,21 @@ int nv_construct_tree(void *h) {
 int nv_velocity(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_velocity(s); });
     return velocity_impl(c, 0);
 }
 
 int nv_pmotion(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_pmotion(s); });
     return velocity_impl(c, 1);
 }
 
 int nv_pinsert(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_pinsert(s); });
     if (c->K == 0) return NV_OK;
     if (int rc = ensure_room(c)) return rc;
     Tangles tg = tangles(c, nullptr, nullptr);
@@ -1450,6 +1663,7 @@ int nv_pinsert(void *h) {
 int nv_pclose(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_pclose(s); });
     if (c->K == 0) return NV_OK;
     if (c->phys.walls) {
         HIPCHK(c, hipEventRecord(c->e2, c->st));
@@ -1473,6 +1687,7 @@ int nv_pclose(void *h) {
 int nv_pclose_tree(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_pclose_tree(s); });
     if (!c->tree_valid) return fail(c, NV_ERR_STATE, "pclose_tree without construct_tree");
     if (c->K != 1) return fail(c, NV_ERR_UNSUPPORTED, "the octree path runs one tangle");
     HIPCHK(c, hipEventRecord(c->e2, c->st));
@@ -1491,6 +1706,7 @@ int nv_pclose_tree(void *h) {
 int nv_precon(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_precon(s); });
     if (c->K == 0) return NV_OK;
     Tangles tg = tangles(c, nullptr, nullptr);
     if (c->phys.walls) {
@@ -1509,6 +1725,7 @@ int nv_precon(void *h) {
 int nv_wall_recon(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_wall_recon(s); });
     // without solid walls there are no pinned points and no wall to reach:
     // the reference's loop changes nothing
     if (c->K == 0 || !c->phys.walls) return NV_OK;
@@ -1519,6 +1736,7 @@ int nv_wall_recon(void *h) {
 int nv_premove(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_premove(s); });
     if (c->K == 0) return NV_OK;
     Tangles tg = tangles(c, nullptr, nullptr);
     if (c->phys.walls) {
@@ -1534,6 +1752,7 @@ int nv_premove(void *h) {
 int nv_enforce_boundary(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_enforce_boundary(s); });
     HIPCHK(c, nvk_wrap(c->st, c->d, c->phys, c->rows));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return NV_OK;
@@ -1542,6 +1761,7 @@ int nv_enforce_boundary(void *h) {
 int nv_empty_tree(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_empty_tree(s); });
     c->tree_valid = false;
     return NV_OK;
 }
@@ -1549,6 +1769,7 @@ int nv_empty_tree(void *h) {
 int nv_nan_check(void *h) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
+    if (forest(c)) return each_sub(c, [](Ctx *s) { return nv_nan_check(s); });
     HIPCHK(c, nvk_nan(c->st, c->d, c->rows, c->ctl));
     return read_ctl(c);
 }
@@ -1557,6 +1778,14 @@ int nv_step(void *h, int32_t nsteps) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     const int total = nsteps;
+    if (forest(c)) {
+        if (noise_missing(c)) return fail(c, NV_ERR_STATE, "normal_velocity noisey_xflow without a noise table (nv_set_noise)");
+        const auto w0 = std::chrono::steady_clock::now();
+        const int rc = step_forest(c, nsteps);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        c->stats.ms_step = total > 0 ? ms / total : 0.0;
+        return rc;
+    }
     if (noise_missing(c)) return fail(c, NV_ERR_STATE, "normal_velocity noisey_xflow without a noise table (nv_set_noise)");
     hipEvent_t s0, s1;
     HIPCHK(c, hipEventCreate(&s0));
@@ -1676,6 +1905,7 @@ int nv_data_dump(void *h, const char *path) {
 int nv_data_restore(void *h, const char *path) {
     Ctx *c = (Ctx *)h;
     if (!c || !path) return NV_ERR_ARG;
+    if (forest(c)) return fail(c, NV_ERR_UNSUPPORTED, "nv_data_restore: upload one tangle first (a batch is loaded)");
     nv_state probe{};
     int rc = nv_var_read(path, &probe);
     if (rc != NV_ERR_CAPACITY && rc != NV_OK) return fail(c, NV_ERR_ARG, std::string("nv_data_restore: cannot read ") + path);
@@ -1739,6 +1969,7 @@ int nv_print_info(void *h, const nv_diag *d, int32_t shots, const char *path) {
 int nv_tree_print(void *h, const char *path) {
     Ctx *c = (Ctx *)h;
     if (!c || !path) return NV_ERR_ARG;
+    if (forest(c)) return fail(c, NV_ERR_UNSUPPORTED, "nv_tree_print: one tangle (a batch has one octree per tangle)");
     if (!c->tree_valid) return fail(c, NV_ERR_STATE, "nv_tree_print: no tree (construct_tree not called)");
     int n = 0;
     if (int rc = nv_tree_dump(c->tree, c->st, nullptr, 0, &n)) return fail(c, rc, "tree dump failed");
@@ -1756,6 +1987,29 @@ int nv_tree_print(void *h, const char *path) {
 int nv_get_recon_events(void *h, nv_recon_event *buf, int32_t cap, int32_t *n) {
     Ctx *c = (Ctx *)h;
     if (!c || !n) return NV_ERR_ARG;
+    if (forest(c)) {
+        // each tangle's log, its tangle index set, in the order below
+        int32_t total = 0, taken = 0;
+        for (size_t k = 0; k < c->subs.size(); ++k) {
+            int32_t nk = 0;
+            nv_recon_event *bk = buf ? buf + taken : nullptr;
+            const int32_t room = buf ? std::max<int32_t>(0, cap - taken) : 0;
+            if (int rc = nv_get_recon_events(c->subs[k], bk, room, &nk))
+                return fail(c, rc, "tangle " + std::to_string(k) + ": " + c->subs[k]->err);
+            const int32_t got = std::min(nk, room);
+            for (int32_t q = 0; q < got; ++q) bk[q].tangle = (int32_t)k;
+            taken += got;
+            total += nk;
+        }
+        if (buf)
+            std::stable_sort(buf, buf + taken, [](const nv_recon_event &a, const nv_recon_event &b) {
+                if (a.itime != b.itime) return a.itime < b.itime;
+                if (a.tangle != b.tangle) return a.tangle < b.tangle;
+                return a.i < b.i;
+            });
+        *n = total;
+        return NV_OK;
+    }
     if (int rc = read_ctl(c)) return rc;
     const int m = std::min(c->ctl_h->nevents, c->ev_cap);
     const int take = std::min(m, (int)cap);
@@ -1778,6 +2032,7 @@ int nv_get_recon_events(void *h, nv_recon_event *buf, int32_t cap, int32_t *n) {
 int nv_get_tree(void *h, nv_tree_node *buf, int32_t cap, int32_t *n) {
     Ctx *c = (Ctx *)h;
     if (!c || !n) return NV_ERR_ARG;
+    if (forest(c)) return fail(c, NV_ERR_UNSUPPORTED, "nv_get_tree: one tangle (a batch has one octree per tangle)");
     if (!c->tree_valid) return fail(c, NV_ERR_STATE, "no tree (construct_tree not called)");
     int rc = nv_tree_dump(c->tree, c->st, buf, cap, n);
     if (rc) return fail(c, rc, "tree dump failed");
@@ -1787,6 +2042,10 @@ int nv_get_tree(void *h, nv_tree_node *buf, int32_t cap, int32_t *n) {
 int nv_get_stats(void *h, nv_stats *s) {
     Ctx *c = (Ctx *)h;
     if (!c || !s) return NV_ERR_ARG;
+    if (forest(c)) {
+        forest_stats(c, s);
+        return NV_OK;
+    }
     c->stats.targets_per_lane = c->prm.precision == NV_PREC_FP32C ? c->tpt : 1;
     c->stats.kahan_block = c->prm.precision == NV_PREC_FP32C ? c->kb : 0;
     c->stats.items_per_cu = c->ipc;
@@ -1799,6 +2058,7 @@ int nv_autotune(void *h, const char *cache_path) {
     Ctx *c = (Ctx *)h;
     if (!c) return NV_ERR_ARG;
     if (c->prm.velocity != NV_VEL_BS) return NV_OK;   // only the direct sum has variants
+    if (forest(c)) return each_sub(c, [&](Ctx *s) { return nv_autotune(s, cache_path); });
     if (c->K == 0) return fail(c, NV_ERR_STATE, "nv_autotune: no state uploaded");
     if (cache_path && !cache_path[0]) cache_path = nullptr;   // "" = no cache
     const bool fp32 = c->prm.precision == NV_PREC_FP32C;
