@@ -1905,7 +1905,6 @@ int nv_data_dump(void *h, const char *path) {
 int nv_data_restore(void *h, const char *path) {
     Ctx *c = (Ctx *)h;
     if (!c || !path) return NV_ERR_ARG;
-    if (forest(c)) return fail(c, NV_ERR_UNSUPPORTED, "nv_data_restore: upload one tangle first (a batch is loaded)");
     nv_state probe{};
     int rc = nv_var_read(path, &probe);
     if (rc != NV_ERR_CAPACITY && rc != NV_OK) return fail(c, NV_ERR_ARG, std::string("nv_data_restore: cannot read ") + path);
@@ -1926,8 +1925,10 @@ int nv_data_restore(void *h, const char *path) {
     hs.pinnedb = i1.data() + 4 * (size_t)n;
     hs.wpinned = w3.data();
     if ((rc = nv_var_read(path, &hs))) return fail(c, NV_ERR_ARG, std::string("nv_data_restore: cannot read ") + path);
-    // the tree, like the reference's, is not part of the restart
+    // the tree, like the reference's, is not part of the restart; the
+    // restart replaces a batch as nv_set_state does
     c->tree_valid = false;
+    drop_subs(c);
     const int32_t pc = hs.pcount;
     return set_batch(c, 1, &pc, &hs, nullptr);
 }
