@@ -5,7 +5,8 @@
   reference's summation order and FP32C, links bit for bit.
 * 64 random rings (16,677 points, dt 5e-6 s, reconnection and re-meshing
   every step), 3 steps: links, slots and the reconnection log bit for bit,
-  positions to a tolerance, for the chunked FP64 kernel and for FP32C.
+  positions to a tolerance, for the chunked FP64 kernel and for FP32C; and
+  FP32C against FP64 on the GPU over 100 steps.
 
 Position differences are taken modulo the periodic box; the tolerances are
 written relative to max|u| times the elapsed time (the displacement scale).
@@ -108,3 +109,26 @@ def test_ring_tangle_trajectory(rings_oracle, prec, tol):
               % (prec, [1, 3][k], err, disp, err / disp, r.recon_count))
         assert err <= tol * disp
     assert [e[:3] for e in ev] == [e[:3] for e in ref_ev]
+
+
+def test_ring_tangle_fp32c_follows_fp64():
+    """The 64-ring tangle over 100 steps (21 reconnections): the FP32C run
+    keeps the FP64 run's links and reconnection log at steps 10, 50 and 100,
+    and its positions within 1e-6 of max|u| times the elapsed time (measured
+    8e-9).  The FP64 run is pinned to the oracle by the test above."""
+    p, st = rings()
+    got = {}
+    for prec in ("fp64", "fp32c"):
+        p.hip_precision = prec
+        got[prec] = run_gpu(p, st, [10, 50, 100])
+    (s64, ev64), (s32, ev32) = got["fp64"], got["fp32c"]
+    assert len(ev64) > 10
+    assert [e[:3] for e in ev32] == [e[:3] for e in ev64]
+    for k, (a, b) in enumerate(zip(s64, s32)):
+        assert b.pcount == a.pcount
+        assert np.array_equal(b.infront, a.infront) and np.array_equal(b.behind, a.behind)
+        act = a.infront != 0
+        disp = np.linalg.norm(a.u[act], axis=1).max() * p.dt * [10, 50, 100][k]
+        err = np.linalg.norm(wrapped_diff(b.x[act], a.x[act], 0.1), axis=1).max()
+        print("rings fp32c vs fp64 step %d: ratio %.3e" % ([10, 50, 100][k], err / disp))
+        assert err <= 1e-6 * disp
