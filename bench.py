@@ -444,7 +444,8 @@ def ensemble_workload():
 def ensemble_steps(p, sts, precision, warmup, steps):
     """Warm-up steps, then `steps` timed steps of the whole batch (wall
     clock around the nv_step calls, each of which synchronises before it
-    returns), split into up to 5 calls for the spread."""
+    returns), split into up to 5 calls for the spread; and the run's
+    reconnections as (step, tangle, i, j)."""
     from nuvort_amd.engine import FilamentSolver
     q = dataclasses.replace(p, hip_precision=precision)
     s = FilamentSolver(q)
@@ -463,9 +464,10 @@ def ensemble_steps(p, sts, precision, warmup, steps):
         per.append((time.perf_counter() - tb) / k * 1e3)
     sec = time.perf_counter() - t0
     s1 = s.stats()
+    events = [(e[0], e[7], e[1], e[2]) for e in s.recon_events()]
     s.close()
     return {"seconds": sec, "point_updates": s1["point_updates"] - s0["point_updates"], "block_ms_per_step": per,
-            "tuning": tune}
+            "tuning": tune, "events": events}
 
 
 def ensemble_velocity(p, sts, precision, reps):
@@ -664,6 +666,12 @@ def main():
         u32 = np.concatenate([g.u_sup[g.infront != 0] for g in g32])
         u64 = np.concatenate([g.u_sup[g.infront != 0] for g in g64])
         acc["fp32c_vs_fp64_gpu"] = rel_err_stats(u64, u32)
+        # the reconnection pairs of the steps both runs took from the same start
+        last = 1 + args.fp64_steps
+        e32 = sorted(e for e in head["events"] if e[0] <= last)
+        e64 = sorted(r64["events"])
+        acc["reconnections_fp32c_vs_fp64"] = {"steps": last, "fp32c": len(e32), "fp64": len(e64),
+                                              "identical": e32 == e64}
     t_kernel = k32["ms"] * 1e-3
     achieved = FLOP_PER_PAIR * k32["pair_evals"] / t_kernel / 1e12
     traffic = None
