@@ -786,6 +786,10 @@ __device__ __forceinline__ void near_term64(double v0, double v1, double v2, con
 // walks first, so that no SIMD idles while long walks finish.
 // targets per lane of the FP32 walk
 constexpr int WALK_TPT = 2;
+// The FP32 walk addresses its records (48-byte walk records, 80-byte full
+// records, 24-byte FP64 circulations) by 32-bit byte offsets: at most this
+// many records (about 25M points).
+constexpr long long WALK32_MAX_RECORDS = 0xFFFFFFFFll / (long long)sizeof(WNode);
 
 // The band around a node's opening radius squared r^2 inside which the walk
 // decides the reference's opening test in FP64 exactly.  The record holds
@@ -838,6 +842,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const double kappa = p.kappa;
     unsigned long long nev = 0, nvis = 0;   // nev: FP64 terms, per lane
     unsigned long long nevw = 0;            // FP32 terms, per wave
+    // visits and FP32 terms of this lane: counted per lane on the vector
+    // unit (a popcount of the wave's mask per block and iteration kept the
+    // scalar unit, which the walk's mask algebra already loads, 4% longer)
+    unsigned visl = 0, evl = 0;
     bool sing = false;
     for (;;) {
     int m, tb;
@@ -886,8 +894,11 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         // both possible successors are fetched while this node is evaluated
         // (the records end with one spare, so neither index needs a clamp)
         const int n1 = cur + 1, n2 = leaf ? cur + 1 : w;
-        const NodeHot a = *(const NodeHot *)(wb + (size_t)n1 * sizeof(Node48));
-        const NodeHot b = *(const NodeHot *)(wb + (size_t)n2 * sizeof(Node48));
+        // (byte offsets in 32 bits: the scalar loads take them as their
+        // offset register, without a 64-bit multiply and add on the scalar
+        // unit, which the walk keeps busy; records stay below 2^31 bytes)
+        const NodeHot a = *(const NodeHot *)(wb + (unsigned)n1 * (unsigned)sizeof(Node48));
+        const NodeHot b = *(const NodeHot *)(wb + (unsigned)n2 * (unsigned)sizeof(Node48));
         // (this file is compiled without machine sinking, which would move
         // these loads down to their use at the end of the iteration)
         // Every lane condition is a 64-bit mask built from ballots of raw
@@ -915,7 +926,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
             // ancestor) changes nothing here: skip its work (a wave-uniform
             // branch; about a sixth of the iterations for each block)
             if (TPT > 1 && mvis == 0ull) continue;
-            nvis += (unsigned long long)__popcll(mvis);
+            visl += lane_in(mvis) ? 1u : 0u;
 #pragma unroll
             for (int k = 0; k < 3; ++k) v[t][k] = nd.cent[k] - xs[t][k];
             dd[t] = __fma_rn(v[t][0], v[t][0], __fma_rn(v[t][1], v[t][1], v[t][2] * v[t][2]));
@@ -935,7 +946,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
             } else {
                 const unsigned long long mband = mvis & ~mgt & ~wave_ballot(ddf < r2lo);
                 if (mband != 0ull) {
-                    const double wdt = wn[cur].width, geo = wn[cur].geo;
+                    const WNode *wc = (const WNode *)((const char *)wn + (unsigned)cur * (unsigned)sizeof(WNode));
+                    const double wdt = wc->width, geo = wc->geo;
                     bool ok = false;
                     if (lane_in(mband)) ok = ref_accepts(s, r[t], nd.cent, sh.s[m], wdt, geo, p.tree_theta, p.extra_corr);
                     mgt |= wave_ballot(ok) & mband;
@@ -951,7 +963,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
             anyfar |= mfar[t];
         }
         if (anyfar != 0ull) {
-            const NodeCold cd = *(const NodeCold *)(wb + (size_t)cur * sizeof(Node48) + sizeof(NodeHot));
+            const NodeCold cd = *(const NodeCold *)(wb + (unsigned)cur * (unsigned)sizeof(Node48) + sizeof(NodeHot));
             const float e0 = cd.e[0], e1 = cd.e[1], e2 = cd.e[2];
             const float kee = 64.0f * 5.96e-8f * cd.ee;
 #pragma unroll
@@ -985,14 +997,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 fa[t][0] += (double)(x0 * fac);
                 fa[t][1] += (double)(x1 * fac);
                 fa[t][2] += (double)(x2 * fac);
-                nevw += (unsigned long long)__popcll(mok);
+                evl += lane_in(mok) ? 1u : 0u;
                 mneed[t] |= mfar[t] & mill;
             }
         }
 #pragma unroll
         for (int t = 0; t < TPT; ++t) anyneed |= mneed[t];
         if (anyneed != 0ull) {
-            const double c64[3] = {circ64[3 * cur], circ64[3 * cur + 1], circ64[3 * cur + 2]};
+            const double *cc = (const double *)((const char *)circ64 + (unsigned)cur * 24u);
+            const double c64[3] = {cc[0], cc[1], cc[2]};
 #pragma unroll
             for (int t = 0; t < TPT; ++t)
                 if (lane_in(mneed[t])) near_term64(v[t][0], v[t][1], v[t][2], c64, dd[t], kappa, nr[t], nev);
@@ -1018,8 +1031,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     }
     }
     if (sing && lane == 0) atomicOr(&ctl->status, NVK_SINGULAR);
-    // nvis counts the visits of the whole wave already
-    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
+    nev += evl;
+    nvis = visl;
+    for (int off = 32; off > 0; off >>= 1) {
+        nev += __shfl_down(nev, off);
+        nvis += __shfl_down(nvis, off);
+    }
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&counters[0], nev + nevw);
         atomicAdd(&counters[1], nvis);
@@ -1896,6 +1913,7 @@ int nv_tree_walk_velocity(TreeBuf *t, hipStream_t st, DState d, Phys p, const Sh
     const int nb64 = (n + 63) / 64, nitems = nb64 * sh.n;
     // the FP32 walk's single opening radius needs width / theta > geo
     const bool fp32 = precision == NV_PREC_FP32C && p.tree_theta <= 1.1;
+    if (fp32 && (long long)t->nwalk + 1 > WALK32_MAX_RECORDS) return NV_ERR_CAPACITY;
     if (fp32) {
         // one spare record past the last: the walk fetches a successor index
         // up to nwalk without a clamp
@@ -2040,8 +2058,9 @@ int nv_tree_graph_prepare(TreeBuf *t, hipStream_t st, int rows) {
     const int top = std::min(prev, t->maxlev);
     int fmax = 1;
     const long long total = plan_levels(t, prev, top, t->g_bound, t->g_wgrid, &fmax);
-    // the close search's records index their successors in CLOSE_SKIP_BITS bits
-    if (total + 1 >= (1ll << CLOSE_SKIP_BITS)) return NV_ERR_UNSUPPORTED;
+    // the close search's records index their successors in CLOSE_SKIP_BITS
+    // bits, the FP32 walk its records by 32-bit byte offsets
+    if (total + 1 >= (1ll << CLOSE_SKIP_BITS) || total + 1 > WALK32_MAX_RECORDS) return NV_ERR_UNSUPPORTED;
     for (int l = 0; l <= top; ++l) t->g_hint[l] = l < prev ? t->lvl[l + 1] - t->lvl[l] : 0;
     if (int rc = grow_nodes(t, st, (int)total, 0)) return rc;
     if (int rc = ensure_flags(t, fmax)) return rc;

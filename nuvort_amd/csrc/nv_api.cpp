@@ -667,6 +667,7 @@ int velocity_impl(Ctx *c, int do_ab) {
         const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
         int rc = nv_tree_walk_velocity(c->tree, c->st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision,
                                        rn * rn, c->part.p, c->ctl, &c->stats);
+        if (rc == NV_ERR_CAPACITY) return fail(c, rc, "FP32 tree walk: more records than 32-bit offsets reach");
         if (rc) return fail(c, NV_ERR_HIP, "tree walk launch failed");
         HIPCHK(c, hipEventRecord(c->e3, c->st));
         timed_walk = true;
@@ -869,9 +870,10 @@ int enqueue_tree_pmotion(Ctx *c) {
     HIPCHK(c, nv_fill_int(st, (int *)c->d_poff.p, 0, 2));   // one long long
     const double rn = std::max(c->prm.near_radius, 2.0) * c->prm.delta;
     HIPCHK(c, hipEventRecord(c->e2, st));
-    if (nv_tree_walk_velocity(c->tree, st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision, rn * rn, c->part.p,
-                              c->ctl, &c->stats))
-        return fail(c, NV_ERR_HIP, "tree walk launch failed");
+    if (int rw = nv_tree_walk_velocity(c->tree, st, c->d, c->phys, c->sh, c->tn[0], R, c->prm.precision, rn * rn,
+                                       c->part.p, c->ctl, &c->stats))
+        return rw == NV_ERR_CAPACITY ? fail(c, rw, "FP32 tree walk: more records than 32-bit offsets reach")
+                                     : fail(c, NV_ERR_HIP, "tree walk launch failed");
     HIPCHK(c, hipEventRecord(c->e3, st));
     if (int rc = enqueue_mirrors(c, c->d, false)) return rc;
     if (int rc = nv_tree_walk_counters_async(c->tree, st)) return fail(c, rc, "tree counters");
