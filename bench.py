@@ -616,8 +616,7 @@ def main():
                          "for scaling HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

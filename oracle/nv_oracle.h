@@ -2,9 +2,9 @@
  * Test infrastructure only: the oracle.
  *
  * A plain-C, FP64, loop-by-loop restatement of NuVort's filament velocity and
- * timestep path (src/run.f90:24-65 and the routines it calls).  It is the
- * checker of the HIP library's parity tests and the CPU baseline timed beside
- * it; nothing in the product links or calls it.  Every function cites the
+ * timestep path (src/run.f90:24-65 and the routines it calls).  The HIP
+ * library's parity tests compare against it, and bench.py times it as the CPU
+ * baseline; nothing in the product links or calls it.  Every function cites the
  * reference lines it restates; evaluation order follows the Fortran source
  * term by term (built with -ffp-contract=off) so that results are
  * bit-identical to the reference compiled without FMA.

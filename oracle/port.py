@@ -1,8 +1,8 @@
 """Test infrastructure only: ctypes access to the oracle's C restatement
 (libnv_oracle.so, built by ``make -C oracle port``).
 
-Used by tests/ as the checker and by bench.py's CPU-baseline leg; the
-product never imports this module.
+Used by tests/ as the parity reference and by bench.py's CPU-baseline leg;
+the product never imports this module.
 """
 from __future__ import annotations
 
