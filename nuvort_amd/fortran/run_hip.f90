@@ -18,8 +18,10 @@ program run_hip
   type(c_ptr) :: ctx
   integer(c_int) :: rc
   ! run.in values the loop itself needs
-  integer :: recon_shots
+  integer :: recon_shots, shots
+  logical :: tree_print
   real(c_double) :: dt, tree_theta
+  character(len=40) :: print_file
   ! the point array as structure of arrays
   integer :: n, itime, recon_count
   real(c_double) :: t
@@ -34,7 +36,7 @@ program run_hip
   call get_command_argument(2, fout)
   call get_command_argument(3, arg) ; read(arg,*) nrun
 
-  call read_params(prm, recon_shots, dt, tree_theta, noise_file)
+  call read_params(prm, recon_shots, dt, tree_theta, noise_file, shots, tree_print)
   rc = nv_create(prm, ctx)
   if (rc /= NV_OK) then
     write (*,*) 'nv_create rejected the run.in parameters, status ', rc
@@ -56,7 +58,14 @@ program run_hip
     itime = itime + 1
     call nv_check(ctx, nv_set_time(ctx, t, int(itime, c_int32_t)), 'set_time')
     call nv_check(ctx, nv_ghostp(ctx), 'boundary.mod:ghostp')
-    if (tree_theta > 0) call nv_check(ctx, nv_construct_tree(ctx), 'tree.mod:construct_tree')
+    if (tree_theta > 0) then
+      call nv_check(ctx, nv_construct_tree(ctx), 'tree.mod:construct_tree')
+      ! the leaf list build_tree appends every shots steps (src/tree.f90:96-102)
+      if (tree_print .and. mod(itime, shots) == 0) then
+        write(unit=print_file, fmt="(a,i3.3,a)") "./data/tree_mesh", itime/shots, ".log"
+        call nv_check(ctx, nv_tree_print(ctx, trim(print_file)//c_null_char), 'tree.mod:build_tree (tree_print)')
+      end if
+    end if
     call nv_check(ctx, nv_pmotion(ctx), 'timestep.mod:pmotion')
     call nv_check(ctx, nv_pinsert(ctx), 'line.mod:pinsert')
     if (mod(itime, recon_shots) == 0) then
@@ -91,9 +100,10 @@ program run_hip
 contains
 
   !> the reference's run.in semantics for the keys this library reads
-  subroutine read_params(p, rshots, dt_out, theta_out, nfile)
+  subroutine read_params(p, rshots, dt_out, theta_out, nfile, shots_out, tprint)
     type(nv_params), intent(out) :: p
-    integer, intent(out) :: rshots
+    integer, intent(out) :: rshots, shots_out
+    logical, intent(out) :: tprint
     real(c_double), intent(out) :: dt_out, theta_out
     character(len=30), intent(out) :: nfile
     character(len=100) :: buffer, label
@@ -108,6 +118,7 @@ contains
     bx = 'periodic' ; by = 'periodic' ; bz = 'periodic' ; vel = ' ' ; nf = 'zero'
     theta = 0 ; tec = .true. ; xflow = 0.5_c_double ; alpha = 0 ; sup = 0
     cutoff = 1E8_c_double ; tzero = 1E8_c_double ; rshots = 1 ; otc = .false.
+    shots_out = 1 ; tprint = .false.
     prec = 'fp64' ; nearr = 8 ; omega = 0 ; nfile = 'set_me!'
     ios = 0
     open(newunit=fh, file='run.in', status='old')
@@ -119,6 +130,8 @@ contains
       buffer = buffer(pos+1:)
       select case (label)
       case ('recon_shots') ; read(buffer, *, iostat=ios) rshots
+      case ('shots') ; read(buffer, *, iostat=ios) shots_out
+      case ('tree_print') ; read(buffer, *, iostat=ios) tprint
       case ('dt') ; read(buffer, *, iostat=ios) dtv
       case ('delta') ; read(buffer, *, iostat=ios) delta
       case ('quant_circ') ; read(buffer, *, iostat=ios) kappa

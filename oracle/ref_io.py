@@ -72,7 +72,9 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
     status and output are returned instead.  With ``output`` every step ends
     with the reference's output phase (diagnostics; var.dat, var%04d.log and
     the ts.log line every shots steps), whose files are returned as bytes
-    under "files".  ``state_path`` (a var.dat) replaces ``st`` as the input."""
+    under "files".  The tree_mesh%03d.log leaf lists build_tree writes with
+    tree_print on (src/tree.f90:96-102) are returned there in every run.
+    ``state_path`` (a var.dat) replaces ``st`` as the input."""
     os.makedirs(os.path.join(workdir, "data"), exist_ok=True)
     for name, text in (files or {}).items():
         with open(os.path.join(workdir, name), "w") as fh:
@@ -119,12 +121,12 @@ def run_reference(workdir: str, st: FilamentState, run_text: str, nsteps: int, d
             k, v = line.split()
             timing[k] = float(v)
     out["timing"] = timing
-    if output:
-        files = {}
-        dd = os.path.join(workdir, "data")
-        for name in sorted(os.listdir(dd)):
-            if name.startswith("var") or name in ("ts.log", "diag.log"):
-                with open(os.path.join(dd, name), "rb") as fh:
-                    files[name] = fh.read()
+    files = {}
+    dd = os.path.join(workdir, "data")
+    for name in sorted(os.listdir(dd)):
+        if name.startswith("tree_mesh") or (output and (name.startswith("var") or name in ("ts.log", "diag.log"))):
+            with open(os.path.join(dd, name), "rb") as fh:
+                files[name] = fh.read()
+    if output or files:
         out["files"] = files
     return out

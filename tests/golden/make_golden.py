@@ -55,11 +55,13 @@ ALL_PHASES = ("ghost", "motion", "insert", "close", "recon", "wall", "remove", "
 
 
 def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, phases=ALL_PHASES,
-         noise_seed: int | None = None) -> None:
+         noise_seed: int | None = None, keep_files=()) -> None:
     """Run the reference build and store the fixture.  The ghost phase (the
     input of every per-phase test) is always kept.  With ``noise_seed`` the
     run reads the noisey_xflow table tangles.noise_table(noise_seed) from
-    ``noise.dat``; the fixture stores the seed, not the table."""
+    ``noise.dat``; the fixture stores the seed, not the table.  ``keep_files``
+    names files of data/ the run writes (tree_print's leaf lists) to store as
+    file__<name>."""
     files = None
     if noise_seed is not None:
         p.noise_file = "noise.dat"
@@ -68,6 +70,8 @@ def make(name: str, st: FilamentState, p: RunParams, nsteps: int, note: str, pha
     with tempfile.TemporaryDirectory(dir="/tmp") as wd:
         out = ref_io.run_reference(wd, st, txt, nsteps, dump_phases=True, threads=1, files=files)
     d = {"run_text": np.array(txt), "nsteps": np.array(nsteps), "note": np.array(note)}
+    for fname in keep_files:
+        d["file__" + fname] = np.frombuffer(out["files"][fname], np.uint8)
     if noise_seed is not None:
         d["noise_seed"] = np.array(noise_seed)
     pack_state("in", st, d)
@@ -282,6 +286,9 @@ def main() -> None:
                  "the output phase over 2 steps, 1 restarted")
     make_restart("restart_walls", tangles.wall_tangle(41), base_params(boundary_x="solid", boundary_z="solid"), 2, 1,
                  "solid x and z walls: pinned points in var.dat, wall_recon_count in ts.log, 1 step restarted")
+    # tree_print (src/tree.f90:96-102): the leaf list of the step's octree
+    make("tree_mesh", rings, base_params(velocity="Tree", tree_theta=0.5, tree_print=True), 1,
+         "Tree theta 0.5 with tree_print: the leaf list of the step's octree", keep_files=("tree_mesh001.log",))
     make("tree_deep", deep_tree_state(d5), base_params(velocity="Tree", tree_theta=0.5), 1,
          "Tree theta 0.5 with point pairs 4e-7 and 5e-12 cm apart (octree deeper than 16 and 32 levels)")
     # solid walls at x = +-L/2 and z = +-L/2, y periodic: pinned lines and

@@ -11,7 +11,11 @@ second run restarted from the var.dat the first one wrote.
     FP64 sums run in another order), and the ts.log line's text;
   * nv_data_dump, then nv_data_restore in a fresh context and more steps:
     bit for bit the run that was not interrupted (FP64; the direct sum's
-    chunks and the octree are deterministic).
+    chunks and the octree are deterministic);
+  * nv_tree_print against the tree_mesh001.log the reference's build_tree
+    wrote (the tree_mesh fixture, tree_print on): every leaf's box and point
+    count, value for value (the reference's list-directed text wraps its
+    records; the values are compared, as a reader of the file reads them).
 """
 import dataclasses
 
@@ -130,3 +134,25 @@ def test_dump_restore_continue_bit_exact(name, tmp_path):
     assert got.recon_count == want.recon_count
     for f in COMPARE:
         assert np.array_equal(getattr(got, f), getattr(want, f)), f
+
+
+def tree_mesh_values(text: str) -> np.ndarray:
+    """A tree_mesh file's records: x0, x1, y0, y1, z0, z1, pcount per leaf."""
+    v = np.array([float(t) for t in text.split()])
+    assert v.size % 7 == 0
+    return v.reshape(-1, 7)
+
+
+def test_tree_print_matches_reference(tmp_path):
+    fx = Fixture("tree_mesh")
+    s = solver(fx.params)
+    s.set_state(fx.state("ghost"))
+    s.construct_tree()
+    out = tmp_path / "tree_mesh001.log"
+    s.tree_print(str(out))
+    s.close()
+    ref = tree_mesh_values(fx.file("tree_mesh001.log").decode())
+    got = tree_mesh_values(out.read_text())
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    assert int((fx.tree()["pcount"] <= 1).sum()) == ref.shape[0]
