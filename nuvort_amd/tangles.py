@@ -69,23 +69,26 @@ def ring(center, radius, normal, delta, phase=0.0, chord=0.98):
 
 
 def random_rings(seed: int, nrings: int, rmin: float, rmax: float, box: float, delta: float,
-                 target_points: int | None = None) -> FilamentState:
+                 target_points: int | None = None, sides: tuple | None = None) -> FilamentState:
     """Rings with radii ~U[rmin, rmax], centres ~U(box), normals uniform on
     the sphere (normalised Gaussian), phase ~U[0, 2pi).  If ``target_points``
-    is given, rings are added until the point count reaches it instead."""
+    is given, rings are added until the point count reaches it instead.
+    ``sides`` gives a box of unequal sides instead of the cube of side box:
+    the centres are drawn in it and the points wrapped into it."""
     g = _rng(seed)
+    lo = -np.asarray(sides, np.float64) / 2.0 if sides is not None else -box / 2.0
     loops, total = [], 0
     k = 0
     while (target_points is None and k < nrings) or (target_points is not None and total < target_points):
         r = g.uniform(rmin, rmax)
-        c = g.uniform(-box / 2.0, box / 2.0, size=3)
+        c = g.uniform(lo, -lo, size=3)
         nv = g.normal(size=3)
         ph = g.uniform(0.0, 2.0 * math.pi)
         pts = ring(c, r, nv, delta, ph)
         loops.append(pts)
         total += len(pts)
         k += 1
-    return _assemble(loops, (box, box, box))
+    return _assemble(loops, tuple(sides) if sides is not None else (box, box, box))
 
 
 def kelvin_lattice(seed: int, box: float = 0.1, nx: int = 4, ny: int = 2, amp: float = 0.002,

@@ -9,8 +9,15 @@ Each case runs a few whole steps (nv_step, recorded) from a seeded tangle in
 FP64 and compares with the oracle's run: point count, links, reconnection
 count and the reconnection log bit for bit, positions to 1e-10 of the box
 (the full-run tolerance of the fixture tests: the chunked FP64 sums and the
-octree's internal moments are rounded in another order).
+octree's internal moments are rounded in another order).  The same cases'
+first-step velocities in FP32C against the oracle's FP64: within 1e-5 per
+point for the direct sum; for the FP32 tree walk within 1e-4 per point and
+1e-5 RMS, the tolerance of its node terms (their FP32 rounding, amplified by
+the cancellation between them).  Relative errors are per point, over the
+points whose speed is at least 1e-3 of the largest.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -63,7 +70,7 @@ def oracle_run(cache, cid, st, p, steps):
 def test_sweep_matches_oracle(oracle_runs, cid, seed, nrings, p, steps):
     from nuvort_amd.engine import FilamentSolver
     box = max(p.box_size)
-    st = tangles.random_rings(seed, nrings, 0.004, 0.01, min(p.box_size), p.delta)
+    st = tangles.random_rings(seed, nrings, 0.004, 0.01, box, p.delta, sides=p.box_size)
     print("%s: %d points, %d steps" % (cid, st.active, steps), flush=True)
     ref, log = oracle_run(oracle_runs, cid, st, p, steps)
     s = FilamentSolver(p, check=False)
@@ -87,3 +94,29 @@ def test_sweep_matches_oracle(oracle_runs, cid, seed, nrings, p, steps):
     assert np.abs(d).max() <= 1e-10 * box
     print("%s: %d points after, %d reconnections, max |dx| %.1e" % (cid, ref.active, ref.recon_count,
                                                                     np.abs(d).max()))
+
+
+@pytest.mark.parametrize("cid,seed,nrings,p,steps", CASES, ids=[c[0] for c in CASES])
+def test_sweep_fp32c_velocity(cid, seed, nrings, p, steps):
+    from nuvort_amd.engine import FilamentSolver
+    st = tangles.random_rings(seed, nrings, 0.004, 0.01, max(p.box_size), p.delta, sides=p.box_size)
+    ph = port.run_phases(st, p, threads=16)
+    assert ph["rc_motion"] == 0
+    ghost, ref = ph["ghost"], ph["motion"]
+    s = FilamentSolver(dataclasses.replace(p, hip_precision="fp32c"), check=False)
+    s.set_state(ghost)
+    if p.tree_theta > 0:
+        s.construct_tree()
+    s.pmotion()
+    got = s.get_state()
+    s.close()
+    act = ref.infront != 0
+    for field in ("u_sup", "u"):
+        a, b = getattr(ref, field)[act], getattr(got, field)[act]
+        mag = np.linalg.norm(a, axis=1)
+        rel = np.linalg.norm(b - a, axis=1)[mag >= 1e-3 * mag.max()] / mag[mag >= 1e-3 * mag.max()]
+        print("%s %s: max rel %.2e, rms %.2e" % (cid, field, rel.max(), np.sqrt(np.mean(rel ** 2))))
+        if p.tree_theta > 0:
+            assert rel.max() <= 1e-4 and np.sqrt(np.mean(rel ** 2)) <= 1e-5
+        else:
+            assert rel.max() <= 1e-5
